@@ -1,0 +1,252 @@
+"""Benchmark of the HIP rasterizer's hot path: one fwd+bwd of a 2048x2048 field of 200k Gaussians.
+
+A step is one pass of the renderer's hot path over the resident field: tile binning (no host
+synchronization), tiled forward, tiled backward with the normalizer recomputed, against a fixed
+synthetic dL/dout.  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+baseline).  `value` is the algorithmic HBM throughput of the whole step:
+
+    B_step = B_bin + B_fwd + B_bwd
+    B_bin  = 52 N + 4 P + 4 (T+1)                     (means, conics, radii in; lists, offsets out)
+    B_fwd  = 4 (T+1) + P (4 + R) + 12 H W             (one list sweep; RGB out)
+    B_bwd  = 12 H W + 2 P (4 + R) + 36 P + 36 N       (grad in; two list sweeps; one 9-float
+                                                        atomic set per pair; gradients)
+with R = 44 bytes per staged Gaussian record, N Gaussians, P (tile, Gaussian) pairs and T tiles.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|single|stress]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+RECORD_BYTES = 44  # algorithmic bytes of one staged Gaussian record
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "metric": ("fwd+bwd 2048x2048 RGB fp32, 200,000 Gaussians: scales log-U[0.5,12] px, aspect "
+               "log-U[1,10] (major s, minor s/a), smoothed random angle field, opacity U[0.1,1], "
+               "normalized, sigma_cutoff 3, tile 16, seed 0", 2048, 2048, 200_000),
+    "single": ("fwd+bwd 1024x1024 RGB fp32, 50,000 Gaussians, same recipe, seed 0",
+               1024, 1024, 50_000),
+    "stress": ("fwd+bwd 4096x4096 RGB fp32, 1,000,000 Gaussians: scales log-normal(ln 2, 1) "
+               "clipped at 64 px, aspect log-U[1,20], seed 0", 4096, 4096, 1_000_000),
+}
+
+
+def algorithmic_bytes(N: int, P: int, T: int, HW: int) -> dict[str, float]:
+    R = RECORD_BYTES
+    b_bin = 52 * N + 4 * P + 4 * (T + 1)
+    b_fwd = 4 * (T + 1) + P * (4 + R) + 12 * HW
+    b_bwd = 12 * HW + 2 * P * (4 + R) + 36 * P + 36 * N
+    return {"bin": b_bin, "fwd": b_fwd, "bwd": b_bwd, "step": b_bin + b_fwd + b_bwd}
+
+
+def make_field(workload: str, device):
+    import torch
+
+    from structsplat_amd import synth
+    from structsplat_amd.gaussians import GaussianField
+
+    if workload == "metric":
+        f = synth.metric_field(0)
+    elif workload == "single":
+        f = synth.single_field(0)
+    else:
+        f = synth.stress_field(0)
+    logits = torch.logit(torch.tensor(f["opacity"], dtype=torch.float64)).float().numpy()
+    field = GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
+                                     opacities=logits, device=device)
+    with torch.no_grad():
+        means = field.means.contiguous()
+        conics = field.conics().contiguous()
+        colors = field.colors.contiguous()
+        radii = field.radii(3.0).contiguous()
+        op = field.opacity_values().contiguous()
+    return means, conics, colors, radii, op
+
+
+def cpu_baseline(workload: str, H: int, W: int, bytes_step: float, reps: int = 2) -> dict:
+    """The oracle's torch restatement of the reference CPU path (render.py _accumulate + torch
+    autograd), fwd+bwd of the same field, timed on this host's cores."""
+    import torch
+
+    from oracle import reference_render as R
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    means, conics, colors, radii, op = (t.cpu() for t in make_field(workload, "cpu"))
+    grad = torch.full((H, W, 3), 1.0 / (3 * H * W))
+
+    def one():
+        leaves = [t.detach().clone().requires_grad_() for t in (means, conics, colors, op)]
+        img = R.render(leaves[0], leaves[1], leaves[2], radii, H, W, chunk=512,
+                       opacities=leaves[3])
+        img.backward(grad)
+
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        one()
+        times.append(time.perf_counter() - t0)
+    sec = statistics.median(times)
+    return {
+        "value": round(bytes_step / sec / 1e9, 4),
+        "unit": "GB/s",
+        "cores": torch.get_num_threads(),
+        "kind": "port",
+        "sample": (f"whole {workload} workload, fwd+bwd of the torch restatement of render.py "
+                   f"(float32, render_chunk 512, autograd), median of {reps}: "
+                   f"{sec * 1e3:.1f} ms per fwd+bwd"),
+        "seconds_per_step": round(sec, 4),
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the renderer's fields are independent units and this "
+              "build runs on a single MI355X; run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from structsplat_amd import render as HR
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    desc, H, W, _ = WORKLOADS[args.workload]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    means, conics, colors, radii, op = make_field(args.workload, dev)
+    N = means.shape[0]
+    T = HR.num_tiles(H, W, 16)
+    gen = torch.Generator(device=dev).manual_seed(0)
+    grad_out = (torch.rand(H, W, 3, device=dev, generator=gen) - 0.5) / (3 * H * W)
+
+    # Size the pair buffer once (outside the timed region), then bin without host syncs.
+    gids0, offsets0 = HR.build_tile_index(means, conics, radii, H, W)
+    P = int(offsets0[-1].item())
+    capacity = P + P // 8 + 1024
+    gids = torch.empty(capacity, dtype=torch.int32, device=dev)
+    offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = HR.workspace(N, H, W, 16, dev)
+    out = torch.empty(H, W, 3, device=dev)
+    grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
+             torch.empty_like(op))
+
+    stream = torch.cuda.current_stream(dev)
+    ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(2)] for k in ("b", "f", "g")}
+    phase_ms = {"bin": [], "fwd": [], "bwd": []}
+
+    def step(record: bool):
+        if record:
+            ev["b"][0].record(stream)
+        HR.build_tile_index_capacity(means, conics, radii, H, W, capacity,
+                                     out=(gids, offsets, status, ws))
+        if record:
+            ev["b"][1].record(stream)
+            ev["f"][0].record(stream)
+        HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8, 16,
+                         False, 3.0, out=out, ws=ws)
+        if record:
+            ev["f"][1].record(stream)
+            ev["g"][0].record(stream)
+        HR.backward_tiled(grad_out, means, conics, colors, radii, op, gids, offsets, H, W, True,
+                          1e-8, 16, False, 3.0, grads=grads, ws=ws)
+        if record:
+            ev["g"][1].record(stream)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    if int(status.item()) != 0:
+        raise RuntimeError("pair capacity exceeded during warmup")
+
+    # Whole-step timing: K back-to-back steps bracketed by device synchronization.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+
+    # Per-phase timing with events on the launch stream (a separate pass over K steps).
+    for _ in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        phase_ms["bin"].append(ev["b"][0].elapsed_time(ev["b"][1]))
+        phase_ms["fwd"].append(ev["f"][0].elapsed_time(ev["f"][1]))
+        phase_ms["bwd"].append(ev["g"][0].elapsed_time(ev["g"][1]))
+    if int(status.item()) != 0:
+        raise RuntimeError("pair capacity exceeded")
+    phase = {k: statistics.mean(v) for k, v in phase_ms.items()}
+
+    P_now = int(offsets[-1].item())
+    b = algorithmic_bytes(N, P_now, T, H * W)
+    value = b["step"] / (ms_per_step * 1e-3) / 1e9
+    bwd_gbs = b["bwd"] / (phase["bwd"] * 1e-3) / 1e9
+    line = {
+        "metric": "fwd+bwd ms & HBM GB/s at 2048^2/200k Gaussians; batched fit steps/s vs plain",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "H": H, "W": W, "n_gaussians": N, "tile_pairs": P_now, "tiles": T,
+            "phase_ms": {k: round(v, 4) for k, v in phase.items()},
+            "algorithmic_bytes": b,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_backward16 (timed with its grad-clearing and record-packing launches)",
+            "achieved": round(bwd_gbs, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.workload, H, W, b["step"])
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

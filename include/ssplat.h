@@ -1,0 +1,139 @@
+/*
+ * ssplat.h - C ABI of the MI355X (gfx950) sorting-free 2D Gaussian rasterizer.
+ *
+ * The library renders StructSplat's normalized weighted sum
+ *     out(p) = sum_i c_i o_i G_i(p) / (sum_i o_i G_i(p) + eps)
+ * and its additive variant out(p) = sum_i c_i o_i G_i(p), forward and backward, over the
+ * clipped per-Gaussian support rectangles of the reference renderer.
+ *
+ * Conventions shared by every entry point:
+ *   - All array arguments are DEVICE pointers to contiguous row-major memory owned by the
+ *     caller.  The library never allocates device memory; scratch space is a caller-owned
+ *     workspace whose size comes from ssplat_workspace_bytes().
+ *   - `stream` is a hipStream_t (passed as void* so this header needs no HIP headers).  All
+ *     work is enqueued on it; no entry point synchronizes with the host, so every call can be
+ *     recorded into a HIP graph.
+ *   - Return value: SSPLAT_OK (0) or an SSPLAT_ERR_* code; ssplat_last_error() returns a
+ *     thread-local message for the last failing call on the calling thread.
+ *   - Field layout (the reference's tensor contract, render_ext.cpp:94-125):
+ *       means (N,2) f32, conics (N,3) f32 = (a, b, c) of inv(Sigma) = [[a,b],[b,c]],
+ *       colors (N,3) f32, radii (N,2) int64 support half-widths, opacities (N,) f32 or NULL
+ *       (NULL means opacity 1).  Images are (H,W,3) f32; coverage/den is (H,W) f32.
+ *   - Tile index: tile_offsets (T+1,) int32 and tile_gids (P,) int32 where
+ *       T = ceil(W/ts) * ceil(H/ts) and tile t owns tile_gids[tile_offsets[t] : tile_offsets[t+1]]
+ *       in ascending Gaussian id.
+ */
+#ifndef SSPLAT_H_
+#define SSPLAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSPLAT_ABI_VERSION 1
+
+enum ssplat_status {
+  SSPLAT_OK = 0,
+  SSPLAT_ERR_INVALID_ARGUMENT = 1, /* bad shape, size, tile size or pointer */
+  SSPLAT_ERR_CAPACITY = 2,         /* tile_gids capacity smaller than the pair count */
+  SSPLAT_ERR_HIP = 3,              /* a HIP runtime call or kernel launch failed */
+  SSPLAT_ERR_UNSUPPORTED = 4       /* valid request this build does not implement */
+};
+
+/* Version of this ABI (SSPLAT_ABI_VERSION of the build). */
+int ssplat_abi_version(void);
+
+/* Message describing the last failing call on this thread ("" after a success). */
+const char* ssplat_last_error(void);
+
+/*
+ * Workspace bytes needed by the calls below for a field of n Gaussians on an HxW image binned
+ * with tile_size.  One workspace may be reused by consecutive calls on one stream.
+ * No reference counterpart: the reference allocates its scratch with the torch caching
+ * allocator inside each call (render_ext.cu:889-919).
+ */
+int ssplat_workspace_bytes(int64_t n, int height, int width, int tile_size, size_t* bytes);
+
+/*
+ * Tile binning, phase 1: per-tile pair counts and their exclusive scan.
+ * Writes tile_offsets[0..T]; tile_offsets[T] is the number of (tile, Gaussian) pairs P.
+ * Replaces the counting half of build_tile_index (render_ext.cpp:139-167,
+ * render_ext.cu:858-911: tile_count_kernel + cub::DeviceScan + the host readback).
+ * ellipse_cull drops pairs whose support-fade weight is exactly zero on the whole tile
+ * (render_ext.cu:97-133, 178-191); callers pass it only together with support fade.
+ */
+int ssplat_tile_index_count(const float* means, const float* conics, const int64_t* radii,
+                            int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                            double sigma_cutoff, int32_t* tile_offsets, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/*
+ * Tile binning, phase 2: fills tile_gids from the offsets written by phase 1 on the same
+ * workspace, in ascending Gaussian id within every tile (the stable order of the reference's
+ * CUB radix sort, render_ext.cu:927-944).  gid_capacity is the length of tile_gids; when it is
+ * smaller than tile_offsets[T] (or the pair count exceeds int32), pairs beyond it are dropped and
+ * *status (a device int32, may be NULL) is set to SSPLAT_ERR_CAPACITY.  A successful fill leaves
+ * *status untouched (sticky), so the caller clears it once and may check it after many calls.
+ * Replaces expand_pairs_kernel + SortPairs + tile_offsets_from_keys_kernel (render_ext.cu:135-218,
+ * :912-951).
+ */
+int ssplat_tile_index_fill(const float* means, const float* conics, const int64_t* radii,
+                           int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                           double sigma_cutoff, const int32_t* tile_offsets, int32_t* tile_gids,
+                           int64_t gid_capacity, int32_t* status, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
+/*
+ * Both phases in one call with no host synchronization (capturable in a HIP graph).
+ * Equivalent of build_tile_index(means, conics, radii, H, W, tile_size, ellipse_cull,
+ * sigma_cutoff) (render_ext.cpp:139-167) with a caller-chosen gid capacity.
+ */
+int ssplat_build_tile_index(const float* means, const float* conics, const int64_t* radii,
+                            int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                            double sigma_cutoff, int32_t* tile_offsets, int32_t* tile_gids,
+                            int64_t gid_capacity, int32_t* status, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/*
+ * Render calls read tile_gids[tile_offsets[t] : tile_offsets[t+1]] clamped to [0, num_gids),
+ * where num_gids is the length of tile_gids, so a stale offset table cannot read past it.
+ * The render kernels are built for tile_size 16 (the reference default, cuda_render.py:258);
+ * other tile sizes return SSPLAT_ERR_UNSUPPORTED.
+ *
+ * Tiled forward.  Writes out (H,W,3) and, when den is not NULL, the raw coverage
+ * den (H,W) = sum_i o_i G_i (no eps).  Every pixel is accumulated in ascending Gaussian id,
+ * so the result is bitwise deterministic.
+ * Replaces forward_tiled (render_ext.cpp:188-211, render_ext.cu:349-431, :1011-1063) and the
+ * coverage forward used by render_cuda_exact_with_coverage (cuda_render.py:229-253).
+ */
+int ssplat_forward_tiled(const float* means, const float* conics, const float* colors,
+                         const int64_t* radii, const float* opacities, int64_t n,
+                         const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
+                         int height, int width, int tile_size, int normalize, double eps,
+                         int support_fade, double sigma_cutoff, float* out, float* den,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Tiled backward.  Given grad_out (H,W,3) = dL/dout, writes dL/dmeans (N,2), dL/dconics (N,3),
+ * dL/dcolors (N,3) and, when opacities and grad_opacities are both non-NULL, dL/dopacities (N,).
+ * The per-pixel normalizer is recomputed from the tile lists (no saved den/out).  Gradient
+ * outputs are overwritten (the library clears them first).
+ * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231), which
+ * takes the forward's saved den/out instead.
+ */
+int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
+                          const float* colors, const int64_t* radii, const float* opacities,
+                          int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
+                          int64_t num_gids, int height, int width, int tile_size, int normalize,
+                          double eps, int support_fade, double sigma_cutoff, float* grad_means,
+                          float* grad_conics, float* grad_colors, float* grad_opacities,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSPLAT_H_ */

@@ -1,0 +1,139 @@
+"""Float64 forward and analytic backward of the render path.  Test infrastructure only.
+
+Every Gaussian is evaluated on every pixel of its clipped support rectangle
+(``tile_index.support_rects``), in float64 from the float32 inputs:
+
+    q = a dx^2 + 2 b dx dy + c dy^2,  raw = exp(-q/2),  tail = exp(-cutoff^2/2)
+    base = max(raw - tail, 0) under support fade, else raw;   w = base * o
+    N_k = sum w c_k,  D = sum w;   out = N/(D+eps) (normalized) or N (additive)
+
+Backward (the per-pixel chain of render_ext.cu:485-530 and :784-818), with G = dL/dout:
+    normalized: dN_k = G_k/(D+eps), dD = -sum_k G_k out_k/(D+eps);  additive: dN_k = G_k, dD = 0
+    dc_ik += dN_k w_i;  dw_i = sum_k dN_k c_ik + dD;  do_i += dw_i base_i
+    dq = dw_i o_i * (0 if fade and raw <= tail else -raw/2)
+    dmx += dq (-2a dx - 2b dy);  dmy += dq (-2b dx - 2c dy)
+    da += dq dx^2;  db += dq 2 dx dy;  dc += dq dy^2
+The support rectangles (rounded means, integer radii) are held fixed, as in the reference
+(render.py:70-71 detaches the means; gaussians.py:306 detaches the radii).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .tile_index import support_rects
+
+_CHUNK_VISITS = 1 << 22
+
+
+def _visit_chunks(x0, x1, y0, y1, valid):
+    """Yield (gid, px, py) int64 arrays covering every (Gaussian, pixel) visit, in chunks."""
+    ids = np.nonzero(valid)[0]
+    if ids.size == 0:
+        return
+    n = (x1[ids] - x0[ids]) * (y1[ids] - y0[ids])
+    csum = np.cumsum(n)
+    start = 0
+    while start < ids.size:
+        base = csum[start - 1] if start > 0 else 0
+        end = int(np.searchsorted(csum, base + _CHUNK_VISITS, side="right"))
+        end = max(end, start + 1)
+        sel = ids[start:end]
+        ns = n[start:end]
+        gid = np.repeat(sel, ns)
+        first = np.cumsum(ns) - ns
+        t = np.arange(int(ns.sum()), dtype=np.int64) - np.repeat(first, ns)
+        tx = np.repeat(x1[sel] - x0[sel], ns)
+        yield gid, np.repeat(x0[sel], ns) + t % tx, np.repeat(y0[sel], ns) + t // tx
+        start = end
+
+
+def _prepare(means, conics, colors, radii, opacities, H, W, rects):
+    means = np.asarray(means, dtype=np.float32).reshape(-1, 2)
+    rect = support_rects(means, radii, H, W) if rects is None else rects
+    m = means.astype(np.float64)
+    q = np.asarray(conics, dtype=np.float32).reshape(-1, 3).astype(np.float64)
+    col = np.asarray(colors, dtype=np.float32).reshape(-1, 3).astype(np.float64)
+    if opacities is None or np.size(opacities) == 0:
+        op = np.ones(m.shape[0])
+    else:
+        op = np.asarray(opacities, dtype=np.float32).reshape(-1).astype(np.float64)
+    return m, q, col, op, rect
+
+
+def _weights(m, q, op, gid, px, py, fade, tail):
+    dx = px.astype(np.float64) - m[gid, 0]
+    dy = py.astype(np.float64) - m[gid, 1]
+    a, b, c = q[gid, 0], q[gid, 1], q[gid, 2]
+    quad = a * dx * dx + 2.0 * b * dx * dy + c * dy * dy
+    raw = np.exp(-0.5 * quad)
+    base = np.maximum(raw - tail, 0.0) if fade else raw
+    return dx, dy, a, b, c, raw, base, base * op[gid]
+
+
+def accumulate(means, conics, colors, radii, opacities, H: int, W: int, support_fade: bool,
+               sigma_cutoff: float, rects=None):
+    """Float64 numerator (H*W, 3) and denominator (H*W,) sums."""
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
+    tail = math.exp(-0.5 * float(sigma_cutoff) ** 2)
+    num = np.zeros((H * W, 3))
+    den = np.zeros(H * W)
+    for gid, px, py in _visit_chunks(*rect):
+        *_, w = _weights(m, q, op, gid, px, py, support_fade, tail)
+        flat = py * W + px
+        for k in range(3):
+            num[:, k] += np.bincount(flat, weights=w * col[gid, k], minlength=H * W)
+        den += np.bincount(flat, weights=w, minlength=H * W)
+    return num, den
+
+
+def forward(means, conics, colors, radii, opacities, H: int, W: int, normalize: bool = True,
+            eps: float = 1e-8, support_fade: bool = False, sigma_cutoff: float = 3.0,
+            rects=None):
+    """Float64 image (H, W, 3) and raw coverage (H, W)."""
+    num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
+                          sigma_cutoff, rects)
+    out = num / (den[:, None] + eps) if normalize else num
+    return out.reshape(H, W, 3), den.reshape(H, W)
+
+
+def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
+             normalize: bool = True, eps: float = 1e-8, support_fade: bool = False,
+             sigma_cutoff: float = 3.0, rects=None):
+    """Analytic float64 gradients (g_means (N,2), g_conics (N,3), g_colors (N,3), g_op (N,))."""
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
+    tail = math.exp(-0.5 * float(sigma_cutoff) ** 2)
+    G = np.asarray(grad_out, dtype=np.float64).reshape(H * W, 3)
+    if normalize:
+        num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
+                              sigma_cutoff, rect)
+        denom = den + eps
+        out = num / denom[:, None]
+        dN = G / denom[:, None]
+        dD = -(G * out).sum(axis=1) / denom
+    else:
+        dN = G
+        dD = np.zeros(H * W)
+    n = m.shape[0]
+    g_m = np.zeros((n, 2))
+    g_q = np.zeros((n, 3))
+    g_c = np.zeros((n, 3))
+    g_o = np.zeros(n)
+    for gid, px, py in _visit_chunks(*rect):
+        dx, dy, a, b, c, raw, base, w = _weights(m, q, op, gid, px, py, support_fade, tail)
+        flat = py * W + px
+        dw = (dN[flat] * col[gid]).sum(axis=1) + dD[flat]
+        for k in range(3):
+            g_c[:, k] += np.bincount(gid, weights=dN[flat, k] * w, minlength=n)
+        g_o += np.bincount(gid, weights=dw * base, minlength=n)
+        dvis = -0.5 * raw
+        if support_fade:
+            dvis = np.where(raw <= tail, 0.0, dvis)
+        dq = dw * op[gid] * dvis
+        g_m[:, 0] += np.bincount(gid, weights=dq * (-2.0 * a * dx - 2.0 * b * dy), minlength=n)
+        g_m[:, 1] += np.bincount(gid, weights=dq * (-2.0 * b * dx - 2.0 * c * dy), minlength=n)
+        g_q[:, 0] += np.bincount(gid, weights=dq * dx * dx, minlength=n)
+        g_q[:, 1] += np.bincount(gid, weights=dq * 2.0 * dx * dy, minlength=n)
+        g_q[:, 2] += np.bincount(gid, weights=dq * dy * dy, minlength=n)
+    return g_m, g_q, g_c, g_o

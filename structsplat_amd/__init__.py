@@ -1,0 +1,29 @@
+"""StructSplat's sorting-free 2D Gaussian rasterizer, rebuilt for MI355X (gfx950).
+
+Public entry points mirror the reference's render path:
+
+* :func:`render_field` - mode dispatcher (render.py:319-414).
+* :func:`render_hip_exact` - the ``render_cuda_exact`` entry (cuda_render.py:256-294).
+* :func:`build_tile_index` - per-tile Gaussian lists (render_ext.cpp:139-167).
+* :class:`GaussianField` - field layout and conic/radii/opacity producer (gaussians.py).
+
+The compute runs in hand-written HIP kernels in ``libssplat_hip.so`` behind a C ABI
+(``include/ssplat.h``); build it with ``python -m structsplat_amd._build``.
+"""
+from .gaussians import GaussianField
+from .render import (
+    build_tile_index,
+    build_tile_index_capacity,
+    render_field,
+    render_hip_exact,
+    render_hip_exact_with_coverage,
+)
+
+__all__ = [
+    "GaussianField",
+    "build_tile_index",
+    "build_tile_index_capacity",
+    "render_field",
+    "render_hip_exact",
+    "render_hip_exact_with_coverage",
+]

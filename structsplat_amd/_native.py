@@ -1,0 +1,105 @@
+"""ctypes binding of ``libssplat_hip.so``, the C ABI declared in ``include/ssplat.h``.
+
+torch is imported before the library is opened: torch ships its own HIP runtime with the same
+soname, and loading torch first makes the library bind to that one runtime instead of a second
+copy.  Every wrapper here takes raw device addresses (ints) and a stream handle; tensors never
+cross the ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (one HIP runtime per process, see module docstring)
+
+from ._build import LIB_PATH
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_CAPACITY = 2
+ERR_HIP = 3
+ERR_UNSUPPORTED = 4
+
+_c_int = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_c_dbl = ctypes.c_double
+_c_ptr = ctypes.c_void_p
+_c_size = ctypes.c_size_t
+
+# Exported symbol -> (restype, argtypes).  Mirrors include/ssplat.h.
+SIGNATURES = {
+    "ssplat_abi_version": (_c_int, []),
+    "ssplat_last_error": (ctypes.c_char_p, []),
+    "ssplat_workspace_bytes": (_c_int, [_c_i64, _c_int, _c_int, _c_int,
+                                        ctypes.POINTER(_c_size)]),
+    "ssplat_tile_index_count": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int,
+                                         _c_int, _c_dbl, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_tile_index_fill": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int,
+                                        _c_int, _c_dbl, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr,
+                                        _c_size, _c_ptr]),
+    "ssplat_build_tile_index": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int,
+                                         _c_int, _c_dbl, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr,
+                                         _c_size, _c_ptr]),
+    "ssplat_forward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
+                                      _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl,
+                                      _c_int, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64,
+                                       _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
+                                       _c_dbl, _c_int, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                       _c_ptr, _c_size, _c_ptr]),
+}
+
+_LIB = None
+_LOCK = threading.Lock()
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Open the in-tree HIP library (fails loudly when it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    with _LOCK:
+        if _LIB is None:
+            if not os.path.exists(LIB_PATH):
+                raise NativeLibraryMissing(
+                    f"StructSplat HIP library not found at {LIB_PATH}; build it with "
+                    "`python -m structsplat_amd._build` (hipcc, gfx950)."
+                )
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (restype, argtypes) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _LIB = lib
+    return _LIB
+
+
+def last_error() -> str:
+    msg = load().ssplat_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    """Raise the Python exception matching a non-zero status code."""
+    if rc == OK:
+        return
+    msg = last_error()
+    if rc == ERR_INVALID_ARGUMENT:
+        raise ValueError(f"{what}: {msg}")
+    if rc == ERR_UNSUPPORTED:
+        raise ValueError(f"{what}: unsupported: {msg}")
+    if rc == ERR_CAPACITY:
+        raise RuntimeError(f"{what}: tile index capacity exceeded: {msg}")
+    raise RuntimeError(f"{what}: HIP error: {msg}")
+
+
+def workspace_bytes(n: int, height: int, width: int, tile_size: int) -> int:
+    out = _c_size(0)
+    check(load().ssplat_workspace_bytes(int(n), int(height), int(width), int(tile_size),
+                                        ctypes.byref(out)), "ssplat_workspace_bytes")
+    return int(out.value)

@@ -1,0 +1,1197 @@
+// ssplat.hip - sorting-free normalized / additive 2D Gaussian rasterizer for gfx950 (MI355X).
+//
+// Everything behind the C ABI of include/ssplat.h lives in this file:
+//   * tile binning   : per-Gaussian clipped support rectangles -> per-tile pair counts (atomics)
+//                      -> hand-written integer exclusive scan -> scatter into per-tile lists
+//                      -> per-tile LDS bitonic sort, which yields ascending Gaussian ids per tile
+//                      (the order the reference obtains from a stable radix sort).
+//   * record packing : one 48-byte record per Gaussian (mean, conic, opacity, color, support
+//                      rectangle) so a staged Gaussian costs three 16-byte loads.
+//   * tiled forward  : one 16x16 tile per 256-thread workgroup, one 8x8 quadrant per wave64,
+//                      records staged through LDS with coalesced loads, a ballot pre-filter per
+//                      wave, and per-pixel register accumulation in ascending Gaussian id.
+//   * tiled backward : recomputes the per-pixel numerator/denominator (nothing is saved by the
+//                      forward), turns them into per-pixel adjoints in LDS, then sweeps the tile
+//                      list again with each wave owning a quarter of the Gaussians over all 256
+//                      pixels (four per lane), DPP wave reductions, one atomic set per
+//                      (tile, Gaussian).
+// No MFMA: the path is a gather/accumulate, not a contraction.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "ssplat.h"
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// Error reporting
+// ------------------------------------------------------------------------------------------
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int success() {
+  g_last_error.clear();
+  return SSPLAT_OK;
+}
+
+#define SSPLAT_LAUNCH_CHECK(name)                                                           \
+  do {                                                                                      \
+    hipError_t err_ = hipGetLastError();                                                    \
+    if (err_ != hipSuccess) {                                                               \
+      return set_error(SSPLAT_ERR_HIP, "%s launch failed: %s", name, hipGetErrorString(err_)); \
+    }                                                                                       \
+  } while (0)
+
+// ------------------------------------------------------------------------------------------
+// Constants
+// ------------------------------------------------------------------------------------------
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems;  // counts per first-level scan block
+constexpr int kMaxScanBlocks = kScanBlock;             // the top-level scan is one block
+constexpr int64_t kMaxScanItems = int64_t(kScanBlock) * kMaxScanBlocks;
+constexpr int kBinThreads = 256;
+constexpr int kSortThreads = 256;
+constexpr int kSortLds = 4096;  // per-tile lists up to this length sort entirely in LDS
+constexpr int kRenderThreads = 256;
+constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
+constexpr int kTile = 16;    // render tile edge (pixels)
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+// -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
+constexpr float kNegHalfLog2e = -0.72134752044448170368f;
+// Clamps that keep the integer support arithmetic exact in int64 (see support_rect).
+constexpr float kCenterClamp = 1.0e12f;
+constexpr int64_t kRadiusClamp = int64_t(1) << 36;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------------------------------------------
+// Shared device helpers
+// ------------------------------------------------------------------------------------------
+
+// Half-open support rectangle [x0, x1) x [y0, y1) of one Gaussian, clipped to the image.
+struct Rect {
+  int x0, x1, y0, y1;
+};
+
+// Clipped support rectangle: centre rounded half-to-even, +-radius, clipped to the image;
+// fully outside -> empty (render.py:62-78).  Non-finite means are dropped (render_ext.cu:28-34).
+// The centre is clamped to +-1e12 and the radii to +-2^36 so the int64 arithmetic cannot
+// overflow; for every mean and radius inside those bounds the result equals the reference's.
+__device__ __forceinline__ Rect support_rect(float mx, float my, int64_t rx, int64_t ry, int H,
+                                             int W) {
+  Rect r{0, 0, 0, 0};
+  if (!isfinite(mx) || !isfinite(my)) return r;
+  const float fx = fminf(fmaxf(rintf(mx), -kCenterClamp), kCenterClamp);
+  const float fy = fminf(fmaxf(rintf(my), -kCenterClamp), kCenterClamp);
+  const int64_t ix = static_cast<int64_t>(fx);
+  const int64_t iy = static_cast<int64_t>(fy);
+  rx = rx < -kRadiusClamp ? -kRadiusClamp : (rx > kRadiusClamp ? kRadiusClamp : rx);
+  ry = ry < -kRadiusClamp ? -kRadiusClamp : (ry > kRadiusClamp ? kRadiusClamp : ry);
+  const int64_t x0 = ix - rx > 0 ? ix - rx : 0;
+  const int64_t x1 = ix + rx < W - 1 ? ix + rx : W - 1;
+  const int64_t y0 = iy - ry > 0 ? iy - ry : 0;
+  const int64_t y1 = iy + ry < H - 1 ? iy + ry : H - 1;
+  if (x1 < x0 || y1 < y0) return r;
+  r.x0 = static_cast<int>(x0);
+  r.x1 = static_cast<int>(x1) + 1;
+  r.y0 = static_cast<int>(y0);
+  r.y1 = static_cast<int>(y1) + 1;
+  return r;
+}
+
+// Conic quadratic in the reference's operation order, no FMA contraction (bit-exact with the
+// oracle's float32 restatement; only the ellipse cull uses it).
+__device__ __forceinline__ float conic_q_exact(float dx, float dy, float a, float b, float c) {
+#pragma clang fp contract(off)
+  return a * dx * dx + 2.0f * b * dx * dy + c * dy * dy;
+}
+
+// Exact continuous minimum of the conic quadratic over an axis-aligned rectangle (pixel-centre
+// coordinates).  Same arithmetic as render_ext.cu:101-133 so the cull decision is bit-exact:
+// a malformed conic keeps the pair (-FLT_MAX), a rectangle containing the mean gives 0, else the
+// minimum over the four edges, each at its clamped 1-D minimiser.
+__device__ __forceinline__ float tile_min_q(float mx, float my, float a, float b, float c,
+                                            float x_lo, float x_hi, float y_lo, float y_hi) {
+#pragma clang fp contract(off)
+  const float det = a * c - b * b;
+  if (!(a > 0.0f && c > 0.0f && det > 0.0f)) return -FLT_MAX;
+  if (mx >= x_lo && mx <= x_hi && my >= y_lo && my <= y_hi) return 0.0f;
+  const float slope_y = -(b / c);
+  const float slope_x = -(b / a);
+  float best = FLT_MAX;
+  float dx = x_lo - mx;
+  float dy = fminf(fmaxf(slope_y * dx, y_lo - my), y_hi - my);
+  best = fminf(best, conic_q_exact(dx, dy, a, b, c));
+  dx = x_hi - mx;
+  dy = fminf(fmaxf(slope_y * dx, y_lo - my), y_hi - my);
+  best = fminf(best, conic_q_exact(dx, dy, a, b, c));
+  dy = y_lo - my;
+  dx = fminf(fmaxf(slope_x * dy, x_lo - mx), x_hi - mx);
+  best = fminf(best, conic_q_exact(dx, dy, a, b, c));
+  dy = y_hi - my;
+  dx = fminf(fmaxf(slope_x * dy, x_lo - mx), x_hi - mx);
+  return fminf(best, conic_q_exact(dx, dy, a, b, c));
+}
+
+// Hardware 2^x (v_exp_f32).  The same instruction evaluates both the per-pixel weight and the
+// fade tail, so a pixel whose quadratic equals sigma_cutoff^2 gets a faded weight of exactly 0.
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Workgroup -> tile mapping that gives each XCD a contiguous run of tiles, so neighbouring
+// tiles (which share most of their Gaussians) hit the same XCD-private L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  const int per = nb / kXcds;
+  const int rem = nb % kXcds;
+  const int xcd = bid % kXcds;
+  const int local = bid / kXcds;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+template <int kCtrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  // Lanes without a source read 0 (old = 0, bound_ctrl off).
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+
+// Sum over the 64 lanes of a wave; the total lands in lane 63.  Must be called with all lanes
+// active.
+__device__ __forceinline__ float wave_sum_lane63(float v) {
+  v += dpp_move<0xb1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x114>(v);  // row_shr:4
+  v += dpp_move<0x118>(v);  // row_shr:8
+  v += dpp_move<0x142>(v);  // row_bcast:15
+  v += dpp_move<0x143>(v);  // row_bcast:31
+  return v;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t count, int32_t value) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+    p[i] = value;
+  }
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t count, float value) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+    p[i] = value;
+  }
+}
+
+// Sticky status: a non-OK source is copied, an OK source leaves the destination untouched, so a
+// caller can clear the word once and check it after many binnings.
+__global__ void k_merge_status(const int32_t* __restrict__ src, int32_t* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && *src != SSPLAT_OK) *dst = *src;
+}
+
+int fill_i32(int32_t* p, int64_t count, int32_t value, hipStream_t stream) {
+  if (count <= 0) return SSPLAT_OK;
+  const int64_t blocks64 = (count + 255) / 256;
+  const int blocks = static_cast<int>(blocks64 < 4096 ? blocks64 : 4096);
+  k_fill_i32<<<blocks, 256, 0, stream>>>(p, count, value);
+  SSPLAT_LAUNCH_CHECK("fill_i32");
+  return SSPLAT_OK;
+}
+
+int fill_f32(float* p, int64_t count, float value, hipStream_t stream) {
+  if (count <= 0) return SSPLAT_OK;
+  const int64_t blocks64 = (count + 255) / 256;
+  const int blocks = static_cast<int>(blocks64 < 4096 ? blocks64 : 4096);
+  k_fill_f32<<<blocks, 256, 0, stream>>>(p, count, value);
+  SSPLAT_LAUNCH_CHECK("fill_f32");
+  return SSPLAT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile binning
+// ------------------------------------------------------------------------------------------
+
+struct BinArgs {
+  const float* means;
+  const float* conics;
+  const int64_t* radii;
+  int n, H, W, ts, tiles_x, cull;
+  float cull_thr;  // cutoff^2 + 1e-5 * max(cutoff^2, 1), computed in float32 on the host
+};
+
+// Enumerates every live (tile, Gaussian) pair of this workgroup's 256 Gaussians.  Each wave owns
+// 64 consecutive Gaussians; their pairs are numbered by a wave-level prefix sum and dealt out
+// to the lanes round-robin, so one large Gaussian does not serialize its wave.
+template <class Emit>
+__device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
+  __shared__ int64_t s_incl[kBinThreads];
+  __shared__ int4 s_rect[kBinThreads];
+  __shared__ int4 s_tile[kBinThreads];
+  __shared__ float4 s_geo[kBinThreads];
+  __shared__ float s_b[kBinThreads];
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
+  const int i = blockIdx.x * kBinThreads + threadIdx.x;
+  int64_t cnt = 0;
+  Rect r{0, 0, 0, 0};
+  int tx0 = 0, ty0 = 0, tw = 1;
+  float mx = 0.0f, my = 0.0f;
+  if (i < a.n) {
+    mx = a.means[2 * i + 0];
+    my = a.means[2 * i + 1];
+    r = support_rect(mx, my, a.radii[2 * i + 0], a.radii[2 * i + 1], a.H, a.W);
+    if (r.x1 > r.x0 && r.y1 > r.y0) {
+      tx0 = r.x0 / a.ts;
+      ty0 = r.y0 / a.ts;
+      const int tx1 = (r.x1 - 1) / a.ts;
+      const int ty1 = (r.y1 - 1) / a.ts;
+      tw = tx1 - tx0 + 1;
+      cnt = int64_t(tw) * (ty1 - ty0 + 1);
+    }
+  }
+  int64_t incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const int64_t total = __shfl(incl, 63, 64);
+  s_incl[threadIdx.x] = incl;
+  s_rect[threadIdx.x] = make_int4(r.x0, r.x1, r.y0, r.y1);
+  s_tile[threadIdx.x] = make_int4(tx0, ty0, tw, i);
+  if (a.cull) {
+    float ca = 0.0f, cb = 0.0f, cc = 0.0f;
+    if (i < a.n) {
+      ca = a.conics[3 * i + 0];
+      cb = a.conics[3 * i + 1];
+      cc = a.conics[3 * i + 2];
+    }
+    s_geo[threadIdx.x] = make_float4(mx, my, ca, cc);
+    s_b[threadIdx.x] = cb;
+  }
+  __syncthreads();
+  for (int64_t k = lane; k < total; k += 64) {
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_incl[wbase + mid] > k) {
+        hi = mid;
+      } else {
+        lo = mid + 1;
+      }
+    }
+    const int o = wbase + lo;
+    const int64_t excl = lo > 0 ? s_incl[o - 1] : 0;
+    const int64_t local = k - excl;
+    const int4 tl = s_tile[o];
+    const int tile_x = tl.x + static_cast<int>(local % tl.z);
+    const int tile_y = tl.y + static_cast<int>(local / tl.z);
+    if (a.cull) {
+      const int4 rc = s_rect[o];
+      const float4 g = s_geo[o];
+      const int rx0 = max(rc.x, tile_x * a.ts);
+      const int rx1 = min(rc.y - 1, (tile_x + 1) * a.ts - 1);
+      const int ry0 = max(rc.z, tile_y * a.ts);
+      const int ry1 = min(rc.w - 1, (tile_y + 1) * a.ts - 1);
+      const float mq = tile_min_q(g.x, g.y, g.z, s_b[o], g.w, static_cast<float>(rx0),
+                                  static_cast<float>(rx1), static_cast<float>(ry0),
+                                  static_cast<float>(ry1));
+      if (mq > a.cull_thr) continue;
+    }
+    emit(tile_y * a.tiles_x + tile_x, tl.w);
+  }
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(BinArgs a, int32_t* __restrict__ counts) {
+  enumerate_pairs(a, [&](int tile, int) { atomicAdd(&counts[tile], 1); });
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(BinArgs a,
+                                                             const int32_t* __restrict__ offsets,
+                                                             int32_t* __restrict__ cursors,
+                                                             int32_t* __restrict__ gids,
+                                                             int64_t capacity,
+                                                             int32_t* __restrict__ status) {
+  enumerate_pairs(a, [&](int tile, int gid) {
+    const int64_t pos = int64_t(offsets[tile]) + atomicAdd(&cursors[tile], 1);
+    if (pos >= 0 && pos < capacity) {
+      gids[pos] = gid;
+    } else {
+      *status = SSPLAT_ERR_CAPACITY;
+    }
+  });
+}
+
+// Exclusive scan of one value per thread over a 1024-thread block; returns the block total
+// through *total.  Reusable back to back (ends with a barrier).
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t s_wave[kScanThreads / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  constexpr int kWaves = kScanThreads / 64;
+  int64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    int64_t s = lane < kWaves ? s_wave[lane] : 0;
+#pragma unroll
+    for (int d = 1; d < kWaves; d <<= 1) {
+      const int64_t y = __shfl_up(s, d, 64);
+      if (lane >= d) s += y;
+    }
+    if (lane < kWaves) s_wave[lane] = s;
+  }
+  __syncthreads();
+  const int64_t prefix = wave > 0 ? s_wave[wave - 1] : 0;
+  *total = s_wave[kWaves - 1];
+  __syncthreads();
+  return prefix + x - v;
+}
+
+// First level: per-block sums of `items` int32 counts.
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int32_t* __restrict__ counts,
+                                                                int64_t items,
+                                                                int64_t* __restrict__ block_sums) {
+  const int64_t base = int64_t(blockIdx.x) * kScanBlock + int64_t(threadIdx.x) * kScanItems;
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < items) s += counts[base + k];
+  }
+  int64_t total;
+  block_exclusive_scan(s, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Second level: exclusive scan of the block sums, in place (one block, <= kMaxScanBlocks sums).
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(int64_t* __restrict__ block_sums,
+                                                           int blocks) {
+  const int base = threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < blocks ? block_sums[base + k] : 0;
+    s += v[k];
+  }
+  int64_t total;
+  int64_t run = block_exclusive_scan(s, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < blocks) block_sums[base + k] = run;
+    run += v[k];
+  }
+}
+
+// Third level: exclusive offsets of every count, written as int32.  A running total that does not
+// fit int32 is saturated at INT32_MAX and flags the workspace status (the tile index stores int32
+// offsets, like the reference), so a host reading tile_offsets[T] sees the overflow.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* __restrict__ counts,
+                                                             int64_t items,
+                                                             const int64_t* __restrict__ block_sums,
+                                                             int32_t* __restrict__ offsets,
+                                                             int32_t* __restrict__ status) {
+  const int64_t base = int64_t(blockIdx.x) * kScanBlock + int64_t(threadIdx.x) * kScanItems;
+  int32_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < items ? counts[base + k] : 0;
+    s += v[k];
+  }
+  int64_t total;
+  int64_t run = block_exclusive_scan(s, &total) + block_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < items) {
+      if (run > INT32_MAX) *status = SSPLAT_ERR_CAPACITY;
+      offsets[base + k] = static_cast<int32_t>(run > INT32_MAX ? INT32_MAX : run);
+    }
+    run += v[k];
+  }
+}
+
+// Ascending sort of a[0, L) by a bitonic network in its "flip" form: every comparator puts the
+// smaller value at the lower index, so virtual +inf padding above L never moves and comparators
+// reaching past L are skipped.  kGlobal: a lives in global memory (lists longer than the LDS
+// buffer), which needs device-scope fences between the stages.
+template <bool kGlobal>
+__device__ void bitonic_sort(int32_t* a, int L) {
+  int p2 = 1;
+  while (p2 < L) p2 <<= 1;
+  const int pairs = p2 >> 1;
+  for (int k = 2; k <= p2; k <<= 1) {
+    const int half = k >> 1;
+    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+      const int i = (p / half) * k + (p % half);
+      const int j = i ^ (k - 1);
+      if (j < L) {
+        const int32_t x = a[i];
+        const int32_t y = a[j];
+        if (x > y) {
+          a[i] = y;
+          a[j] = x;
+        }
+      }
+    }
+    if (kGlobal) __threadfence();
+    __syncthreads();
+    for (int m = k >> 2; m >= 1; m >>= 1) {
+      for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+        const int i = (p / m) * 2 * m + (p % m);
+        const int j = i + m;
+        if (j < L) {
+          const int32_t x = a[i];
+          const int32_t y = a[j];
+          if (x > y) {
+            a[i] = y;
+            a[j] = x;
+          }
+        }
+      }
+      if (kGlobal) __threadfence();
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict__ gids,
+                                                             const int32_t* __restrict__ offsets,
+                                                             int num_tiles, int64_t capacity) {
+  __shared__ int32_t buf[kSortLds];
+  for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
+    int64_t s = offsets[t];
+    int64_t e = offsets[t + 1];
+    s = s < 0 ? 0 : (s > capacity ? capacity : s);
+    e = e < s ? s : (e > capacity ? capacity : e);
+    const int L = static_cast<int>(e - s);
+    if (L <= 1) continue;
+    if (L <= kSortLds) {
+      for (int i = threadIdx.x; i < L; i += blockDim.x) buf[i] = gids[s + i];
+      __syncthreads();
+      bitonic_sort<false>(buf, L);
+      for (int i = threadIdx.x; i < L; i += blockDim.x) gids[s + i] = buf[i];
+      __syncthreads();
+    } else {
+      bitonic_sort<true>(gids + s, L);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Rendering
+// ------------------------------------------------------------------------------------------
+
+// One packed Gaussian: three 16-byte chunks.
+//   c0 = (mx, my, a, 2b)   c1 = (c, opacity, r, g)   c2 = (b_color, xr, yr, gid)
+// xr = x0 | x1 << 16 and yr = y0 | y1 << 16 hold the half-open clipped support rectangle.
+struct __align__(16) GaussRec {
+  float4 c0, c1, c2;
+};
+static_assert(sizeof(GaussRec) == 48, "GaussRec must be 48 bytes");
+
+__global__ void k_prepare(const float* __restrict__ means, const float* __restrict__ conics,
+                          const float* __restrict__ colors, const int64_t* __restrict__ radii,
+                          const float* __restrict__ opacities, int n, int H, int W,
+                          GaussRec* __restrict__ recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float mx = means[2 * i + 0];
+  const float my = means[2 * i + 1];
+  const Rect r = support_rect(mx, my, radii[2 * i + 0], radii[2 * i + 1], H, W);
+  uint32_t xr = 0, yr = 0;
+  if (r.x1 > r.x0 && r.y1 > r.y0) {
+    xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
+    yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
+  }
+  GaussRec g;
+  g.c0 = make_float4(mx, my, conics[3 * i + 0], 2.0f * conics[3 * i + 1]);
+  g.c1 = make_float4(conics[3 * i + 2], opacities ? opacities[i] : 1.0f, colors[3 * i + 0],
+                     colors[3 * i + 1]);
+  g.c2 = make_float4(colors[3 * i + 2], __uint_as_float(xr), __uint_as_float(yr),
+                     __int_as_float(i));
+  recs[i] = g;
+}
+
+struct RenderArgs {
+  const GaussRec* recs;
+  const int32_t* gids;
+  const int32_t* offsets;
+  int64_t num_gids;
+  int n;  // number of records; list entries outside [0, n) stage as empty records
+  int H, W, tiles_x;
+  float eps;
+  float tail_exp;  // sigma_cutoff^2 * kNegHalfLog2e
+};
+
+// Clamped [start, end) of a tile's list.
+__device__ __forceinline__ void tile_range(const RenderArgs& a, int tile, int64_t* start,
+                                           int64_t* end) {
+  int64_t s = a.offsets[tile];
+  int64_t e = a.offsets[tile + 1];
+  s = s < 0 ? 0 : (s > a.num_gids ? a.num_gids : s);
+  e = e < s ? s : (e > a.num_gids ? a.num_gids : e);
+  *start = s;
+  *end = e;
+}
+
+struct StageLds {
+  float4 c0[kStage];
+  float4 c1[kStage];
+  float4 c2[kStage];
+};
+
+// Coalesced staging of up to kStage records of the tile list into LDS (three 16-byte loads
+// per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
+// stages an empty rectangle, which no pixel and no quadrant test ever selects.
+__device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds& st, int64_t batch,
+                                            int m) {
+  if (static_cast<int>(threadIdx.x) < m) {
+    const int g = a.gids[batch + threadIdx.x];
+    if (g >= 0 && g < a.n) {
+      const GaussRec* src = a.recs + g;
+      st.c0[threadIdx.x] = src->c0;
+      st.c1[threadIdx.x] = src->c1;
+      st.c2[threadIdx.x] = src->c2;
+    } else {
+      st.c0[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st.c1[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st.c2[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  }
+}
+
+// Accumulates numerator (r,g,b) and denominator of one pixel per lane over the whole tile list,
+// in ascending Gaussian id.  Each wave owns the 8x8 quadrant (qx0, qy0); a ballot over the staged
+// batch keeps only the records whose rectangle meets the quadrant, and records covering the whole
+// quadrant skip the per-pixel rectangle test.
+template <bool kFade>
+__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds& st, int tile,
+                                                    int qx0, int qy0, int px, int py,
+                                                    float tail, float4& acc) {
+  const int lane = threadIdx.x & 63;
+  const float fx = static_cast<float>(px);
+  const float fy = static_cast<float>(py);
+  int64_t start, end;
+  tile_range(a, tile, &start, &end);
+  float nr = 0.0f, ng = 0.0f, nb = 0.0f, d = 0.0f;
+  for (int64_t batch = start; batch < end; batch += kStage) {
+    const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
+    stage_batch(a, st, batch, m);
+    __syncthreads();
+    uint64_t hit[kStage / 64], full[kStage / 64];
+#pragma unroll
+    for (int k = 0; k < kStage / 64; ++k) {
+      const int j = lane + 64 * k;
+      bool h = false, f = false;
+      if (j < m) {
+        const float4 v = st.c2[j];
+        const uint32_t xr = __float_as_uint(v.y);
+        const uint32_t yr = __float_as_uint(v.z);
+        const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
+        h = x0 < qx0 + 8 && x1 > qx0 && y0 < qy0 + 8 && y1 > qy0;
+        f = x0 <= qx0 && x1 >= qx0 + 8 && y0 <= qy0 && y1 >= qy0 + 8;
+      }
+      hit[k] = __ballot(h);
+      full[k] = __ballot(f);
+    }
+#pragma unroll
+    for (int k = 0; k < kStage / 64; ++k) {
+      uint64_t mask = hit[k];
+      while (mask) {
+        const int b = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        mask &= mask - 1;
+        const int j = 64 * k + b;
+        const float4 v0 = st.c0[j];
+        const float4 v1 = st.c1[j];
+        const float4 v2 = st.c2[j];
+        bool inside = true;
+        if (!((full[k] >> b) & 1)) {
+          const uint32_t xr = __float_as_uint(v2.y);
+          const uint32_t yr = __float_as_uint(v2.z);
+          const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
+          inside = static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0) &&
+                   static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0);
+        }
+        if (inside) {
+          const float dx = fx - v0.x;
+          const float dy = fy - v0.y;
+          const float t = fmaf(v0.z, dx, v0.w * dy);
+          const float q = fmaf(t, dx, v1.x * dy * dy);
+          const float raw = exp2_hw(q * kNegHalfLog2e);
+          const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
+          const float w = base * v1.y;
+          nr = fmaf(w, v1.z, nr);
+          ng = fmaf(w, v1.w, ng);
+          nb = fmaf(w, v2.x, nb);
+          d += w;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  acc = make_float4(nr, ng, nb, d);
+}
+
+template <bool kFade, bool kNormalize>
+__global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
+                                                              float* __restrict__ out,
+                                                              float* __restrict__ den) {
+  __shared__ StageLds st;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile_x = tile % a.tiles_x;
+  const int tile_y = tile / a.tiles_x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int qx0 = tile_x * kTile + (wave & 1) * 8;
+  const int qy0 = tile_y * kTile + (wave >> 1) * 8;
+  const int px = qx0 + (lane & 7);
+  const int py = qy0 + (lane >> 3);
+  const float tail = kFade ? exp2_hw(a.tail_exp) : 0.0f;
+  float4 acc;
+  accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
+  if (px < a.W && py < a.H) {
+    const int64_t flat = int64_t(py) * a.W + px;
+    if (den) den[flat] = acc.w;
+    if (kNormalize) {
+      const float denom = acc.w + a.eps;
+      out[flat * 3 + 0] = acc.x / denom;
+      out[flat * 3 + 1] = acc.y / denom;
+      out[flat * 3 + 2] = acc.z / denom;
+    } else {
+      out[flat * 3 + 0] = acc.x;
+      out[flat * 3 + 1] = acc.y;
+      out[flat * 3 + 2] = acc.z;
+    }
+  }
+}
+
+struct GradArgs {
+  const float* grad_out;
+  float* grad_means;
+  float* grad_conics;
+  float* grad_colors;
+  float* grad_opacities;  // NULL when the field has no opacities
+};
+
+template <bool kFade, bool kNormalize>
+__global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, GradArgs g) {
+  __shared__ StageLds st;
+  __shared__ float4 s_adj[kTile * kTile];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile_x = tile % a.tiles_x;
+  const int tile_y = tile / a.tiles_x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int lx = lane & 7;
+  const int ly = lane >> 3;
+  const int tx0 = tile_x * kTile;
+  const int ty0 = tile_y * kTile;
+  const float tail = kFade ? exp2_hw(a.tail_exp) : 0.0f;
+
+  // Phase 1: recompute N_k and D for one pixel per lane (same code and order as the forward),
+  // then the per-pixel adjoints dL/dN_k and dL/dD.
+  {
+    const int qx0 = tx0 + (wave & 1) * 8;
+    const int qy0 = ty0 + (wave >> 1) * 8;
+    const int px = qx0 + lx;
+    const int py = qy0 + ly;
+    float4 acc;
+    accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
+    float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (px < a.W && py < a.H) {
+      const int64_t flat = int64_t(py) * a.W + px;
+      const float gr = g.grad_out[flat * 3 + 0];
+      const float gg = g.grad_out[flat * 3 + 1];
+      const float gb = g.grad_out[flat * 3 + 2];
+      if (kNormalize) {
+        const float denom = acc.w + a.eps;
+        const float o_r = acc.x / denom;
+        const float o_g = acc.y / denom;
+        const float o_b = acc.z / denom;
+        adj = make_float4(gr / denom, gg / denom, gb / denom,
+                          -(gr * o_r + gg * o_g + gb * o_b) / denom);
+      } else {
+        adj = make_float4(gr, gg, gb, 0.0f);
+      }
+    }
+    s_adj[(py - ty0) * kTile + (px - tx0)] = adj;
+  }
+  __syncthreads();
+
+  // Phase 2: each lane owns the pixel (lx, ly) of all four quadrants.
+  float4 adj[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    adj[s] = s_adj[((s >> 1) * 8 + ly) * kTile + (s & 1) * 8 + lx];
+  }
+  const bool has_opacity = g.grad_opacities != nullptr;
+  int64_t start, end;
+  tile_range(a, tile, &start, &end);
+  for (int64_t batch = start; batch < end; batch += kStage) {
+    const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
+    stage_batch(a, st, batch, m);
+    __syncthreads();
+    for (int j = wave; j < m; j += kRenderThreads / 64) {
+      const float4 v0 = st.c0[j];
+      const float4 v1 = st.c1[j];
+      const float4 v2 = st.c2[j];
+      const uint32_t xr = __builtin_amdgcn_readfirstlane(__float_as_uint(v2.y));
+      const uint32_t yr = __builtin_amdgcn_readfirstlane(__float_as_uint(v2.z));
+      const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
+      const float mx = v0.x, my = v0.y, ca = v0.z, cb2 = v0.w, cc = v1.x;
+      const float cr = v1.z, cg = v1.w, cbl = v2.x;
+      float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+      float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int qx0 = tx0 + (s & 1) * 8;
+        const int qy0 = ty0 + (s >> 1) * 8;
+        if (!(x0 < qx0 + 8 && x1 > qx0 && y0 < qy0 + 8 && y1 > qy0)) continue;
+        const bool full = x0 <= qx0 && x1 >= qx0 + 8 && y0 <= qy0 && y1 >= qy0 + 8;
+        const int px = qx0 + lx;
+        const int py = qy0 + ly;
+        const bool inside = full || (static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0) &&
+                                     static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0));
+        if (inside) {
+          const float dx = static_cast<float>(px) - mx;
+          const float dy = static_cast<float>(py) - my;
+          const float t = fmaf(ca, dx, cb2 * dy);
+          const float q = fmaf(t, dx, cc * dy * dy);
+          const float raw = exp2_hw(q * kNegHalfLog2e);
+          const float4 ad = adj[s];
+          const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
+          float base, dqr;
+          if (kFade) {
+            base = fmaxf(raw - tail, 0.0f);
+            dqr = raw > tail ? dw * raw : 0.0f;
+          } else {
+            base = raw;
+            dqr = dw * raw;
+          }
+          gcr = fmaf(ad.x, base, gcr);
+          gcg = fmaf(ad.y, base, gcg);
+          gcb = fmaf(ad.z, base, gcb);
+          gop = fmaf(dw, base, gop);
+          sx = fmaf(dqr, dx, sx);
+          sy = fmaf(dqr, dy, sy);
+          const float dqx = dqr * dx;
+          sxx = fmaf(dqx, dx, sxx);
+          sxy = fmaf(dqx, dy, sxy);
+          syy = fmaf(dqr * dy, dy, syy);
+        }
+      }
+      sx = wave_sum_lane63(sx);
+      sy = wave_sum_lane63(sy);
+      sxx = wave_sum_lane63(sxx);
+      sxy = wave_sum_lane63(sxy);
+      syy = wave_sum_lane63(syy);
+      gcr = wave_sum_lane63(gcr);
+      gcg = wave_sum_lane63(gcg);
+      gcb = wave_sum_lane63(gcb);
+      gop = wave_sum_lane63(gop);
+      if (lane == 63) {
+        // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries.
+        const int gid = __float_as_int(v2.w);
+        const float op = v1.y;
+        const float cb = 0.5f * cb2;
+        const float gmx = op * (ca * sx + cb * sy);
+        const float gmy = op * (cb * sx + cc * sy);
+        const float ga = -0.5f * op * sxx;
+        const float gb = -op * sxy;
+        const float gc = -0.5f * op * syy;
+        float* gm = g.grad_means + 2 * int64_t(gid);
+        float* gq = g.grad_conics + 3 * int64_t(gid);
+        float* gk = g.grad_colors + 3 * int64_t(gid);
+        if (gmx != 0.0f) unsafeAtomicAdd(gm + 0, gmx);
+        if (gmy != 0.0f) unsafeAtomicAdd(gm + 1, gmy);
+        if (ga != 0.0f) unsafeAtomicAdd(gq + 0, ga);
+        if (gb != 0.0f) unsafeAtomicAdd(gq + 1, gb);
+        if (gc != 0.0f) unsafeAtomicAdd(gq + 2, gc);
+        if (gcr != 0.0f) unsafeAtomicAdd(gk + 0, op * gcr);
+        if (gcg != 0.0f) unsafeAtomicAdd(gk + 1, op * gcg);
+        if (gcb != 0.0f) unsafeAtomicAdd(gk + 2, op * gcb);
+        if (has_opacity && gop != 0.0f) unsafeAtomicAdd(g.grad_opacities + gid, gop);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side argument handling
+// ------------------------------------------------------------------------------------------
+
+struct Dims {
+  int tiles_x, tiles_y, num_tiles;
+};
+
+int check_dims(int64_t n, int H, int W, int ts, Dims* d) {
+  if (n < 0 || n > INT32_MAX) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
+  if (H <= 0 || W <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be positive");
+  }
+  if (H > 65535 || W > 65535) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be <= 65535");
+  }
+  if (ts <= 0 || ts * ts > 1024) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
+                     "tile_size must be positive with tile_size^2 <= 1024");
+  }
+  const int64_t tx = (int64_t(W) + ts - 1) / ts;
+  const int64_t ty = (int64_t(H) + ts - 1) / ts;
+  const int64_t t = tx * ty;
+  if (t + 1 > kMaxScanItems || t + 1 > INT32_MAX) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile grid of %lld tiles is too large",
+                     static_cast<long long>(t));
+  }
+  d->tiles_x = static_cast<int>(tx);
+  d->tiles_y = static_cast<int>(ty);
+  d->num_tiles = static_cast<int>(t);
+  return SSPLAT_OK;
+}
+
+struct Workspace {
+  GaussRec* recs;
+  int32_t* counts;   // num_tiles + 1 per-tile counters (binning), reused as scatter cursors
+  int64_t* bsums;    // first-level scan block sums
+  int32_t* status;   // internal status word of the last binning
+  size_t bytes;
+};
+
+Workspace layout(void* base, int64_t n, int num_tiles) {
+  Workspace w{};
+  size_t off = 0;
+  const size_t recs = off;
+  off = align_up(off + size_t(n) * sizeof(GaussRec), 256);
+  const size_t counts = off;
+  off = align_up(off + size_t(num_tiles + 1) * sizeof(int32_t), 256);
+  const size_t bsums = off;
+  const size_t blocks = (size_t(num_tiles) + 1 + kScanBlock - 1) / kScanBlock;
+  off = align_up(off + blocks * sizeof(int64_t), 256);
+  const size_t status = off;
+  off = align_up(off + sizeof(int32_t), 256);
+  char* b = static_cast<char*>(base);
+  w.recs = reinterpret_cast<GaussRec*>(b + recs);
+  w.counts = reinterpret_cast<int32_t*>(b + counts);
+  w.bsums = reinterpret_cast<int64_t*>(b + bsums);
+  w.status = reinterpret_cast<int32_t*>(b + status);
+  w.bytes = off;
+  return w;
+}
+
+int check_workspace(void* ws, size_t ws_bytes, int64_t n, int num_tiles, Workspace* w) {
+  *w = layout(ws, n, num_tiles);
+  if (ws == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace is NULL");
+  if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+  }
+  if (ws_bytes < w->bytes) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace has %zu bytes, needs %zu", ws_bytes,
+                     w->bytes);
+  }
+  return SSPLAT_OK;
+}
+
+int make_bin_args(const float* means, const float* conics, const int64_t* radii, int64_t n,
+                  int H, int W, int ts, int cull, double sigma_cutoff, const Dims& d,
+                  BinArgs* a) {
+  if (n > 0 && (means == nullptr || radii == nullptr || (cull && conics == nullptr))) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/conics/radii must not be NULL");
+  }
+  if (!std::isfinite(sigma_cutoff)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "sigma_cutoff must be finite");
+  }
+  a->means = means;
+  a->conics = conics;
+  a->radii = radii;
+  a->n = static_cast<int>(n);
+  a->H = H;
+  a->W = W;
+  a->ts = ts;
+  a->tiles_x = d.tiles_x;
+  a->cull = cull ? 1 : 0;
+  // Same float32 arithmetic as the reference: cutoff2 = float(sigma^2), guard 1e-5*max(cutoff2,1)
+  // (render_ext.cu:187-188, :920).
+  const float cutoff2 = static_cast<float>(sigma_cutoff * sigma_cutoff);
+  volatile float guard = 1e-5f * (cutoff2 > 1.0f ? cutoff2 : 1.0f);
+  volatile float thr = cutoff2 + guard;
+  a->cull_thr = thr;
+  return SSPLAT_OK;
+}
+
+int run_count(const BinArgs& a, const Dims& d, const Workspace& w, int32_t* offsets,
+              hipStream_t stream) {
+  const int64_t items = int64_t(d.num_tiles) + 1;
+  int rc = fill_i32(w.counts, items, 0, stream);
+  if (rc) return rc;
+  rc = fill_i32(w.status, 1, SSPLAT_OK, stream);
+  if (rc) return rc;
+  if (a.n > 0) {
+    k_bin_count<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(a, w.counts);
+    SSPLAT_LAUNCH_CHECK("k_bin_count");
+  }
+  const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
+  k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(w.counts, items, w.bsums);
+  SSPLAT_LAUNCH_CHECK("k_scan_partials");
+  k_scan_top<<<1, kScanThreads, 0, stream>>>(w.bsums, blocks);
+  SSPLAT_LAUNCH_CHECK("k_scan_top");
+  k_scan_apply<<<blocks, kScanThreads, 0, stream>>>(w.counts, items, w.bsums, offsets, w.status);
+  SSPLAT_LAUNCH_CHECK("k_scan_apply");
+  return SSPLAT_OK;
+}
+
+int run_fill(const BinArgs& a, const Dims& d, const Workspace& w, const int32_t* offsets,
+             int32_t* gids, int64_t capacity, int32_t* status, hipStream_t stream) {
+  const int64_t items = int64_t(d.num_tiles) + 1;
+  int rc = fill_i32(w.counts, items, 0, stream);
+  if (rc) return rc;
+  if (a.n > 0) {
+    k_bin_scatter<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(
+        a, offsets, w.counts, gids, capacity, w.status);
+    SSPLAT_LAUNCH_CHECK("k_bin_scatter");
+    k_sort_tiles<<<d.num_tiles, kSortThreads, 0, stream>>>(gids, offsets, d.num_tiles, capacity);
+    SSPLAT_LAUNCH_CHECK("k_sort_tiles");
+  }
+  if (status) {
+    k_merge_status<<<1, 64, 0, stream>>>(w.status, status);
+    SSPLAT_LAUNCH_CHECK("k_merge_status");
+  }
+  return SSPLAT_OK;
+}
+
+int check_render_inputs(const float* means, const float* conics, const float* colors,
+                        const int64_t* radii, int64_t n, const int32_t* gids,
+                        const int32_t* offsets, int64_t num_gids, int ts, int normalize,
+                        double eps) {
+  if (n > 0 && (means == nullptr || conics == nullptr || colors == nullptr || radii == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/conics/colors/radii must not be NULL");
+  }
+  if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  if (num_gids < 0 || (num_gids > 0 && gids == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or num_gids negative");
+  }
+  if (ts != kTile) {
+    return set_error(SSPLAT_ERR_UNSUPPORTED, "the render kernels are built for tile_size %d, got %d",
+                     kTile, ts);
+  }
+  if (normalize && !(std::isfinite(eps) && eps > 0.0)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "normalization_eps must be finite and > 0");
+  }
+  return SSPLAT_OK;
+}
+
+RenderArgs make_render_args(const Workspace& w, int64_t n, const int32_t* gids,
+                            const int32_t* offsets, int64_t num_gids, int H, int W, const Dims& d,
+                            double eps, double sigma_cutoff) {
+  RenderArgs r;
+  r.recs = w.recs;
+  r.gids = gids;
+  r.offsets = offsets;
+  r.num_gids = num_gids;
+  r.n = static_cast<int>(n);
+  r.H = H;
+  r.W = W;
+  r.tiles_x = d.tiles_x;
+  r.eps = static_cast<float>(eps);
+  r.tail_exp = static_cast<float>(sigma_cutoff * sigma_cutoff) * kNegHalfLog2e;
+  return r;
+}
+
+int run_prepare(const float* means, const float* conics, const float* colors,
+                const int64_t* radii, const float* opacities, int64_t n, int H, int W,
+                const Workspace& w, hipStream_t stream) {
+  if (n == 0) return SSPLAT_OK;
+  k_prepare<<<static_cast<int>((n + 255) / 256), 256, 0, stream>>>(
+      means, conics, colors, radii, opacities, static_cast<int>(n), H, W, w.recs);
+  SSPLAT_LAUNCH_CHECK("k_prepare");
+  return SSPLAT_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+
+extern "C" {
+
+int ssplat_abi_version(void) { return SSPLAT_ABI_VERSION; }
+
+const char* ssplat_last_error(void) { return g_last_error.c_str(); }
+
+int ssplat_workspace_bytes(int64_t n, int height, int width, int tile_size, size_t* bytes) {
+  if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  Dims d;
+  const int rc = check_dims(n, height, width, tile_size, &d);
+  if (rc) return rc;
+  *bytes = layout(nullptr, n, d.num_tiles).bytes;
+  return success();
+}
+
+int ssplat_tile_index_count(const float* means, const float* conics, const int64_t* radii,
+                            int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                            double sigma_cutoff, int32_t* tile_offsets, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  Dims d;
+  int rc = check_dims(n, height, width, tile_size, &d);
+  if (rc) return rc;
+  if (tile_offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  Workspace w;
+  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
+  if (rc) return rc;
+  BinArgs a;
+  rc = make_bin_args(means, conics, radii, n, height, width, tile_size, ellipse_cull, sigma_cutoff,
+                     d, &a);
+  if (rc) return rc;
+  rc = run_count(a, d, w, tile_offsets, static_cast<hipStream_t>(stream));
+  if (rc) return rc;
+  return success();
+}
+
+int ssplat_tile_index_fill(const float* means, const float* conics, const int64_t* radii,
+                           int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                           double sigma_cutoff, const int32_t* tile_offsets, int32_t* tile_gids,
+                           int64_t gid_capacity, int32_t* status, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  Dims d;
+  int rc = check_dims(n, height, width, tile_size, &d);
+  if (rc) return rc;
+  if (tile_offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  if (gid_capacity < 0 || (gid_capacity > 0 && tile_gids == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or capacity negative");
+  }
+  Workspace w;
+  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
+  if (rc) return rc;
+  BinArgs a;
+  rc = make_bin_args(means, conics, radii, n, height, width, tile_size, ellipse_cull, sigma_cutoff,
+                     d, &a);
+  if (rc) return rc;
+  rc = run_fill(a, d, w, tile_offsets, tile_gids, gid_capacity, status,
+                static_cast<hipStream_t>(stream));
+  if (rc) return rc;
+  return success();
+}
+
+int ssplat_build_tile_index(const float* means, const float* conics, const int64_t* radii,
+                            int64_t n, int height, int width, int tile_size, int ellipse_cull,
+                            double sigma_cutoff, int32_t* tile_offsets, int32_t* tile_gids,
+                            int64_t gid_capacity, int32_t* status, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  const int rc = ssplat_tile_index_count(means, conics, radii, n, height, width, tile_size,
+                                         ellipse_cull, sigma_cutoff, tile_offsets, workspace,
+                                         workspace_bytes, stream);
+  if (rc) return rc;
+  return ssplat_tile_index_fill(means, conics, radii, n, height, width, tile_size, ellipse_cull,
+                                sigma_cutoff, tile_offsets, tile_gids, gid_capacity, status,
+                                workspace, workspace_bytes, stream);
+}
+
+int ssplat_forward_tiled(const float* means, const float* conics, const float* colors,
+                         const int64_t* radii, const float* opacities, int64_t n,
+                         const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
+                         int height, int width, int tile_size, int normalize, double eps,
+                         int support_fade, double sigma_cutoff, float* out, float* den,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+  Dims d;
+  int rc = check_dims(n, height, width, tile_size, &d);
+  if (rc) return rc;
+  rc = check_render_inputs(means, conics, colors, radii, n, tile_gids, tile_offsets, num_gids,
+                           tile_size, normalize, eps);
+  if (rc) return rc;
+  if (out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "out is NULL");
+  Workspace w;
+  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  rc = run_prepare(means, conics, colors, radii, opacities, n, height, width, w, s);
+  if (rc) return rc;
+  const RenderArgs ra =
+      make_render_args(w, n, tile_gids, tile_offsets, num_gids, height, width, d, eps,
+                       sigma_cutoff);
+  const int fade = support_fade ? 1 : 0;
+  const int norm = normalize ? 1 : 0;
+  if (fade && norm) {
+    k_forward16<true, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else if (fade) {
+    k_forward16<true, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else if (norm) {
+    k_forward16<false, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else {
+    k_forward16<false, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  }
+  SSPLAT_LAUNCH_CHECK("k_forward16");
+  return success();
+}
+
+int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
+                          const float* colors, const int64_t* radii, const float* opacities,
+                          int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
+                          int64_t num_gids, int height, int width, int tile_size, int normalize,
+                          double eps, int support_fade, double sigma_cutoff, float* grad_means,
+                          float* grad_conics, float* grad_colors, float* grad_opacities,
+                          void* workspace, size_t workspace_bytes, void* stream) {
+  Dims d;
+  int rc = check_dims(n, height, width, tile_size, &d);
+  if (rc) return rc;
+  rc = check_render_inputs(means, conics, colors, radii, n, tile_gids, tile_offsets, num_gids,
+                           tile_size, normalize, eps);
+  if (rc) return rc;
+  if (grad_out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_out is NULL");
+  if (n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
+  }
+  Workspace w;
+  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return success();
+  if ((rc = fill_f32(grad_means, 2 * n, 0.0f, s))) return rc;
+  if ((rc = fill_f32(grad_conics, 3 * n, 0.0f, s))) return rc;
+  if ((rc = fill_f32(grad_colors, 3 * n, 0.0f, s))) return rc;
+  float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
+  if (gop && (rc = fill_f32(gop, n, 0.0f, s))) return rc;
+  rc = run_prepare(means, conics, colors, radii, opacities, n, height, width, w, s);
+  if (rc) return rc;
+  const RenderArgs ra =
+      make_render_args(w, n, tile_gids, tile_offsets, num_gids, height, width, d, eps,
+                       sigma_cutoff);
+  GradArgs ga{grad_out, grad_means, grad_conics, grad_colors, gop};
+  const int fade = support_fade ? 1 : 0;
+  const int norm = normalize ? 1 : 0;
+  if (fade && norm) {
+    k_backward16<true, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else if (fade) {
+    k_backward16<true, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else if (norm) {
+    k_backward16<false, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else {
+    k_backward16<false, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  }
+  SSPLAT_LAUNCH_CHECK("k_backward16");
+  return success();
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+"""Fit step around the HIP renderer: render -> L1 + SSIM loss -> backward -> Adam -> log-scale clamp.
+
+This is the reference's hot loop reduced to the renderer's neighbours (fit.py:3182-3375):
+``_render`` (fit.py:1043-1056), ``_weighted_pixel_loss`` with ``pixel_loss="l1"`` (fit.py:800-823),
+``loss = (1 - w) L1 + w (1 - SSIM)`` (fit.py:3219-3244), ``opt.step()`` with the parameter groups
+of gaussians.py:203-214, and the log-scale clamp to [ln 0.35, ln max(H, W)] (fit.py:3033,
+:3359-3375).  Densification, pruning and the other research events of the reference fitter are
+not part of this path.
+
+Two drivers run the same arithmetic:
+
+* ``fit(..., graph=False)`` - the plain per-image loop (one eager step per iteration).
+* ``fit(..., graph=True)`` - the whole step captured once as a HIP graph and replayed.
+
+Both bin without host synchronization into a capacity-sized pair buffer (``RenderPlan``).  The
+binning status word is sticky; it is checked once per window of steps, and a window that
+overflowed is rolled back (parameters and Adam state restored) and replayed with a larger
+capacity, so the result equals a run with unlimited capacity.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from . import render as HR
+from .gaussians import GaussianField
+from .metrics import SSIMTargetStats, psnr_tensor, ssim
+
+
+@dataclass
+class FitConfig:
+    iters: int = 1000
+    lr_means: float = 1e-2
+    lr_scales: float = 5e-3
+    lr_rot: float = 5e-3
+    lr_color: float = 5e-3
+    lr_opacity: float = 5e-3
+    ssim_weight: float = 0.2       # loss = (1 - w) * L1 + w * (1 - SSIM)
+    sigma_cutoff: float = 3.0
+    support_fade: bool = False
+    normalization_eps: float = 1e-8
+    normalize: bool = True
+    window: int = 25               # steps between capacity checks
+    capacity_slack: float = 1.5    # pair-buffer capacity / current pair count
+
+
+class RenderPlan:
+    """Preallocated pair buffer, offsets, sticky status and workspace for one field shape."""
+
+    def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16):
+        self.n, self.H, self.W, self.tile_size = int(n), int(H), int(W), int(tile_size)
+        self.capacity = max(int(capacity), 1)
+        T = HR.num_tiles(H, W, tile_size)
+        self.gids = torch.empty(self.capacity, dtype=torch.int32, device=device)
+        self.offsets = torch.empty(T + 1, dtype=torch.int32, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+        self.ws = HR.workspace(n, H, W, tile_size, device)
+
+    def buffers(self):
+        return self.gids, self.offsets, self.status, self.ws
+
+
+class _PlannedRender(Function):
+    """Autograd node of one planned (synchronization-free) render; same gradients as
+    ``render._ExactRenderHip``."""
+
+    @staticmethod
+    def forward(ctx, means, conics, colors, radii, opacities, plan, normalize, eps, fade,
+                sigma_cutoff):
+        means = means.contiguous()
+        conics = conics.contiguous()
+        colors = colors.contiguous()
+        radii = radii.contiguous()
+        opacities = opacities.contiguous()
+        cull = bool(fade)
+        gids, offsets, _ = HR.build_tile_index_capacity(
+            means, conics, radii, plan.H, plan.W, plan.capacity, plan.tile_size, cull,
+            sigma_cutoff, out=plan.buffers())
+        out, _ = HR.forward_tiled(means, conics, colors, radii, opacities, gids, offsets, plan.H,
+                                  plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
+                                  ws=plan.ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities)
+        ctx.plan = plan
+        ctx.args = (normalize, eps, fade, sigma_cutoff)
+        return out
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_out):
+        means, conics, colors, radii, opacities = ctx.saved_tensors
+        plan = ctx.plan
+        normalize, eps, fade, sigma_cutoff = ctx.args
+        g_m, g_q, g_c, g_o = HR.backward_tiled(
+            grad_out.contiguous(), means, conics, colors, radii, opacities, plan.gids,
+            plan.offsets, plan.H, plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
+            ws=plan.ws)
+        if opacities.numel() == 0:
+            g_o = None
+        return g_m, g_q, g_c, None, g_o, None, None, None, None, None
+
+
+def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> torch.Tensor:
+    op = field.opacity_values()
+    if op is None:
+        op = field.means.new_empty(0)
+    return _PlannedRender.apply(field.means, field.conics(), field.colors,
+                                field.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
+                                float(cfg.normalization_eps), bool(cfg.support_fade),
+                                float(cfg.sigma_cutoff))
+
+
+def loss_fn(img: torch.Tensor, target: torch.Tensor, stats: SSIMTargetStats | None,
+            w: float) -> torch.Tensor:
+    """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1")."""
+    l1 = (img - target).abs().mean()
+    if w == 0.0:
+        return l1
+    return (1.0 - w) * l1 + w * (1.0 - ssim(img, stats=stats))
+
+
+def pair_count(field: GaussianField, H: int, W: int, cfg: FitConfig, tile_size: int = 16) -> int:
+    """Exact (tile, Gaussian) pair count of the current field (one host readback)."""
+    with torch.no_grad():
+        _, offsets = HR.build_tile_index(field.means, field.conics(), field.radii(cfg.sigma_cutoff),
+                                         H, W, tile_size, bool(cfg.support_fade),
+                                         cfg.sigma_cutoff)
+    return int(offsets[-1].item())
+
+
+def make_optimizer(field: GaussianField, cfg: FitConfig) -> torch.optim.Adam:
+    groups = field.parameter_groups(cfg.lr_means, cfg.lr_scales, cfg.lr_rot, cfg.lr_color,
+                                    cfg.lr_opacity)
+    return torch.optim.Adam(groups, capturable=True)
+
+
+class _Snapshot:
+    def __init__(self, field: GaussianField, opt: torch.optim.Adam):
+        self.params = [p.detach().clone() for p in field.parameters()]
+        self.state = {id(p): {k: (v.clone() if torch.is_tensor(v) else v)
+                              for k, v in opt.state[p].items()} for p in field.parameters()
+                      if p in opt.state}
+
+    def restore(self, field: GaussianField, opt: torch.optim.Adam) -> None:
+        with torch.no_grad():
+            for p, saved in zip(field.parameters(), self.params):
+                p.copy_(saved)
+                st = self.state.get(id(p))
+                if st is not None:
+                    for k, v in st.items():
+                        if torch.is_tensor(v):
+                            opt.state[p][k].copy_(v)
+
+
+def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool = False,
+        verbose: bool = False) -> dict:
+    """Fit `field` to `target` (H, W, 3) for cfg.iters Adam steps; returns history and PSNR."""
+    H, W = target.shape[:2]
+    dev = target.device
+    field.trainable()
+    opt = make_optimizer(field, cfg)
+    stats = SSIMTargetStats(target) if cfg.ssim_weight != 0.0 else None
+    lo, hi = math.log(0.35), math.log(max(H, W))
+    plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev)
+    state = {"loss": None}
+
+    def step():
+        opt.zero_grad(set_to_none=False)
+        img = planned_render(field, plan, cfg)
+        loss = loss_fn(img, target, stats, cfg.ssim_weight)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            field.log_scales.clamp_(min=lo, max=hi)
+        state["loss"] = loss.detach()
+
+    done = 0
+    graph_obj = None
+    regrows = 0
+    while done < cfg.iters:
+        n = min(cfg.window, cfg.iters - done)
+        snap = _Snapshot(field, opt) if done > 0 else None
+        plan.status.zero_()
+        if graph and done >= 2:
+            if graph_obj is None:
+                graph_obj = _capture(step)
+            for _ in range(n):
+                graph_obj.replay()
+        elif graph:
+            # The graphed driver's first two steps run eagerly on a side stream (torch's capture
+            # recipe): they create Adam's state and the gradient buffers the graph reuses.
+            n = min(n, 2 - done)
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                for _ in range(n):
+                    step()
+            torch.cuda.current_stream(dev).wait_stream(side)
+        else:
+            for _ in range(n):
+                step()
+        if int(plan.status.item()) != 0:
+            if snap is None:
+                raise RuntimeError("pair capacity exceeded in the first steps")
+            snap.restore(field, opt)
+            plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg) * 2,
+                              dev)
+            graph_obj = None
+            regrows += 1
+            continue
+        done += n
+        # Keep headroom: regrow (and re-capture) when the pair count approaches capacity.
+        p = pair_count(field, H, W, cfg)
+        if p * 1.15 > plan.capacity:
+            plan = RenderPlan(field.n, H, W, _capacity(p, cfg), dev)
+            graph_obj = None
+            regrows += 1
+        if verbose:
+            print(f"step {done}: loss {float(state['loss']):.6f} pairs {p}", flush=True)
+    with torch.no_grad():
+        img = HR.render_hip_exact(field.means, field.conics(), field.colors,
+                                  field.radii(cfg.sigma_cutoff), H, W,
+                                  opacities=field.opacity_values(), normalize=cfg.normalize,
+                                  eps=cfg.normalization_eps, support_fade=cfg.support_fade,
+                                  sigma_cutoff=cfg.sigma_cutoff)
+        final_psnr = float(psnr_tensor(img, target))
+    return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
+            "capacity": plan.capacity}
+
+
+def _capacity(pairs: int, cfg: FitConfig) -> int:
+    return int(pairs * cfg.capacity_slack) + 4096
+
+
+def _capture(step) -> torch.cuda.CUDAGraph:
+    """Capture one full step (render, loss, backward, Adam, clamp) as a HIP graph.
+
+    The caller has already run two eager steps on a side stream; they are real optimization
+    steps (counted in the iteration budget) and they create the Adam state and gradient buffers
+    that the captured step updates in place.
+    """
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        step()
+    torch.cuda.synchronize()
+    return g

@@ -1,0 +1,146 @@
+"""Field layout and producer of the renderer inputs (the reference's ``GaussianField``).
+
+A field stores means (N,2), log-scales (N,2), rotations (N,), colors (N,3) and optional opacity
+logits (N,) (gaussians.py:15-29).  The renderer consumes what this class produces:
+``conics()`` (inverse covariance entries (a, b, c)), ``radii()`` (int64 support half-widths) and
+``opacity_values()`` (sigmoid of the logits).  The formulas follow gaussians.py:231-321 so the
+integer radii, and therefore the tile lists, are identical to the reference's.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class GaussianField:
+    def __init__(self, means, log_scales, rotations, colors, opacities=None,
+                 filter_variance=None):
+        self.means = means
+        self.log_scales = log_scales
+        self.rotations = rotations
+        self.colors = colors
+        self.opacities = opacities
+        # Optional non-trainable isotropic covariance variance (px^2) per row.
+        self.filter_variance = filter_variance
+
+    @classmethod
+    def from_numpy(cls, means, scales, angles, colors, opacities=None, device="cpu",
+                   dtype=torch.float32, filter_variance=None):
+        """Build a field from NumPy arrays; scales are clipped to >= 1e-3 (gaussians.py:31-55)."""
+        def t(a):
+            return torch.as_tensor(np.asarray(a), device=device, dtype=dtype).clone()
+
+        scales = np.clip(np.asarray(scales), 1e-3, None)
+        opacity_t = None if opacities is None else t(opacities).reshape(-1)
+        fv = None if filter_variance is None else t(filter_variance).reshape(-1)
+        return cls(t(means), torch.log(t(scales)), t(angles).reshape(-1), t(colors), opacity_t,
+                   fv)
+
+    @property
+    def n(self) -> int:
+        return self.means.shape[0]
+
+    def trainable(self) -> "GaussianField":
+        params = [self.means, self.log_scales, self.rotations, self.colors]
+        if self.opacities is not None:
+            params.append(self.opacities)
+        for p in params:
+            p.requires_grad_(True)
+        return self
+
+    def detached(self) -> "GaussianField":
+        def d(x):
+            return None if x is None else x.detach().clone()
+
+        return GaussianField(d(self.means), d(self.log_scales), d(self.rotations),
+                             d(self.colors), d(self.opacities), d(self.filter_variance))
+
+    def parameters(self) -> list[torch.Tensor]:
+        params = [self.means, self.log_scales, self.rotations, self.colors]
+        if self.opacities is not None:
+            params.append(self.opacities)
+        return params
+
+    def parameter_groups(self, lr_means, lr_scales, lr_rot, lr_color, lr_opacity=1e-2):
+        """Adam parameter groups (gaussians.py:203-214)."""
+        groups = [
+            {"params": [self.means], "lr": lr_means},
+            {"params": [self.log_scales], "lr": lr_scales},
+            {"params": [self.rotations], "lr": lr_rot},
+            {"params": [self.colors], "lr": lr_color},
+        ]
+        if self.opacities is not None:
+            groups.append({"params": [self.opacities], "lr": lr_opacity})
+        return groups
+
+    def scales(self):
+        return torch.exp(self.log_scales)
+
+    def opacity_values(self):
+        """sigmoid(logits) or None (= opacity 1) (gaussians.py:270-273)."""
+        if self.opacities is None:
+            return None
+        return torch.sigmoid(self.opacities)
+
+    def conics(self, dilation: float = 0.0):
+        """(a, b, c) of inv(Sigma) = [[a, b], [b, c]] with Sigma = R diag(sx^2, sy^2) R^T + d I
+        (gaussians.py:275-300)."""
+        if dilation < 0.0:
+            raise ValueError(f"dilation must be >= 0, got {dilation}")
+        s = self.scales()
+        sx2 = s[:, 0] ** 2
+        sy2 = s[:, 1] ** 2
+        if self.filter_variance is not None:
+            v = self.filter_variance.to(device=s.device, dtype=s.dtype).reshape(-1)
+            sx2 = sx2 + v
+            sy2 = sy2 + v
+        inv_sx2 = 1.0 / (sx2 + dilation)
+        inv_sy2 = 1.0 / (sy2 + dilation)
+        c = torch.cos(self.rotations)
+        sn = torch.sin(self.rotations)
+        a = c * c * inv_sx2 + sn * sn * inv_sy2
+        b = c * sn * (inv_sx2 - inv_sy2)
+        cc = sn * sn * inv_sx2 + c * c * inv_sy2
+        return torch.stack([a, b, cc], dim=1)
+
+    def radii(self, sigma_cutoff: float, dilation: float = 0.0):
+        """Per-axis support half-widths ceil(sigma_cutoff * sqrt(Sigma_xx)), ... clamped >= 1,
+        int64, detached (gaussians.py:302-321)."""
+        if dilation < 0.0:
+            raise ValueError(f"dilation must be >= 0, got {dilation}")
+        with torch.no_grad():
+            s2 = self.scales() ** 2
+            if self.filter_variance is not None:
+                s2 = s2 + self.filter_variance.to(device=s2.device, dtype=s2.dtype).reshape(-1, 1)
+            s2 = s2 + dilation
+            c = torch.cos(self.rotations)
+            sn = torch.sin(self.rotations)
+            var_x = c * c * s2[:, 0] + sn * sn * s2[:, 1]
+            var_y = sn * sn * s2[:, 0] + c * c * s2[:, 1]
+            r = sigma_cutoff * torch.sqrt(torch.stack([var_x, var_y], dim=1))
+            return torch.clamp(r.ceil().long(), min=1)
+
+    @torch.no_grad()
+    def save(self, path: str) -> None:
+        data = {
+            "means": self.means.detach().cpu().numpy(),
+            "log_scales": self.log_scales.detach().cpu().numpy(),
+            "rotations": self.rotations.detach().cpu().numpy(),
+            "colors": self.colors.detach().cpu().numpy(),
+        }
+        if self.opacities is not None:
+            data["opacities"] = self.opacities.detach().cpu().numpy()
+        if self.filter_variance is not None:
+            data["filter_variance"] = self.filter_variance.detach().cpu().numpy()
+        np.savez(path, **data)
+
+    @classmethod
+    def load(cls, path: str, device="cpu") -> "GaussianField":
+        z = np.load(path)
+
+        def t(k):
+            return torch.as_tensor(z[k], device=device, dtype=torch.float32)
+
+        return cls(t("means"), t("log_scales"), t("rotations"), t("colors"),
+                   t("opacities") if "opacities" in z.files else None,
+                   t("filter_variance") if "filter_variance" in z.files else None)

@@ -1,0 +1,394 @@
+"""Render entry points of the HIP rasterizer: the drop-in for StructSplat's render path.
+
+The reference renders through ``render_field`` (render.py:319-414), whose CUDA modes go through
+``render_cuda_exact`` -> ``_ExactRenderCuda`` -> a pybind11 extension (cuda_render.py:52-294,
+render_ext.cpp:139-317).  This module keeps those Python names, arguments, mode strings and error
+phrases and routes every mode to the gfx950 kernels of ``libssplat_hip.so`` through the C ABI:
+
+* ``render_field(...)`` - mode dispatcher.  ``normalized`` / ``additive`` and all of the
+  reference's CUDA mode names (plus ``hip``, ``hip_additive``) run the HIP kernels on GPU tensors.
+  There is no CPU path: CPU tensors raise the reference's "requires CUDA tensors" error.
+* ``render_hip_exact(...)`` - the ``render_cuda_exact`` signature (cuda_render.py:256-294).
+* ``build_tile_index(...)`` - the native ``build_tile_index`` (render_ext.cpp:139-167).
+* ``render_hip_exact_with_coverage(...)`` - ``render_cuda_exact_with_coverage``
+  (cuda_render.py:229-253): normalized RGB plus raw coverage, no autograd.
+
+Differences from the reference by design: the backward recomputes the per-pixel normalizer
+instead of saving ``den``/``out`` from the forward; tile lists are built without a radix sort
+(ascending Gaussian id per tile, like the reference's stable sort); the forward is bitwise
+deterministic.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from . import _native
+
+_EPS = 1e-8
+DEFAULT_TILE_SIZE = 16
+_INT32_SATURATED = 2**31 - 1
+
+_NORMALIZED_MODES = (
+    "normalized", "hip", "hip_normalized", "hip_tiled", "hip_tiled_normalized",
+    "cuda", "cuda_normalized", "cuda_block_reduce", "cuda_tiled", "cuda_tiled_normalized",
+)
+_ADDITIVE_MODES = (
+    "additive", "hip_additive", "hip_tiled_additive", "cuda_additive", "cuda_tiled_additive",
+)
+
+_DEVICE_MESSAGE = (
+    "StructSplat HIP renderer requires CUDA tensors (torch's device type for ROCm GPUs); "
+    "pass device='cuda'."
+)
+
+
+def _validate_normalization_eps(value: float) -> float:
+    """eps must be finite and positive (render.py:33-37)."""
+    eps = float(value)
+    if not math.isfinite(eps) or eps <= 0.0:
+        raise ValueError(f"normalization_eps must be finite and > 0, got {value}")
+    return eps
+
+
+def _resolve_support_fade_alpha(support_fade: bool, support_fade_alpha: float | None = None
+                                ) -> float:
+    """Fade strength in [0, 1] (render.py:94-98)."""
+    if support_fade_alpha is None:
+        return 1.0 if support_fade else 0.0
+    return max(0.0, min(1.0, float(support_fade_alpha)))
+
+
+def _stream(device: torch.device) -> int:
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None or t.numel() == 0:
+        return None
+    return int(t.data_ptr())
+
+
+def _check_field(means, conics, colors, radii, opacities, need_colors: bool = True) -> None:
+    """Tensor contract of the reference extension (render_ext.cpp:94-125, cuda_render.py:274-277)."""
+    if not means.is_cuda:
+        raise RuntimeError(_DEVICE_MESSAGE)
+    tensors = (means, conics, colors) if need_colors else (means, conics)
+    if any(t.dtype != torch.float32 for t in tensors):
+        raise RuntimeError("StructSplat HIP renderer currently supports float32 tensors only.")
+    n = means.shape[0]
+    if means.dim() != 2 or means.shape[1] != 2:
+        raise ValueError("means must be (N, 2)")
+    if conics.dim() != 2 or conics.shape != (n, 3):
+        raise ValueError("conics must be (N, 3)")
+    if need_colors and (colors.dim() != 2 or colors.shape != (n, 3)):
+        raise ValueError("colors must be (N, 3)")
+    if radii.dim() != 2 or radii.shape != (n, 2):
+        raise ValueError("radii must be (N, 2)")
+    if radii.dtype not in (torch.int64, torch.int32):
+        raise ValueError("radii must be an integer tensor (int64)")
+    if opacities is not None and opacities.numel() not in (0, n):
+        raise ValueError("opacities must be empty or length N")
+    if opacities is not None and opacities.numel() and opacities.dtype != torch.float32:
+        raise RuntimeError("opacities must be empty or float32")
+    for t in (conics, colors, radii) + ((opacities,) if opacities is not None else ()):
+        if t.numel() and t.device != means.device:
+            raise RuntimeError("all field tensors must be on the same device")
+
+
+def _check_image(height: int, width: int, tile_size: int) -> None:
+    if int(height) <= 0 or int(width) <= 0:
+        raise ValueError("height and width must be positive")
+    if int(tile_size) <= 0 or int(tile_size) ** 2 > 1024:
+        raise ValueError("tile_size must be positive with tile_size^2 <= 1024")
+
+
+def _radii64(radii: torch.Tensor) -> torch.Tensor:
+    return radii.detach().to(torch.int64).contiguous()
+
+
+def num_tiles(height: int, width: int, tile_size: int = DEFAULT_TILE_SIZE) -> int:
+    ts = int(tile_size)
+    return ((int(width) + ts - 1) // ts) * ((int(height) + ts - 1) // ts)
+
+
+def workspace(n: int, height: int, width: int, tile_size: int, device) -> torch.Tensor:
+    """Caller-owned scratch for one call sequence (the library never allocates)."""
+    nbytes = _native.workspace_bytes(n, height, width, tile_size)
+    return torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+
+
+@torch.no_grad()
+def build_tile_index(means, conics, radii, H: int, W: int, tile_size: int = DEFAULT_TILE_SIZE,
+                     ellipse_cull: bool = False, sigma_cutoff: float = 3.0):
+    """Per-tile Gaussian lists: ``(gids int32 (P,), offsets int32 (T+1,))``.
+
+    Mirrors the reference's native ``build_tile_index`` (render_ext.cpp:139-167,
+    render_ext.cu:858-952): tile ``t`` owns ``gids[offsets[t]:offsets[t+1]]`` in ascending Gaussian
+    id.  Culled pairs are never written, so ``gids`` has exactly ``offsets[-1]`` entries (the
+    reference keeps culled pairs as a sorted-out tail beyond ``offsets[-1]``).  One host
+    synchronization sizes ``gids`` exactly, like the reference's readback (render_ext.cu:910);
+    ``build_tile_index_capacity`` is the synchronization-free variant.
+    """
+    _check_field(means, conics, conics, radii, None, need_colors=False)
+    _check_image(H, W, tile_size)
+    lib = _native.load()
+    dev = means.device
+    means = means.detach().contiguous()
+    conics = conics.detach().contiguous()
+    radii = _radii64(radii)
+    n = means.shape[0]
+    T = num_tiles(H, W, tile_size)
+    offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
+    ws = workspace(n, H, W, tile_size, dev)
+    stream = _stream(dev)
+    args = (_ptr(means), _ptr(conics), _ptr(radii), n, int(H), int(W), int(tile_size),
+            int(bool(ellipse_cull)), float(sigma_cutoff))
+    _native.check(lib.ssplat_tile_index_count(*args, _ptr(offsets), _ptr(ws), ws.numel(), stream),
+                  "ssplat_tile_index_count")
+    total = int(offsets[T].item())
+    if total >= _INT32_SATURATED or total < 0:
+        raise RuntimeError("tile index pair count exceeds int32; reduce N or image size")
+    gids = torch.empty(total, dtype=torch.int32, device=dev)
+    _native.check(lib.ssplat_tile_index_fill(*args, _ptr(offsets), _ptr(gids), total, None,
+                                             _ptr(ws), ws.numel(), stream),
+                  "ssplat_tile_index_fill")
+    return gids, offsets
+
+
+@torch.no_grad()
+def build_tile_index_capacity(means, conics, radii, H: int, W: int, capacity: int,
+                              tile_size: int = DEFAULT_TILE_SIZE, ellipse_cull: bool = False,
+                              sigma_cutoff: float = 3.0, out=None):
+    """Synchronization-free binning into a caller-sized gid buffer (HIP-graph capturable).
+
+    Returns ``(gids (capacity,), offsets (T+1,), status (1,))``; ``status`` is a device int32 that
+    reads 0 when every pair fit and 2 (capacity exceeded) otherwise.  ``out`` may pass
+    preallocated ``(gids, offsets, status, workspace)`` tensors to reuse; the status word is
+    sticky (an overflow stays recorded until the caller clears it).
+    """
+    _check_field(means, conics, conics, radii, None, need_colors=False)
+    _check_image(H, W, tile_size)
+    lib = _native.load()
+    dev = means.device
+    n = means.shape[0]
+    T = num_tiles(H, W, tile_size)
+    if out is None:
+        gids = torch.empty(max(int(capacity), 1), dtype=torch.int32, device=dev)
+        offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        ws = workspace(n, H, W, tile_size, dev)
+    else:
+        gids, offsets, status, ws = out
+    means = means.detach().contiguous()
+    conics = conics.detach().contiguous()
+    radii = _radii64(radii)
+    _native.check(lib.ssplat_build_tile_index(
+        _ptr(means), _ptr(conics), _ptr(radii), n, int(H), int(W), int(tile_size),
+        int(bool(ellipse_cull)), float(sigma_cutoff), _ptr(offsets), _ptr(gids),
+        int(min(capacity, gids.numel())), _ptr(status), _ptr(ws), ws.numel(), _stream(dev)),
+        "ssplat_build_tile_index")
+    return gids, offsets, status
+
+
+def forward_tiled(means, conics, colors, radii, opacities, gids, offsets, H: int, W: int,
+                  normalize: bool, eps: float, tile_size: int, support_fade: bool,
+                  sigma_cutoff: float, out=None, den=None, ws=None):
+    """One tiled forward through the C ABI; returns ``(out (H,W,3), den (H,W) or None)``."""
+    lib = _native.load()
+    dev = means.device
+    n = means.shape[0]
+    if out is None:
+        out = torch.empty(int(H), int(W), 3, dtype=torch.float32, device=dev)
+    if ws is None:
+        ws = workspace(n, H, W, tile_size, dev)
+    _native.check(lib.ssplat_forward_tiled(
+        _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities), n, _ptr(gids),
+        _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size), int(bool(normalize)),
+        float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(out), _ptr(den),
+        _ptr(ws), ws.numel(), _stream(dev)), "ssplat_forward_tiled")
+    return out, den
+
+
+def backward_tiled(grad_out, means, conics, colors, radii, opacities, gids, offsets, H: int,
+                   W: int, normalize: bool, eps: float, tile_size: int, support_fade: bool,
+                   sigma_cutoff: float, grads=None, ws=None):
+    """One tiled backward; returns ``(g_means, g_conics, g_colors, g_opacities or None)``."""
+    lib = _native.load()
+    dev = means.device
+    n = means.shape[0]
+    has_opacity = opacities is not None and opacities.numel() > 0
+    if grads is None:
+        grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
+                 torch.empty_like(opacities) if has_opacity else None)
+    g_means, g_conics, g_colors, g_opac = grads
+    if ws is None:
+        ws = workspace(n, H, W, tile_size, dev)
+    _native.check(lib.ssplat_backward_tiled(
+        _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
+        n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size),
+        int(bool(normalize)), float(eps), int(bool(support_fade)), float(sigma_cutoff),
+        _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac), _ptr(ws), ws.numel(),
+        _stream(dev)), "ssplat_backward_tiled")
+    return g_means, g_conics, g_colors, g_opac
+
+
+class _ExactRenderHip(Function):
+    """Autograd boundary of the HIP renderer (the role of ``_ExactRenderCuda``,
+    cuda_render.py:52-174): gradients for means, conics, colors and opacities only."""
+
+    @staticmethod
+    def forward(ctx, means, conics, colors, radii, opacities, height, width, normalize, eps,
+                tile_size, support_fade, sigma_cutoff, tile_ellipse_cull):
+        means = means.contiguous()
+        conics = conics.contiguous()
+        colors = colors.contiguous()
+        radii = _radii64(radii)
+        opacities = opacities.contiguous()
+        # Culling is exact only under support fade (cuda_render.py:80-85).
+        cull = bool(tile_ellipse_cull) and bool(support_fade)
+        gids, offsets = build_tile_index(means, conics, radii, int(height), int(width),
+                                         int(tile_size), cull, float(sigma_cutoff))
+        ws = workspace(means.shape[0], height, width, tile_size, means.device)
+        out, _ = forward_tiled(means, conics, colors, radii, opacities, gids, offsets,
+                               int(height), int(width), bool(normalize), float(eps),
+                               int(tile_size), bool(support_fade), float(sigma_cutoff), ws=ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, gids, offsets)
+        ctx.height = int(height)
+        ctx.width = int(width)
+        ctx.normalize = bool(normalize)
+        ctx.eps = float(eps)
+        ctx.tile_size = int(tile_size)
+        ctx.support_fade = bool(support_fade)
+        ctx.sigma_cutoff = float(sigma_cutoff)
+        return out
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_out):
+        means, conics, colors, radii, opacities, gids, offsets = ctx.saved_tensors
+        need_means, need_conics, need_colors, _, need_opac = ctx.needs_input_grad[:5]
+        none_tail = (None,) * 8
+        if not (need_means or need_conics or need_colors or need_opac):
+            return (None,) * 5 + none_tail
+        g_means, g_conics, g_colors, g_opac = backward_tiled(
+            grad_out.contiguous(), means, conics, colors, radii, opacities, gids, offsets,
+            ctx.height, ctx.width, ctx.normalize, ctx.eps, ctx.tile_size, ctx.support_fade,
+            ctx.sigma_cutoff)
+        if opacities.numel() == 0 or not need_opac:
+            g_opac = None
+        return (
+            g_means if need_means else None,
+            g_conics if need_conics else None,
+            g_colors if need_colors else None,
+            None,
+            g_opac,
+        ) + none_tail
+
+
+def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=None,
+                     normalize: bool = True, eps: float = 1e-8, tiled: bool = True,
+                     tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool = False,
+                     sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
+                     tile_index_backend: str = "hip", tile_ellipse_cull: bool = True):
+    """Exact normalized/additive render on the GPU (signature of ``render_cuda_exact``,
+    cuda_render.py:256-294).
+
+    ``tiled`` and ``backward_variant`` are accepted for drop-in compatibility: the HIP path is
+    always tiled and has one backward.  ``tile_index_backend`` accepts ``"hip"`` (or the
+    reference's ``"cuda"``); the reference's torch-op builder exists here only as the oracle.
+    """
+    _check_field(means, conics, colors, radii, opacities)
+    if backward_variant not in ("baseline", "block_reduce"):
+        raise ValueError(
+            f"backward_variant must be baseline or block_reduce, got {backward_variant!r}"
+        )
+    if tile_index_backend not in ("hip", "cuda"):
+        raise ValueError(f"tile_index_backend must be hip or cuda, got {tile_index_backend!r}")
+    _check_image(H, W, tile_size)
+    if normalize:
+        eps = _validate_normalization_eps(eps)
+    if opacities is None:
+        opacities = means.new_empty(0)
+    return _ExactRenderHip.apply(means, conics, colors, radii, opacities, int(H), int(W),
+                                 bool(normalize), float(eps), int(tile_size), bool(support_fade),
+                                 float(sigma_cutoff), bool(tile_ellipse_cull))
+
+
+@torch.no_grad()
+def render_hip_exact_with_coverage(means, conics, colors, radii, H: int, W: int,
+                                   opacities=None, eps: float = 1e-8,
+                                   support_fade: bool = False, sigma_cutoff: float = 3.0):
+    """Normalized RGB and raw coverage ``den = sum o G`` (no eps) from one forward, no autograd
+    (``render_cuda_exact_with_coverage``, cuda_render.py:229-253)."""
+    if not means.is_cuda:
+        raise RuntimeError("joint HIP rendering requires CUDA tensors")
+    if any(t.dtype != torch.float32 for t in (means, conics, colors)):
+        raise RuntimeError("joint HIP rendering requires float32 tensors")
+    eps = _validate_normalization_eps(eps)
+    _check_field(means, conics, colors, radii, opacities)
+    if opacities is None:
+        opacities = means.new_empty(0)
+    means = means.contiguous()
+    conics = conics.contiguous()
+    colors = colors.contiguous()
+    radii = _radii64(radii)
+    opacities = opacities.contiguous()
+    gids, offsets = build_tile_index(means, conics, radii, int(H), int(W), DEFAULT_TILE_SIZE,
+                                     False, float(sigma_cutoff))
+    den = torch.empty(int(H), int(W), dtype=torch.float32, device=means.device)
+    out, den = forward_tiled(means, conics, colors, radii, opacities, gids, offsets, int(H),
+                             int(W), True, eps, DEFAULT_TILE_SIZE, bool(support_fade),
+                             float(sigma_cutoff), den=den)
+    return out, den
+
+
+def render_field(means, conics, colors, radii, H: int, W: int,
+                 chunk: int = 4096, mode: str = "normalized", opacities=None,
+                 scales=None, rotations=None, support_fade: bool = False,
+                 sigma_cutoff: float = 3.0, color_grads=None,
+                 support_fade_alpha: float | None = None,
+                 checkpoint_chunks: bool = False,
+                 normalization_eps: float = _EPS):
+    """Mode dispatcher with the reference's signature (render.py:319-414).
+
+    ``chunk`` and ``checkpoint_chunks`` bound the reference renderer's memory and have no effect
+    here.  Affine colors (``color_grads``) and fractional fade strengths are not implemented by
+    the HIP kernels and raise ``ValueError`` instead of silently changing the renderer.
+    """
+    del chunk, checkpoint_chunks
+    fade_alpha = _resolve_support_fade_alpha(support_fade, support_fade_alpha)
+    if mode in ("gsplat", "cuda_gsplat"):
+        if scales is None or rotations is None:
+            raise ValueError("renderer='gsplat' requires scales and rotations")
+        raise ValueError(
+            "renderer='gsplat' (the GaussianImage++ alpha/sum comparator) is not part of the "
+            "HIP renderer"
+        )
+    if mode not in _NORMALIZED_MODES and mode not in _ADDITIVE_MODES:
+        raise ValueError(
+            f"unknown renderer {mode!r}; expected normalized, additive, hip, hip_additive, cuda, "
+            "cuda_block_reduce, cuda_additive, cuda_tiled, or cuda_tiled_additive"
+        )
+    if not means.is_cuda:
+        raise RuntimeError(_DEVICE_MESSAGE)
+    if color_grads is not None:
+        raise ValueError(
+            "affine color rendering (color_grads) is not implemented by the HIP renderer; "
+            f"got renderer={mode!r}"
+        )
+    if fade_alpha not in (0.0, 1.0):
+        raise ValueError(
+            f"fractional support_fade_alpha={fade_alpha} is not implemented by the HIP renderer"
+        )
+    if mode in _NORMALIZED_MODES:
+        eps = _validate_normalization_eps(normalization_eps)
+        return render_hip_exact(means, conics, colors, radii, H, W, opacities=opacities,
+                                normalize=True, eps=eps, support_fade=fade_alpha > 0.0,
+                                sigma_cutoff=sigma_cutoff)
+    return render_hip_exact(means, conics, colors, radii, H, W, opacities=opacities,
+                            normalize=False, eps=_EPS, support_fade=fade_alpha > 0.0,
+                            sigma_cutoff=sigma_cutoff)

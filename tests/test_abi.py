@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every symbol include/ssplat.h declares (no GPU calls)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from structsplat_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ssplat.h")
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(ssplat_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_bound_symbols():
+    assert set(_declared_symbols()) == set(_native.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_workspace_query():
+    lib = _native.load()
+    assert lib.ssplat_abi_version() == 1
+    nbytes = _native.workspace_bytes(1000, 256, 256, 16)
+    # 1000 records of 48 B + (256 tiles + 1) counters + scan sums + status, 256-byte aligned.
+    assert nbytes >= 1000 * 48 + 257 * 4
+    assert nbytes % 256 == 0
+
+
+@pytest.mark.parametrize("args,phrase", [
+    ((10, 0, 5, 16), "height and width must be positive"),
+    ((10, 5, 5, 0), "tile_size must be positive"),
+    ((10, 5, 5, 33), "tile_size must be positive"),
+    ((-1, 5, 5, 16), "n must be"),
+])
+def test_invalid_arguments_report_errors(args, phrase):
+    with pytest.raises(ValueError, match=phrase):
+        _native.workspace_bytes(*args)
+
+
+def test_null_output_pointer_is_rejected():
+    lib = _native.load()
+    rc = lib.ssplat_workspace_bytes(10, 5, 5, 16, None)
+    assert rc == _native.ERR_INVALID_ARGUMENT
+    assert "NULL" in _native.last_error()
+    assert isinstance(ctypes.c_size_t(0).value, int)

@@ -1,0 +1,121 @@
+"""GPU parity of the HIP path against the CPU oracle (tile lists bit-exact, forward vs float64,
+backward vs the float64 analytic gradients)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact64, tile_index
+from structsplat_amd import render as HR
+from tests.helpers import field_tensors, random_field, rel_err, scaled_err, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(**kw):
+    print("case:", ", ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.mark.parametrize("ts", [4, 8, 16, 32])
+@pytest.mark.parametrize("cull", [False, True])
+def test_tile_index_bit_exact(cuda, ts, cull):
+    _print_case(test="tile_index", ts=ts, cull=cull)
+    N, H, W = 300, 67, 45
+    f = random_field(21, N, H, W, scale_lo=0.6, scale_hi=9.0, margin=6.0,
+                     aniso=(1.0, 10.0, 0.4, 2.0))
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, ts, cull, 3.0)
+    torch.cuda.synchronize()
+    ref_g, ref_o = tile_index.build_tile_index(to_np(means), to_np(conics), to_np(radii), H, W,
+                                               ts, cull, 3.0)
+    assert offsets.dtype == torch.int32 and gids.dtype == torch.int32
+    assert to_np(offsets).tolist() == ref_o.tolist()
+    assert to_np(gids).tolist() == ref_g.tolist()
+
+
+def test_tile_index_reference_golden(cuda):
+    # test_render.py:167-172
+    means = torch.tensor([[2.0, 2.0], [20.0, 20.0]], device=cuda)
+    radii = torch.tensor([[1, 1], [1, 1]], dtype=torch.long, device=cuda)
+    conics = torch.ones(2, 3, device=cuda)
+    gids, offsets = HR.build_tile_index(means, conics, radii, 8, 8, 4)
+    assert offsets.tolist() == [0, 1, 1, 1, 1]
+    assert gids.tolist() == [0]
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("opacity", [False, True])
+def test_forward_matches_fp64(cuda, normalize, fade, opacity):
+    _print_case(test="forward", normalize=normalize, fade=fade, opacity=opacity)
+    N, H, W = 400, 70, 93
+    f = random_field(7, N, H, W, scale_lo=0.7, scale_hi=6.0, opacity=opacity)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    out = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op,
+                              normalize=normalize, support_fade=fade)
+    torch.cuda.synchronize()
+    ref, _ = exact64.forward(to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op),
+                             H, W, normalize, 1e-8, fade, 3.0)
+    got = to_np(out)
+    assert np.isfinite(got).all()
+    err = scaled_err(got, ref)
+    print("scaled err", err, "elementwise rel err (floor 1e-3)", rel_err(got, ref, 1e-3))
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True])
+def test_backward_matches_fp64(cuda, normalize, fade):
+    _print_case(test="backward", normalize=normalize, fade=fade)
+    N, H, W = 250, 61, 77
+    f = random_field(12, N, H, W, scale_lo=0.8, scale_hi=5.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    for t in (means, conics, colors, op):
+        t.requires_grad_(True)
+    out = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op,
+                              normalize=normalize, support_fade=fade)
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(3))
+    (out * g).sum().backward()
+    torch.cuda.synchronize()
+    ref = exact64.backward(to_np(g), to_np(means), to_np(conics), to_np(colors), to_np(radii),
+                           to_np(op), H, W, normalize, 1e-8, fade, 3.0)
+    for name, got, exp in zip(("means", "conics", "colors", "opacities"),
+                              (means.grad, conics.grad, colors.grad, op.grad), ref):
+        err = scaled_err(to_np(got), exp)
+        print(name, "scaled err", err)
+        assert err <= 1e-4, name
+
+
+def test_empty_field_renders_zero(cuda):
+    H, W = 8, 9
+    z2 = torch.zeros(0, 2, device=cuda)
+    z3 = torch.zeros(0, 3, device=cuda)
+    r = torch.zeros(0, 2, dtype=torch.long, device=cuda)
+    out = HR.render_field(z2, z3, z3, r, H, W, mode="hip")
+    torch.cuda.synchronize()
+    assert out.shape == (H, W, 3)
+    assert torch.count_nonzero(out) == 0
+
+
+def test_nonfinite_means_are_dropped(cuda):
+    H, W = 12, 12
+    f = {"means": np.array([[5.0, 5.0], [np.nan, 3.0], [1e12, 1e12], [np.inf, 2.0]], np.float32),
+         "scales": np.full((4, 2), 1.5, np.float32), "angles": np.zeros(4, np.float32),
+         "colors": np.ones((4, 3), np.float32), "opacity": None}
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    out = HR.render_field(means, conics, colors, radii, H, W, mode="hip")
+    ref, _ = exact64.forward(to_np(means), to_np(conics), to_np(colors), to_np(radii), None, H, W)
+    torch.cuda.synchronize()
+    assert torch.isfinite(out).all()
+    assert scaled_err(to_np(out), ref) <= 1e-5
+
+
+def test_forward_is_deterministic(cuda):
+    N, H, W = 2000, 128, 160
+    f = random_field(5, N, H, W, scale_lo=0.7, scale_hi=8.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    a = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op)
+    b = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)

@@ -1,0 +1,247 @@
+"""Pins the CPU oracle's forward (oracle.reference_render, oracle.exact64) against the reference's
+own closed-form and behavioural tests (test_render.py, test_normalization_epsilon.py,
+test_gaussians.py) and an independent dense O(N H W) renderer."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact64
+from oracle import reference_render as R
+from structsplat_amd.gaussians import GaussianField
+from tests.helpers import field_tensors, random_field, to_np
+
+
+def _render(g, H, W, **kw):
+    return R.render(g.means, g.conics(), g.colors, g.radii(kw.pop("cutoff", 3.0)), H, W, **kw)
+
+
+def test_element_budget_preserves_chunk_floor_and_unit():
+    # test_render.py:11-14
+    assert R.element_budget(1) == 64 * 4096
+    assert R.element_budget(64) == 64 * 4096
+    assert R.element_budget(512) == 512 * 4096
+
+
+def test_two_gaussian_blend_and_grad():
+    # test_render.py:17-27
+    g = GaussianField.from_numpy(np.array([[3.0, 3.0], [6.0, 3.0]]), np.full((2, 2), 1.5),
+                                 np.zeros(2), np.array([[1.0, 0, 0], [0, 0, 1.0]])).trainable()
+    img = _render(g, 10, 10)
+    left, right = img[3, 3], img[3, 6]
+    assert left[0] > left[2] and right[2] > right[0]
+    (img - 0.5).abs().mean().backward()
+    assert g.means.grad is not None and g.colors.grad is not None
+
+
+def test_gaussian_activity_marks_visible_support():
+    # test_render.py:30-39
+    g = GaussianField.from_numpy(np.array([[3.0, 3.0], [50.0, 50.0]]), np.full((2, 2), 1.5),
+                                 np.zeros(2), np.ones((2, 3)))
+    activity = R.gaussian_activity(g.means, g.conics(), g.radii(3.0), 10, 10, chunk=1)
+    assert activity.shape == (2,)
+    assert activity[0] > 0 and activity[1] == 0
+
+
+def test_additive_renderer_runs_and_differs():
+    # test_render.py:42-50
+    g = GaussianField.from_numpy(np.array([[3.0, 3.0], [3.5, 3.0]]), np.full((2, 2), 1.5),
+                                 np.zeros(2), np.array([[0.5, 0, 0], [0.5, 0, 0]]))
+    norm = R.render(g.means, g.conics(), g.colors, g.radii(3.0), 10, 10)
+    add = R.render_additive(g.means, g.conics(), g.colors, g.radii(3.0), 10, 10)
+    assert torch.isfinite(add).all()
+    assert add[3, 3, 0] > norm[3, 3, 0]
+
+
+def test_support_fade_zeroes_tail_at_cutoff():
+    # test_render.py:108-120: exactly 0 at 3 sigma, strictly between 0 and the hard weight inside.
+    g = GaussianField.from_numpy(np.array([[5.0, 5.0]]), np.full((1, 2), 1.0), np.zeros(1),
+                                 np.array([[1.0, 0.0, 0.0]]))
+    args = (g.means, g.conics(), g.colors, g.radii(3.0), 11, 11)
+    hard = R.render_additive(*args)
+    faded = R.render_additive(*args, support_fade=True, sigma_cutoff=3.0)
+    assert hard[5, 8, 0] > 0
+    assert faded[5, 8, 0] == 0
+    assert 0 < faded[5, 7, 0] < hard[5, 7, 0]
+
+
+def test_support_fade_reduces_activity():
+    # test_render.py:123-133
+    g = GaussianField.from_numpy(np.array([[5.0, 5.0]]), np.full((1, 2), 1.0), np.zeros(1),
+                                 np.ones((1, 3)))
+    hard = R.gaussian_activity(g.means, g.conics(), g.radii(3.0), 11, 11)
+    faded = R.gaussian_activity(g.means, g.conics(), g.radii(3.0), 11, 11, support_fade=True)
+    assert 0 < faded[0] < hard[0]
+
+
+def _naive_render(g, H, W, opacities=None):
+    """Dense O(N*H*W) renderer with no support window (test_render.py:535-551)."""
+    ys, xs = torch.meshgrid(torch.arange(H, dtype=torch.float64),
+                            torch.arange(W, dtype=torch.float64), indexing="ij")
+    conics = g.conics().double()
+    means = g.means.double()
+    colors = g.colors.double()
+    num = torch.zeros(H, W, 3, dtype=torch.float64)
+    den = torch.zeros(H, W, 1, dtype=torch.float64)
+    for i in range(g.n):
+        dx = xs - means[i, 0]
+        dy = ys - means[i, 1]
+        a, b, c = conics[i]
+        w = torch.exp(-0.5 * (a * dx * dx + 2 * b * dx * dy + c * dy * dy))
+        if opacities is not None:
+            w = w * opacities[i]
+        num += w[..., None] * colors[i]
+        den += w[..., None]
+    return num / (den + 1e-8)
+
+
+def test_render_matches_naive_reference():
+    # test_render.py:554-564: sigma_cutoff 6 leaves ~exp(-18) tails outside the support.
+    rng = np.random.default_rng(0)
+    N, H, W = 40, 24, 32
+    means = np.stack([rng.uniform(-4, W + 4, N), rng.uniform(-4, H + 4, N)], 1)
+    scales = np.exp(rng.uniform(np.log(0.6), np.log(6.0), (N, 2)))
+    g = GaussianField.from_numpy(means, scales, rng.uniform(0, np.pi, N), rng.random((N, 3)))
+    fast = R.render(g.means, g.conics(), g.colors, g.radii(6.0), H, W, chunk=1)
+    naive = _naive_render(g, H, W)
+    assert torch.allclose(fast.double(), naive, atol=2e-3)
+    f64, _ = exact64.forward(to_np(g.means), to_np(g.conics()), to_np(g.colors),
+                             to_np(g.radii(6.0)), None, H, W)
+    assert np.allclose(f64, naive.numpy(), atol=2e-3)
+
+
+def test_offimage_gaussian_keeps_inimage_support():
+    # test_render.py:567-580
+    g = GaussianField.from_numpy(np.array([[25.0, 5.0]]), np.array([[10.0, 10.0]]), np.zeros(1),
+                                 np.array([[1.0, 0.0, 0.0]])).trainable()
+    activity = R.gaussian_activity(g.means, g.conics(), g.radii(3.0), 10, 10)
+    assert activity[0] > 0
+    img = R.render(g.means, g.conics(), g.colors, g.radii(3.0), 10, 10)
+    assert img[5, 9, 0] > 0.5
+    img.sum().backward()
+    assert g.means.grad.abs().sum() > 0
+
+
+def test_opacity_changes_normalized_blend():
+    # test_render.py:583-593
+    g = GaussianField.from_numpy(np.array([[3.0, 3.0], [3.0, 3.0]]), np.full((2, 2), 1.5),
+                                 np.zeros(2), np.array([[1.0, 0, 0], [0, 0, 1.0]]))
+    equal = R.render(g.means, g.conics(), g.colors, g.radii(3.0), 10, 10)
+    biased = R.render(g.means, g.conics(), g.colors, g.radii(3.0), 10, 10,
+                      opacities=torch.tensor([0.9, 0.1]))
+    assert equal[3, 3, 0] < biased[3, 3, 0]
+
+
+def _weak_support_field():
+    return GaussianField.from_numpy(np.array([[0.0, 0.0]], np.float32),
+                                    np.array([[1.0, 1.0]], np.float32), np.zeros(1, np.float32),
+                                    np.array([[0.8, 0.4, 0.2]], np.float32))
+
+
+def test_weak_support_matches_analytic_epsilon_attenuation():
+    # test_normalization_epsilon.py:39-53: one Gaussian 7 sigma away renders c w / (w + eps).
+    f = _weak_support_field()
+    common = (f.means, f.conics(), f.colors, f.radii(8.0), 1, 8)
+    implicit = R.render(*common, sigma_cutoff=8.0)
+    explicit = R.render(*common, sigma_cutoff=8.0, normalization_eps=1e-8)
+    repaired = R.render(*common, sigma_cutoff=8.0, normalization_eps=1e-12)
+    weight = math.exp(-0.5 * 7.0 ** 2)
+    assert torch.equal(implicit, explicit)
+    assert torch.allclose(explicit[0, 7], f.colors[0] * weight / (weight + 1e-8), atol=1e-7,
+                          rtol=1e-6)
+    assert torch.allclose(repaired[0, 7], f.colors[0] * weight / (weight + 1e-12), atol=1e-6,
+                          rtol=1e-6)
+    assert repaired[0, 7, 0] > explicit[0, 7, 0] + 0.7
+    out, _ = exact64.forward(to_np(f.means), to_np(f.conics()), to_np(f.colors),
+                             to_np(f.radii(8.0)), None, 1, 8, True, 1e-12, False, 8.0)
+    assert np.allclose(out[0, 7], to_np(f.colors[0]).astype(np.float64) * weight
+                       / (weight + 1e-12), rtol=1e-12)
+
+
+def test_zero_denominator_stays_finite_black_and_invalid_epsilon_fails():
+    # test_normalization_epsilon.py:56-67
+    z2, z3 = torch.zeros(0, 2), torch.zeros(0, 3)
+    radii = torch.zeros(0, 2, dtype=torch.long)
+    image = R.render(z2, z3, z3, radii, 3, 4, normalization_eps=1e-12)
+    assert torch.isfinite(image).all() and torch.count_nonzero(image) == 0
+    for bad in (0.0, -1e-12, float("inf"), float("nan")):
+        with pytest.raises(ValueError, match="normalization_eps must be finite and > 0"):
+            R.render(z2, z3, z3, radii, 3, 4, normalization_eps=bad)
+
+
+def test_additive_render_is_independent_of_normalization_eps():
+    # test_normalization_epsilon.py:70-84 (the additive path takes no eps at all).
+    f = _weak_support_field()
+    out1, _ = exact64.forward(to_np(f.means), to_np(f.conics()), to_np(f.colors),
+                              to_np(f.radii(8.0)), None, 1, 8, False, 1e-8, False, 8.0)
+    out2, _ = exact64.forward(to_np(f.means), to_np(f.conics()), to_np(f.colors),
+                              to_np(f.radii(8.0)), None, 1, 8, False, 1e-12, False, 8.0)
+    assert np.array_equal(out1, out2)
+
+
+def test_slice_budget_does_not_change_the_image():
+    f = random_field(3, 300, 40, 50, scale_lo=0.7, scale_hi=9.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, "cpu")
+    small = R.render(means, conics, colors, radii, 40, 50, chunk=1, opacities=op)
+    x0, y0, Tx, n = R.tile_bounds(means, radii, 40, 50)
+    assert len(list(R.flat_tile_slices(n, 4096))) > 1  # several slices really happen
+    big = R.accumulate(means, conics, colors, radii, 40, 50, 10**6, op, True)
+    assert torch.allclose(small, big, atol=1e-6, rtol=1e-6)
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True])
+def test_exact64_forward_equals_torch_restatement(normalize, fade):
+    """Two independent float64 implementations (index_add over slices vs bincount over chunks)."""
+    f = random_field(9, 500, 45, 61, scale_lo=0.6, scale_hi=7.0, margin=5.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, "cpu")
+    ref = R.accumulate(means.double(), conics.double(), colors.double(), radii, 45, 61, 1,
+                       op.double(), normalize, fade, 3.0)
+    out, _ = exact64.forward(to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op),
+                             45, 61, normalize, 1e-8, fade, 3.0)
+    assert np.allclose(out, ref.numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_nonfinite_and_huge_means_are_dropped():
+    # render_ext.cu:25-34 semantics; test_render.py:517-532 (no out-of-range access).
+    means = torch.tensor([[5.0, 5.0], [float("nan"), 3.0], [1e12, 1e12], [float("inf"), 1.0]])
+    radii = torch.full((4, 2), 5, dtype=torch.long)
+    x0, y0, Tx, n = R.tile_bounds(means, radii, 12, 12)
+    assert n.tolist()[1:] == [0, 0, 0] and n[0] > 0
+
+
+def _explicit_inv_cov(scale, theta):
+    c, s = np.cos(theta), np.sin(theta)
+    rot = np.array([[c, -s], [s, c]])
+    return np.linalg.inv(rot @ np.diag(scale ** 2) @ rot.T)
+
+
+def test_conics_match_explicit_inverse():
+    # test_gaussians.py:9-23
+    scales = np.array([[3.0, 1.0], [2.0, 5.0], [4.0, 4.0]])
+    angles = np.array([0.7, -1.2, 0.3])
+    g = GaussianField.from_numpy(np.zeros((3, 2)), scales, angles, np.zeros((3, 3)))
+    conics = g.conics().numpy()
+    for i in range(3):
+        got = np.array([[conics[i, 0], conics[i, 1]], [conics[i, 1], conics[i, 2]]])
+        assert np.allclose(got, _explicit_inv_cov(scales[i], angles[i]), atol=1e-5)
+    oracle_conics = R.conics_from_rs(g.log_scales, g.rotations).numpy()
+    assert np.array_equal(oracle_conics, conics)
+
+
+def test_radii_positive_and_bound_the_cutoff_ellipse():
+    # test_gaussians.py:26-33 plus the AABB property of gaussians.py:302-321.
+    rng = np.random.default_rng(4)
+    scales = np.exp(rng.uniform(np.log(0.3), np.log(20.0), (200, 2)))
+    angles = rng.uniform(0, np.pi, 200)
+    g = GaussianField.from_numpy(np.zeros((200, 2)), scales, angles, np.zeros((200, 3)))
+    radii = g.radii(3.0).numpy()
+    assert (radii >= 1).all()
+    assert np.array_equal(radii, R.radii_from_rs(g.log_scales, g.rotations, 3.0).numpy())
+    for i in range(200):
+        cov = np.linalg.inv(_explicit_inv_cov(scales[i], angles[i]))
+        assert radii[i, 0] >= 3.0 * np.sqrt(cov[0, 0]) - 1e-3
+        assert radii[i, 1] >= 3.0 * np.sqrt(cov[1, 1]) - 1e-3

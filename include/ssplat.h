@@ -132,6 +132,65 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           float* grad_conics, float* grad_colors, float* grad_opacities,
                           void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Batches: many independent fields (images of their own size, each with its own Gaussians)
+ * rendered by one launch per kernel.  The fields' Gaussians are concatenated in field order
+ * (means, conics, colors, radii, opacities: rows first_gaussian .. first_gaussian+num_gaussians),
+ * their images are concatenated in field order (out/grad_out: pixels first_pixel ..), and their
+ * tile grids likewise (tile_offsets has total_tiles + 1 entries).
+ * No reference counterpart: the reference fits many images with one process per image
+ * (batch.py:163-245); this replaces the process farm with one ragged launch.
+ */
+typedef struct ssplat_field {
+  int32_t height;         /* set by the caller */
+  int32_t width;          /* set by the caller */
+  int64_t first_gaussian; /* filled by ssplat_batch_plan */
+  int64_t num_gaussians;  /* set by the caller */
+  int64_t first_pixel;    /* filled by ssplat_batch_plan */
+  int64_t first_tile;     /* filled by ssplat_batch_plan */
+  int32_t tiles_x;        /* filled by ssplat_batch_plan */
+  int32_t num_tiles;      /* filled by ssplat_batch_plan */
+} ssplat_field;
+
+/*
+ * Host-only: completes a HOST table of fields (offsets of every field in the concatenated
+ * Gaussians, pixels and tiles) and returns the totals and the workspace size of the batch.  The
+ * caller copies the completed table to device memory for the batch calls below.
+ */
+int ssplat_batch_plan(ssplat_field* fields, int num_fields, int tile_size,
+                      int64_t* total_gaussians, int64_t* total_tiles, int64_t* total_pixels,
+                      size_t* workspace_bytes);
+
+/* Batched ssplat_build_tile_index; `fields` is the DEVICE copy of the planned table. */
+int ssplat_batch_build_tile_index(const ssplat_field* fields, int num_fields,
+                                  int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                                  const float* means, const float* conics, const int64_t* radii,
+                                  int ellipse_cull, double sigma_cutoff, int32_t* tile_offsets,
+                                  int32_t* tile_gids, int64_t gid_capacity, int32_t* status,
+                                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batched ssplat_forward_tiled; out (and den) hold every field's image, concatenated. */
+int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
+                               int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                               const float* means, const float* conics, const float* colors,
+                               const int64_t* radii, const float* opacities,
+                               const int32_t* tile_gids, const int32_t* tile_offsets,
+                               int64_t num_gids, int normalize, double eps, int support_fade,
+                               double sigma_cutoff, float* out, float* den, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* Batched ssplat_backward_tiled. */
+int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
+                                int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                                const float* grad_out, const float* means, const float* conics,
+                                const float* colors, const int64_t* radii,
+                                const float* opacities, const int32_t* tile_gids,
+                                const int32_t* tile_offsets, int64_t num_gids, int normalize,
+                                double eps, int support_fade, double sigma_cutoff,
+                                float* grad_means, float* grad_conics, float* grad_colors,
+                                float* grad_opacities, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

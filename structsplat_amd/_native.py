@@ -48,7 +48,37 @@ SIGNATURES = {
                                        _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
                                        _c_dbl, _c_int, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                        _c_ptr, _c_size, _c_ptr]),
+    "ssplat_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(_c_size)]),
+    "ssplat_batch_build_tile_index": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
+                                               _c_ptr, _c_ptr, _c_int, _c_dbl, _c_ptr, _c_ptr,
+                                               _c_i64, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_batch_forward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
+                                            _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                            _c_i64, _c_int, _c_dbl, _c_int, _c_dbl, _c_ptr,
+                                            _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_batch_backward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
+                                             _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                             _c_ptr, _c_i64, _c_int, _c_dbl, _c_int, _c_dbl,
+                                             _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
+                                             _c_ptr]),
 }
+
+
+class Field(ctypes.Structure):
+    """ssplat_field of include/ssplat.h (one image of a batch)."""
+
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("first_gaussian", ctypes.c_int64),
+        ("num_gaussians", ctypes.c_int64),
+        ("first_pixel", ctypes.c_int64),
+        ("first_tile", ctypes.c_int64),
+        ("tiles_x", ctypes.c_int32),
+        ("num_tiles", ctypes.c_int32),
+    ]
 
 _LIB = None
 _LOCK = threading.Lock()

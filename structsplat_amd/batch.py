@@ -1,0 +1,348 @@
+"""Ragged batches: many independent fields fitted by one launch per kernel.
+
+The reference fits many images with a process farm, one ``fit`` per image (batch.py:83-245).  Here
+all fields share one set of kernel launches: their Gaussians are concatenated (rows in field
+order), their images are concatenated (pixels in field order), and a device field table tells every
+kernel which image a tile or a Gaussian belongs to (``ssplat_batch_*`` in include/ssplat.h).
+
+* ``Batch`` - planned field table, workspace and launchers (binning, forward, backward).
+* ``fit_batch(fields, targets, cfg, graph=True)`` - one Adam fit over all fields at once, the
+  whole step captured as a HIP graph; per-field losses are summed, so every field gets exactly
+  the gradient of its own loss and Adam (elementwise) treats every field's rows as its own.
+* ``fit_each(fields, targets, cfg)`` - the plain per-image loop over the same fields (one field per
+  launch, no graph), the baseline the batched fit is compared with.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from . import _native
+from . import fit as F
+from .gaussians import GaussianField
+from .metrics import C1, C2, gaussian_window, psnr_tensor
+from .render import _ptr, _stream
+
+
+class Batch:
+    """Planned layout of a batch of fields of sizes `sizes` [(H, W)] with `counts` Gaussians."""
+
+    def __init__(self, sizes, counts, device, tile_size: int = 16):
+        lib = _native.load()
+        nf = len(sizes)
+        if nf == 0 or len(counts) != nf:
+            raise ValueError("a batch needs one Gaussian count per field size")
+        table = (_native.Field * nf)()
+        for i, ((h, w), n) in enumerate(zip(sizes, counts)):
+            table[i].height = int(h)
+            table[i].width = int(w)
+            table[i].num_gaussians = int(n)
+        g, t, px = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        wsb = ctypes.c_size_t(0)
+        _native.check(lib.ssplat_batch_plan(ctypes.addressof(table), nf, int(tile_size),
+                                            ctypes.byref(g), ctypes.byref(t), ctypes.byref(px),
+                                            ctypes.byref(wsb)), "ssplat_batch_plan")
+        self.device = torch.device(device)
+        self.tile_size = int(tile_size)
+        self.num_fields = nf
+        self.table = table
+        self.total_gaussians = int(g.value)
+        self.total_tiles = int(t.value)
+        self.total_pixels = int(px.value)
+        raw = bytes(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table)))
+        self.fields_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+        self.workspace = torch.empty(max(int(wsb.value), 256), dtype=torch.uint8,
+                                     device=self.device)
+        self.sizes = [(int(h), int(w)) for h, w in sizes]
+        self.gauss_slices = [slice(table[i].first_gaussian,
+                                   table[i].first_gaussian + table[i].num_gaussians)
+                             for i in range(nf)]
+        self.pixel_slices = [slice(table[i].first_pixel,
+                                   table[i].first_pixel + table[i].height * table[i].width)
+                             for i in range(nf)]
+
+    def _head(self):
+        return (_ptr(self.fields_dev), self.num_fields, self.total_gaussians, self.total_tiles,
+                self.tile_size)
+
+    def images(self, flat: torch.Tensor):
+        """Per-field (H, W, 3) views of a concatenated (total_pixels, 3) image buffer."""
+        return [flat[s].view(h, w, 3) for s, (h, w) in zip(self.pixel_slices, self.sizes)]
+
+    def build_tile_index(self, means, conics, radii, capacity: int, cull: bool = False,
+                         sigma_cutoff: float = 3.0, out=None):
+        lib = _native.load()
+        if out is None:
+            gids = torch.empty(max(int(capacity), 1), dtype=torch.int32, device=self.device)
+            offsets = torch.empty(self.total_tiles + 1, dtype=torch.int32, device=self.device)
+            status = torch.zeros(1, dtype=torch.int32, device=self.device)
+        else:
+            gids, offsets, status = out
+        _native.check(lib.ssplat_batch_build_tile_index(
+            *self._head(), _ptr(means), _ptr(conics), _ptr(radii), int(bool(cull)),
+            float(sigma_cutoff), _ptr(offsets), _ptr(gids), int(min(capacity, gids.numel())),
+            _ptr(status), _ptr(self.workspace), self.workspace.numel(), _stream(self.device)),
+            "ssplat_batch_build_tile_index")
+        return gids, offsets, status
+
+    def pair_count(self, means, conics, radii, cull: bool = False, sigma_cutoff: float = 3.0):
+        """Exact pair count (binning into a large buffer, one host readback)."""
+        gids, offsets, status = self.build_tile_index(means, conics, radii, 1, cull,
+                                                      sigma_cutoff)
+        return int(offsets[-1].item())
+
+    def forward(self, means, conics, colors, radii, opacities, gids, offsets, normalize=True,
+                eps=1e-8, support_fade=False, sigma_cutoff=3.0, out=None, den=None):
+        lib = _native.load()
+        if out is None:
+            out = torch.empty(self.total_pixels, 3, dtype=torch.float32, device=self.device)
+        _native.check(lib.ssplat_batch_forward_tiled(
+            *self._head(), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
+            _ptr(opacities), _ptr(gids), _ptr(offsets), gids.numel(), int(bool(normalize)),
+            float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(out), _ptr(den),
+            _ptr(self.workspace), self.workspace.numel(), _stream(self.device)),
+            "ssplat_batch_forward_tiled")
+        return out
+
+    def backward(self, grad_out, means, conics, colors, radii, opacities, gids, offsets,
+                 normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0):
+        lib = _native.load()
+        has_op = opacities is not None and opacities.numel() > 0
+        g_m = torch.empty_like(means)
+        g_q = torch.empty_like(conics)
+        g_c = torch.empty_like(colors)
+        g_o = torch.empty_like(opacities) if has_op else None
+        _native.check(lib.ssplat_batch_backward_tiled(
+            *self._head(), _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
+            _ptr(opacities), _ptr(gids), _ptr(offsets), gids.numel(), int(bool(normalize)),
+            float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(g_m), _ptr(g_q),
+            _ptr(g_c), _ptr(g_o), _ptr(self.workspace), self.workspace.numel(),
+            _stream(self.device)), "ssplat_batch_backward_tiled")
+        return g_m, g_q, g_c, g_o
+
+
+class BatchPlan:
+    """Pair buffer, offsets and sticky status of a batch (capturable binning)."""
+
+    def __init__(self, batch: Batch, capacity: int):
+        self.batch = batch
+        self.capacity = max(int(capacity), 1)
+        dev = batch.device
+        self.gids = torch.empty(self.capacity, dtype=torch.int32, device=dev)
+        self.offsets = torch.empty(batch.total_tiles + 1, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+
+
+class _BatchRender(Function):
+    @staticmethod
+    def forward(ctx, means, conics, colors, radii, opacities, plan, normalize, eps, fade,
+                sigma_cutoff):
+        b = plan.batch
+        means, conics, colors = means.contiguous(), conics.contiguous(), colors.contiguous()
+        radii, opacities = radii.contiguous(), opacities.contiguous()
+        b.build_tile_index(means, conics, radii, plan.capacity, fade, sigma_cutoff,
+                           out=(plan.gids, plan.offsets, plan.status))
+        out = b.forward(means, conics, colors, radii, opacities, plan.gids, plan.offsets,
+                        normalize, eps, fade, sigma_cutoff)
+        ctx.save_for_backward(means, conics, colors, radii, opacities)
+        ctx.plan = plan
+        ctx.args = (normalize, eps, fade, sigma_cutoff)
+        return out
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_out):
+        means, conics, colors, radii, opacities = ctx.saved_tensors
+        plan = ctx.plan
+        g_m, g_q, g_c, g_o = plan.batch.backward(grad_out.contiguous(), means, conics, colors,
+                                                 radii, opacities, plan.gids, plan.offsets,
+                                                 *ctx.args)
+        if opacities.numel() == 0:
+            g_o = None
+        return g_m, g_q, g_c, None, g_o, None, None, None, None, None
+
+
+def concat_fields(fields: list[GaussianField]) -> GaussianField:
+    """One field holding every field's rows in order (opacity logits required on all or none)."""
+    has_op = [f.opacities is not None for f in fields]
+    if any(has_op) and not all(has_op):
+        raise ValueError("either every field has opacities or none has")
+
+    def cat(name):
+        return torch.cat([getattr(f, name).detach() for f in fields], 0).clone()
+
+    return GaussianField(cat("means"), cat("log_scales"), cat("rotations"), cat("colors"),
+                         cat("opacities") if all(has_op) else None)
+
+
+class _SSIMGroup:
+    """Fields of one image size stacked as (B, 3, H, W) for one blur per statistic."""
+
+    def __init__(self, idx, batch: Batch, targets_flat: torch.Tensor):
+        self.idx = list(idx)
+        h, w = batch.sizes[self.idx[0]]
+        self.h, self.w = h, w
+        self.sl = slice(batch.pixel_slices[self.idx[0]].start,
+                        batch.pixel_slices[self.idx[-1]].stop)
+        self.wh, self.wv = gaussian_window(11, 1.5, targets_flat.device, targets_flat.dtype)
+        t = self._bchw(targets_flat)
+        with torch.no_grad():
+            self.t = t
+            self.mu_t = self._blur(t)
+            self.mu_t2 = self.mu_t * self.mu_t
+            self.sig_t = self._blur(t * t) - self.mu_t2
+
+    def _bchw(self, flat):
+        return flat[self.sl].view(len(self.idx), self.h, self.w, 3).permute(0, 3, 1, 2)
+
+    def _blur(self, x):
+        x = torch.nn.functional.conv2d(x, self.wh, padding=(0, 5), groups=3)
+        return torch.nn.functional.conv2d(x, self.wv, padding=(5, 0), groups=3)
+
+    def ssim_sum(self, img_flat):
+        """Sum over the group's fields of each field's mean SSIM."""
+        p = self._bchw(img_flat)
+        mu_p = self._blur(p)
+        mu_p2 = mu_p * mu_p
+        mu_pt = mu_p * self.mu_t
+        sig_p = self._blur(p * p) - mu_p2
+        sig_pt = self._blur(p * self.t) - mu_pt
+        s = ((2 * mu_pt + C1) * (2 * sig_pt + C2)) / ((mu_p2 + self.mu_t2 + C1) *
+                                                     (sig_p + self.sig_t + C2))
+        return s.mean(dim=(1, 2, 3)).sum()
+
+
+@dataclass
+class BatchResult:
+    psnr: list
+    seconds: float
+    steps: int
+
+
+def _check_sorted_by_size(batch: Batch):
+    groups = []
+    for i, s in enumerate(batch.sizes):
+        if groups and batch.sizes[groups[-1][-1]] == s:
+            groups[-1].append(i)
+        elif any(batch.sizes[g[0]] == s for g in groups):
+            raise ValueError("fields of one image size must be adjacent in the batch")
+        else:
+            groups.append([i])
+    return groups
+
+
+def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig,
+              graph: bool = True) -> BatchResult:
+    """Fit every field to its target in one batched step per iteration (fields of one image size
+    must be adjacent).  The fitted rows are copied back into `fields`."""
+    dev = targets[0].device
+    sizes = [tuple(t.shape[:2]) for t in targets]
+    batch = Batch(sizes, [f.n for f in fields], dev)
+    groups = _check_sorted_by_size(batch)
+    big = concat_fields(fields).trainable()
+    targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
+    npix = torch.zeros(batch.total_pixels, 1, device=dev)
+    for s in batch.pixel_slices:
+        npix[s] = 1.0 / (3 * (s.stop - s.start))
+    ssim_groups = [_SSIMGroup(g, batch, targets_flat) for g in groups]
+    opt = F.make_optimizer(big, cfg)
+    lo = math.log(0.35)
+    hi = torch.cat([torch.full((f.n, 1), math.log(max(h, w)), device=dev)
+                    for f, (h, w) in zip(fields, sizes)], 0)
+    w = cfg.ssim_weight
+
+    def count():
+        with torch.no_grad():
+            return batch.pair_count(big.means, big.conics(), big.radii(cfg.sigma_cutoff),
+                                    cfg.support_fade, cfg.sigma_cutoff)
+
+    plan = BatchPlan(batch, int(count() * cfg.capacity_slack) + 4096)
+
+    def step():
+        opt.zero_grad(set_to_none=False)
+        op = big.opacity_values()
+        op = big.means.new_empty(0) if op is None else op
+        img = _BatchRender.apply(big.means, big.conics(), big.colors,
+                                 big.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
+                                 float(cfg.normalization_eps), bool(cfg.support_fade),
+                                 float(cfg.sigma_cutoff))
+        loss = (1.0 - w) * ((img - targets_flat).abs() * npix).sum()
+        if w != 0.0:
+            loss = loss + w * (len(fields) - sum(g.ssim_sum(img) for g in ssim_groups))
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            big.log_scales.clamp_(min=lo)
+            torch.minimum(big.log_scales, hi, out=big.log_scales)
+
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    done = 0
+    g = None
+    while done < cfg.iters:
+        n = min(cfg.window, cfg.iters - done)
+        snap = F._Snapshot(big, opt)
+        plan.status.zero_()
+        if graph and done >= 2:
+            if g is None:
+                g = F._capture(step)
+            for _ in range(n):
+                g.replay()
+        elif graph:
+            n = min(n, 2 - done)
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                for _ in range(n):
+                    step()
+            torch.cuda.current_stream(dev).wait_stream(side)
+        else:
+            for _ in range(n):
+                step()
+        if int(plan.status.item()) != 0:
+            snap.restore(big, opt)
+            plan = BatchPlan(batch, 2 * int(count() * cfg.capacity_slack) + 4096)
+            g = None
+            continue
+        done += n
+        p = count()
+        if p * 1.15 > plan.capacity:
+            plan = BatchPlan(batch, int(p * cfg.capacity_slack) + 4096)
+            g = None
+    torch.cuda.synchronize(dev)
+    seconds = time.perf_counter() - t0
+    with torch.no_grad():
+        op = big.opacity_values()
+        op = big.means.new_empty(0) if op is None else op
+        gids, offsets, _ = batch.build_tile_index(big.means, big.conics(),
+                                                  big.radii(cfg.sigma_cutoff), plan.capacity,
+                                                  cfg.support_fade, cfg.sigma_cutoff)
+        img = batch.forward(big.means, big.conics(), big.colors, big.radii(cfg.sigma_cutoff), op,
+                            gids, offsets, cfg.normalize, cfg.normalization_eps,
+                            cfg.support_fade, cfg.sigma_cutoff)
+        psnrs = [float(psnr_tensor(im, t)) for im, t in zip(batch.images(img), targets)]
+        for f, sl in zip(fields, batch.gauss_slices):
+            for name in ("means", "log_scales", "rotations", "colors", "opacities"):
+                src = getattr(big, name)
+                if src is not None:
+                    setattr(f, name, src.detach()[sl].clone())
+    return BatchResult(psnrs, seconds, cfg.iters)
+
+
+def fit_each(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig
+             ) -> BatchResult:
+    """The plain per-image loop: each field fitted on its own, eagerly, one after the other."""
+    dev = targets[0].device
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    psnrs = []
+    for f, t in zip(fields, targets):
+        psnrs.append(F.fit(f, t, cfg, graph=False)["psnr"])
+    torch.cuda.synchronize(dev)
+    return BatchResult(psnrs, time.perf_counter() - t0, cfg.iters * len(fields))

@@ -224,6 +224,49 @@ int fill_f32(float* p, int64_t count, float value, hipStream_t stream) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Fields: one image and its Gaussians.  A batch concatenates the fields' Gaussians, tiles and
+// pixels; every kernel finds its field by binary search over the (device) field table.  A single
+// field travels by value in the kernel arguments instead.
+// ------------------------------------------------------------------------------------------
+
+struct FieldTable {
+  const ssplat_field* fields;  // device table of num_fields entries, or NULL for `single`
+  int num_fields;
+  ssplat_field single;
+};
+
+// Field owning global tile `tile` (the last field whose first_tile <= tile).
+__device__ __forceinline__ ssplat_field field_of_tile(const FieldTable& t, int tile) {
+  if (t.fields == nullptr) return t.single;
+  int lo = 0, hi = t.num_fields - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.fields[mid].first_tile <= tile) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return t.fields[lo];
+}
+
+// Field owning global Gaussian `i` (the last field whose first_gaussian <= i; fields are
+// contiguous, so an empty field never wins over the next one).
+__device__ __forceinline__ ssplat_field field_of_gaussian(const FieldTable& t, int i) {
+  if (t.fields == nullptr) return t.single;
+  int lo = 0, hi = t.num_fields - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.fields[mid].first_gaussian <= i) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return t.fields[lo];
+}
+
+// ------------------------------------------------------------------------------------------
 // Tile binning
 // ------------------------------------------------------------------------------------------
 
@@ -231,7 +274,8 @@ struct BinArgs {
   const float* means;
   const float* conics;
   const int64_t* radii;
-  int n, H, W, ts, tiles_x, cull;
+  FieldTable fields;
+  int n, ts, cull;
   float cull_thr;  // cutoff^2 + 1e-5 * max(cutoff^2, 1), computed in float32 on the host
 };
 
@@ -243,6 +287,7 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
   __shared__ int64_t s_incl[kBinThreads];
   __shared__ int4 s_rect[kBinThreads];
   __shared__ int4 s_tile[kBinThreads];
+  __shared__ int2 s_grid[kBinThreads];
   __shared__ float4 s_geo[kBinThreads];
   __shared__ float s_b[kBinThreads];
   const int lane = threadIdx.x & 63;
@@ -251,11 +296,15 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
   int64_t cnt = 0;
   Rect r{0, 0, 0, 0};
   int tx0 = 0, ty0 = 0, tw = 1;
+  int first_tile = 0, tiles_x = 1;
   float mx = 0.0f, my = 0.0f;
   if (i < a.n) {
+    const ssplat_field f = field_of_gaussian(a.fields, i);
+    first_tile = static_cast<int>(f.first_tile);
+    tiles_x = f.tiles_x;
     mx = a.means[2 * i + 0];
     my = a.means[2 * i + 1];
-    r = support_rect(mx, my, a.radii[2 * i + 0], a.radii[2 * i + 1], a.H, a.W);
+    r = support_rect(mx, my, a.radii[2 * i + 0], a.radii[2 * i + 1], f.height, f.width);
     if (r.x1 > r.x0 && r.y1 > r.y0) {
       tx0 = r.x0 / a.ts;
       ty0 = r.y0 / a.ts;
@@ -275,6 +324,7 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
   s_incl[threadIdx.x] = incl;
   s_rect[threadIdx.x] = make_int4(r.x0, r.x1, r.y0, r.y1);
   s_tile[threadIdx.x] = make_int4(tx0, ty0, tw, i);
+  s_grid[threadIdx.x] = make_int2(first_tile, tiles_x);
   if (a.cull) {
     float ca = 0.0f, cb = 0.0f, cc = 0.0f;
     if (i < a.n) {
@@ -314,7 +364,8 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
                                   static_cast<float>(ry1));
       if (mq > a.cull_thr) continue;
     }
-    emit(tile_y * a.tiles_x + tile_x, tl.w);
+    const int2 grid = s_grid[o];
+    emit(grid.x + tile_y * grid.y + tile_x, tl.w);
   }
 }
 
@@ -513,13 +564,14 @@ static_assert(sizeof(GaussRec) == 48, "GaussRec must be 48 bytes");
 
 __global__ void k_prepare(const float* __restrict__ means, const float* __restrict__ conics,
                           const float* __restrict__ colors, const int64_t* __restrict__ radii,
-                          const float* __restrict__ opacities, int n, int H, int W,
+                          const float* __restrict__ opacities, int n, FieldTable fields,
                           GaussRec* __restrict__ recs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const ssplat_field f = field_of_gaussian(fields, i);
   const float mx = means[2 * i + 0];
   const float my = means[2 * i + 1];
-  const Rect r = support_rect(mx, my, radii[2 * i + 0], radii[2 * i + 1], H, W);
+  const Rect r = support_rect(mx, my, radii[2 * i + 0], radii[2 * i + 1], f.height, f.width);
   uint32_t xr = 0, yr = 0;
   if (r.x1 > r.x0 && r.y1 > r.y0) {
     xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
@@ -540,7 +592,7 @@ struct RenderArgs {
   const int32_t* offsets;
   int64_t num_gids;
   int n;  // number of records; list entries outside [0, n) stage as empty records
-  int H, W, tiles_x;
+  FieldTable fields;
   float eps;
   float tail_exp;  // sigma_cutoff^2 * kNegHalfLog2e
 };
@@ -660,8 +712,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
                                                               float* __restrict__ den) {
   __shared__ StageLds st;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tile_x = tile % a.tiles_x;
-  const int tile_y = tile / a.tiles_x;
+  const ssplat_field f = field_of_tile(a.fields, tile);
+  const int local = tile - static_cast<int>(f.first_tile);
+  const int tile_x = local % f.tiles_x;
+  const int tile_y = local / f.tiles_x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int qx0 = tile_x * kTile + (wave & 1) * 8;
@@ -671,8 +725,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
   const float tail = kFade ? exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
   accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
-  if (px < a.W && py < a.H) {
-    const int64_t flat = int64_t(py) * a.W + px;
+  if (px < f.width && py < f.height) {
+    const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
     if (den) den[flat] = acc.w;
     if (kNormalize) {
       const float denom = acc.w + a.eps;
@@ -700,8 +754,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   __shared__ StageLds st;
   __shared__ float4 s_adj[kTile * kTile];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tile_x = tile % a.tiles_x;
-  const int tile_y = tile / a.tiles_x;
+  const ssplat_field f = field_of_tile(a.fields, tile);
+  const int local = tile - static_cast<int>(f.first_tile);
+  const int tile_x = local % f.tiles_x;
+  const int tile_y = local / f.tiles_x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int lx = lane & 7;
@@ -720,8 +776,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
     float4 acc;
     accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (px < a.W && py < a.H) {
-      const int64_t flat = int64_t(py) * a.W + px;
+    if (px < f.width && py < f.height) {
+      const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
       const float gr = g.grad_out[flat * 3 + 0];
       const float gg = g.grad_out[flat * 3 + 1];
       const float gb = g.grad_out[flat * 3 + 2];
@@ -843,32 +899,75 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
 // Host-side argument handling
 // ------------------------------------------------------------------------------------------
 
-struct Dims {
-  int tiles_x, tiles_y, num_tiles;
+// Geometry of one call: the field table (one by-value field, or a device table) and totals.
+struct Plan {
+  FieldTable table;
+  int64_t n;      // Gaussians over all fields
+  int num_tiles;  // tiles over all fields
+  int ts;
 };
 
-int check_dims(int64_t n, int H, int W, int ts, Dims* d) {
-  if (n < 0 || n > INT32_MAX) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
+int check_tile_size(int ts) {
+  if (ts <= 0 || ts * ts > 1024) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
+                     "tile_size must be positive with tile_size^2 <= 1024");
+  }
+  return SSPLAT_OK;
+}
+
+int check_image(int H, int W) {
   if (H <= 0 || W <= 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be positive");
   }
   if (H > 65535 || W > 65535) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be <= 65535");
   }
-  if (ts <= 0 || ts * ts > 1024) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
-                     "tile_size must be positive with tile_size^2 <= 1024");
+  return SSPLAT_OK;
+}
+
+int check_totals(int64_t n, int64_t tiles) {
+  if (n < 0 || n > INT32_MAX) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
+  if (tiles <= 0 || tiles + 1 > kMaxScanItems) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile grid of %lld tiles is out of range",
+                     static_cast<long long>(tiles));
   }
+  return SSPLAT_OK;
+}
+
+int make_single_plan(int64_t n, int H, int W, int ts, Plan* p) {
+  int rc;
+  if (n < 0 || n > INT32_MAX) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
+  if ((rc = check_image(H, W)) || (rc = check_tile_size(ts))) return rc;
   const int64_t tx = (int64_t(W) + ts - 1) / ts;
   const int64_t ty = (int64_t(H) + ts - 1) / ts;
-  const int64_t t = tx * ty;
-  if (t + 1 > kMaxScanItems || t + 1 > INT32_MAX) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile grid of %lld tiles is too large",
-                     static_cast<long long>(t));
+  if ((rc = check_totals(n, tx * ty))) return rc;
+  ssplat_field f{};
+  f.height = H;
+  f.width = W;
+  f.first_gaussian = 0;
+  f.num_gaussians = n;
+  f.first_pixel = 0;
+  f.first_tile = 0;
+  f.tiles_x = static_cast<int32_t>(tx);
+  f.num_tiles = static_cast<int32_t>(tx * ty);
+  p->table = FieldTable{nullptr, 0, f};
+  p->n = n;
+  p->num_tiles = f.num_tiles;
+  p->ts = ts;
+  return SSPLAT_OK;
+}
+
+int make_batch_plan(const ssplat_field* fields, int num_fields, int64_t total_gaussians,
+                    int64_t total_tiles, int ts, Plan* p) {
+  int rc;
+  if (fields == nullptr || num_fields <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the field table must hold at least one field");
   }
-  d->tiles_x = static_cast<int>(tx);
-  d->tiles_y = static_cast<int>(ty);
-  d->num_tiles = static_cast<int>(t);
+  if ((rc = check_tile_size(ts)) || (rc = check_totals(total_gaussians, total_tiles))) return rc;
+  p->table = FieldTable{fields, num_fields, ssplat_field{}};
+  p->n = total_gaussians;
+  p->num_tiles = static_cast<int>(total_tiles);
+  p->ts = ts;
   return SSPLAT_OK;
 }
 
@@ -901,8 +1000,8 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   return w;
 }
 
-int check_workspace(void* ws, size_t ws_bytes, int64_t n, int num_tiles, Workspace* w) {
-  *w = layout(ws, n, num_tiles);
+int check_workspace(void* ws, size_t ws_bytes, const Plan& p, Workspace* w) {
+  *w = layout(ws, p.n, p.num_tiles);
   if (ws == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace is NULL");
   if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
@@ -914,10 +1013,9 @@ int check_workspace(void* ws, size_t ws_bytes, int64_t n, int num_tiles, Workspa
   return SSPLAT_OK;
 }
 
-int make_bin_args(const float* means, const float* conics, const int64_t* radii, int64_t n,
-                  int H, int W, int ts, int cull, double sigma_cutoff, const Dims& d,
-                  BinArgs* a) {
-  if (n > 0 && (means == nullptr || radii == nullptr || (cull && conics == nullptr))) {
+int make_bin_args(const float* means, const float* conics, const int64_t* radii, const Plan& p,
+                  int cull, double sigma_cutoff, BinArgs* a) {
+  if (p.n > 0 && (means == nullptr || radii == nullptr || (cull && conics == nullptr))) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/conics/radii must not be NULL");
   }
   if (!std::isfinite(sigma_cutoff)) {
@@ -926,11 +1024,9 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
   a->means = means;
   a->conics = conics;
   a->radii = radii;
-  a->n = static_cast<int>(n);
-  a->H = H;
-  a->W = W;
-  a->ts = ts;
-  a->tiles_x = d.tiles_x;
+  a->fields = p.table;
+  a->n = static_cast<int>(p.n);
+  a->ts = p.ts;
   a->cull = cull ? 1 : 0;
   // Same float32 arithmetic as the reference: cutoff2 = float(sigma^2), guard 1e-5*max(cutoff2,1)
   // (render_ext.cu:187-188, :920).
@@ -941,9 +1037,9 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
   return SSPLAT_OK;
 }
 
-int run_count(const BinArgs& a, const Dims& d, const Workspace& w, int32_t* offsets,
+int run_count(const BinArgs& a, const Plan& p, const Workspace& w, int32_t* offsets,
               hipStream_t stream) {
-  const int64_t items = int64_t(d.num_tiles) + 1;
+  const int64_t items = int64_t(p.num_tiles) + 1;
   int rc = fill_i32(w.counts, items, 0, stream);
   if (rc) return rc;
   rc = fill_i32(w.status, 1, SSPLAT_OK, stream);
@@ -962,16 +1058,16 @@ int run_count(const BinArgs& a, const Dims& d, const Workspace& w, int32_t* offs
   return SSPLAT_OK;
 }
 
-int run_fill(const BinArgs& a, const Dims& d, const Workspace& w, const int32_t* offsets,
+int run_fill(const BinArgs& a, const Plan& p, const Workspace& w, const int32_t* offsets,
              int32_t* gids, int64_t capacity, int32_t* status, hipStream_t stream) {
-  const int64_t items = int64_t(d.num_tiles) + 1;
+  const int64_t items = int64_t(p.num_tiles) + 1;
   int rc = fill_i32(w.counts, items, 0, stream);
   if (rc) return rc;
   if (a.n > 0) {
     k_bin_scatter<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(
         a, offsets, w.counts, gids, capacity, w.status);
     SSPLAT_LAUNCH_CHECK("k_bin_scatter");
-    k_sort_tiles<<<d.num_tiles, kSortThreads, 0, stream>>>(gids, offsets, d.num_tiles, capacity);
+    k_sort_tiles<<<p.num_tiles, kSortThreads, 0, stream>>>(gids, offsets, p.num_tiles, capacity);
     SSPLAT_LAUNCH_CHECK("k_sort_tiles");
   }
   if (status) {
@@ -981,20 +1077,52 @@ int run_fill(const BinArgs& a, const Dims& d, const Workspace& w, const int32_t*
   return SSPLAT_OK;
 }
 
+int tile_index_count(const float* means, const float* conics, const int64_t* radii,
+                     const Plan& p, int cull, double sigma_cutoff, int32_t* offsets, void* ws,
+                     size_t ws_bytes, void* stream) {
+  int rc;
+  if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  BinArgs a;
+  if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
+  if ((rc = run_count(a, p, w, offsets, static_cast<hipStream_t>(stream)))) return rc;
+  return success();
+}
+
+int tile_index_fill(const float* means, const float* conics, const int64_t* radii,
+                    const Plan& p, int cull, double sigma_cutoff, const int32_t* offsets,
+                    int32_t* gids, int64_t capacity, int32_t* status, void* ws, size_t ws_bytes,
+                    void* stream) {
+  int rc;
+  if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  if (capacity < 0 || (capacity > 0 && gids == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or capacity negative");
+  }
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  BinArgs a;
+  if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
+  if ((rc = run_fill(a, p, w, offsets, gids, capacity, status,
+                     static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
+}
+
 int check_render_inputs(const float* means, const float* conics, const float* colors,
-                        const int64_t* radii, int64_t n, const int32_t* gids,
-                        const int32_t* offsets, int64_t num_gids, int ts, int normalize,
-                        double eps) {
-  if (n > 0 && (means == nullptr || conics == nullptr || colors == nullptr || radii == nullptr)) {
+                        const int64_t* radii, const Plan& p, const int32_t* gids,
+                        const int32_t* offsets, int64_t num_gids, int normalize, double eps) {
+  if (p.n > 0 && (means == nullptr || conics == nullptr || colors == nullptr || radii == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/conics/colors/radii must not be NULL");
   }
   if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
   if (num_gids < 0 || (num_gids > 0 && gids == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or num_gids negative");
   }
-  if (ts != kTile) {
+  if (p.ts != kTile) {
     return set_error(SSPLAT_ERR_UNSUPPORTED, "the render kernels are built for tile_size %d, got %d",
-                     kTile, ts);
+                     kTile, p.ts);
   }
   if (normalize && !(std::isfinite(eps) && eps > 0.0)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "normalization_eps must be finite and > 0");
@@ -1002,31 +1130,98 @@ int check_render_inputs(const float* means, const float* conics, const float* co
   return SSPLAT_OK;
 }
 
-RenderArgs make_render_args(const Workspace& w, int64_t n, const int32_t* gids,
-                            const int32_t* offsets, int64_t num_gids, int H, int W, const Dims& d,
-                            double eps, double sigma_cutoff) {
+RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gids,
+                            const int32_t* offsets, int64_t num_gids, double eps,
+                            double sigma_cutoff) {
   RenderArgs r;
   r.recs = w.recs;
   r.gids = gids;
   r.offsets = offsets;
   r.num_gids = num_gids;
-  r.n = static_cast<int>(n);
-  r.H = H;
-  r.W = W;
-  r.tiles_x = d.tiles_x;
+  r.n = static_cast<int>(p.n);
+  r.fields = p.table;
   r.eps = static_cast<float>(eps);
   r.tail_exp = static_cast<float>(sigma_cutoff * sigma_cutoff) * kNegHalfLog2e;
   return r;
 }
 
 int run_prepare(const float* means, const float* conics, const float* colors,
-                const int64_t* radii, const float* opacities, int64_t n, int H, int W,
-                const Workspace& w, hipStream_t stream) {
-  if (n == 0) return SSPLAT_OK;
-  k_prepare<<<static_cast<int>((n + 255) / 256), 256, 0, stream>>>(
-      means, conics, colors, radii, opacities, static_cast<int>(n), H, W, w.recs);
+                const int64_t* radii, const float* opacities, const Plan& p, const Workspace& w,
+                hipStream_t stream) {
+  if (p.n == 0) return SSPLAT_OK;
+  k_prepare<<<static_cast<int>((p.n + 255) / 256), 256, 0, stream>>>(
+      means, conics, colors, radii, opacities, static_cast<int>(p.n), p.table, w.recs);
   SSPLAT_LAUNCH_CHECK("k_prepare");
   return SSPLAT_OK;
+}
+
+int forward_tiled(const Plan& p, const float* means, const float* conics, const float* colors,
+                  const int64_t* radii, const float* opacities, const int32_t* gids,
+                  const int32_t* offsets, int64_t num_gids, int normalize, double eps,
+                  int support_fade, double sigma_cutoff, float* out, float* den, void* ws,
+                  size_t ws_bytes, void* stream) {
+  int rc;
+  if ((rc = check_render_inputs(means, conics, colors, radii, p, gids, offsets, num_gids,
+                                normalize, eps))) {
+    return rc;
+  }
+  if (out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "out is NULL");
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, sigma_cutoff);
+  if (support_fade && normalize) {
+    k_forward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else if (support_fade) {
+    k_forward16<true, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else if (normalize) {
+    k_forward16<false, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  } else {
+    k_forward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  }
+  SSPLAT_LAUNCH_CHECK("k_forward16");
+  return success();
+}
+
+int backward_tiled(const Plan& p, const float* grad_out, const float* means, const float* conics,
+                   const float* colors, const int64_t* radii, const float* opacities,
+                   const int32_t* gids, const int32_t* offsets, int64_t num_gids, int normalize,
+                   double eps, int support_fade, double sigma_cutoff, float* grad_means,
+                   float* grad_conics, float* grad_colors, float* grad_opacities, void* ws,
+                   size_t ws_bytes, void* stream) {
+  int rc;
+  if ((rc = check_render_inputs(means, conics, colors, radii, p, gids, offsets, num_gids,
+                                normalize, eps))) {
+    return rc;
+  }
+  if (grad_out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_out is NULL");
+  if (p.n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
+  }
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.n == 0) return success();
+  if ((rc = fill_f32(grad_means, 2 * p.n, 0.0f, s))) return rc;
+  if ((rc = fill_f32(grad_conics, 3 * p.n, 0.0f, s))) return rc;
+  if ((rc = fill_f32(grad_colors, 3 * p.n, 0.0f, s))) return rc;
+  float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
+  if (gop && (rc = fill_f32(gop, p.n, 0.0f, s))) return rc;
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, sigma_cutoff);
+  GradArgs ga{grad_out, grad_means, grad_conics, grad_colors, gop};
+  if (support_fade && normalize) {
+    k_backward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else if (support_fade) {
+    k_backward16<true, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else if (normalize) {
+    k_backward16<false, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  } else {
+    k_backward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  }
+  SSPLAT_LAUNCH_CHECK("k_backward16");
+  return success();
 }
 
 }  // namespace
@@ -1043,10 +1238,10 @@ const char* ssplat_last_error(void) { return g_last_error.c_str(); }
 
 int ssplat_workspace_bytes(int64_t n, int height, int width, int tile_size, size_t* bytes) {
   if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
-  Dims d;
-  const int rc = check_dims(n, height, width, tile_size, &d);
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  *bytes = layout(nullptr, n, d.num_tiles).bytes;
+  *bytes = layout(nullptr, p.n, p.num_tiles).bytes;
   return success();
 }
 
@@ -1054,20 +1249,11 @@ int ssplat_tile_index_count(const float* means, const float* conics, const int64
                             int64_t n, int height, int width, int tile_size, int ellipse_cull,
                             double sigma_cutoff, int32_t* tile_offsets, void* workspace,
                             size_t workspace_bytes, void* stream) {
-  Dims d;
-  int rc = check_dims(n, height, width, tile_size, &d);
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  if (tile_offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
-  Workspace w;
-  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
-  if (rc) return rc;
-  BinArgs a;
-  rc = make_bin_args(means, conics, radii, n, height, width, tile_size, ellipse_cull, sigma_cutoff,
-                     d, &a);
-  if (rc) return rc;
-  rc = run_count(a, d, w, tile_offsets, static_cast<hipStream_t>(stream));
-  if (rc) return rc;
-  return success();
+  return tile_index_count(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                          workspace, workspace_bytes, stream);
 }
 
 int ssplat_tile_index_fill(const float* means, const float* conics, const int64_t* radii,
@@ -1075,24 +1261,11 @@ int ssplat_tile_index_fill(const float* means, const float* conics, const int64_
                            double sigma_cutoff, const int32_t* tile_offsets, int32_t* tile_gids,
                            int64_t gid_capacity, int32_t* status, void* workspace,
                            size_t workspace_bytes, void* stream) {
-  Dims d;
-  int rc = check_dims(n, height, width, tile_size, &d);
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  if (tile_offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
-  if (gid_capacity < 0 || (gid_capacity > 0 && tile_gids == nullptr)) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or capacity negative");
-  }
-  Workspace w;
-  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
-  if (rc) return rc;
-  BinArgs a;
-  rc = make_bin_args(means, conics, radii, n, height, width, tile_size, ellipse_cull, sigma_cutoff,
-                     d, &a);
-  if (rc) return rc;
-  rc = run_fill(a, d, w, tile_offsets, tile_gids, gid_capacity, status,
-                static_cast<hipStream_t>(stream));
-  if (rc) return rc;
-  return success();
+  return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
 }
 
 int ssplat_build_tile_index(const float* means, const float* conics, const int64_t* radii,
@@ -1100,13 +1273,14 @@ int ssplat_build_tile_index(const float* means, const float* conics, const int64
                             double sigma_cutoff, int32_t* tile_offsets, int32_t* tile_gids,
                             int64_t gid_capacity, int32_t* status, void* workspace,
                             size_t workspace_bytes, void* stream) {
-  const int rc = ssplat_tile_index_count(means, conics, radii, n, height, width, tile_size,
-                                         ellipse_cull, sigma_cutoff, tile_offsets, workspace,
-                                         workspace_bytes, stream);
+  Plan p;
+  int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  return ssplat_tile_index_fill(means, conics, radii, n, height, width, tile_size, ellipse_cull,
-                                sigma_cutoff, tile_offsets, tile_gids, gid_capacity, status,
-                                workspace, workspace_bytes, stream);
+  rc = tile_index_count(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                        workspace, workspace_bytes, stream);
+  if (rc) return rc;
+  return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
 }
 
 int ssplat_forward_tiled(const float* means, const float* conics, const float* colors,
@@ -1115,35 +1289,12 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          int height, int width, int tile_size, int normalize, double eps,
                          int support_fade, double sigma_cutoff, float* out, float* den,
                          void* workspace, size_t workspace_bytes, void* stream) {
-  Dims d;
-  int rc = check_dims(n, height, width, tile_size, &d);
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  rc = check_render_inputs(means, conics, colors, radii, n, tile_gids, tile_offsets, num_gids,
-                           tile_size, normalize, eps);
-  if (rc) return rc;
-  if (out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "out is NULL");
-  Workspace w;
-  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
-  if (rc) return rc;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  rc = run_prepare(means, conics, colors, radii, opacities, n, height, width, w, s);
-  if (rc) return rc;
-  const RenderArgs ra =
-      make_render_args(w, n, tile_gids, tile_offsets, num_gids, height, width, d, eps,
-                       sigma_cutoff);
-  const int fade = support_fade ? 1 : 0;
-  const int norm = normalize ? 1 : 0;
-  if (fade && norm) {
-    k_forward16<true, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  } else if (fade) {
-    k_forward16<true, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  } else if (norm) {
-    k_forward16<false, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  } else {
-    k_forward16<false, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  }
-  SSPLAT_LAUNCH_CHECK("k_forward16");
-  return success();
+  return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
+                       num_gids, normalize, eps, support_fade, sigma_cutoff, out, den, workspace,
+                       workspace_bytes, stream);
 }
 
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
@@ -1153,45 +1304,98 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           double eps, int support_fade, double sigma_cutoff, float* grad_means,
                           float* grad_conics, float* grad_colors, float* grad_opacities,
                           void* workspace, size_t workspace_bytes, void* stream) {
-  Dims d;
-  int rc = check_dims(n, height, width, tile_size, &d);
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  rc = check_render_inputs(means, conics, colors, radii, n, tile_gids, tile_offsets, num_gids,
-                           tile_size, normalize, eps);
-  if (rc) return rc;
-  if (grad_out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_out is NULL");
-  if (n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
+                        tile_offsets, num_gids, normalize, eps, support_fade, sigma_cutoff,
+                        grad_means, grad_conics, grad_colors, grad_opacities, workspace,
+                        workspace_bytes, stream);
+}
+
+int ssplat_batch_plan(ssplat_field* fields, int num_fields, int tile_size,
+                      int64_t* total_gaussians, int64_t* total_tiles, int64_t* total_pixels,
+                      size_t* workspace_bytes) {
+  int rc;
+  if (fields == nullptr || num_fields <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the field table must hold at least one field");
   }
-  Workspace w;
-  rc = check_workspace(workspace, workspace_bytes, n, d.num_tiles, &w);
-  if (rc) return rc;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (n == 0) return success();
-  if ((rc = fill_f32(grad_means, 2 * n, 0.0f, s))) return rc;
-  if ((rc = fill_f32(grad_conics, 3 * n, 0.0f, s))) return rc;
-  if ((rc = fill_f32(grad_colors, 3 * n, 0.0f, s))) return rc;
-  float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
-  if (gop && (rc = fill_f32(gop, n, 0.0f, s))) return rc;
-  rc = run_prepare(means, conics, colors, radii, opacities, n, height, width, w, s);
-  if (rc) return rc;
-  const RenderArgs ra =
-      make_render_args(w, n, tile_gids, tile_offsets, num_gids, height, width, d, eps,
-                       sigma_cutoff);
-  GradArgs ga{grad_out, grad_means, grad_conics, grad_colors, gop};
-  const int fade = support_fade ? 1 : 0;
-  const int norm = normalize ? 1 : 0;
-  if (fade && norm) {
-    k_backward16<true, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
-  } else if (fade) {
-    k_backward16<true, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
-  } else if (norm) {
-    k_backward16<false, true><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
-  } else {
-    k_backward16<false, false><<<d.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  if ((rc = check_tile_size(tile_size))) return rc;
+  int64_t g = 0, t = 0, px = 0;
+  for (int i = 0; i < num_fields; ++i) {
+    ssplat_field& f = fields[i];
+    if ((rc = check_image(f.height, f.width))) return rc;
+    if (f.num_gaussians < 0) {
+      return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "field %d has a negative Gaussian count", i);
+    }
+    const int64_t tx = (int64_t(f.width) + tile_size - 1) / tile_size;
+    const int64_t ty = (int64_t(f.height) + tile_size - 1) / tile_size;
+    f.first_gaussian = g;
+    f.first_pixel = px;
+    f.first_tile = t;
+    f.tiles_x = static_cast<int32_t>(tx);
+    f.num_tiles = static_cast<int32_t>(tx * ty);
+    g += f.num_gaussians;
+    t += tx * ty;
+    px += int64_t(f.height) * f.width;
+    if ((rc = check_totals(g, t))) return rc;
   }
-  SSPLAT_LAUNCH_CHECK("k_backward16");
+  if (total_gaussians) *total_gaussians = g;
+  if (total_tiles) *total_tiles = t;
+  if (total_pixels) *total_pixels = px;
+  if (workspace_bytes) *workspace_bytes = layout(nullptr, g, static_cast<int>(t)).bytes;
   return success();
+}
+
+int ssplat_batch_build_tile_index(const ssplat_field* fields, int num_fields,
+                                  int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                                  const float* means, const float* conics, const int64_t* radii,
+                                  int ellipse_cull, double sigma_cutoff, int32_t* tile_offsets,
+                                  int32_t* tile_gids, int64_t gid_capacity, int32_t* status,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+  Plan p;
+  int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
+  if (rc) return rc;
+  rc = tile_index_count(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                        workspace, workspace_bytes, stream);
+  if (rc) return rc;
+  return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
+}
+
+int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
+                               int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                               const float* means, const float* conics, const float* colors,
+                               const int64_t* radii, const float* opacities,
+                               const int32_t* tile_gids, const int32_t* tile_offsets,
+                               int64_t num_gids, int normalize, double eps, int support_fade,
+                               double sigma_cutoff, float* out, float* den, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+  Plan p;
+  const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
+  if (rc) return rc;
+  return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
+                       num_gids, normalize, eps, support_fade, sigma_cutoff, out, den, workspace,
+                       workspace_bytes, stream);
+}
+
+int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
+                                int64_t total_gaussians, int64_t total_tiles, int tile_size,
+                                const float* grad_out, const float* means, const float* conics,
+                                const float* colors, const int64_t* radii,
+                                const float* opacities, const int32_t* tile_gids,
+                                const int32_t* tile_offsets, int64_t num_gids, int normalize,
+                                double eps, int support_fade, double sigma_cutoff,
+                                float* grad_means, float* grad_conics, float* grad_colors,
+                                float* grad_opacities, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+  Plan p;
+  const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
+  if (rc) return rc;
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
+                        tile_offsets, num_gids, normalize, eps, support_fade, sigma_cutoff,
+                        grad_means, grad_conics, grad_colors, grad_opacities, workspace,
+                        workspace_bytes, stream);
 }
 
 }  // extern "C"

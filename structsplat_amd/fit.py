@@ -149,10 +149,12 @@ class _Snapshot:
             for p, saved in zip(field.parameters(), self.params):
                 p.copy_(saved)
                 st = self.state.get(id(p))
-                if st is not None:
-                    for k, v in st.items():
-                        if torch.is_tensor(v):
-                            opt.state[p][k].copy_(v)
+                if st is None:
+                    opt.state.pop(p, None)  # snapshot predates Adam's lazy state
+                    continue
+                for k, v in st.items():
+                    if torch.is_tensor(v):
+                        opt.state[p][k].copy_(v)
 
 
 def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool = False,
@@ -182,7 +184,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     regrows = 0
     while done < cfg.iters:
         n = min(cfg.window, cfg.iters - done)
-        snap = _Snapshot(field, opt) if done > 0 else None
+        snap = _Snapshot(field, opt)
         plan.status.zero_()
         if graph and done >= 2:
             if graph_obj is None:
@@ -203,8 +205,6 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
             for _ in range(n):
                 step()
         if int(plan.status.item()) != 0:
-            if snap is None:
-                raise RuntimeError("pair capacity exceeded in the first steps")
             snap.restore(field, opt)
             plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg) * 2,
                               dev)

@@ -110,6 +110,52 @@ def cpu_baseline(workload: str, H: int, W: int, bytes_step: float, reps: int = 2
     }
 
 
+def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
+    """Batched fitting: `num_fields` fields, sides uniform over {256, 512, 768, 1024}, Gaussian
+    counts log-uniform in [1k, 60k], fit-target recipe; the batched graph-captured fit against
+    the plain per-image loop, same steps, same initial fields (fields sorted by size)."""
+    import numpy as np
+    import torch
+
+    from structsplat_amd import batch as B
+    from structsplat_amd import fit as F
+    from structsplat_amd import synth
+    from structsplat_amd.gaussians import GaussianField
+
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([256, 512, 768, 1024], num_fields)
+    counts = np.exp(rng.uniform(np.log(1000), np.log(60000), num_fields)).astype(int)
+    order = np.argsort(sides, kind="stable")
+    sides, counts = sides[order], counts[order]
+    dev = torch.device("cuda:0")
+    targets, inits = [], []
+    for k, (s, n) in enumerate(zip(sides, counts)):
+        t = synth.fit_target(1000 + k, int(s), int(s))
+        targets.append(torch.tensor(t, device=dev))
+        inits.append(synth.fit_init_field(t, int(n), 1000 + k))
+
+    def fields():
+        return [GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
+                                         opacities=f["opacity"], device=dev) for f in inits]
+
+    cfg = F.FitConfig(iters=steps, window=max(steps // 2, 1))
+    batched = B.fit_batch(fields(), targets, cfg, graph=True)
+    plain = B.fit_each(fields(), targets, cfg)
+    diff = float(np.max(np.abs(np.array(batched.psnr) - np.array(plain.psnr))))
+    return {
+        "fields": num_fields,
+        "steps_per_field": steps,
+        "total_gaussians": int(counts.sum()),
+        "total_pixels": int((sides.astype(np.int64) ** 2).sum()),
+        "batched_steps_per_s": round(steps / batched.seconds, 3),
+        "plain_steps_per_s": round(steps / plain.seconds, 3),
+        "speedup": round(plain.seconds / batched.seconds, 3),
+        "psnr_max_abs_diff_db": round(diff, 5),
+        "mean_psnr_db": round(float(np.mean(batched.psnr)), 3),
+        "note": "a step advances every field by one Adam step",
+    }
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as fh:
@@ -128,6 +174,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=20,
+                    help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the renderer's fields are independent units and this "
@@ -242,6 +290,8 @@ def main() -> int:
             "traffic": None,
         },
     }
+    if args.batched_steps > 0:
+        line["batched"] = batched_fit(args.batched_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, H, W, b["step"])
     print(json.dumps(line), flush=True)

@@ -103,6 +103,11 @@ int ssplat_build_tile_index(const float* means, const float* conics, const int64
  * The render kernels are built for tile_size 16 (the reference default, cuda_render.py:258);
  * other tile sizes return SSPLAT_ERR_UNSUPPORTED.
  *
+ * support_fade_alpha in [0, 1] (clamped) subtracts alpha * exp(-cutoff^2/2) from every weight,
+ * clamped at 0 (render.py:94-107): 0 renders the plain Gaussian, 1 is the reference's
+ * support_fade (weight exactly 0 at sigma_cutoff), values in between are the reference's
+ * fractional fade strength.
+ *
  * Tiled forward.  Writes out (H,W,3) and, when den is not NULL, the raw coverage
  * den (H,W) = sum_i o_i G_i (no eps).  Every pixel is accumulated in ascending Gaussian id,
  * so the result is bitwise deterministic.
@@ -113,7 +118,7 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          const int64_t* radii, const float* opacities, int64_t n,
                          const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
                          int height, int width, int tile_size, int normalize, double eps,
-                         int support_fade, double sigma_cutoff, float* out, float* den,
+                         double support_fade_alpha, double sigma_cutoff, float* out, float* den,
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /*
@@ -128,7 +133,8 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           const float* colors, const int64_t* radii, const float* opacities,
                           int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
-                          double eps, int support_fade, double sigma_cutoff, float* grad_means,
+                          double eps, double support_fade_alpha, double sigma_cutoff,
+                          float* grad_means,
                           float* grad_conics, float* grad_colors, float* grad_opacities,
                           void* workspace, size_t workspace_bytes, void* stream);
 
@@ -175,8 +181,9 @@ int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
                                const float* means, const float* conics, const float* colors,
                                const int64_t* radii, const float* opacities,
                                const int32_t* tile_gids, const int32_t* tile_offsets,
-                               int64_t num_gids, int normalize, double eps, int support_fade,
-                               double sigma_cutoff, float* out, float* den, void* workspace,
+                               int64_t num_gids, int normalize, double eps,
+                               double support_fade_alpha, double sigma_cutoff, float* out,
+                               float* den, void* workspace,
                                size_t workspace_bytes, void* stream);
 
 /* Batched ssplat_backward_tiled. */
@@ -186,7 +193,7 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 const float* colors, const int64_t* radii,
                                 const float* opacities, const int32_t* tile_gids,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
-                                double eps, int support_fade, double sigma_cutoff,
+                                double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,
                                 float* grad_opacities, void* workspace, size_t workspace_bytes,
                                 void* stream);

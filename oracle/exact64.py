@@ -3,8 +3,9 @@
 Every Gaussian is evaluated on every pixel of its clipped support rectangle
 (``tile_index.support_rects``), in float64 from the float32 inputs:
 
-    q = a dx^2 + 2 b dx dy + c dy^2,  raw = exp(-q/2),  tail = exp(-cutoff^2/2)
+    q = a dx^2 + 2 b dx dy + c dy^2,  raw = exp(-q/2),  tail = alpha exp(-cutoff^2/2)
     base = max(raw - tail, 0) under support fade, else raw;   w = base * o
+(alpha = 1 for support_fade=True, a number in [0, 1] for a fractional fade, render.py:94-107)
     N_k = sum w c_k,  D = sum w;   out = N/(D+eps) (normalized) or N (additive)
 
 Backward (the per-pixel chain of render_ext.cu:485-530 and :784-818), with G = dL/dout:
@@ -62,6 +63,14 @@ def _prepare(means, conics, colors, radii, opacities, H, W, rects):
     return m, q, col, op, rect
 
 
+def _tail(support_fade, sigma_cutoff) -> tuple[bool, float]:
+    if isinstance(support_fade, bool):
+        alpha = 1.0 if support_fade else 0.0
+    else:
+        alpha = max(0.0, min(1.0, float(support_fade)))
+    return alpha > 0.0, alpha * math.exp(-0.5 * float(sigma_cutoff) ** 2)
+
+
 def _weights(m, q, op, gid, px, py, fade, tail):
     dx = px.astype(np.float64) - m[gid, 0]
     dy = py.astype(np.float64) - m[gid, 1]
@@ -76,11 +85,11 @@ def accumulate(means, conics, colors, radii, opacities, H: int, W: int, support_
                sigma_cutoff: float, rects=None):
     """Float64 numerator (H*W, 3) and denominator (H*W,) sums."""
     m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
-    tail = math.exp(-0.5 * float(sigma_cutoff) ** 2)
+    fade, tail = _tail(support_fade, sigma_cutoff)
     num = np.zeros((H * W, 3))
     den = np.zeros(H * W)
     for gid, px, py in _visit_chunks(*rect):
-        *_, w = _weights(m, q, op, gid, px, py, support_fade, tail)
+        *_, w = _weights(m, q, op, gid, px, py, fade, tail)
         flat = py * W + px
         for k in range(3):
             num[:, k] += np.bincount(flat, weights=w * col[gid, k], minlength=H * W)
@@ -103,7 +112,7 @@ def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
              sigma_cutoff: float = 3.0, rects=None):
     """Analytic float64 gradients (g_means (N,2), g_conics (N,3), g_colors (N,3), g_op (N,))."""
     m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
-    tail = math.exp(-0.5 * float(sigma_cutoff) ** 2)
+    fade, tail = _tail(support_fade, sigma_cutoff)
     G = np.asarray(grad_out, dtype=np.float64).reshape(H * W, 3)
     if normalize:
         num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
@@ -121,14 +130,14 @@ def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
     g_c = np.zeros((n, 3))
     g_o = np.zeros(n)
     for gid, px, py in _visit_chunks(*rect):
-        dx, dy, a, b, c, raw, base, w = _weights(m, q, op, gid, px, py, support_fade, tail)
+        dx, dy, a, b, c, raw, base, w = _weights(m, q, op, gid, px, py, fade, tail)
         flat = py * W + px
         dw = (dN[flat] * col[gid]).sum(axis=1) + dD[flat]
         for k in range(3):
             g_c[:, k] += np.bincount(gid, weights=dN[flat, k] * w, minlength=n)
         g_o += np.bincount(gid, weights=dw * base, minlength=n)
         dvis = -0.5 * raw
-        if support_fade:
+        if fade:
             dvis = np.where(raw <= tail, 0.0, dvis)
         dq = dw * op[gid] * dvis
         g_m[:, 0] += np.bincount(gid, weights=dq * (-2.0 * a * dx - 2.0 * b * dy), minlength=n)

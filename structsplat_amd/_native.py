@@ -43,10 +43,10 @@ SIGNATURES = {
                                          _c_size, _c_ptr]),
     "ssplat_forward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
                                       _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl,
-                                      _c_int, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+                                      _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64,
                                        _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
-                                       _c_dbl, _c_int, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                       _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                        _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
@@ -56,11 +56,11 @@ SIGNATURES = {
                                                _c_i64, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_forward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
                                             _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                            _c_i64, _c_int, _c_dbl, _c_int, _c_dbl, _c_ptr,
+                                            _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_ptr,
                                             _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_backward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
                                              _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                             _c_ptr, _c_i64, _c_int, _c_dbl, _c_int, _c_dbl,
+                                             _c_ptr, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl,
                                              _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
                                              _c_ptr]),
 }

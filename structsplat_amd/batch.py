@@ -27,7 +27,7 @@ from . import _native
 from . import fit as F
 from .gaussians import GaussianField
 from .metrics import C1, C2, gaussian_window, psnr_tensor
-from .render import _ptr, _stream
+from .render import _fade_alpha, _ptr, _stream
 
 
 class Batch:
@@ -105,7 +105,7 @@ class Batch:
         _native.check(lib.ssplat_batch_forward_tiled(
             *self._head(), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
             _ptr(opacities), _ptr(gids), _ptr(offsets), gids.numel(), int(bool(normalize)),
-            float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(out), _ptr(den),
+            float(eps), _fade_alpha(support_fade), float(sigma_cutoff), _ptr(out), _ptr(den),
             _ptr(self.workspace), self.workspace.numel(), _stream(self.device)),
             "ssplat_batch_forward_tiled")
         return out
@@ -121,7 +121,7 @@ class Batch:
         _native.check(lib.ssplat_batch_backward_tiled(
             *self._head(), _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
             _ptr(opacities), _ptr(gids), _ptr(offsets), gids.numel(), int(bool(normalize)),
-            float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(g_m), _ptr(g_q),
+            float(eps), _fade_alpha(support_fade), float(sigma_cutoff), _ptr(g_m), _ptr(g_q),
             _ptr(g_c), _ptr(g_o), _ptr(self.workspace), self.workspace.numel(),
             _stream(self.device)), "ssplat_batch_backward_tiled")
         return g_m, g_q, g_c, g_o
@@ -146,7 +146,8 @@ class _BatchRender(Function):
         b = plan.batch
         means, conics, colors = means.contiguous(), conics.contiguous(), colors.contiguous()
         radii, opacities = radii.contiguous(), opacities.contiguous()
-        b.build_tile_index(means, conics, radii, plan.capacity, fade, sigma_cutoff,
+        b.build_tile_index(means, conics, radii, plan.capacity, _fade_alpha(fade) == 1.0,
+                           sigma_cutoff,
                            out=(plan.gids, plan.offsets, plan.status))
         out = b.forward(means, conics, colors, radii, opacities, plan.gids, plan.offsets,
                         normalize, eps, fade, sigma_cutoff)
@@ -260,7 +261,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     def count():
         with torch.no_grad():
             return batch.pair_count(big.means, big.conics(), big.radii(cfg.sigma_cutoff),
-                                    cfg.support_fade, cfg.sigma_cutoff)
+                                    _fade_alpha(cfg.support_fade) == 1.0, cfg.sigma_cutoff)
 
     plan = BatchPlan(batch, int(count() * cfg.capacity_slack) + 4096)
 
@@ -270,7 +271,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
         op = big.means.new_empty(0) if op is None else op
         img = _BatchRender.apply(big.means, big.conics(), big.colors,
                                  big.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
-                                 float(cfg.normalization_eps), bool(cfg.support_fade),
+                                 float(cfg.normalization_eps), _fade_alpha(cfg.support_fade),
                                  float(cfg.sigma_cutoff))
         loss = (1.0 - w) * ((img - targets_flat).abs() * npix).sum()
         if w != 0.0:
@@ -322,7 +323,8 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
         op = big.means.new_empty(0) if op is None else op
         gids, offsets, _ = batch.build_tile_index(big.means, big.conics(),
                                                   big.radii(cfg.sigma_cutoff), plan.capacity,
-                                                  cfg.support_fade, cfg.sigma_cutoff)
+                                                  _fade_alpha(cfg.support_fade) == 1.0,
+                                                  cfg.sigma_cutoff)
         img = batch.forward(big.means, big.conics(), big.colors, big.radii(cfg.sigma_cutoff), op,
                             gids, offsets, cfg.normalize, cfg.normalization_eps,
                             cfg.support_fade, cfg.sigma_cutoff)

@@ -13,7 +13,8 @@
 //   * tiled backward : recomputes the per-pixel numerator/denominator (nothing is saved by the
 //                      forward), turns them into per-pixel adjoints in LDS, then sweeps the tile
 //                      list again with each wave owning a quarter of the Gaussians over all 256
-//                      pixels (four per lane), DPP wave reductions, one atomic set per
+//                      pixels (four per lane), a transposed permlane/DPP wave reduction that
+//                      leaves one gradient component per lane, one atomic set per
 //                      (tile, Gaussian).
 // No MFMA: the path is a gather/accumulate, not a contraction.
 
@@ -183,6 +184,42 @@ __device__ __forceinline__ float wave_sum_lane63(float v) {
   v += dpp_move<0x142>(v);  // row_bcast:15
   v += dpp_move<0x143>(v);  // row_bcast:31
   return v;
+}
+
+// x + y after exchanging the upper 32 lanes of x with the lower 32 lanes of y
+// (v_permlane32_swap): lanes 0-31 get x(l) + x(l+32), lanes 32-63 get y(l-32) + y(l).
+__device__ __forceinline__ float swap32_add(float x, float y) {
+  const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                  false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+
+// Same with the odd 16-lane rows of x exchanged with the even rows of y (v_permlane16_swap):
+// row 0 gets x's rows 0+1, row 1 y's rows 0+1, row 2 x's rows 2+3, row 3 y's rows 2+3.
+__device__ __forceinline__ float swap16_add(float x, float y) {
+  const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                  false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+
+// Transposed reduction of eight per-lane values over a wave64: on return, lane 8k+7 holds the
+// wave sum of v[k].  Each step halves the lanes a value spans while pairing values, so eight
+// values cost 18 VALU operations instead of 8 independent 6-step trees.  `hi8` is
+// (lane & 8) != 0.  Must be called with all lanes active.
+__device__ __forceinline__ float wave_sum8_transposed(const float (&v)[8], bool hi8) {
+  const float a0 = swap32_add(v[0], v[4]);  // lanes 0-31: v0, 32-63: v4
+  const float a1 = swap32_add(v[1], v[5]);
+  const float a2 = swap32_add(v[2], v[6]);
+  const float a3 = swap32_add(v[3], v[7]);
+  const float b0 = swap16_add(a0, a2);      // rows: v0, v2, v4, v6
+  const float b1 = swap16_add(a1, a3);      // rows: v1, v3, v5, v7
+  const float t0 = b0 + dpp_move<0x128>(b0);  // row_ror:8
+  const float t1 = b1 + dpp_move<0x128>(b1);
+  float c = hi8 ? t1 : t0;                   // lanes 0-7 of a row: b0's value, 8-15: b1's
+  c += dpp_move<0xb1>(c);                    // quad_perm [1,0,3,2]
+  c += dpp_move<0x4e>(c);                    // quad_perm [2,3,0,1]
+  c += dpp_move<0x114>(c);                   // row_shr:4 -> lanes 7 and 15 of each row
+  return c;
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t count, int32_t value) {
@@ -594,7 +631,8 @@ struct RenderArgs {
   int n;  // number of records; list entries outside [0, n) stage as empty records
   FieldTable fields;
   float eps;
-  float tail_exp;  // sigma_cutoff^2 * kNegHalfLog2e
+  float tail_exp;    // sigma_cutoff^2 * kNegHalfLog2e
+  float fade_alpha;  // support-fade strength in [0, 1]; the faded tail is alpha * exp(-cutoff^2/2)
 };
 
 // Clamped [start, end) of a tile's list.
@@ -722,7 +760,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
   const int qy0 = tile_y * kTile + (wave >> 1) * 8;
   const int px = qx0 + (lane & 7);
   const int py = qy0 + (lane >> 3);
-  const float tail = kFade ? exp2_hw(a.tail_exp) : 0.0f;
+  const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
   accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
@@ -764,7 +802,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   const int ly = lane >> 3;
   const int tx0 = tile_x * kTile;
   const int ty0 = tile_y * kTile;
-  const float tail = kFade ? exp2_hw(a.tail_exp) : 0.0f;
+  const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
 
   // Phase 1: recompute N_k and D for one pixel per lane (same code and order as the forward),
   // then the per-pixel adjoints dL/dN_k and dL/dD.
@@ -803,6 +841,15 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
     adj[s] = s_adj[((s >> 1) * 8 + ly) * kTile + (s & 1) * 8 + lx];
   }
   const bool has_opacity = g.grad_opacities != nullptr;
+  // Lane roles of the transposed reduction: lane 8k+7 owns gradient component k
+  // (0-1 mean x/y, 2-4 conic a/b/c, 5-7 color r/g/b).
+  const bool hi8 = (lane & 8) != 0;
+  const int comp = lane >> 3;
+  const bool result_lane = (lane & 7) == 7;
+  float* const comp_base = comp < 2 ? g.grad_means : (comp < 5 ? g.grad_conics : g.grad_colors);
+  const int comp_stride = comp < 2 ? 2 : 3;
+  const int comp_offset = comp < 2 ? comp : (comp < 5 ? comp - 2 : comp - 5);
+  const float comp_scale = (comp == 2 || comp == 4) ? -0.5f : (comp == 3 ? -1.0f : 1.0f);
   int64_t start, end;
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kStage) {
@@ -858,37 +905,25 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
           syy = fmaf(dqr * dy, dy, syy);
         }
       }
-      sx = wave_sum_lane63(sx);
-      sy = wave_sum_lane63(sy);
-      sxx = wave_sum_lane63(sxx);
-      sxy = wave_sum_lane63(sxy);
-      syy = wave_sum_lane63(syy);
-      gcr = wave_sum_lane63(gcr);
-      gcg = wave_sum_lane63(gcg);
-      gcb = wave_sum_lane63(gcb);
-      gop = wave_sum_lane63(gop);
-      if (lane == 63) {
-        // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries.
-        const int gid = __float_as_int(v2.w);
-        const float op = v1.y;
-        const float cb = 0.5f * cb2;
-        const float gmx = op * (ca * sx + cb * sy);
-        const float gmy = op * (cb * sx + cc * sy);
-        const float ga = -0.5f * op * sxx;
-        const float gb = -op * sxy;
-        const float gc = -0.5f * op * syy;
-        float* gm = g.grad_means + 2 * int64_t(gid);
-        float* gq = g.grad_conics + 3 * int64_t(gid);
-        float* gk = g.grad_colors + 3 * int64_t(gid);
-        if (gmx != 0.0f) unsafeAtomicAdd(gm + 0, gmx);
-        if (gmy != 0.0f) unsafeAtomicAdd(gm + 1, gmy);
-        if (ga != 0.0f) unsafeAtomicAdd(gq + 0, ga);
-        if (gb != 0.0f) unsafeAtomicAdd(gq + 1, gb);
-        if (gc != 0.0f) unsafeAtomicAdd(gq + 2, gc);
-        if (gcr != 0.0f) unsafeAtomicAdd(gk + 0, op * gcr);
-        if (gcg != 0.0f) unsafeAtomicAdd(gk + 1, op * gcg);
-        if (gcb != 0.0f) unsafeAtomicAdd(gk + 2, op * gcb);
-        if (has_opacity && gop != 0.0f) unsafeAtomicAdd(g.grad_opacities + gid, gop);
+      // Reduce the eight partials to lanes 8k+7 (k = component), the opacity partial to lane 63.
+      const float vals[8] = {sx, sy, sxx, sxy, syy, gcr, gcg, gcb};
+      const float red = wave_sum8_transposed(vals, hi8);
+      const float partner = dpp_move<0x128>(red);  // lane 7 <-> lane 15: Sx and Sy meet
+      const float gop_sum = has_opacity ? wave_sum_lane63(gop) : 0.0f;
+      // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries:
+      //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
+      //   g_c = -op/2 Syy, g_color = op C.
+      const int gid = __float_as_int(v2.w);
+      const float op = v1.y;
+      const float cb = 0.5f * cb2;
+      const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
+      const float cross = comp < 2 ? partner * cb : 0.0f;
+      const float gval = op * fmaf(red, own_scale, cross);
+      if (result_lane && gval != 0.0f) {
+        unsafeAtomicAdd(comp_base + comp_stride * int64_t(gid) + comp_offset, gval);
+      }
+      if (has_opacity && lane == 63 && gop_sum != 0.0f) {
+        unsafeAtomicAdd(g.grad_opacities + gid, gop_sum);
       }
     }
     __syncthreads();
@@ -1110,6 +1145,15 @@ int tile_index_fill(const float* means, const float* conics, const int64_t* radi
   return success();
 }
 
+// Support-fade strength, clamped to [0, 1] like the reference (render.py:94-98).
+int check_fade(double alpha, double* out) {
+  if (!std::isfinite(alpha)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "support_fade_alpha must be finite");
+  }
+  *out = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+  return SSPLAT_OK;
+}
+
 int check_render_inputs(const float* means, const float* conics, const float* colors,
                         const int64_t* radii, const Plan& p, const int32_t* gids,
                         const int32_t* offsets, int64_t num_gids, int normalize, double eps) {
@@ -1132,7 +1176,7 @@ int check_render_inputs(const float* means, const float* conics, const float* co
 
 RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gids,
                             const int32_t* offsets, int64_t num_gids, double eps,
-                            double sigma_cutoff) {
+                            double fade_alpha, double sigma_cutoff) {
   RenderArgs r;
   r.recs = w.recs;
   r.gids = gids;
@@ -1142,6 +1186,7 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
   r.fields = p.table;
   r.eps = static_cast<float>(eps);
   r.tail_exp = static_cast<float>(sigma_cutoff * sigma_cutoff) * kNegHalfLog2e;
+  r.fade_alpha = static_cast<float>(fade_alpha);
   return r;
 }
 
@@ -1158,9 +1203,12 @@ int run_prepare(const float* means, const float* conics, const float* colors,
 int forward_tiled(const Plan& p, const float* means, const float* conics, const float* colors,
                   const int64_t* radii, const float* opacities, const int32_t* gids,
                   const int32_t* offsets, int64_t num_gids, int normalize, double eps,
-                  int support_fade, double sigma_cutoff, float* out, float* den, void* ws,
-                  size_t ws_bytes, void* stream) {
+                  double support_fade_alpha, double sigma_cutoff, float* out, float* den,
+                  void* ws, size_t ws_bytes, void* stream) {
   int rc;
+  double fade_alpha;
+  if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
+  const bool support_fade = fade_alpha > 0.0;
   if ((rc = check_render_inputs(means, conics, colors, radii, p, gids, offsets, num_gids,
                                 normalize, eps))) {
     return rc;
@@ -1170,7 +1218,7 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, sigma_cutoff);
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
   if (support_fade && normalize) {
     k_forward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
   } else if (support_fade) {
@@ -1187,10 +1235,13 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
 int backward_tiled(const Plan& p, const float* grad_out, const float* means, const float* conics,
                    const float* colors, const int64_t* radii, const float* opacities,
                    const int32_t* gids, const int32_t* offsets, int64_t num_gids, int normalize,
-                   double eps, int support_fade, double sigma_cutoff, float* grad_means,
-                   float* grad_conics, float* grad_colors, float* grad_opacities, void* ws,
-                   size_t ws_bytes, void* stream) {
+                   double eps, double support_fade_alpha, double sigma_cutoff,
+                   float* grad_means, float* grad_conics, float* grad_colors,
+                   float* grad_opacities, void* ws, size_t ws_bytes, void* stream) {
   int rc;
+  double fade_alpha;
+  if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
+  const bool support_fade = fade_alpha > 0.0;
   if ((rc = check_render_inputs(means, conics, colors, radii, p, gids, offsets, num_gids,
                                 normalize, eps))) {
     return rc;
@@ -1209,7 +1260,7 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
   if (gop && (rc = fill_f32(gop, p.n, 0.0f, s))) return rc;
   if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, sigma_cutoff);
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
   GradArgs ga{grad_out, grad_means, grad_conics, grad_colors, gop};
   if (support_fade && normalize) {
     k_backward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
@@ -1287,28 +1338,29 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          const int64_t* radii, const float* opacities, int64_t n,
                          const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
                          int height, int width, int tile_size, int normalize, double eps,
-                         int support_fade, double sigma_cutoff, float* out, float* den,
-                         void* workspace, size_t workspace_bytes, void* stream) {
+                         double support_fade_alpha, double sigma_cutoff, float* out,
+                         float* den, void* workspace, size_t workspace_bytes, void* stream) {
   Plan p;
   const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
   return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
-                       num_gids, normalize, eps, support_fade, sigma_cutoff, out, den, workspace,
-                       workspace_bytes, stream);
+                       num_gids, normalize, eps, support_fade_alpha, sigma_cutoff, out, den,
+                       workspace, workspace_bytes, stream);
 }
 
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
                           const float* colors, const int64_t* radii, const float* opacities,
                           int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
-                          double eps, int support_fade, double sigma_cutoff, float* grad_means,
+                          double eps, double support_fade_alpha, double sigma_cutoff,
+                          float* grad_means,
                           float* grad_conics, float* grad_colors, float* grad_opacities,
                           void* workspace, size_t workspace_bytes, void* stream) {
   Plan p;
   const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
   return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
-                        tile_offsets, num_gids, normalize, eps, support_fade, sigma_cutoff,
+                        tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, workspace,
                         workspace_bytes, stream);
 }
@@ -1368,15 +1420,16 @@ int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
                                const float* means, const float* conics, const float* colors,
                                const int64_t* radii, const float* opacities,
                                const int32_t* tile_gids, const int32_t* tile_offsets,
-                               int64_t num_gids, int normalize, double eps, int support_fade,
-                               double sigma_cutoff, float* out, float* den, void* workspace,
+                               int64_t num_gids, int normalize, double eps,
+                               double support_fade_alpha, double sigma_cutoff, float* out,
+                               float* den, void* workspace,
                                size_t workspace_bytes, void* stream) {
   Plan p;
   const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
   if (rc) return rc;
   return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
-                       num_gids, normalize, eps, support_fade, sigma_cutoff, out, den, workspace,
-                       workspace_bytes, stream);
+                       num_gids, normalize, eps, support_fade_alpha, sigma_cutoff, out, den,
+                       workspace, workspace_bytes, stream);
 }
 
 int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
@@ -1385,7 +1438,7 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 const float* colors, const int64_t* radii,
                                 const float* opacities, const int32_t* tile_gids,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
-                                double eps, int support_fade, double sigma_cutoff,
+                                double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,
                                 float* grad_opacities, void* workspace, size_t workspace_bytes,
                                 void* stream) {
@@ -1393,7 +1446,7 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
   const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
   if (rc) return rc;
   return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
-                        tile_offsets, num_gids, normalize, eps, support_fade, sigma_cutoff,
+                        tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, workspace,
                         workspace_bytes, stream);
 }

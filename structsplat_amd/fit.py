@@ -41,7 +41,7 @@ class FitConfig:
     lr_opacity: float = 5e-3
     ssim_weight: float = 0.2       # loss = (1 - w) * L1 + w * (1 - SSIM)
     sigma_cutoff: float = 3.0
-    support_fade: bool = False
+    support_fade: bool | float = False  # bool, or a fade strength in [0, 1]
     normalization_eps: float = 1e-8
     normalize: bool = True
     window: int = 25               # steps between capacity checks
@@ -76,7 +76,7 @@ class _PlannedRender(Function):
         colors = colors.contiguous()
         radii = radii.contiguous()
         opacities = opacities.contiguous()
-        cull = bool(fade)
+        cull = HR._fade_alpha(fade) == 1.0
         gids, offsets, _ = HR.build_tile_index_capacity(
             means, conics, radii, plan.H, plan.W, plan.capacity, plan.tile_size, cull,
             sigma_cutoff, out=plan.buffers())
@@ -109,7 +109,7 @@ def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> to
         op = field.means.new_empty(0)
     return _PlannedRender.apply(field.means, field.conics(), field.colors,
                                 field.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
-                                float(cfg.normalization_eps), bool(cfg.support_fade),
+                                float(cfg.normalization_eps), HR._fade_alpha(cfg.support_fade),
                                 float(cfg.sigma_cutoff))
 
 
@@ -126,7 +126,8 @@ def pair_count(field: GaussianField, H: int, W: int, cfg: FitConfig, tile_size: 
     """Exact (tile, Gaussian) pair count of the current field (one host readback)."""
     with torch.no_grad():
         _, offsets = HR.build_tile_index(field.means, field.conics(), field.radii(cfg.sigma_cutoff),
-                                         H, W, tile_size, bool(cfg.support_fade),
+                                         H, W, tile_size,
+                                         HR._fade_alpha(cfg.support_fade) == 1.0,
                                          cfg.sigma_cutoff)
     return int(offsets[-1].item())
 

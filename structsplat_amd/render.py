@@ -106,6 +106,13 @@ def _check_image(height: int, width: int, tile_size: int) -> None:
         raise ValueError("tile_size must be positive with tile_size^2 <= 1024")
 
 
+def _fade_alpha(support_fade) -> float:
+    """Fade strength for the C ABI: a bool maps to 0/1, a number is clamped to [0, 1]."""
+    if isinstance(support_fade, bool):
+        return 1.0 if support_fade else 0.0
+    return max(0.0, min(1.0, float(support_fade)))
+
+
 def _radii64(radii: torch.Tensor) -> torch.Tensor:
     return radii.detach().to(torch.int64).contiguous()
 
@@ -208,7 +215,7 @@ def forward_tiled(means, conics, colors, radii, opacities, gids, offsets, H: int
     _native.check(lib.ssplat_forward_tiled(
         _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities), n, _ptr(gids),
         _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size), int(bool(normalize)),
-        float(eps), int(bool(support_fade)), float(sigma_cutoff), _ptr(out), _ptr(den),
+        float(eps), _fade_alpha(support_fade), float(sigma_cutoff), _ptr(out), _ptr(den),
         _ptr(ws), ws.numel(), _stream(dev)), "ssplat_forward_tiled")
     return out, den
 
@@ -230,7 +237,7 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, gids, offs
     _native.check(lib.ssplat_backward_tiled(
         _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
         n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size),
-        int(bool(normalize)), float(eps), int(bool(support_fade)), float(sigma_cutoff),
+        int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
         _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac), _ptr(ws), ws.numel(),
         _stream(dev)), "ssplat_backward_tiled")
     return g_means, g_conics, g_colors, g_opac
@@ -248,21 +255,23 @@ class _ExactRenderHip(Function):
         colors = colors.contiguous()
         radii = _radii64(radii)
         opacities = opacities.contiguous()
-        # Culling is exact only under support fade (cuda_render.py:80-85).
-        cull = bool(tile_ellipse_cull) and bool(support_fade)
+        # Culling is exact only under full support fade (cuda_render.py:80-85): with alpha = 1
+        # the faded weight is exactly 0 beyond sigma_cutoff.
+        alpha = _fade_alpha(support_fade)
+        cull = bool(tile_ellipse_cull) and alpha == 1.0
         gids, offsets = build_tile_index(means, conics, radii, int(height), int(width),
                                          int(tile_size), cull, float(sigma_cutoff))
         ws = workspace(means.shape[0], height, width, tile_size, means.device)
         out, _ = forward_tiled(means, conics, colors, radii, opacities, gids, offsets,
                                int(height), int(width), bool(normalize), float(eps),
-                               int(tile_size), bool(support_fade), float(sigma_cutoff), ws=ws)
+                               int(tile_size), alpha, float(sigma_cutoff), ws=ws)
         ctx.save_for_backward(means, conics, colors, radii, opacities, gids, offsets)
         ctx.height = int(height)
         ctx.width = int(width)
         ctx.normalize = bool(normalize)
         ctx.eps = float(eps)
         ctx.tile_size = int(tile_size)
-        ctx.support_fade = bool(support_fade)
+        ctx.support_fade = alpha
         ctx.sigma_cutoff = float(sigma_cutoff)
         return out
 
@@ -291,11 +300,14 @@ class _ExactRenderHip(Function):
 
 def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=None,
                      normalize: bool = True, eps: float = 1e-8, tiled: bool = True,
-                     tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool = False,
+                     tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool | float = False,
                      sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
                      tile_index_backend: str = "hip", tile_ellipse_cull: bool = True):
     """Exact normalized/additive render on the GPU (signature of ``render_cuda_exact``,
     cuda_render.py:256-294).
+
+    ``support_fade`` is the reference's bool (full fade) or a fade strength in [0, 1]
+    (render.py:94-107).
 
     ``tiled`` and ``backward_variant`` are accepted for drop-in compatibility: the HIP path is
     always tiled and has one backward.  ``tile_index_backend`` accepts ``"hip"`` (or the
@@ -314,14 +326,16 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
     if opacities is None:
         opacities = means.new_empty(0)
     return _ExactRenderHip.apply(means, conics, colors, radii, opacities, int(H), int(W),
-                                 bool(normalize), float(eps), int(tile_size), bool(support_fade),
-                                 float(sigma_cutoff), bool(tile_ellipse_cull))
+                                 bool(normalize), float(eps), int(tile_size),
+                                 _fade_alpha(support_fade), float(sigma_cutoff),
+                                 bool(tile_ellipse_cull))
 
 
 @torch.no_grad()
 def render_hip_exact_with_coverage(means, conics, colors, radii, H: int, W: int,
                                    opacities=None, eps: float = 1e-8,
-                                   support_fade: bool = False, sigma_cutoff: float = 3.0):
+                                   support_fade: bool | float = False,
+                                   sigma_cutoff: float = 3.0):
     """Normalized RGB and raw coverage ``den = sum o G`` (no eps) from one forward, no autograd
     (``render_cuda_exact_with_coverage``, cuda_render.py:229-253)."""
     if not means.is_cuda:
@@ -341,7 +355,7 @@ def render_hip_exact_with_coverage(means, conics, colors, radii, H: int, W: int,
                                      False, float(sigma_cutoff))
     den = torch.empty(int(H), int(W), dtype=torch.float32, device=means.device)
     out, den = forward_tiled(means, conics, colors, radii, opacities, gids, offsets, int(H),
-                             int(W), True, eps, DEFAULT_TILE_SIZE, bool(support_fade),
+                             int(W), True, eps, DEFAULT_TILE_SIZE, _fade_alpha(support_fade),
                              float(sigma_cutoff), den=den)
     return out, den
 
@@ -356,8 +370,9 @@ def render_field(means, conics, colors, radii, H: int, W: int,
     """Mode dispatcher with the reference's signature (render.py:319-414).
 
     ``chunk`` and ``checkpoint_chunks`` bound the reference renderer's memory and have no effect
-    here.  Affine colors (``color_grads``) and fractional fade strengths are not implemented by
-    the HIP kernels and raise ``ValueError`` instead of silently changing the renderer.
+    here.  Fractional fade strengths run in the HIP kernels (the reference falls back to torch for
+    them, render.py:341-358).  Affine colors (``color_grads``) are not implemented by the HIP
+    kernels and raise ``ValueError`` instead of silently changing the renderer.
     """
     del chunk, checkpoint_chunks
     fade_alpha = _resolve_support_fade_alpha(support_fade, support_fade_alpha)
@@ -380,15 +395,11 @@ def render_field(means, conics, colors, radii, H: int, W: int,
             "affine color rendering (color_grads) is not implemented by the HIP renderer; "
             f"got renderer={mode!r}"
         )
-    if fade_alpha not in (0.0, 1.0):
-        raise ValueError(
-            f"fractional support_fade_alpha={fade_alpha} is not implemented by the HIP renderer"
-        )
     if mode in _NORMALIZED_MODES:
         eps = _validate_normalization_eps(normalization_eps)
         return render_hip_exact(means, conics, colors, radii, H, W, opacities=opacities,
-                                normalize=True, eps=eps, support_fade=fade_alpha > 0.0,
+                                normalize=True, eps=eps, support_fade=fade_alpha,
                                 sigma_cutoff=sigma_cutoff)
     return render_hip_exact(means, conics, colors, radii, H, W, opacities=opacities,
-                            normalize=False, eps=_EPS, support_fade=fade_alpha > 0.0,
+                            normalize=False, eps=_EPS, support_fade=fade_alpha,
                             sigma_cutoff=sigma_cutoff)

@@ -45,7 +45,7 @@ def test_tile_index_reference_golden(cuda):
 
 
 @pytest.mark.parametrize("normalize", [True, False])
-@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("fade", [False, True, 0.5])
 @pytest.mark.parametrize("opacity", [False, True])
 def test_forward_matches_fp64(cuda, normalize, fade, opacity):
     _print_case(test="forward", normalize=normalize, fade=fade, opacity=opacity)
@@ -65,7 +65,7 @@ def test_forward_matches_fp64(cuda, normalize, fade, opacity):
 
 
 @pytest.mark.parametrize("normalize", [True, False])
-@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("fade", [False, True, 0.5])
 def test_backward_matches_fp64(cuda, normalize, fade):
     _print_case(test="backward", normalize=normalize, fade=fade)
     N, H, W = 250, 61, 77
@@ -119,3 +119,32 @@ def test_forward_is_deterministic(cuda):
     b = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+def test_capacity_status_is_sticky_and_exact_capacity_is_enough(cuda):
+    N, H, W = 500, 80, 96
+    f = random_field(17, N, H, W, scale_lo=0.7, scale_hi=7.0)
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    ref_g, ref_o = HR.build_tile_index(means, conics, radii, H, W)
+    P = int(ref_o[-1])
+    gids, offsets, status = HR.build_tile_index_capacity(means, conics, radii, H, W, P)
+    torch.cuda.synchronize()
+    assert int(status.item()) == 0
+    assert torch.equal(offsets, ref_o) and torch.equal(gids[:P], ref_g)
+    small = HR.build_tile_index_capacity(means, conics, radii, H, W, P // 2)
+    torch.cuda.synchronize()
+    assert int(small[2].item()) == 2  # capacity exceeded
+    # The word stays set after a later binning that fits (sticky until the caller clears it).
+    HR.build_tile_index_capacity(means, conics, radii, H, W, P, out=(gids, offsets, small[2],
+                                 HR.workspace(N, H, W, 16, cuda)))
+    torch.cuda.synchronize()
+    assert int(small[2].item()) == 2
+
+
+def test_render_kernels_reject_other_tile_sizes(cuda):
+    f = random_field(1, 10, 16, 16)
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    gids, offsets = HR.build_tile_index(means, conics, radii, 16, 16, 8)
+    with pytest.raises(ValueError, match="tile_size"):
+        HR.forward_tiled(means, conics, colors, radii, None, gids, offsets, 16, 16, True, 1e-8,
+                         8, False, 3.0)

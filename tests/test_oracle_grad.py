@@ -13,7 +13,7 @@ from tests.helpers import field_tensors, random_field, to_np
 
 
 @pytest.mark.parametrize("normalize", [True, False])
-@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("fade", [False, True, 0.6])
 @pytest.mark.parametrize("opacity", [False, True])
 def test_analytic_backward_matches_float64_autograd(normalize, fade, opacity):
     H, W = 23, 29
@@ -23,7 +23,8 @@ def test_analytic_backward_matches_float64_autograd(normalize, fade, opacity):
     leaves = [means.double().requires_grad_(), conics.double().requires_grad_(),
               colors.double().requires_grad_()]
     opd = op.double().requires_grad_() if op is not None else None
-    img = R.accumulate(*leaves, radii, H, W, 1, opd, normalize, fade, 3.0)
+    alpha = None if isinstance(fade, bool) else fade
+    img = R.accumulate(*leaves, radii, H, W, 1, opd, normalize, bool(fade), 3.0, alpha)
     (img * torch.from_numpy(G)).sum().backward()
     got = exact64.backward(G, to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op),
                            H, W, normalize, 1e-8, fade, 3.0)

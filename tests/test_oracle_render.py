@@ -193,13 +193,14 @@ def test_slice_budget_does_not_change_the_image():
 
 
 @pytest.mark.parametrize("normalize", [True, False])
-@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("fade", [False, True, 0.4])
 def test_exact64_forward_equals_torch_restatement(normalize, fade):
     """Two independent float64 implementations (index_add over slices vs bincount over chunks)."""
     f = random_field(9, 500, 45, 61, scale_lo=0.6, scale_hi=7.0, margin=5.0, opacity=True)
     means, conics, colors, radii, op = field_tensors(f, "cpu")
+    alpha = None if isinstance(fade, bool) else fade
     ref = R.accumulate(means.double(), conics.double(), colors.double(), radii, 45, 61, 1,
-                       op.double(), normalize, fade, 3.0)
+                       op.double(), normalize, bool(fade), 3.0, alpha)
     out, _ = exact64.forward(to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op),
                              45, 61, normalize, 1e-8, fade, 3.0)
     assert np.allclose(out, ref.numpy(), rtol=1e-12, atol=1e-14)

@@ -1,0 +1,64 @@
+"""Host-side contract of render_field / render_hip_exact (no GPU needed): the reference's error
+phrases and argument validation (test_render.py:136-164, test_normalization_epsilon.py:33-67)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from structsplat_amd import GaussianField, render_field, render_hip_exact
+from structsplat_amd.render import _resolve_support_fade_alpha, _validate_normalization_eps
+
+
+def _field():
+    g = GaussianField.from_numpy(np.array([[3.0, 3.0]]), np.full((1, 2), 1.5), np.zeros(1),
+                                 np.array([[1.0, 0, 0]]))
+    return g.means, g.conics(), g.colors, g.radii(3.0)
+
+
+@pytest.mark.parametrize("mode", ["hip", "normalized", "additive", "cuda", "cuda_tiled",
+                                  "cuda_block_reduce", "cuda_additive", "cuda_tiled_additive"])
+def test_cpu_tensors_are_rejected_with_the_reference_phrase(mode):
+    with pytest.raises(RuntimeError, match="requires CUDA tensors"):
+        render_field(*_field(), 10, 10, mode=mode)
+
+
+def test_gsplat_requires_scales_and_rotations():
+    with pytest.raises(ValueError, match="requires scales and rotations"):
+        render_field(*_field(), 10, 10, mode="gsplat")
+
+
+def test_unknown_mode_is_a_value_error():
+    with pytest.raises(ValueError, match="unknown renderer"):
+        render_field(*_field(), 10, 10, mode="nope")
+
+
+@pytest.mark.parametrize("value", [0.0, -1e-12, float("inf"), float("nan")])
+def test_normalization_eps_validation(value):
+    with pytest.raises(ValueError, match="normalization_eps must be finite and > 0"):
+        _validate_normalization_eps(value)
+    assert _validate_normalization_eps(1e-12) == 1e-12
+
+
+def test_render_hip_exact_rejects_cpu_and_non_float32():
+    m, q, c, r = _field()
+    with pytest.raises(RuntimeError, match="requires CUDA tensors"):
+        render_hip_exact(m, q, c, r, 10, 10)
+
+
+def test_fade_alpha_resolution_matches_reference():
+    # render.py:94-98
+    assert _resolve_support_fade_alpha(False) == 0.0
+    assert _resolve_support_fade_alpha(True) == 1.0
+    assert _resolve_support_fade_alpha(True, 0.25) == 0.25
+    assert _resolve_support_fade_alpha(False, 7.0) == 1.0
+    assert _resolve_support_fade_alpha(False, -1.0) == 0.0
+
+
+def test_field_producer_dilation_validation():
+    g = GaussianField.from_numpy(np.zeros((2, 2)), np.ones((2, 2)), np.zeros(2), np.zeros((2, 3)))
+    with pytest.raises(ValueError, match="dilation must be >= 0"):
+        g.conics(-1.0)
+    with pytest.raises(ValueError, match="dilation must be >= 0"):
+        g.radii(3.0, -0.5)
+    assert g.radii(3.0).dtype == torch.int64

@@ -263,7 +263,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
             return batch.pair_count(big.means, big.conics(), big.radii(cfg.sigma_cutoff),
                                     _fade_alpha(cfg.support_fade) == 1.0, cfg.sigma_cutoff)
 
-    plan = BatchPlan(batch, int(count() * cfg.capacity_slack) + 4096)
+    plan = BatchPlan(batch, F._capacity(count(), cfg))
 
     def step():
         opt.zero_grad(set_to_none=False)
@@ -308,13 +308,13 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                 step()
         if int(plan.status.item()) != 0:
             snap.restore(big, opt)
-            plan = BatchPlan(batch, 2 * int(count() * cfg.capacity_slack) + 4096)
+            plan = BatchPlan(batch, max(2 * plan.capacity, F._capacity(count(), cfg)))
             g = None
             continue
         done += n
         p = count()
         if p * 1.15 > plan.capacity:
-            plan = BatchPlan(batch, int(p * cfg.capacity_slack) + 4096)
+            plan = BatchPlan(batch, F._headroom(p, cfg))
             g = None
     torch.cuda.synchronize(dev)
     seconds = time.perf_counter() - t0

@@ -46,6 +46,7 @@ class FitConfig:
     normalize: bool = True
     window: int = 25               # steps between capacity checks
     capacity_slack: float = 1.5    # pair-buffer capacity / current pair count
+    capacity_floor: int = 4096     # minimum pair-buffer capacity
 
 
 class RenderPlan:
@@ -207,7 +208,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
                 step()
         if int(plan.status.item()) != 0:
             snap.restore(field, opt)
-            plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg) * 2,
+            plan = RenderPlan(field.n, H, W,
+                              max(2 * plan.capacity, _capacity(pair_count(field, H, W, cfg), cfg)),
                               dev)
             graph_obj = None
             regrows += 1
@@ -216,7 +218,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         # Keep headroom: regrow (and re-capture) when the pair count approaches capacity.
         p = pair_count(field, H, W, cfg)
         if p * 1.15 > plan.capacity:
-            plan = RenderPlan(field.n, H, W, _capacity(p, cfg), dev)
+            plan = RenderPlan(field.n, H, W, _headroom(p, cfg), dev)
             graph_obj = None
             regrows += 1
         if verbose:
@@ -233,7 +235,12 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
 
 
 def _capacity(pairs: int, cfg: FitConfig) -> int:
-    return int(pairs * cfg.capacity_slack) + 4096
+    return max(int(pairs * cfg.capacity_slack), cfg.capacity_floor, 1)
+
+
+def _headroom(pairs: int, cfg: FitConfig) -> int:
+    """Capacity after a regrow: the configured slack, but never less than 1.3x the pairs."""
+    return max(_capacity(pairs, cfg), int(pairs * 1.3) + 1)
 
 
 def _capture(step) -> torch.cuda.CUDAGraph:

@@ -86,7 +86,8 @@ def test_capacity_overflow_rolls_back_and_matches(cuda):
     a, target = _problem(cuda, seed=7)
     b, _ = _problem(cuda, seed=7)
     ok = F.fit(a, target, F.FitConfig(iters=12, window=4), graph=False)
-    tight = F.fit(b, target, F.FitConfig(iters=12, window=4, capacity_slack=0.5), graph=False)
+    tight = F.fit(b, target, F.FitConfig(iters=12, window=4, capacity_slack=0.5,
+                                         capacity_floor=1), graph=False)
     assert tight["regrows"] >= 1
     assert abs(ok["psnr"] - tight["psnr"]) < 0.01
     assert np.isfinite(tight["loss"])

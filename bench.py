@@ -1,8 +1,8 @@
 """Benchmark of the HIP rasterizer's hot path: one fwd+bwd of a 2048x2048 field of 200k Gaussians.
 
 A step is one pass of the renderer's hot path over the resident field: tile binning (no host
-synchronization), tiled forward, tiled backward with the normalizer recomputed, against a fixed
-synthetic dL/dout.  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+synchronization), tiled forward (image and coverage), tiled backward from the saved image and
+coverage, against a fixed synthetic dL/dout.  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
 baseline).  `value` is the algorithmic HBM throughput of the whole step:
 
     B_step = B_bin + B_fwd + B_bwd
@@ -10,7 +10,11 @@ baseline).  `value` is the algorithmic HBM throughput of the whole step:
     B_fwd  = 4 (T+1) + P (4 + R) + 12 H W             (one list sweep; RGB out)
     B_bwd  = 12 H W + 2 P (4 + R) + 36 P + 36 N       (grad in; two list sweeps; one 9-float
                                                         atomic set per pair; gradients)
-with R = 44 bytes per staged Gaussian record, N Gaussians, P (tile, Gaussian) pairs and T tiles.
+with R = 44 bytes of Gaussian parameters per pair visit (mean 8, conic 12, color 12, opacity 4,
+radii 8), N Gaussians, P (tile, Gaussian) pairs and T tiles.  This is the byte model of a backward
+that recomputes the normalizer.  The implementation reads the forward's saved coverage and image
+instead (16 H W more in the forward and backward, one list sweep less), which counts 37 MB more
+at the metric workload; the smaller model is reported.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|single|stress]
 """
@@ -28,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-RECORD_BYTES = 44  # algorithmic bytes of one staged Gaussian record
+RECORD_BYTES = 44  # algorithmic bytes of one Gaussian's parameters
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -207,6 +211,7 @@ def main() -> int:
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = HR.workspace(N, H, W, 16, dev)
     out = torch.empty(H, W, 3, device=dev)
+    den = torch.empty(H, W, device=dev)
     grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
              torch.empty_like(op))
 
@@ -223,12 +228,12 @@ def main() -> int:
             ev["b"][1].record(stream)
             ev["f"][0].record(stream)
         HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8, 16,
-                         False, 3.0, out=out, ws=ws)
+                         False, 3.0, out=out, den=den, ws=ws)
         if record:
             ev["f"][1].record(stream)
             ev["g"][0].record(stream)
-        HR.backward_tiled(grad_out, means, conics, colors, radii, op, gids, offsets, H, W, True,
-                          1e-8, 16, False, 3.0, grads=grads, ws=ws)
+        HR.backward_tiled(grad_out, means, conics, colors, radii, op, den, out, gids, offsets, H,
+                          W, True, 1e-8, 16, False, 3.0, grads=grads, ws=ws)
         if record:
             ev["g"][1].record(stream)
 

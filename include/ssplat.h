@@ -122,15 +122,17 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /*
- * Tiled backward.  Given grad_out (H,W,3) = dL/dout, writes dL/dmeans (N,2), dL/dconics (N,3),
- * dL/dcolors (N,3) and, when opacities and grad_opacities are both non-NULL, dL/dopacities (N,).
- * The per-pixel normalizer is recomputed from the tile lists (no saved den/out).  Gradient
+ * Tiled backward.  Given grad_out (H,W,3) = dL/dout and the forward's den (H,W) and out (H,W,3)
+ * of the same inputs, writes dL/dmeans (N,2), dL/dconics (N,3), dL/dcolors (N,3) and, when
+ * opacities and grad_opacities are both non-NULL, dL/dopacities (N,).  den and out are read only
+ * when normalize is set (the per-pixel adjoints are dL/dN_k = G_k/(den+eps) and
+ * dL/dden = -sum_k G_k out_k/(den+eps)); an additive call may pass NULL for both.  Gradient
  * outputs are overwritten (the library clears them first).
- * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231), which
- * takes the forward's saved den/out instead.
+ * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231).
  */
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
                           const float* colors, const int64_t* radii, const float* opacities,
+                          const float* den, const float* out,
                           int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
                           double eps, double support_fade_alpha, double sigma_cutoff,
@@ -186,12 +188,13 @@ int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
                                float* den, void* workspace,
                                size_t workspace_bytes, void* stream);
 
-/* Batched ssplat_backward_tiled. */
+/* Batched ssplat_backward_tiled; den and out are the batched forward's concatenated outputs. */
 int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 int64_t total_gaussians, int64_t total_tiles, int tile_size,
                                 const float* grad_out, const float* means, const float* conics,
                                 const float* colors, const int64_t* radii,
-                                const float* opacities, const int32_t* tile_gids,
+                                const float* opacities, const float* den, const float* out,
+                                const int32_t* tile_gids,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
                                 double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,

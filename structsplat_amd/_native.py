@@ -44,8 +44,8 @@ SIGNATURES = {
     "ssplat_forward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
                                       _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                       _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
-    "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64,
-                                       _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
+    "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                       _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
                                        _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                        _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, ctypes.POINTER(_c_i64),
@@ -60,7 +60,7 @@ SIGNATURES = {
                                             _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_backward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
                                              _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                             _c_ptr, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl,
+                                             _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl,
                                              _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
                                              _c_ptr]),
 }

@@ -110,8 +110,10 @@ class Batch:
             "ssplat_batch_forward_tiled")
         return out
 
-    def backward(self, grad_out, means, conics, colors, radii, opacities, gids, offsets,
-                 normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0):
+    def backward(self, grad_out, means, conics, colors, radii, opacities, den, out, gids,
+                 offsets, normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0):
+        """Batched backward; ``den``/``out`` are the batched forward's outputs (``None`` allowed
+        for additive renders)."""
         lib = _native.load()
         has_op = opacities is not None and opacities.numel() > 0
         g_m = torch.empty_like(means)
@@ -120,8 +122,9 @@ class Batch:
         g_o = torch.empty_like(opacities) if has_op else None
         _native.check(lib.ssplat_batch_backward_tiled(
             *self._head(), _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
-            _ptr(opacities), _ptr(gids), _ptr(offsets), gids.numel(), int(bool(normalize)),
-            float(eps), _fade_alpha(support_fade), float(sigma_cutoff), _ptr(g_m), _ptr(g_q),
+            _ptr(opacities), _ptr(den), _ptr(out), _ptr(gids), _ptr(offsets), gids.numel(),
+            int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
+            _ptr(g_m), _ptr(g_q),
             _ptr(g_c), _ptr(g_o), _ptr(self.workspace), self.workspace.numel(),
             _stream(self.device)), "ssplat_batch_backward_tiled")
         return g_m, g_q, g_c, g_o
@@ -137,6 +140,7 @@ class BatchPlan:
         self.gids = torch.empty(self.capacity, dtype=torch.int32, device=dev)
         self.offsets = torch.empty(batch.total_tiles + 1, dtype=torch.int32, device=dev)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.den = torch.empty(batch.total_pixels, dtype=torch.float32, device=dev)
 
 
 class _BatchRender(Function):
@@ -150,8 +154,8 @@ class _BatchRender(Function):
                            sigma_cutoff,
                            out=(plan.gids, plan.offsets, plan.status))
         out = b.forward(means, conics, colors, radii, opacities, plan.gids, plan.offsets,
-                        normalize, eps, fade, sigma_cutoff)
-        ctx.save_for_backward(means, conics, colors, radii, opacities)
+                        normalize, eps, fade, sigma_cutoff, den=plan.den)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, plan.den, out)
         ctx.plan = plan
         ctx.args = (normalize, eps, fade, sigma_cutoff)
         return out
@@ -159,11 +163,11 @@ class _BatchRender(Function):
     @staticmethod
     @once_differentiable
     def backward(ctx, grad_out):
-        means, conics, colors, radii, opacities = ctx.saved_tensors
+        means, conics, colors, radii, opacities, den, out = ctx.saved_tensors
         plan = ctx.plan
         g_m, g_q, g_c, g_o = plan.batch.backward(grad_out.contiguous(), means, conics, colors,
-                                                 radii, opacities, plan.gids, plan.offsets,
-                                                 *ctx.args)
+                                                 radii, opacities, den, out, plan.gids,
+                                                 plan.offsets, *ctx.args)
         if opacities.numel() == 0:
             g_o = None
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None

@@ -71,14 +71,21 @@ constexpr int kScanBlock = kScanThreads * kScanItems;  // counts per first-level
 constexpr int kMaxScanBlocks = kScanBlock;             // the top-level scan is one block
 constexpr int64_t kMaxScanItems = int64_t(kScanBlock) * kMaxScanBlocks;
 constexpr int kBinThreads = 256;
+// Up to this many tile counters are scanned by one workgroup (one launch); larger grids use the
+// three-launch scan.
+constexpr int kSingleScanPerThread = 16;
+constexpr int64_t kSingleScanItems = int64_t(1) << 16;
 constexpr int kSortThreads = 256;
-constexpr int kSortLds = 4096;  // per-tile lists up to this length sort entirely in LDS
+constexpr int kSortLds = 4096;      // lists up to this length sort in the block's LDS
+constexpr int kSortWaveLds = 1024;  // lists up to this length sort in one wave's LDS slice
 constexpr int kRenderThreads = 256;
 constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
 constexpr int kTile = 16;    // render tile edge (pixels)
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+static_assert(kRenderThreads == kTile * kTile, "one render thread per tile pixel");
 // -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
 constexpr float kNegHalfLog2e = -0.72134752044448170368f;
+constexpr float kInvNegHalfLog2e = -1.38629436111989061883f;  // 1 / kNegHalfLog2e = -2 ln 2
 // Clamps that keep the integer support arithmetic exact in int64 (see support_rect).
 constexpr float kCenterClamp = 1.0e12f;
 constexpr int64_t kRadiusClamp = int64_t(1) << 36;
@@ -406,33 +413,14 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
   }
 }
 
-__global__ __launch_bounds__(kBinThreads) void k_bin_count(BinArgs a, int32_t* __restrict__ counts) {
-  enumerate_pairs(a, [&](int tile, int) { atomicAdd(&counts[tile], 1); });
-}
-
-__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(BinArgs a,
-                                                             const int32_t* __restrict__ offsets,
-                                                             int32_t* __restrict__ cursors,
-                                                             int32_t* __restrict__ gids,
-                                                             int64_t capacity,
-                                                             int32_t* __restrict__ status) {
-  enumerate_pairs(a, [&](int tile, int gid) {
-    const int64_t pos = int64_t(offsets[tile]) + atomicAdd(&cursors[tile], 1);
-    if (pos >= 0 && pos < capacity) {
-      gids[pos] = gid;
-    } else {
-      *status = SSPLAT_ERR_CAPACITY;
-    }
-  });
-}
-
-// Exclusive scan of one value per thread over a 1024-thread block; returns the block total
+// Exclusive scan of one value per thread over a kThreads-thread block; returns the block total
 // through *total.  Reusable back to back (ends with a barrier).
+template <int kThreads>
 __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
-  __shared__ int64_t s_wave[kScanThreads / 64];
+  constexpr int kWaves = kThreads / 64;
+  __shared__ int64_t s_wave[kWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  constexpr int kWaves = kScanThreads / 64;
   int64_t x = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -457,6 +445,110 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
   return prefix + x - v;
 }
 
+// Clears the tile counters and the binning status word.
+__global__ void k_bin_reset(int32_t* __restrict__ counts, int64_t items,
+                            int32_t* __restrict__ status) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < items; i += stride) {
+    counts[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *status = SSPLAT_OK;
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(BinArgs a, int32_t* __restrict__ counts) {
+  enumerate_pairs(a, [&](int tile, int) { atomicAdd(&counts[tile], 1); });
+}
+
+// One-workgroup exclusive scan of counts[0, items) into offsets, for grids of up to
+// kSingleScanItems counters: each thread scans kSingleScanPerThread consecutive counters of a
+// kScanThreads * kSingleScanPerThread chunk (16-byte loads and stores), with a running carry
+// between chunks.  Saturates at INT32_MAX and flags *status like k_scan_apply.  The counters are
+// left as they are: the scatter counts them back down to zero.
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(const int32_t* __restrict__ counts,
+                                                              int64_t items,
+                                                              int32_t* __restrict__ offsets,
+                                                              int32_t* __restrict__ status) {
+  constexpr int kPer = kSingleScanPerThread;
+  constexpr int kChunk = kScanThreads * kPer;
+  static_assert(kPer % 4 == 0, "counters are loaded four at a time");
+  // The workspace (counts) is 256-byte aligned; the caller's offsets may not be.
+  const bool offsets_aligned = (reinterpret_cast<uintptr_t>(offsets) & 15) == 0;
+  int64_t carry = 0;
+  bool saturated = false;
+  for (int64_t base = 0; base < items; base += kChunk) {
+    const int64_t i0 = base + int64_t(threadIdx.x) * kPer;
+    int32_t v[kPer];
+    int64_t sum = 0;
+    if (i0 + kPer <= items) {
+      // counts is 256-byte aligned and i0 a multiple of 4: aligned 16-byte loads.
+#pragma unroll
+      for (int k = 0; k < kPer; k += 4) {
+        const int4 q = *reinterpret_cast<const int4*>(counts + i0 + k);
+        v[k] = q.x;
+        v[k + 1] = q.y;
+        v[k + 2] = q.z;
+        v[k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) v[k] = i0 + k < items ? counts[i0 + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) sum += v[k];
+    int64_t total;
+    int64_t run = carry + block_exclusive_scan<kScanThreads>(sum, &total);
+    int32_t o[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      saturated |= (i0 + k < items) & (run > INT32_MAX);
+      o[k] = static_cast<int32_t>(run > INT32_MAX ? INT32_MAX : run);
+      run += v[k];
+    }
+    if (offsets_aligned && i0 + kPer <= items) {
+#pragma unroll
+      for (int k = 0; k < kPer; k += 4) {
+        *reinterpret_cast<int4*>(offsets + i0 + k) = make_int4(o[k], o[k + 1], o[k + 2], o[k + 3]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        if (i0 + k < items) offsets[i0 + k] = o[k];
+      }
+    }
+    carry += total;
+  }
+  if (saturated) *status = SSPLAT_ERR_CAPACITY;
+}
+
+// Scatter cursors rebuilt from a finished offsets table: cursor[t] = offsets[t+1] - offsets[t]
+// (for a fill issued on its own, after the counting call's counters may have been reused).
+__global__ void k_cursors_from_offsets(const int32_t* __restrict__ offsets, int num_tiles,
+                                       int32_t* __restrict__ cursors) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num_tiles) return;
+  const int32_t d = offsets[t + 1] - offsets[t];
+  cursors[t] = d > 0 ? d : 0;
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(BinArgs a,
+                                                             const int32_t* __restrict__ offsets,
+                                                             int32_t* __restrict__ cursors,
+                                                             int32_t* __restrict__ gids,
+                                                             int64_t capacity,
+                                                             int32_t* __restrict__ status) {
+  // Each tile's cursor holds its pair count and is counted down to zero: the pairs land in
+  // [offsets[t], offsets[t+1]) in arbitrary order (the per-tile sort orders them) and the counters
+  // end at zero.
+  enumerate_pairs(a, [&](int tile, int gid) {
+    const int64_t pos = int64_t(offsets[tile]) + atomicSub(&cursors[tile], 1) - 1;
+    if (pos >= 0 && pos < capacity) {
+      gids[pos] = gid;
+    } else {
+      *status = SSPLAT_ERR_CAPACITY;
+    }
+  });
+}
+
 // First level: per-block sums of `items` int32 counts.
 __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int32_t* __restrict__ counts,
                                                                 int64_t items,
@@ -468,7 +560,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int32_t* _
     if (base + k < items) s += counts[base + k];
   }
   int64_t total;
-  block_exclusive_scan(s, &total);
+  block_exclusive_scan<kScanThreads>(s, &total);
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
@@ -484,7 +576,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(int64_t* __restrict__
     s += v[k];
   }
   int64_t total;
-  int64_t run = block_exclusive_scan(s, &total);
+  int64_t run = block_exclusive_scan<kScanThreads>(s, &total);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < blocks) block_sums[base + k] = run;
@@ -509,7 +601,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* __re
     s += v[k];
   }
   int64_t total;
-  int64_t run = block_exclusive_scan(s, &total) + block_sums[blockIdx.x];
+  int64_t run = block_exclusive_scan<kScanThreads>(s, &total) + block_sums[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < items) {
@@ -564,26 +656,132 @@ __device__ void bitonic_sort(int32_t* a, int L) {
   }
 }
 
+// Orders the LDS accesses of one wave: its lanes run in lockstep and LDS executes a wave's
+// operations in issue order, so a release/acquire pair (no s_barrier) is enough between the
+// compare-exchange stages of a wave-private sort.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Ascending bitonic sort of up to 64 values held one per lane (INT32_MAX padding sorts last).
+// Must be called with all 64 lanes active.
+__device__ __forceinline__ int32_t wave_sort64(int32_t v, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int32_t p = __shfl_xor(v, j, 64);
+      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = keep_min ? min(v, p) : max(v, p);
+    }
+  }
+  return v;
+}
+
+// Flip-form bitonic sort of a[0, L) by the 64 lanes of one wave (a in the wave's LDS slice).
+__device__ void wave_sort_lds(int32_t* a, int L, int lane) {
+  int p2 = 1;
+  while (p2 < L) p2 <<= 1;
+  const int pairs = p2 >> 1;
+  for (int k = 2; k <= p2; k <<= 1) {
+    const int half = k >> 1;
+    for (int p = lane; p < pairs; p += 64) {
+      const int i = (p / half) * k + (p % half);
+      const int j = i ^ (k - 1);
+      if (j < L) {
+        const int32_t x = a[i];
+        const int32_t y = a[j];
+        if (x > y) {
+          a[i] = y;
+          a[j] = x;
+        }
+      }
+    }
+    wave_lds_sync();
+    for (int m = k >> 2; m >= 1; m >>= 1) {
+      for (int p = lane; p < pairs; p += 64) {
+        const int i = (p / m) * 2 * m + (p % m);
+        const int j = i + m;
+        if (j < L) {
+          const int32_t x = a[i];
+          const int32_t y = a[j];
+          if (x > y) {
+            a[i] = y;
+            a[j] = x;
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+__device__ __forceinline__ int tile_segment(const int32_t* offsets, int t, int64_t capacity,
+                                            int64_t* start) {
+  int64_t s = offsets[t];
+  int64_t e = offsets[t + 1];
+  s = s < 0 ? 0 : (s > capacity ? capacity : s);
+  e = e < s ? s : (e > capacity ? capacity : e);
+  *start = s;
+  return static_cast<int>(e - s);
+}
+
+// Sorts every tile's gid list and merges the binning status.  One wave per tile: lists of <= 64 entries sort in registers
+// (shuffle bitonic, no barriers), lists of <= kSortWaveLds entries in the wave's LDS slice; the
+// rare longer lists are sorted by the whole block afterwards (LDS up to kSortLds entries, global
+// memory beyond).
 __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict__ gids,
                                                              const int32_t* __restrict__ offsets,
-                                                             int num_tiles, int64_t capacity) {
+                                                             int num_tiles, int64_t capacity,
+                                                             const int32_t* __restrict__ ws_status,
+                                                             int32_t* __restrict__ status) {
+  constexpr int kWaves = kSortThreads / 64;
+  // The binning's status is final (the scatter ran before this launch): merge it into the
+  // caller's sticky status word.
+  if (status != nullptr && blockIdx.x == 0 && threadIdx.x == 0 && *ws_status != SSPLAT_OK) {
+    *status = *ws_status;
+  }
   __shared__ int32_t buf[kSortLds];
-  for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
-    int64_t s = offsets[t];
-    int64_t e = offsets[t + 1];
-    s = s < 0 ? 0 : (s > capacity ? capacity : s);
-    e = e < s ? s : (e > capacity ? capacity : e);
-    const int L = static_cast<int>(e - s);
-    if (L <= 1) continue;
-    if (L <= kSortLds) {
-      for (int i = threadIdx.x; i < L; i += blockDim.x) buf[i] = gids[s + i];
-      __syncthreads();
-      bitonic_sort<false>(buf, L);
-      for (int i = threadIdx.x; i < L; i += blockDim.x) gids[s + i] = buf[i];
-      __syncthreads();
-    } else {
-      bitonic_sort<true>(gids + s, L);
+  __shared__ int s_big[kWaves];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int base = blockIdx.x * kWaves; base < num_tiles; base += gridDim.x * kWaves) {
+    const int t = base + wave;
+    int64_t s = 0;
+    const int L = t < num_tiles ? tile_segment(offsets, t, capacity, &s) : 0;
+    if (L <= 64) {
+      if (L > 1) {
+        int32_t v = lane < L ? gids[s + lane] : INT32_MAX;
+        v = wave_sort64(v, lane);
+        if (lane < L) gids[s + lane] = v;
+      }
+    } else if (L <= kSortWaveLds) {
+      int32_t* a = buf + wave * kSortWaveLds;
+      for (int i = lane; i < L; i += 64) a[i] = gids[s + i];
+      wave_lds_sync();
+      wave_sort_lds(a, L, lane);
+      for (int i = lane; i < L; i += 64) gids[s + i] = a[i];
     }
+    if (lane == 0) s_big[wave] = L > kSortWaveLds ? t : -1;
+    __syncthreads();
+    for (int w = 0; w < kWaves; ++w) {
+      const int tb = s_big[w];
+      if (tb < 0) continue;
+      int64_t sb = 0;
+      const int Lb = tile_segment(offsets, tb, capacity, &sb);
+      if (Lb <= kSortLds) {
+        for (int i = threadIdx.x; i < Lb; i += blockDim.x) buf[i] = gids[sb + i];
+        __syncthreads();
+        bitonic_sort<false>(buf, Lb);
+        for (int i = threadIdx.x; i < Lb; i += blockDim.x) gids[sb + i] = buf[i];
+      } else {
+        bitonic_sort<true>(gids + sb, Lb);
+      }
+      __syncthreads();
+    }
+    __syncthreads();
   }
 }
 
@@ -591,20 +789,46 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict
 // Rendering
 // ------------------------------------------------------------------------------------------
 
-// One packed Gaussian: three 16-byte chunks.
-//   c0 = (mx, my, a, 2b)   c1 = (c, opacity, r, g)   c2 = (b_color, xr, yr, gid)
+// One packed Gaussian: four 16-byte chunks.
+//   c0 = (mx, my, qa, qb)       conic in the exponent's units: the visit computes
+//   c1 = (qc, xr, yr, gid)        e = qa dx^2 + qb dx dy + qc dy^2 and weights with exp2(e)
+//   c2 = (o r, o g, o b, o)     opacity-premultiplied color and opacity (accumulation)
+//   c3 = (r, g, b, 0)           plain color (gradient sweep)
+// Without support fade the record holds (qa, qb, qc) = -log2(e)/2 * (a, 2b, c) (the fold saves
+// a multiply per visit); with fade it holds (a, 2b, c) and the kernels scale the quadratic so
+// that a pixel at exactly sigma_cutoff evaluates the same 2^x as the fade tail.
 // xr = x0 | x1 << 16 and yr = y0 | y1 << 16 hold the half-open clipped support rectangle.
 struct __align__(16) GaussRec {
-  float4 c0, c1, c2;
+  float4 c0, c1, c2, c3;
 };
-static_assert(sizeof(GaussRec) == 48, "GaussRec must be 48 bytes");
+static_assert(sizeof(GaussRec) == 64, "GaussRec must be 64 bytes");
 
+// Gradient buffers the backward's record packing clears (all NULL in the forward).
+struct GradClear {
+  float* means;
+  float* conics;
+  float* colors;
+  float* opacities;
+};
+
+template <bool kFold>
 __global__ void k_prepare(const float* __restrict__ means, const float* __restrict__ conics,
                           const float* __restrict__ colors, const int64_t* __restrict__ radii,
                           const float* __restrict__ opacities, int n, FieldTable fields,
-                          GaussRec* __restrict__ recs) {
+                          GaussRec* __restrict__ recs, GradClear clear) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (clear.means != nullptr) {
+    clear.means[2 * i + 0] = 0.0f;
+    clear.means[2 * i + 1] = 0.0f;
+    clear.conics[3 * i + 0] = 0.0f;
+    clear.conics[3 * i + 1] = 0.0f;
+    clear.conics[3 * i + 2] = 0.0f;
+    clear.colors[3 * i + 0] = 0.0f;
+    clear.colors[3 * i + 1] = 0.0f;
+    clear.colors[3 * i + 2] = 0.0f;
+    if (clear.opacities != nullptr) clear.opacities[i] = 0.0f;
+  }
   const ssplat_field f = field_of_gaussian(fields, i);
   const float mx = means[2 * i + 0];
   const float my = means[2 * i + 1];
@@ -614,12 +838,15 @@ __global__ void k_prepare(const float* __restrict__ means, const float* __restri
     xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
     yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
   }
+  const float scale = kFold ? kNegHalfLog2e : 1.0f;
+  const float op = opacities ? opacities[i] : 1.0f;
+  const float cr = colors[3 * i + 0], cg = colors[3 * i + 1], cb = colors[3 * i + 2];
   GaussRec g;
-  g.c0 = make_float4(mx, my, conics[3 * i + 0], 2.0f * conics[3 * i + 1]);
-  g.c1 = make_float4(conics[3 * i + 2], opacities ? opacities[i] : 1.0f, colors[3 * i + 0],
-                     colors[3 * i + 1]);
-  g.c2 = make_float4(colors[3 * i + 2], __uint_as_float(xr), __uint_as_float(yr),
+  g.c0 = make_float4(mx, my, scale * conics[3 * i + 0], scale * (2.0f * conics[3 * i + 1]));
+  g.c1 = make_float4(scale * conics[3 * i + 2], __uint_as_float(xr), __uint_as_float(yr),
                      __int_as_float(i));
+  g.c2 = make_float4(op * cr, op * cg, op * cb, op);
+  g.c3 = make_float4(cr, cg, cb, 0.0f);
   recs[i] = g;
 }
 
@@ -650,9 +877,10 @@ struct StageLds {
   float4 c0[kStage];
   float4 c1[kStage];
   float4 c2[kStage];
+  float4 c3[kStage];
 };
 
-// Coalesced staging of up to kStage records of the tile list into LDS (three 16-byte loads
+// Coalesced staging of up to kStage records of the tile list into LDS (four 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
 __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds& st, int64_t batch,
@@ -664,10 +892,13 @@ __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds& st, i
       st.c0[threadIdx.x] = src->c0;
       st.c1[threadIdx.x] = src->c1;
       st.c2[threadIdx.x] = src->c2;
+      st.c3[threadIdx.x] = src->c3;
     } else {
-      st.c0[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      st.c1[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      st.c2[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st.c0[threadIdx.x] = z;
+      st.c1[threadIdx.x] = z;
+      st.c2[threadIdx.x] = z;
+      st.c3[threadIdx.x] = z;
     }
   }
 }
@@ -696,7 +927,7 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
       const int j = lane + 64 * k;
       bool h = false, f = false;
       if (j < m) {
-        const float4 v = st.c2[j];
+        const float4 v = st.c1[j];
         const uint32_t xr = __float_as_uint(v.y);
         const uint32_t yr = __float_as_uint(v.z);
         const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
@@ -718,8 +949,8 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
         const float4 v2 = st.c2[j];
         bool inside = true;
         if (!((full[k] >> b) & 1)) {
-          const uint32_t xr = __float_as_uint(v2.y);
-          const uint32_t yr = __float_as_uint(v2.z);
+          const uint32_t xr = __float_as_uint(v1.y);
+          const uint32_t yr = __float_as_uint(v1.z);
           const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
           inside = static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0) &&
                    static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0);
@@ -728,14 +959,13 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
           const float dx = fx - v0.x;
           const float dy = fy - v0.y;
           const float t = fmaf(v0.z, dx, v0.w * dy);
-          const float q = fmaf(t, dx, v1.x * dy * dy);
-          const float raw = exp2_hw(q * kNegHalfLog2e);
+          const float e = fmaf(t, dx, v1.x * dy * dy);
+          const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
           const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
-          const float w = base * v1.y;
-          nr = fmaf(w, v1.z, nr);
-          ng = fmaf(w, v1.w, ng);
-          nb = fmaf(w, v2.x, nb);
-          d += w;
+          nr = fmaf(base, v2.x, nr);
+          ng = fmaf(base, v2.y, ng);
+          nb = fmaf(base, v2.z, nb);
+          d = fmaf(base, v2.w, d);
         }
       }
     }
@@ -781,6 +1011,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
 
 struct GradArgs {
   const float* grad_out;
+  const float* den;  // forward coverage (read when normalizing)
+  const float* out;  // forward image (read when normalizing)
   float* grad_means;
   float* grad_conics;
   float* grad_colors;
@@ -804,15 +1036,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   const int ty0 = tile_y * kTile;
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
 
-  // Phase 1: recompute N_k and D for one pixel per lane (same code and order as the forward),
-  // then the per-pixel adjoints dL/dN_k and dL/dD.
+  // Per-pixel adjoints from the forward's saved outputs (render_ext.cu:496-510):
+  //   dL/dN_k = G_k/(D+eps), dL/dD = -sum_k G_k out_k/(D+eps)  (additive: G_k and 0).
   {
-    const int qx0 = tx0 + (wave & 1) * 8;
-    const int qy0 = ty0 + (wave >> 1) * 8;
-    const int px = qx0 + lx;
-    const int py = qy0 + ly;
-    float4 acc;
-    accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
+    const int px = tx0 + (threadIdx.x & (kTile - 1));
+    const int py = ty0 + (threadIdx.x >> 4);
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (px < f.width && py < f.height) {
       const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
@@ -820,17 +1048,15 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
       const float gg = g.grad_out[flat * 3 + 1];
       const float gb = g.grad_out[flat * 3 + 2];
       if (kNormalize) {
-        const float denom = acc.w + a.eps;
-        const float o_r = acc.x / denom;
-        const float o_g = acc.y / denom;
-        const float o_b = acc.z / denom;
-        adj = make_float4(gr / denom, gg / denom, gb / denom,
-                          -(gr * o_r + gg * o_g + gb * o_b) / denom);
+        const float denom = g.den[flat] + a.eps;
+        const float dot = gr * g.out[flat * 3 + 0] + gg * g.out[flat * 3 + 1] +
+                          gb * g.out[flat * 3 + 2];
+        adj = make_float4(gr / denom, gg / denom, gb / denom, -dot / denom);
       } else {
         adj = make_float4(gr, gg, gb, 0.0f);
       }
     }
-    s_adj[(py - ty0) * kTile + (px - tx0)] = adj;
+    s_adj[threadIdx.x] = adj;
   }
   __syncthreads();
 
@@ -860,29 +1086,29 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
       const float4 v0 = st.c0[j];
       const float4 v1 = st.c1[j];
       const float4 v2 = st.c2[j];
-      const uint32_t xr = __builtin_amdgcn_readfirstlane(__float_as_uint(v2.y));
-      const uint32_t yr = __builtin_amdgcn_readfirstlane(__float_as_uint(v2.z));
+      const float4 v3 = st.c3[j];
+      const uint32_t xr = __builtin_amdgcn_readfirstlane(__float_as_uint(v1.y));
+      const uint32_t yr = __builtin_amdgcn_readfirstlane(__float_as_uint(v1.z));
       const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
-      const float mx = v0.x, my = v0.y, ca = v0.z, cb2 = v0.w, cc = v1.x;
-      const float cr = v1.z, cg = v1.w, cbl = v2.x;
+      const float mx = v0.x, my = v0.y, qa = v0.z, qb = v0.w, qc = v1.x;
+      const float cr = v3.x, cg = v3.y, cbl = v3.z;
       float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
       float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
+      // One straight-line block per quadrant: the per-lane rectangle test alone decides which
+      // lanes accumulate (a quadrant the rectangle misses has no lane inside and is skipped by
+      // the exec-mask branch), so the accumulators stay in the same registers throughout.
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const int qx0 = tx0 + (s & 1) * 8;
-        const int qy0 = ty0 + (s >> 1) * 8;
-        if (!(x0 < qx0 + 8 && x1 > qx0 && y0 < qy0 + 8 && y1 > qy0)) continue;
-        const bool full = x0 <= qx0 && x1 >= qx0 + 8 && y0 <= qy0 && y1 >= qy0 + 8;
-        const int px = qx0 + lx;
-        const int py = qy0 + ly;
-        const bool inside = full || (static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0) &&
-                                     static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0));
+        const int px = tx0 + (s & 1) * 8 + lx;
+        const int py = ty0 + (s >> 1) * 8 + ly;
+        const bool inside = (static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0)) &
+                            (static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0));
         if (inside) {
           const float dx = static_cast<float>(px) - mx;
           const float dy = static_cast<float>(py) - my;
-          const float t = fmaf(ca, dx, cb2 * dy);
-          const float q = fmaf(t, dx, cc * dy * dy);
-          const float raw = exp2_hw(q * kNegHalfLog2e);
+          const float t = fmaf(qa, dx, qb * dy);
+          const float e = fmaf(t, dx, qc * dy * dy);
+          const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
           const float4 ad = adj[s];
           const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
           float base, dqr;
@@ -913,9 +1139,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
       // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries:
       //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
       //   g_c = -op/2 Syy, g_color = op C.
-      const int gid = __float_as_int(v2.w);
-      const float op = v1.y;
-      const float cb = 0.5f * cb2;
+      // The record's quadratic is the conic times -log2(e)/2 when folded; undo that here.
+      const int gid = __float_as_int(v1.w);
+      const float op = v2.w;
+      const float unfold = kFade ? 1.0f : kInvNegHalfLog2e;
+      const float ca = qa * unfold, cc = qc * unfold, cb = (0.5f * unfold) * qb;
       const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
       const float cross = comp < 2 ? partner * cb : 0.0f;
       const float gval = op * fmaf(red, own_scale, cross);
@@ -1072,16 +1300,26 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
   return SSPLAT_OK;
 }
 
+// Counts and scans: offsets[0..T] written, the counters left holding the per-tile counts (the
+// scatter's cursors).  Three launches (reset, count, one-workgroup scan) up to kSingleScanItems
+// counters, five beyond.
 int run_count(const BinArgs& a, const Plan& p, const Workspace& w, int32_t* offsets,
               hipStream_t stream) {
   const int64_t items = int64_t(p.num_tiles) + 1;
-  int rc = fill_i32(w.counts, items, 0, stream);
-  if (rc) return rc;
-  rc = fill_i32(w.status, 1, SSPLAT_OK, stream);
-  if (rc) return rc;
+  {
+    const int64_t blocks64 = (items + 255) / 256;
+    const int blocks = static_cast<int>(blocks64 < 4096 ? blocks64 : 4096);
+    k_bin_reset<<<blocks, 256, 0, stream>>>(w.counts, items, w.status);
+    SSPLAT_LAUNCH_CHECK("k_bin_reset");
+  }
   if (a.n > 0) {
     k_bin_count<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(a, w.counts);
     SSPLAT_LAUNCH_CHECK("k_bin_count");
+  }
+  if (items <= kSingleScanItems) {
+    k_scan_single<<<1, kScanThreads, 0, stream>>>(w.counts, items, offsets, w.status);
+    SSPLAT_LAUNCH_CHECK("k_scan_single");
+    return SSPLAT_OK;
   }
   const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
   k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(w.counts, items, w.bsums);
@@ -1093,19 +1331,27 @@ int run_count(const BinArgs& a, const Plan& p, const Workspace& w, int32_t* offs
   return SSPLAT_OK;
 }
 
+// Scatter + per-tile sort (which also merges the status).  rebuild_cursors: the counters do not
+// hold run_count's per-tile counts (a fill issued on its own), so they are rebuilt from offsets.
 int run_fill(const BinArgs& a, const Plan& p, const Workspace& w, const int32_t* offsets,
-             int32_t* gids, int64_t capacity, int32_t* status, hipStream_t stream) {
-  const int64_t items = int64_t(p.num_tiles) + 1;
-  int rc = fill_i32(w.counts, items, 0, stream);
-  if (rc) return rc;
+             int32_t* gids, int64_t capacity, int32_t* status, bool rebuild_cursors,
+             hipStream_t stream) {
+  if (rebuild_cursors && p.num_tiles > 0) {
+    k_cursors_from_offsets<<<(p.num_tiles + 255) / 256, 256, 0, stream>>>(offsets, p.num_tiles,
+                                                                          w.counts);
+    SSPLAT_LAUNCH_CHECK("k_cursors_from_offsets");
+  }
   if (a.n > 0) {
     k_bin_scatter<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(
         a, offsets, w.counts, gids, capacity, w.status);
     SSPLAT_LAUNCH_CHECK("k_bin_scatter");
-    k_sort_tiles<<<p.num_tiles, kSortThreads, 0, stream>>>(gids, offsets, p.num_tiles, capacity);
-    SSPLAT_LAUNCH_CHECK("k_sort_tiles");
   }
-  if (status) {
+  const int sort_blocks = (p.num_tiles + kSortThreads / 64 - 1) / (kSortThreads / 64);
+  if (a.n > 0 && sort_blocks > 0) {
+    k_sort_tiles<<<sort_blocks, kSortThreads, 0, stream>>>(gids, offsets, p.num_tiles, capacity,
+                                                            w.status, status);
+    SSPLAT_LAUNCH_CHECK("k_sort_tiles");
+  } else if (status) {
     k_merge_status<<<1, 64, 0, stream>>>(w.status, status);
     SSPLAT_LAUNCH_CHECK("k_merge_status");
   }
@@ -1128,7 +1374,7 @@ int tile_index_count(const float* means, const float* conics, const int64_t* rad
 int tile_index_fill(const float* means, const float* conics, const int64_t* radii,
                     const Plan& p, int cull, double sigma_cutoff, const int32_t* offsets,
                     int32_t* gids, int64_t capacity, int32_t* status, void* ws, size_t ws_bytes,
-                    void* stream) {
+                    bool rebuild_cursors, void* stream) {
   int rc;
   if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
   if (capacity < 0 || (capacity > 0 && gids == nullptr)) {
@@ -1138,7 +1384,7 @@ int tile_index_fill(const float* means, const float* conics, const int64_t* radi
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   BinArgs a;
   if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
-  if ((rc = run_fill(a, p, w, offsets, gids, capacity, status,
+  if ((rc = run_fill(a, p, w, offsets, gids, capacity, status, rebuild_cursors,
                      static_cast<hipStream_t>(stream)))) {
     return rc;
   }
@@ -1190,12 +1436,20 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
   return r;
 }
 
+// Packs the records (and clears the gradient buffers named in `clear`); without support fade
+// the conic is folded with -log2(e)/2.
 int run_prepare(const float* means, const float* conics, const float* colors,
                 const int64_t* radii, const float* opacities, const Plan& p, const Workspace& w,
-                hipStream_t stream) {
+                bool support_fade, GradClear clear, hipStream_t stream) {
   if (p.n == 0) return SSPLAT_OK;
-  k_prepare<<<static_cast<int>((p.n + 255) / 256), 256, 0, stream>>>(
-      means, conics, colors, radii, opacities, static_cast<int>(p.n), p.table, w.recs);
+  const int blocks = static_cast<int>((p.n + 255) / 256);
+  if (support_fade) {
+    k_prepare<false><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
+                                                 static_cast<int>(p.n), p.table, w.recs, clear);
+  } else {
+    k_prepare<true><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
+                                                static_cast<int>(p.n), p.table, w.recs, clear);
+  }
   SSPLAT_LAUNCH_CHECK("k_prepare");
   return SSPLAT_OK;
 }
@@ -1217,7 +1471,10 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   Workspace w;
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade,
+                        GradClear{nullptr, nullptr, nullptr, nullptr}, s))) {
+    return rc;
+  }
   const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
   if (support_fade && normalize) {
     k_forward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
@@ -1234,7 +1491,7 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
 
 int backward_tiled(const Plan& p, const float* grad_out, const float* means, const float* conics,
                    const float* colors, const int64_t* radii, const float* opacities,
-                   const int32_t* gids, const int32_t* offsets, int64_t num_gids, int normalize,
+                   const float* den, const float* out, const int32_t* gids, const int32_t* offsets, int64_t num_gids, int normalize,
                    double eps, double support_fade_alpha, double sigma_cutoff,
                    float* grad_means, float* grad_conics, float* grad_colors,
                    float* grad_opacities, void* ws, size_t ws_bytes, void* stream) {
@@ -1247,6 +1504,10 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
     return rc;
   }
   if (grad_out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_out is NULL");
+  if (normalize && (den == nullptr || out == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
+                     "normalized backward needs the forward's den and out");
+  }
   if (p.n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
   }
@@ -1254,14 +1515,13 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (p.n == 0) return success();
-  if ((rc = fill_f32(grad_means, 2 * p.n, 0.0f, s))) return rc;
-  if ((rc = fill_f32(grad_conics, 3 * p.n, 0.0f, s))) return rc;
-  if ((rc = fill_f32(grad_colors, 3 * p.n, 0.0f, s))) return rc;
   float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
-  if (gop && (rc = fill_f32(gop, p.n, 0.0f, s))) return rc;
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, s))) return rc;
+  const GradClear clear{grad_means, grad_conics, grad_colors, gop};
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, clear, s))) {
+    return rc;
+  }
   const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
-  GradArgs ga{grad_out, grad_means, grad_conics, grad_colors, gop};
+  GradArgs ga{grad_out, den, out, grad_means, grad_conics, grad_colors, gop};
   if (support_fade && normalize) {
     k_backward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
   } else if (support_fade) {
@@ -1316,7 +1576,7 @@ int ssplat_tile_index_fill(const float* means, const float* conics, const int64_
   const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
   return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, true, stream);
 }
 
 int ssplat_build_tile_index(const float* means, const float* conics, const int64_t* radii,
@@ -1331,7 +1591,8 @@ int ssplat_build_tile_index(const float* means, const float* conics, const int64
                         workspace, workspace_bytes, stream);
   if (rc) return rc;
   return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, false,
+                         stream);
 }
 
 int ssplat_forward_tiled(const float* means, const float* conics, const float* colors,
@@ -1350,7 +1611,7 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
 
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
                           const float* colors, const int64_t* radii, const float* opacities,
-                          int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
+                          const float* den, const float* out, int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
                           double eps, double support_fade_alpha, double sigma_cutoff,
                           float* grad_means,
@@ -1359,7 +1620,7 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
   Plan p;
   const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, workspace,
                         workspace_bytes, stream);
@@ -1412,7 +1673,8 @@ int ssplat_batch_build_tile_index(const ssplat_field* fields, int num_fields,
                         workspace, workspace_bytes, stream);
   if (rc) return rc;
   return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                         tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
+                         tile_gids, gid_capacity, status, workspace, workspace_bytes, false,
+                         stream);
 }
 
 int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
@@ -1436,7 +1698,8 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 int64_t total_gaussians, int64_t total_tiles, int tile_size,
                                 const float* grad_out, const float* means, const float* conics,
                                 const float* colors, const int64_t* radii,
-                                const float* opacities, const int32_t* tile_gids,
+                                const float* opacities, const float* den, const float* out,
+                                const int32_t* tile_gids,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
                                 double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,
@@ -1445,7 +1708,7 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
   Plan p;
   const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
   if (rc) return rc;
-  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, tile_gids,
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, workspace,
                         workspace_bytes, stream);

@@ -60,6 +60,7 @@ class RenderPlan:
         self.offsets = torch.empty(T + 1, dtype=torch.int32, device=device)
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
         self.ws = HR.workspace(n, H, W, tile_size, device)
+        self.den = torch.empty(H, W, dtype=torch.float32, device=device)
 
     def buffers(self):
         return self.gids, self.offsets, self.status, self.ws
@@ -81,10 +82,10 @@ class _PlannedRender(Function):
         gids, offsets, _ = HR.build_tile_index_capacity(
             means, conics, radii, plan.H, plan.W, plan.capacity, plan.tile_size, cull,
             sigma_cutoff, out=plan.buffers())
-        out, _ = HR.forward_tiled(means, conics, colors, radii, opacities, gids, offsets, plan.H,
-                                  plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
-                                  ws=plan.ws)
-        ctx.save_for_backward(means, conics, colors, radii, opacities)
+        out, den = HR.forward_tiled(means, conics, colors, radii, opacities, gids, offsets,
+                                    plan.H, plan.W, normalize, eps, plan.tile_size, fade,
+                                    sigma_cutoff, den=plan.den, ws=plan.ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, den, out)
         ctx.plan = plan
         ctx.args = (normalize, eps, fade, sigma_cutoff)
         return out
@@ -92,11 +93,11 @@ class _PlannedRender(Function):
     @staticmethod
     @once_differentiable
     def backward(ctx, grad_out):
-        means, conics, colors, radii, opacities = ctx.saved_tensors
+        means, conics, colors, radii, opacities, den, out = ctx.saved_tensors
         plan = ctx.plan
         normalize, eps, fade, sigma_cutoff = ctx.args
         g_m, g_q, g_c, g_o = HR.backward_tiled(
-            grad_out.contiguous(), means, conics, colors, radii, opacities, plan.gids,
+            grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, plan.gids,
             plan.offsets, plan.H, plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
             ws=plan.ws)
         if opacities.numel() == 0:

@@ -13,10 +13,9 @@ phrases and routes every mode to the gfx950 kernels of ``libssplat_hip.so`` thro
 * ``render_hip_exact_with_coverage(...)`` - ``render_cuda_exact_with_coverage``
   (cuda_render.py:229-253): normalized RGB plus raw coverage, no autograd.
 
-Differences from the reference by design: the backward recomputes the per-pixel normalizer
-instead of saving ``den``/``out`` from the forward; tile lists are built without a radix sort
-(ascending Gaussian id per tile, like the reference's stable sort); the forward is bitwise
-deterministic.
+As in the reference, the backward reads the forward's saved ``den``/``out``.  Differences by
+design: tile lists are built without a radix sort (ascending Gaussian id per tile, like the
+reference's stable sort); the forward is bitwise deterministic.
 """
 from __future__ import annotations
 
@@ -220,10 +219,14 @@ def forward_tiled(means, conics, colors, radii, opacities, gids, offsets, H: int
     return out, den
 
 
-def backward_tiled(grad_out, means, conics, colors, radii, opacities, gids, offsets, H: int,
-                   W: int, normalize: bool, eps: float, tile_size: int, support_fade: bool,
-                   sigma_cutoff: float, grads=None, ws=None):
-    """One tiled backward; returns ``(g_means, g_conics, g_colors, g_opacities or None)``."""
+def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, gids, offsets,
+                   H: int, W: int, normalize: bool, eps: float, tile_size: int,
+                   support_fade: bool, sigma_cutoff: float, grads=None, ws=None):
+    """One tiled backward; returns ``(g_means, g_conics, g_colors, g_opacities or None)``.
+
+    ``den`` and ``out`` are the forward's coverage and image for the same inputs (the
+    reference's saved tensors, cuda_render.py:111-113); additive renders may pass ``None``.
+    """
     lib = _native.load()
     dev = means.device
     n = means.shape[0]
@@ -236,7 +239,7 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, gids, offs
         ws = workspace(n, H, W, tile_size, dev)
     _native.check(lib.ssplat_backward_tiled(
         _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
-        n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size),
+        _ptr(den), _ptr(out), n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size),
         int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
         _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac), _ptr(ws), ws.numel(),
         _stream(dev)), "ssplat_backward_tiled")
@@ -262,10 +265,11 @@ class _ExactRenderHip(Function):
         gids, offsets = build_tile_index(means, conics, radii, int(height), int(width),
                                          int(tile_size), cull, float(sigma_cutoff))
         ws = workspace(means.shape[0], height, width, tile_size, means.device)
+        den = torch.empty(int(height), int(width), dtype=torch.float32, device=means.device)
         out, _ = forward_tiled(means, conics, colors, radii, opacities, gids, offsets,
                                int(height), int(width), bool(normalize), float(eps),
-                               int(tile_size), alpha, float(sigma_cutoff), ws=ws)
-        ctx.save_for_backward(means, conics, colors, radii, opacities, gids, offsets)
+                               int(tile_size), alpha, float(sigma_cutoff), den=den, ws=ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, den, out, gids, offsets)
         ctx.height = int(height)
         ctx.width = int(width)
         ctx.normalize = bool(normalize)
@@ -278,14 +282,14 @@ class _ExactRenderHip(Function):
     @staticmethod
     @once_differentiable
     def backward(ctx, grad_out):
-        means, conics, colors, radii, opacities, gids, offsets = ctx.saved_tensors
+        means, conics, colors, radii, opacities, den, out, gids, offsets = ctx.saved_tensors
         need_means, need_conics, need_colors, _, need_opac = ctx.needs_input_grad[:5]
         none_tail = (None,) * 8
         if not (need_means or need_conics or need_colors or need_opac):
             return (None,) * 5 + none_tail
         g_means, g_conics, g_colors, g_opac = backward_tiled(
-            grad_out.contiguous(), means, conics, colors, radii, opacities, gids, offsets,
-            ctx.height, ctx.width, ctx.normalize, ctx.eps, ctx.tile_size, ctx.support_fade,
+            grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, gids,
+            offsets, ctx.height, ctx.width, ctx.normalize, ctx.eps, ctx.tile_size, ctx.support_fade,
             ctx.sigma_cutoff)
         if opacities.numel() == 0 or not need_opac:
             g_opac = None

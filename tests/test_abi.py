@@ -32,8 +32,8 @@ def test_abi_version_and_workspace_query():
     lib = _native.load()
     assert lib.ssplat_abi_version() == 1
     nbytes = _native.workspace_bytes(1000, 256, 256, 16)
-    # 1000 records of 48 B + (256 tiles + 1) counters + scan sums + status, 256-byte aligned.
-    assert nbytes >= 1000 * 48 + 257 * 4
+    # 1000 records of 64 B + (256 tiles + 1) counters + scan sums + status, 256-byte aligned.
+    assert nbytes >= 1000 * 64 + 257 * 4
     assert nbytes % 256 == 0
 
 
@@ -54,3 +54,16 @@ def test_null_output_pointer_is_rejected():
     assert rc == _native.ERR_INVALID_ARGUMENT
     assert "NULL" in _native.last_error()
     assert isinstance(ctypes.c_size_t(0).value, int)
+
+
+def test_normalized_backward_requires_saved_forward_outputs():
+    """Argument checks run on the host before any device work: a normalized backward without the
+    forward's den/out is rejected (the reference's backward_tiled requires both,
+    render_ext.cpp:288-293)."""
+    lib = _native.load()
+    dummy = ctypes.create_string_buffer(64)
+    p = ctypes.cast(dummy, ctypes.c_void_p)
+    rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, None, None, 1, p, p, 1, 16, 16, 16, 1,
+                                   1e-8, 0.0, 3.0, p, p, p, None, p, 64, None)
+    assert rc == _native.ERR_INVALID_ARGUMENT
+    assert "den and out" in _native.last_error()

@@ -62,14 +62,18 @@ def test_batched_backward_matches_per_field(cuda):
     gen = torch.Generator(device=cuda).manual_seed(5)
     grad = torch.rand(b.total_pixels, 3, device=cuda, generator=gen)
     gids, offsets, _ = b.build_tile_index(means, conics, radii, 1 << 16)
-    gm, gq, gc, go = b.backward(grad, means, conics, colors, radii, op, gids, offsets)
+    den = torch.empty(b.total_pixels, device=cuda)
+    out = b.forward(means, conics, colors, radii, op, gids, offsets, den=den)
+    gm, gq, gc, go = b.backward(grad, means, conics, colors, radii, op, den, out, gids, offsets)
     torch.cuda.synchronize()
     for k, (p, (h, w)) in enumerate(zip(parts, SIZES)):
         if COUNTS[k] == 0:
             continue
         fg, fo = HR.build_tile_index(p[0], p[1], p[3], h, w)
-        ref = HR.backward_tiled(grad[b.pixel_slices[k]].view(h, w, 3).contiguous(), *p[:5], fg,
-                                fo, h, w, True, 1e-8, 16, False, 3.0)
+        fden = torch.empty(h, w, device=cuda)
+        fout, _ = HR.forward_tiled(*p[:5], fg, fo, h, w, True, 1e-8, 16, False, 3.0, den=fden)
+        ref = HR.backward_tiled(grad[b.pixel_slices[k]].view(h, w, 3).contiguous(), *p[:5], fden,
+                                fout, fg, fo, h, w, True, 1e-8, 16, False, 3.0)
         sl = b.gauss_slices[k]
         for got, exp in zip((gm[sl], gq[sl], gc[sl], go[sl]), ref):
             scale = exp.abs().max().clamp_min(1e-30)

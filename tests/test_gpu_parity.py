@@ -34,6 +34,29 @@ def test_tile_index_bit_exact(cuda, ts, cull):
     assert to_np(gids).tolist() == ref_g.tolist()
 
 
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("H,W,ts", [(1100, 1100, 4), (300, 200, 4)])
+def test_tile_index_scan_paths_bit_exact(cuda, H, W, ts, fused):
+    """Both scans of the counts: 75,626 counters take the three-launch scan, 3,751 the
+    one-workgroup scan; both through the two-call and the one-call binning."""
+    _print_case(test="tile_index_scan_paths", H=H, W=W, ts=ts, fused=fused)
+    f = random_field(33, 3000, H, W, scale_lo=0.6, scale_hi=12.0, margin=8.0)
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    ref_g, ref_o = tile_index.build_tile_index(to_np(means), to_np(conics), to_np(radii), H, W,
+                                               ts, False, 3.0)
+    if fused:
+        gids, offsets, status = HR.build_tile_index_capacity(means, conics, radii, H, W,
+                                                             len(ref_g) + 7, ts)
+        torch.cuda.synchronize()
+        assert int(status.item()) == 0
+        gids = gids[:len(ref_g)]
+    else:
+        gids, offsets = HR.build_tile_index(means, conics, radii, H, W, ts)
+        torch.cuda.synchronize()
+    assert to_np(offsets).tolist() == ref_o.tolist()
+    assert to_np(gids).tolist() == ref_g.tolist()
+
+
 def test_tile_index_reference_golden(cuda):
     # test_render.py:167-172
     means = torch.tensor([[2.0, 2.0], [20.0, 20.0]], device=cuda)
@@ -148,3 +171,28 @@ def test_render_kernels_reject_other_tile_sizes(cuda):
     with pytest.raises(ValueError, match="tile_size"):
         HR.forward_tiled(means, conics, colors, radii, None, gids, offsets, 16, 16, True, 1e-8,
                          8, False, 3.0)
+
+
+@pytest.mark.parametrize("n_dense", [70, 900, 2000, 6000])
+def test_tile_index_long_lists_bit_exact(cuda, n_dense):
+    """Dense tiles: lists of ~n_dense entries exercise every sort path (register, wave LDS,
+    block LDS, global memory)."""
+    _print_case(test="long lists", n_dense=n_dense)
+    rng = np.random.default_rng(n_dense)
+    H, W = 40, 52
+    dense = np.stack([rng.uniform(17, 30, n_dense), rng.uniform(17, 30, n_dense)], 1)
+    sparse = np.stack([rng.uniform(0, W, 200), rng.uniform(0, H, 200)], 1)
+    means_np = np.concatenate([dense, sparse]).astype(np.float32)
+    order = rng.permutation(len(means_np))  # interleave dense and sparse ids
+    f = {"means": means_np[order],
+         "scales": np.exp(rng.uniform(np.log(0.5), np.log(3.0), (len(order), 2))).astype(np.float32),
+         "angles": rng.uniform(0, np.pi, len(order)).astype(np.float32),
+         "colors": rng.random((len(order), 3)).astype(np.float32), "opacity": None}
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, 16)
+    torch.cuda.synchronize()
+    ref_g, ref_o = tile_index.build_tile_index(to_np(means), to_np(conics), to_np(radii), H, W,
+                                               16)
+    assert np.diff(ref_o).max() >= min(n_dense, 64)
+    assert to_np(offsets).tolist() == ref_o.tolist()
+    assert to_np(gids).tolist() == ref_g.tolist()

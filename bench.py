@@ -2,8 +2,8 @@
 
 A step is one pass of the renderer's hot path over the resident field: tile binning (no host
 synchronization), tiled forward (image and coverage), tiled backward from the saved image and
-coverage, against a fixed synthetic dL/dout.  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
-baseline).  `value` is the algorithmic HBM throughput of the whole step:
+coverage, against a fixed synthetic dL/dout.  Prints one JSON line (metric, value, roofline of
+the dominant kernel, CPU baseline).  `value` is the algorithmic HBM throughput of the whole step:
 
     B_step = B_bin + B_fwd + B_bwd
     B_bin  = 52 N + 4 P + 4 (T+1)                     (means, conics, radii in; lists, offsets out)
@@ -16,7 +16,14 @@ that recomputes the normalizer.  The implementation reads the forward's saved co
 instead (16 H W more in the forward and backward, one list sweep less), which counts 37 MB more
 at the metric workload; the smaller model is reported.
 
+The roofline is reported for k_backward16, the step's dominant kernel: its algorithmic bytes per
+launch (B_bwd) over its average duration, measured by HIP events that the library records on the
+launch stream around that kernel alone.  `traffic` is the kernel's measured HBM bytes per launch
+(rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy; `--copy-probe` is that copy)
+when profiles/hbm_traffic.json holds a measurement of this workload, else null.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|single|stress]
+       python bench.py --copy-probe MIB     (one device copy of MIB MiB, for counter calibration)
 """
 from __future__ import annotations
 
@@ -34,6 +41,7 @@ if ROOT not in sys.path:
 
 RECORD_BYTES = 44  # algorithmic bytes of one Gaussian's parameters
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 WORKLOADS = {
     "metric": ("fwd+bwd 2048x2048 RGB fp32, 200,000 Gaussians: scales log-U[0.5,12] px, aspect "
@@ -160,6 +168,33 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
     }
 
 
+def copy_probe(mib: int) -> int:
+    """One device-to-device copy of `mib` MiB (the calibration kernel of the HBM counters)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}))
+    return 0
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from profiles/hbm_traffic.json, if it covers `workload`."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get(kernel)
+    if data.get("workload") != workload or not entry:
+        return None
+    return entry.get("hbm_bytes_per_launch")
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as fh:
@@ -180,7 +215,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20,
                     help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the renderer's fields are independent units and this "
               "build runs on a single MI355X; run with --gpus 1.", file=sys.stderr)
@@ -262,10 +301,27 @@ def main() -> int:
         raise RuntimeError("pair capacity exceeded")
     phase = {k: statistics.mean(v) for k, v in phase_ms.items()}
 
+    # Kernel-only durations: the library's event pairs around k_forward16 / k_backward16.
+    from structsplat_amd import _native
+
+    _native.kernel_timing_read(_native.KERNEL_FORWARD)
+    _native.kernel_timing_read(_native.KERNEL_BACKWARD)
+    _native.kernel_timing(True)
+    try:
+        for _ in range(args.steps):
+            step(False)
+        torch.cuda.synchronize()
+    finally:
+        _native.kernel_timing(False)
+    nf, fwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)
+    nb, bwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_BACKWARD)
+    kernel_ms = {"k_forward16": fwd_kernel_ms / max(nf, 1),
+                 "k_backward16": bwd_kernel_ms / max(nb, 1)}
+
     P_now = int(offsets[-1].item())
     b = algorithmic_bytes(N, P_now, T, H * W)
     value = b["step"] / (ms_per_step * 1e-3) / 1e9
-    bwd_gbs = b["bwd"] / (phase["bwd"] * 1e-3) / 1e9
+    bwd_gbs = b["bwd"] / (kernel_ms["k_backward16"] * 1e-3) / 1e9
     line = {
         "metric": "fwd+bwd ms & HBM GB/s at 2048^2/200k Gaussians; batched fit steps/s vs plain",
         "value": round(value, 3),
@@ -283,16 +339,17 @@ def main() -> int:
             "workload": desc,
             "H": H, "W": W, "n_gaussians": N, "tile_pairs": P_now, "tiles": T,
             "phase_ms": {k: round(v, 4) for k, v in phase.items()},
+            "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "algorithmic_bytes": b,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_backward16 (timed with its grad-clearing and record-packing launches)",
+            "kernel": "k_backward16",
             "achieved": round(bwd_gbs, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, "k_backward16"),
         },
     }
     if args.batched_steps > 0:

@@ -50,6 +50,17 @@ int ssplat_abi_version(void);
 const char* ssplat_last_error(void);
 
 /*
+ * Kernel timing, a measurement hook with no reference counterpart.  While enabled, every
+ * forward / backward call records a HIP event pair on its stream around its render-kernel
+ * launch (k_forward16 / k_backward16, not the record packing).  ssplat_kernel_timing_read
+ * waits for the recorded pairs of one kernel, returns their count and summed milliseconds, and
+ * releases them.  Not for use while a stream is being captured into a HIP graph.
+ */
+enum ssplat_kernel { SSPLAT_KERNEL_FORWARD = 0, SSPLAT_KERNEL_BACKWARD = 1 };
+int ssplat_kernel_timing_enable(int enable);
+int ssplat_kernel_timing_read(int kernel, int64_t* launches, double* total_ms);
+
+/*
  * Workspace bytes needed by the calls below for a field of n Gaussians on an HxW image binned
  * with tile_size.  One workspace may be reused by consecutive calls on one stream.
  * No reference counterpart: the reference allocates its scratch with the torch caching

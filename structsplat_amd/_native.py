@@ -48,6 +48,9 @@ SIGNATURES = {
                                        _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
                                        _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                        _c_ptr, _c_size, _c_ptr]),
+    "ssplat_kernel_timing_enable": (_c_int, [_c_int]),
+    "ssplat_kernel_timing_read": (_c_int, [_c_int, ctypes.POINTER(_c_i64),
+                                           ctypes.POINTER(_c_dbl)]),
     "ssplat_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_size)]),
@@ -133,3 +136,22 @@ def workspace_bytes(n: int, height: int, width: int, tile_size: int) -> int:
     check(load().ssplat_workspace_bytes(int(n), int(height), int(width), int(tile_size),
                                         ctypes.byref(out)), "ssplat_workspace_bytes")
     return int(out.value)
+
+
+KERNEL_FORWARD = 0
+KERNEL_BACKWARD = 1
+
+
+def kernel_timing(enable: bool) -> None:
+    """Switch the library's render-kernel event timing on or off (outside graph capture)."""
+    check(load().ssplat_kernel_timing_enable(int(bool(enable))), "ssplat_kernel_timing_enable")
+
+
+def kernel_timing_read(kernel: int) -> tuple[int, float]:
+    """(launches, summed milliseconds) of the timed launches of one render kernel since the last
+    read; waits for them to finish."""
+    n = _c_i64(0)
+    ms = _c_dbl(0.0)
+    check(load().ssplat_kernel_timing_read(int(kernel), ctypes.byref(n), ctypes.byref(ms)),
+          "ssplat_kernel_timing_read")
+    return int(n.value), float(ms.value)

@@ -26,7 +26,9 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "ssplat.h"
 
@@ -60,6 +62,47 @@ int success() {
       return set_error(SSPLAT_ERR_HIP, "%s launch failed: %s", name, hipGetErrorString(err_)); \
     }                                                                                       \
   } while (0)
+
+// ------------------------------------------------------------------------------------------
+// Kernel timing (measurement hook): event pairs recorded on the launch stream around every
+// render-kernel launch while enabled, summed and released by ssplat_kernel_timing_read.
+// ------------------------------------------------------------------------------------------
+
+struct KernelTimer {
+  std::mutex mu;
+  bool enabled = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs[2];  // [0] forward, [1] backward
+};
+
+KernelTimer& kernel_timer() {
+  static KernelTimer t;
+  return t;
+}
+
+// Records the start event of one timed launch (returns false when timing is off or fails).
+bool timer_begin(int kernel, hipStream_t stream, hipEvent_t* start) {
+  KernelTimer& t = kernel_timer();
+  std::lock_guard<std::mutex> lock(t.mu);
+  if (!t.enabled) return false;
+  if (hipEventCreate(start) != hipSuccess) return false;
+  if (hipEventRecord(*start, stream) != hipSuccess) {
+    (void)hipEventDestroy(*start);
+    return false;
+  }
+  (void)kernel;
+  return true;
+}
+
+void timer_end(int kernel, hipStream_t stream, hipEvent_t start) {
+  KernelTimer& t = kernel_timer();
+  hipEvent_t stop;
+  if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream) != hipSuccess) {
+    (void)hipEventDestroy(start);
+    return;
+  }
+  std::lock_guard<std::mutex> lock(t.mu);
+  t.pairs[kernel].emplace_back(start, stop);
+}
 
 // ------------------------------------------------------------------------------------------
 // Constants
@@ -873,44 +916,56 @@ __device__ __forceinline__ void tile_range(const RenderArgs& a, int tile, int64_
   *end = e;
 }
 
+// The first kChunks 16-byte chunks of up to kStage records: the forward stages chunks 0-2, the
+// backward all four.
+template <int kChunks>
 struct StageLds {
-  float4 c0[kStage];
-  float4 c1[kStage];
-  float4 c2[kStage];
-  float4 c3[kStage];
+  float4 c[kChunks][kStage];
 };
 
-// Coalesced staging of up to kStage records of the tile list into LDS (four 16-byte loads
+// Coalesced staging of up to kStage records of the tile list into LDS (kChunks 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
-__device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds& st, int64_t batch,
-                                            int m) {
+template <int kChunks>
+__device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks>& st,
+                                            int64_t batch, int m) {
   if (static_cast<int>(threadIdx.x) < m) {
     const int g = a.gids[batch + threadIdx.x];
     if (g >= 0 && g < a.n) {
-      const GaussRec* src = a.recs + g;
-      st.c0[threadIdx.x] = src->c0;
-      st.c1[threadIdx.x] = src->c1;
-      st.c2[threadIdx.x] = src->c2;
-      st.c3[threadIdx.x] = src->c3;
+      const float4* src = reinterpret_cast<const float4*>(a.recs + g);
+#pragma unroll
+      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = src[c];
     } else {
-      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      st.c0[threadIdx.x] = z;
-      st.c1[threadIdx.x] = z;
-      st.c2[threadIdx.x] = z;
-      st.c3[threadIdx.x] = z;
+#pragma unroll
+      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
 }
 
+// Pixels of the 8x8 quadrant at (qx0, qy0) inside the half-open rectangle packed in (xr, yr), as
+// a 64-bit mask with bit 8*ly + lx for the pixel (qx0 + lx, qy0 + ly).
+__device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int qx0, int qy0) {
+  const int lx = max(static_cast<int>(xr & 0xffff) - qx0, 0);
+  const int hx = min(static_cast<int>(xr >> 16) - qx0, 8);
+  const int ly = max(static_cast<int>(yr & 0xffff) - qy0, 0);
+  const int hy = min(static_cast<int>(yr >> 16) - qy0, 8);
+  if (hx <= lx || hy <= ly) return 0;
+  const uint32_t row = ((1u << hx) - (1u << lx)) * 0x01010101u;  // the column run in every row
+  const uint64_t rows = (uint64_t(row) << 32) | row;
+  const uint64_t below_hy = hy == 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * hy)) - 1;
+  const uint64_t below_ly = (uint64_t(1) << (8 * ly)) - 1;
+  return rows & below_hy & ~below_ly;
+}
+
 // Accumulates numerator (r,g,b) and denominator of one pixel per lane over the whole tile list,
-// in ascending Gaussian id.  Each wave owns the 8x8 quadrant (qx0, qy0); a ballot over the staged
-// batch keeps only the records whose rectangle meets the quadrant, and records covering the whole
-// quadrant skip the per-pixel rectangle test.
+// in ascending Gaussian id.  Each wave owns the 8x8 quadrant (qx0, qy0).  For every staged record
+// the wave first computes (one record per lane) the mask of quadrant pixels inside its rectangle
+// into its LDS slice; a ballot keeps the records that meet the quadrant, records covering all of
+// it skip the pixel test, and for the others each lane reads its bit of the mask.
 template <bool kFade>
-__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds& st, int tile,
-                                                    int qx0, int qy0, int px, int py,
-                                                    float tail, float4& acc) {
+__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<3>& st,
+                                                    uint32_t* s_mask, int tile, int qx0, int qy0,
+                                                    int px, int py, float tail, float4& acc) {
   const int lane = threadIdx.x & 63;
   const float fx = static_cast<float>(px);
   const float fy = static_cast<float>(py);
@@ -925,18 +980,19 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
 #pragma unroll
     for (int k = 0; k < kStage / 64; ++k) {
       const int j = lane + 64 * k;
-      bool h = false, f = false;
+      uint64_t qm = 0;
       if (j < m) {
-        const float4 v = st.c1[j];
-        const uint32_t xr = __float_as_uint(v.y);
-        const uint32_t yr = __float_as_uint(v.z);
-        const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
-        h = x0 < qx0 + 8 && x1 > qx0 && y0 < qy0 + 8 && y1 > qy0;
-        f = x0 <= qx0 && x1 >= qx0 + 8 && y0 <= qy0 && y1 >= qy0 + 8;
+        const float4 v = st.c[1][j];
+        qm = quadrant_mask(__float_as_uint(v.y), __float_as_uint(v.z), qx0, qy0);
       }
-      hit[k] = __ballot(h);
-      full[k] = __ballot(f);
+      s_mask[2 * j] = static_cast<uint32_t>(qm);  // pixels of lanes 0-31
+      s_mask[2 * j + 1] = static_cast<uint32_t>(qm >> 32);  // lanes 32-63
+      hit[k] = __ballot(qm != 0);
+      full[k] = __ballot(qm == ~uint64_t(0));
     }
+    wave_lds_sync();  // the masks are read back by every lane of this wave
+    const uint32_t lane_bit_index = lane & 31;
+    const uint32_t* my_half = s_mask + (lane >> 5);
 #pragma unroll
     for (int k = 0; k < kStage / 64; ++k) {
       uint64_t mask = hit[k];
@@ -944,16 +1000,12 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
         const int b = __ffsll(static_cast<unsigned long long>(mask)) - 1;
         mask &= mask - 1;
         const int j = 64 * k + b;
-        const float4 v0 = st.c0[j];
-        const float4 v1 = st.c1[j];
-        const float4 v2 = st.c2[j];
+        const float4 v0 = st.c[0][j];
+        const float4 v1 = st.c[1][j];
+        const float4 v2 = st.c[2][j];
         bool inside = true;
         if (!((full[k] >> b) & 1)) {
-          const uint32_t xr = __float_as_uint(v1.y);
-          const uint32_t yr = __float_as_uint(v1.z);
-          const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
-          inside = static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0) &&
-                   static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0);
+          inside = __builtin_amdgcn_ubfe(my_half[2 * j], lane_bit_index, 1) != 0;
         }
         if (inside) {
           const float dx = fx - v0.x;
@@ -978,7 +1030,8 @@ template <bool kFade, bool kNormalize>
 __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
                                                               float* __restrict__ out,
                                                               float* __restrict__ den) {
-  __shared__ StageLds st;
+  __shared__ StageLds<3> st;
+  __shared__ uint32_t s_mask[kRenderThreads / 64][2 * kStage];  // per-wave quadrant masks
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
@@ -992,7 +1045,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
-  accumulate_quadrant<kFade>(a, st, tile, qx0, qy0, px, py, tail, acc);
+  accumulate_quadrant<kFade>(a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
     const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
     if (den) den[flat] = acc.w;
@@ -1021,7 +1074,7 @@ struct GradArgs {
 
 template <bool kFade, bool kNormalize>
 __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, GradArgs g) {
-  __shared__ StageLds st;
+  __shared__ StageLds<4> st;
   __shared__ float4 s_adj[kTile * kTile];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
@@ -1083,10 +1136,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
     stage_batch(a, st, batch, m);
     __syncthreads();
     for (int j = wave; j < m; j += kRenderThreads / 64) {
-      const float4 v0 = st.c0[j];
-      const float4 v1 = st.c1[j];
-      const float4 v2 = st.c2[j];
-      const float4 v3 = st.c3[j];
+      const float4 v0 = st.c[0][j];
+      const float4 v1 = st.c[1][j];
+      const float4 v2 = st.c[2][j];
+      const float4 v3 = st.c[3][j];
       const uint32_t xr = __builtin_amdgcn_readfirstlane(__float_as_uint(v1.y));
       const uint32_t yr = __builtin_amdgcn_readfirstlane(__float_as_uint(v1.z));
       const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
@@ -1475,7 +1528,10 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
                         GradClear{nullptr, nullptr, nullptr, nullptr}, s))) {
     return rc;
   }
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha,
+                                        sigma_cutoff);
+  hipEvent_t t0;
+  const bool timed = timer_begin(0, s, &t0);
   if (support_fade && normalize) {
     k_forward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
   } else if (support_fade) {
@@ -1486,6 +1542,7 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
     k_forward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
   }
   SSPLAT_LAUNCH_CHECK("k_forward16");
+  if (timed) timer_end(0, s, t0);
   return success();
 }
 
@@ -1520,8 +1577,11 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, clear, s))) {
     return rc;
   }
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
+  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha,
+                                        sigma_cutoff);
   GradArgs ga{grad_out, den, out, grad_means, grad_conics, grad_colors, gop};
+  hipEvent_t t0;
+  const bool timed = timer_begin(1, s, &t0);
   if (support_fade && normalize) {
     k_backward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
   } else if (support_fade) {
@@ -1532,6 +1592,7 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
     k_backward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
   }
   SSPLAT_LAUNCH_CHECK("k_backward16");
+  if (timed) timer_end(1, s, t0);
   return success();
 }
 
@@ -1546,6 +1607,46 @@ extern "C" {
 int ssplat_abi_version(void) { return SSPLAT_ABI_VERSION; }
 
 const char* ssplat_last_error(void) { return g_last_error.c_str(); }
+
+int ssplat_kernel_timing_enable(int enable) {
+  KernelTimer& t = kernel_timer();
+  std::lock_guard<std::mutex> lock(t.mu);
+  t.enabled = enable != 0;
+  return success();
+}
+
+int ssplat_kernel_timing_read(int kernel, int64_t* launches, double* total_ms) {
+  if (kernel != SSPLAT_KERNEL_FORWARD && kernel != SSPLAT_KERNEL_BACKWARD) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "kernel must be 0 (forward) or 1 (backward)");
+  }
+  if (launches == nullptr || total_ms == nullptr) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "launches and total_ms must not be NULL");
+  }
+  KernelTimer& t = kernel_timer();
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+  {
+    std::lock_guard<std::mutex> lock(t.mu);
+    pairs.swap(t.pairs[kernel]);
+  }
+  double sum = 0.0;
+  int rc = SSPLAT_OK;
+  for (auto& pr : pairs) {
+    float ms = 0.0f;
+    if (rc == SSPLAT_OK) {
+      hipError_t e = hipEventSynchronize(pr.second);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+      if (e != hipSuccess) {
+        rc = set_error(SSPLAT_ERR_HIP, "kernel timing: %s", hipGetErrorString(e));
+      }
+    }
+    sum += ms;
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  *launches = static_cast<int64_t>(pairs.size());
+  *total_ms = sum;
+  return rc == SSPLAT_OK ? success() : rc;
+}
 
 int ssplat_workspace_bytes(int64_t n, int height, int width, int tile_size, size_t* bytes) {
   if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");

@@ -67,3 +67,10 @@ def test_normalized_backward_requires_saved_forward_outputs():
                                    1e-8, 0.0, 3.0, p, p, p, None, p, 64, None)
     assert rc == _native.ERR_INVALID_ARGUMENT
     assert "den and out" in _native.last_error()
+
+
+def test_kernel_timing_rejects_unknown_kernels():
+    lib = _native.load()
+    assert lib.ssplat_kernel_timing_read(7, None, None) == _native.ERR_INVALID_ARGUMENT
+    n, ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)  # nothing recorded: no GPU call
+    assert (n, ms) == (0, 0.0)

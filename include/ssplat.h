@@ -212,6 +212,27 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 float* grad_opacities, void* workspace, size_t workspace_bytes,
                                 void* stream);
 
+/*
+ * Fit loss: (1 - w) * L1 + w * (1 - SSIM) of an RGB image against its target, and its gradient
+ * with respect to the image (the reference computes it with torch in fit.py:800-823 and
+ * metrics.py:88-151; no native counterpart).  L1 is the mean absolute difference over the 3 H W
+ * values; SSIM the mean of the SSIM map with an 11-tap Gaussian window (sigma 1.5) applied per
+ * channel as two zero-padded 1-D passes, C1 = 0.01^2, C2 = 0.03^2.  image, target and
+ * grad_image are (H, W, 3) float32; *loss is one device float.  grad_image may be NULL (loss
+ * only).  ssim_weight w is in [0, 1].  The loss is summed in a fixed order (deterministic).
+ */
+int ssplat_loss_workspace_bytes(int64_t total_pixels, int64_t total_tiles, size_t* bytes);
+int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int width,
+                        double ssim_weight, float* loss, float* grad_image, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Batched loss: the sum of every field's loss over concatenated images (a field table planned
+   with tile_size 16 by ssplat_batch_plan, copied to the device). */
+int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
+                              int64_t total_pixels, const float* image, const float* target,
+                              double ssim_weight, float* loss, float* grad_image,
+                              void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

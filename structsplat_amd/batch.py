@@ -26,7 +26,8 @@ from torch.autograd.function import once_differentiable
 from . import _native
 from . import fit as F
 from .gaussians import GaussianField
-from .metrics import C1, C2, gaussian_window, psnr_tensor
+from .loss import batch_l1_ssim_loss
+from .metrics import psnr_tensor
 from .render import _fade_alpha, _ptr, _stream
 
 
@@ -186,43 +187,6 @@ def concat_fields(fields: list[GaussianField]) -> GaussianField:
                          cat("opacities") if all(has_op) else None)
 
 
-class _SSIMGroup:
-    """Fields of one image size stacked as (B, 3, H, W) for one blur per statistic."""
-
-    def __init__(self, idx, batch: Batch, targets_flat: torch.Tensor):
-        self.idx = list(idx)
-        h, w = batch.sizes[self.idx[0]]
-        self.h, self.w = h, w
-        self.sl = slice(batch.pixel_slices[self.idx[0]].start,
-                        batch.pixel_slices[self.idx[-1]].stop)
-        self.wh, self.wv = gaussian_window(11, 1.5, targets_flat.device, targets_flat.dtype)
-        t = self._bchw(targets_flat)
-        with torch.no_grad():
-            self.t = t
-            self.mu_t = self._blur(t)
-            self.mu_t2 = self.mu_t * self.mu_t
-            self.sig_t = self._blur(t * t) - self.mu_t2
-
-    def _bchw(self, flat):
-        return flat[self.sl].view(len(self.idx), self.h, self.w, 3).permute(0, 3, 1, 2)
-
-    def _blur(self, x):
-        x = torch.nn.functional.conv2d(x, self.wh, padding=(0, 5), groups=3)
-        return torch.nn.functional.conv2d(x, self.wv, padding=(5, 0), groups=3)
-
-    def ssim_sum(self, img_flat):
-        """Sum over the group's fields of each field's mean SSIM."""
-        p = self._bchw(img_flat)
-        mu_p = self._blur(p)
-        mu_p2 = mu_p * mu_p
-        mu_pt = mu_p * self.mu_t
-        sig_p = self._blur(p * p) - mu_p2
-        sig_pt = self._blur(p * self.t) - mu_pt
-        s = ((2 * mu_pt + C1) * (2 * sig_pt + C2)) / ((mu_p2 + self.mu_t2 + C1) *
-                                                     (sig_p + self.sig_t + C2))
-        return s.mean(dim=(1, 2, 3)).sum()
-
-
 @dataclass
 class BatchResult:
     psnr: list
@@ -230,32 +194,15 @@ class BatchResult:
     steps: int
 
 
-def _check_sorted_by_size(batch: Batch):
-    groups = []
-    for i, s in enumerate(batch.sizes):
-        if groups and batch.sizes[groups[-1][-1]] == s:
-            groups[-1].append(i)
-        elif any(batch.sizes[g[0]] == s for g in groups):
-            raise ValueError("fields of one image size must be adjacent in the batch")
-        else:
-            groups.append([i])
-    return groups
-
-
 def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig,
               graph: bool = True) -> BatchResult:
-    """Fit every field to its target in one batched step per iteration (fields of one image size
-    must be adjacent).  The fitted rows are copied back into `fields`."""
+    """Fit every field to its target in one batched step per iteration.  The fitted rows are
+    copied back into `fields`."""
     dev = targets[0].device
     sizes = [tuple(t.shape[:2]) for t in targets]
     batch = Batch(sizes, [f.n for f in fields], dev)
-    groups = _check_sorted_by_size(batch)
     big = concat_fields(fields).trainable()
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
-    npix = torch.zeros(batch.total_pixels, 1, device=dev)
-    for s in batch.pixel_slices:
-        npix[s] = 1.0 / (3 * (s.stop - s.start))
-    ssim_groups = [_SSIMGroup(g, batch, targets_flat) for g in groups]
     opt = F.make_optimizer(big, cfg)
     lo = math.log(0.35)
     hi = torch.cat([torch.full((f.n, 1), math.log(max(h, w)), device=dev)
@@ -277,9 +224,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                                  big.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
                                  float(cfg.normalization_eps), _fade_alpha(cfg.support_fade),
                                  float(cfg.sigma_cutoff))
-        loss = (1.0 - w) * ((img - targets_flat).abs() * npix).sum()
-        if w != 0.0:
-            loss = loss + w * (len(fields) - sum(g.ssim_sum(img) for g in ssim_groups))
+        loss = batch_l1_ssim_loss(batch, img, targets_flat, w)
         loss.backward()
         opt.step()
         with torch.no_grad():

@@ -1,0 +1,448 @@
+// loss.hip - the fit step's image loss on gfx950: (1 - w) * L1 + w * (1 - SSIM) and its gradient
+// with respect to the rendered image, for one image or a batch of images (fields) of any sizes.
+//
+// Definitions (the reference's fit loss, fit.py:800-823 with pixel_loss "l1", and its SSIM,
+// metrics.py:88-151): per image of n = 3 H W values,
+//   L1   = sum |p - t| / n
+//   SSIM = sum S / n,  S = (2 mu_p mu_t + C1)(2 s_pt + C2) / ((mu_p^2 + mu_t^2 + C1)(s_p + s_t + C2))
+// with mu, s from an 11-tap Gaussian window (sigma 1.5) applied per channel as two zero-padded
+// 1-D passes, C1 = 0.01^2, C2 = 0.03^2.  A batch's loss is the sum of its images' losses.
+//
+// Gradient.  With A1 = 2 mu_p mu_t + C1, A2 = 2 s_pt + C2, B1 = mu_p^2 + mu_t^2 + C1,
+// B2 = s_p + s_t + C2 and the window moments mu_p = G*p, m2 = G*p^2, mpt = G*(p t):
+//   dS/dmu_p = 2 mu_t (A2 - A1) / (B1 B2) - 2 mu_p S (1/B1 - 1/B2)
+//   dS/dm2   = -S / B2
+//   dS/dmpt  = 2 A1 / (B1 B2)
+// and, the window being symmetric, dSSIM/dp = (G*a + 2 p G*b + t G*c) / n for the three maps
+// a, b, c above (zero outside the image).
+//
+// Kernels, one 256-thread workgroup per 16x16 tile (the render tiles of the field table):
+//   k_loss_stats : stages p and t with a 5-pixel halo in LDS, blurs p, t, p^2, t^2, p t
+//                  horizontally then vertically (sliding windows in registers), writes a, b, c,
+//                  the L1 gradient (without SSIM) and the tile's partial loss.
+//   k_loss_grad  : stages a, b, c with the halo, blurs them and adds the SSIM gradient.
+//   k_loss_total : one workgroup sums the tile partials in a fixed order (deterministic loss).
+
+#include <cmath>
+#include <cstdint>
+
+#include "common.h"
+
+namespace {
+
+using namespace ssplat_detail;
+
+constexpr int kLT = 16;               // tile edge
+constexpr int kHalo = 5;              // window radius
+constexpr int kWin = 2 * kHalo + 1;   // 11 taps
+constexpr int kSpan = kLT + 2 * kHalo;  // 26: staged rows / columns
+constexpr int kLoadRow = 3 * kSpan;     // 78 floats per staged row (3 channels)
+constexpr int kLossThreads = 256;
+constexpr int kHSeg = 8;              // horizontal pass: outputs per thread
+constexpr int kVSeg = 4;              // vertical pass: outputs per thread
+constexpr int kHTasks = 3 * kSpan * (kLT / kHSeg);  // 156 (channel, row, segment) tasks
+constexpr int kVTasks = 3 * kLT * (kLT / kVSeg);    // 192 (channel, column, segment) tasks
+constexpr float kC1 = 0.01f * 0.01f;
+constexpr float kC2 = 0.03f * 0.03f;
+static_assert(kHTasks <= kLossThreads && kVTasks <= kLossThreads, "one task per thread");
+
+struct Window {
+  float w[kWin];
+};
+
+struct LossArgs {
+  FieldTable fields;
+  const float* img;     // (pixels, 3)
+  const float* target;  // (pixels, 3)
+  float* grad;          // (pixels, 3) or NULL (loss only)
+  float* abc;           // 3 maps of (pixels, 3): dS/dmu_p, dS/dm2, dS/dmpt
+  float* partial;       // one float per tile
+  int64_t pixels;       // pixels over all fields
+  float w_l1;           // 1 - ssim_weight
+  float w_ssim;         // ssim_weight
+  Window win;
+};
+
+struct TileGeo {
+  int x0, y0, W, H;
+  int64_t first_pixel;
+  float inv_n;  // 1 / (3 H W)
+};
+
+__device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
+  const ssplat_field f = field_of_tile(a.fields, tile);
+  const int local = tile - static_cast<int>(f.first_tile);
+  TileGeo g;
+  g.x0 = (local % f.tiles_x) * kLT;
+  g.y0 = (local / f.tiles_x) * kLT;
+  g.W = f.width;
+  g.H = f.height;
+  g.first_pixel = f.first_pixel;
+  g.inv_n = 1.0f / (3.0f * static_cast<float>(f.height) * static_cast<float>(f.width));
+  return g;
+}
+
+// Stages rows y0-5 .. y0+20, columns x0-5 .. x0+20 of the 3-channel map `src` into
+// dst[ch][row][col], zero outside the image.  Each staged row is 78 contiguous floats.
+__device__ __forceinline__ void stage_halo(const float* __restrict__ src, const TileGeo& g,
+                                           float (*dst)[kSpan][kSpan + 1]) {
+  for (int i = threadIdx.x; i < kSpan * kLoadRow; i += kLossThreads) {
+    const int row = i / kLoadRow;
+    const int c3 = i - row * kLoadRow;
+    const int col = c3 / 3;
+    const int ch = c3 - col * 3;
+    const int gx = g.x0 - kHalo + col;
+    const int gy = g.y0 - kHalo + row;
+    float v = 0.0f;
+    if (gx >= 0 && gx < g.W && gy >= 0 && gy < g.H) {
+      v = src[(g.first_pixel + int64_t(gy) * g.W + gx) * 3 + ch];
+    }
+    dst[ch][row][col] = v;
+  }
+}
+
+// Sum of `vals` over the workgroup (returned to thread 0).
+__device__ float block_sum(float v) {
+  __shared__ float s_red[kLossThreads / 64];
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kLossThreads / 64; ++w) t += s_red[w];
+  }
+  return t;
+}
+
+template <bool kSSIM>
+__global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const TileGeo g = tile_geo(a, tile);
+  float l1 = 0.0f, ssim_sum = 0.0f;
+  if (!kSSIM) {
+    // L1 only: one pixel per thread.
+    const int x = g.x0 + (threadIdx.x & (kLT - 1));
+    const int y = g.y0 + (threadIdx.x >> 4);
+    if (x < g.W && y < g.H) {
+      const int64_t base = (g.first_pixel + int64_t(y) * g.W + x) * 3;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float d = a.img[base + ch] - a.target[base + ch];
+        l1 += fabsf(d);
+        if (a.grad) {
+          const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+          a.grad[base + ch] = a.w_l1 * g.inv_n * sg;
+        }
+      }
+    }
+    const float t = block_sum(l1);
+    if (threadIdx.x == 0) a.partial[tile] = a.w_l1 * g.inv_n * t;
+    return;
+  }
+
+  __shared__ float s_p[3][kSpan][kSpan + 1];
+  __shared__ float s_t[3][kSpan][kSpan + 1];
+  __shared__ float s_h[5][3][kSpan][kLT + 1];  // horizontally blurred p, t, p^2, t^2, p t
+  stage_halo(a.img, g, s_p);
+  stage_halo(a.target, g, s_t);
+  __syncthreads();
+
+  // Horizontal pass: task = (channel, staged row, 8-column segment).
+  if (threadIdx.x < kHTasks) {
+    const int ch = threadIdx.x / (kSpan * 2);
+    const int rem = threadIdx.x - ch * (kSpan * 2);
+    const int row = rem >> 1;
+    const int c0 = (rem & 1) * kHSeg;
+    float pv[kHSeg + kWin - 1], tv[kHSeg + kWin - 1];
+#pragma unroll
+    for (int i = 0; i < kHSeg + kWin - 1; ++i) {
+      pv[i] = s_p[ch][row][c0 + i];
+      tv[i] = s_t[ch][row][c0 + i];
+    }
+#pragma unroll
+    for (int o = 0; o < kHSeg; ++o) {
+      float hp = 0.0f, ht = 0.0f, hpp = 0.0f, htt = 0.0f, hpt = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kWin; ++k) {
+        const float w = a.win.w[k];
+        const float p = pv[o + k], t = tv[o + k];
+        const float wp = w * p, wt = w * t;
+        hp += wp;
+        ht += wt;
+        hpp = fmaf(wp, p, hpp);
+        htt = fmaf(wt, t, htt);
+        hpt = fmaf(wp, t, hpt);
+      }
+      s_h[0][ch][row][c0 + o] = hp;
+      s_h[1][ch][row][c0 + o] = ht;
+      s_h[2][ch][row][c0 + o] = hpp;
+      s_h[3][ch][row][c0 + o] = htt;
+      s_h[4][ch][row][c0 + o] = hpt;
+    }
+  }
+  __syncthreads();
+
+  // Vertical pass: task = (channel, column, 4-row segment); then S and its derivatives.
+  if (threadIdx.x < kVTasks) {
+    const int ch = threadIdx.x >> 6;
+    const int col = threadIdx.x & (kLT - 1);
+    const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
+    float st[5][kVSeg];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      float v[kVSeg + kWin - 1];
+#pragma unroll
+      for (int i = 0; i < kVSeg + kWin - 1; ++i) v[i] = s_h[q][ch][r0 + i][col];
+#pragma unroll
+      for (int o = 0; o < kVSeg; ++o) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
+        st[q][o] = acc;
+      }
+    }
+    const int x = g.x0 + col;
+#pragma unroll
+    for (int o = 0; o < kVSeg; ++o) {
+      const int y = g.y0 + r0 + o;
+      if (x >= g.W || y >= g.H) continue;
+      const float mu_p = st[0][o], mu_t = st[1][o];
+      const float mu_p2 = mu_p * mu_p, mu_t2 = mu_t * mu_t, mu_pt = mu_p * mu_t;
+      const float s_pp = st[2][o] - mu_p2;
+      const float s_tt = st[3][o] - mu_t2;
+      const float s_pt = st[4][o] - mu_pt;
+      const float A1 = 2.0f * mu_pt + kC1;
+      const float A2 = 2.0f * s_pt + kC2;
+      const float B1 = mu_p2 + mu_t2 + kC1;
+      const float B2 = s_pp + s_tt + kC2;
+      const float inv_d = 1.0f / (B1 * B2);
+      const float S = A1 * A2 * inv_d;
+      const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
+      ssim_sum += S;
+      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+      if (a.grad) {
+        const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
+        a.abc[at] = da;
+        a.abc[a.pixels * 3 + at] = -S * inv_b2;
+        a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
+      }
+      const float d = s_p[ch][kHalo + r0 + o][kHalo + col] - s_t[ch][kHalo + r0 + o][kHalo + col];
+      l1 += fabsf(d);
+    }
+  }
+  const float l1_tot = block_sum(l1);
+  __syncthreads();
+  const float ssim_tot = block_sum(ssim_sum);
+  if (threadIdx.x == 0) {
+    a.partial[tile] = g.inv_n * (a.w_l1 * l1_tot - a.w_ssim * ssim_tot);
+  }
+}
+
+// dL/dp = w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n.
+__global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const TileGeo g = tile_geo(a, tile);
+  __shared__ float s_m[3][3][kSpan][kSpan + 1];  // a, b, c staged with the halo
+  __shared__ float s_h[3][3][kSpan][kLT + 1];
+  stage_halo(a.abc, g, s_m[0]);
+  stage_halo(a.abc + a.pixels * 3, g, s_m[1]);
+  stage_halo(a.abc + 2 * a.pixels * 3, g, s_m[2]);
+  __syncthreads();
+  if (threadIdx.x < kHTasks) {
+    const int ch = threadIdx.x / (kSpan * 2);
+    const int rem = threadIdx.x - ch * (kSpan * 2);
+    const int row = rem >> 1;
+    const int c0 = (rem & 1) * kHSeg;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float v[kHSeg + kWin - 1];
+#pragma unroll
+      for (int i = 0; i < kHSeg + kWin - 1; ++i) v[i] = s_m[q][ch][row][c0 + i];
+#pragma unroll
+      for (int o = 0; o < kHSeg; ++o) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
+        s_h[q][ch][row][c0 + o] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kVTasks) {
+    const int ch = threadIdx.x >> 6;
+    const int col = threadIdx.x & (kLT - 1);
+    const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
+    float bl[3][kVSeg];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float v[kVSeg + kWin - 1];
+#pragma unroll
+      for (int i = 0; i < kVSeg + kWin - 1; ++i) v[i] = s_h[q][ch][r0 + i][col];
+#pragma unroll
+      for (int o = 0; o < kVSeg; ++o) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
+        bl[q][o] = acc;
+      }
+    }
+    const int x = g.x0 + col;
+#pragma unroll
+    for (int o = 0; o < kVSeg; ++o) {
+      const int y = g.y0 + r0 + o;
+      if (x >= g.W || y >= g.H) continue;
+      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+      const float p = a.img[at], t = a.target[at];
+      const float d = p - t;
+      const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+      const float dssim = bl[0][o] + 2.0f * p * bl[1][o] + t * bl[2][o];
+      a.grad[at] = g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim);
+    }
+  }
+}
+
+// loss = w_ssim * num_fields + sum of the tile partials, summed in float64 in a fixed order.
+__global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ partial,
+                                                     int64_t tiles, double constant,
+                                                     float* __restrict__ loss) {
+  __shared__ double s_red[1024 / 64];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < tiles; i += 1024) v += partial[i];
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = constant;
+    for (int w = 0; w < 1024 / 64; ++w) t += s_red[w];
+    *loss = static_cast<float>(t);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+
+struct LossLayout {
+  float* abc;
+  float* partial;
+  size_t bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+LossLayout loss_layout(void* base, int64_t pixels, int64_t tiles) {
+  LossLayout l{};
+  const size_t abc_bytes = align256(size_t(pixels) * 9 * sizeof(float));
+  const size_t part_bytes = align256(size_t(tiles) * sizeof(float));
+  char* b = static_cast<char*>(base);
+  l.abc = reinterpret_cast<float*>(b);
+  l.partial = reinterpret_cast<float*>(b + abc_bytes);
+  l.bytes = abc_bytes + part_bytes;
+  return l;
+}
+
+Window gaussian_window() {
+  // The reference's window (metrics.py:71-88), built in float32: exp(-x^2 / (2 sigma^2)) for
+  // x = -5..5, sigma = 1.5, divided by its sum.
+  Window w;
+  float sum = 0.0f;
+  for (int k = 0; k < kWin; ++k) {
+    const float x = static_cast<float>(k - kHalo);
+    w.w[k] = std::exp(-(x * x) / (2.0f * 1.5f * 1.5f));
+    sum += w.w[k];
+  }
+  for (int k = 0; k < kWin; ++k) w.w[k] /= sum;
+  return w;
+}
+
+int run_loss(const FieldTable& table, int num_fields, int64_t tiles, int64_t pixels,
+             const float* img, const float* target, double ssim_weight, float* loss, float* grad,
+             void* ws, size_t ws_bytes, void* stream) {
+  if (img == nullptr || target == nullptr || loss == nullptr) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "image, target and loss must not be NULL");
+  }
+  if (!(std::isfinite(ssim_weight) && ssim_weight >= 0.0 && ssim_weight <= 1.0)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "ssim_weight must be in [0, 1]");
+  }
+  if (tiles <= 0 || tiles > INT32_MAX || pixels <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the images must have pixels");
+  }
+  const LossLayout l = loss_layout(ws, pixels, tiles);
+  if (ws == nullptr || reinterpret_cast<uintptr_t>(ws) % 256 != 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace must be non-NULL and 256-byte aligned");
+  }
+  if (ws_bytes < l.bytes) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace has %zu bytes, needs %zu", ws_bytes,
+                     l.bytes);
+  }
+  LossArgs a;
+  a.fields = table;
+  a.img = img;
+  a.target = target;
+  a.grad = grad;
+  a.abc = l.abc;
+  a.partial = l.partial;
+  a.pixels = pixels;
+  a.w_ssim = static_cast<float>(ssim_weight);
+  a.w_l1 = static_cast<float>(1.0 - ssim_weight);
+  a.win = gaussian_window();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = static_cast<int>(tiles);
+  if (ssim_weight > 0.0) {
+    k_loss_stats<true><<<grid, kLossThreads, 0, s>>>(a);
+    SSPLAT_LAUNCH_CHECK("k_loss_stats");
+    if (grad != nullptr) {
+      k_loss_grad<<<grid, kLossThreads, 0, s>>>(a);
+      SSPLAT_LAUNCH_CHECK("k_loss_grad");
+    }
+  } else {
+    k_loss_stats<false><<<grid, kLossThreads, 0, s>>>(a);
+    SSPLAT_LAUNCH_CHECK("k_loss_stats");
+  }
+  k_loss_total<<<1, 1024, 0, s>>>(l.partial, tiles, ssim_weight * num_fields, loss);
+  SSPLAT_LAUNCH_CHECK("k_loss_total");
+  return success();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssplat_loss_workspace_bytes(int64_t total_pixels, int64_t total_tiles, size_t* bytes) {
+  if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  if (total_pixels <= 0 || total_tiles <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "pixels and tiles must be positive");
+  }
+  *bytes = loss_layout(nullptr, total_pixels, total_tiles).bytes;
+  return success();
+}
+
+int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int width,
+                        double ssim_weight, float* loss, float* grad_image, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  if (height <= 0 || width <= 0 || height > 65535 || width > 65535) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be in [1, 65535]");
+  }
+  ssplat_field f{};
+  f.height = height;
+  f.width = width;
+  f.tiles_x = (width + kLT - 1) / kLT;
+  f.num_tiles = f.tiles_x * ((height + kLT - 1) / kLT);
+  const int64_t tiles = int64_t(f.tiles_x) * ((height + kLT - 1) / kLT);
+  return run_loss(FieldTable{nullptr, 0, f}, 1, tiles, int64_t(height) * width, image, target,
+                  ssim_weight, loss, grad_image, workspace, workspace_bytes, stream);
+}
+
+int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
+                              int64_t total_pixels, const float* image, const float* target,
+                              double ssim_weight, float* loss, float* grad_image,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  if (fields == nullptr || num_fields <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "fields must be a non-empty device table");
+  }
+  return run_loss(FieldTable{fields, num_fields, ssplat_field{}}, num_fields, total_tiles,
+                  total_pixels, image, target, ssim_weight, loss, grad_image, workspace,
+                  workspace_bytes, stream);
+}
+
+}  // extern "C"

@@ -30,13 +30,13 @@
 #include <string>
 #include <vector>
 
-#include "ssplat.h"
-
-namespace {
+#include "common.h"
 
 // ------------------------------------------------------------------------------------------
-// Error reporting
+// Error reporting (shared with the other sources of the library through common.h)
 // ------------------------------------------------------------------------------------------
+
+namespace ssplat_detail {
 
 thread_local std::string g_last_error;
 
@@ -55,13 +55,13 @@ int success() {
   return SSPLAT_OK;
 }
 
-#define SSPLAT_LAUNCH_CHECK(name)                                                           \
-  do {                                                                                      \
-    hipError_t err_ = hipGetLastError();                                                    \
-    if (err_ != hipSuccess) {                                                               \
-      return set_error(SSPLAT_ERR_HIP, "%s launch failed: %s", name, hipGetErrorString(err_)); \
-    }                                                                                       \
-  } while (0)
+const char* last_error() { return g_last_error.c_str(); }
+
+}  // namespace ssplat_detail
+
+namespace {
+
+using namespace ssplat_detail;
 
 // ------------------------------------------------------------------------------------------
 // Kernel timing (measurement hook): event pairs recorded on the launch stream around every
@@ -124,7 +124,6 @@ constexpr int kSortWaveLds = 1024;  // lists up to this length sort in one wave'
 constexpr int kRenderThreads = 256;
 constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
 constexpr int kTile = 16;    // render tile edge (pixels)
-constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin across them
 static_assert(kRenderThreads == kTile * kTile, "one render thread per tile pixel");
 // -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
 constexpr float kNegHalfLog2e = -0.72134752044448170368f;
@@ -207,16 +206,6 @@ __device__ __forceinline__ float tile_min_q(float mx, float my, float a, float b
 // Hardware 2^x (v_exp_f32).  The same instruction evaluates both the per-pixel weight and the
 // fade tail, so a pixel whose quadratic equals sigma_cutoff^2 gets a faded weight of exactly 0.
 __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
-
-// Workgroup -> tile mapping that gives each XCD a contiguous run of tiles, so neighbouring
-// tiles (which share most of their Gaussians) hit the same XCD-private L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-  const int per = nb / kXcds;
-  const int rem = nb % kXcds;
-  const int xcd = bid % kXcds;
-  const int local = bid / kXcds;
-  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
-}
 
 template <int kCtrl>
 __device__ __forceinline__ float dpp_move(float v) {
@@ -308,49 +297,6 @@ int fill_f32(float* p, int64_t count, float value, hipStream_t stream) {
   k_fill_f32<<<blocks, 256, 0, stream>>>(p, count, value);
   SSPLAT_LAUNCH_CHECK("fill_f32");
   return SSPLAT_OK;
-}
-
-// ------------------------------------------------------------------------------------------
-// Fields: one image and its Gaussians.  A batch concatenates the fields' Gaussians, tiles and
-// pixels; every kernel finds its field by binary search over the (device) field table.  A single
-// field travels by value in the kernel arguments instead.
-// ------------------------------------------------------------------------------------------
-
-struct FieldTable {
-  const ssplat_field* fields;  // device table of num_fields entries, or NULL for `single`
-  int num_fields;
-  ssplat_field single;
-};
-
-// Field owning global tile `tile` (the last field whose first_tile <= tile).
-__device__ __forceinline__ ssplat_field field_of_tile(const FieldTable& t, int tile) {
-  if (t.fields == nullptr) return t.single;
-  int lo = 0, hi = t.num_fields - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (t.fields[mid].first_tile <= tile) {
-      lo = mid;
-    } else {
-      hi = mid - 1;
-    }
-  }
-  return t.fields[lo];
-}
-
-// Field owning global Gaussian `i` (the last field whose first_gaussian <= i; fields are
-// contiguous, so an empty field never wins over the next one).
-__device__ __forceinline__ ssplat_field field_of_gaussian(const FieldTable& t, int i) {
-  if (t.fields == nullptr) return t.single;
-  int lo = 0, hi = t.num_fields - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (t.fields[mid].first_gaussian <= i) {
-      lo = mid;
-    } else {
-      hi = mid - 1;
-    }
-  }
-  return t.fields[lo];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1606,7 +1552,7 @@ extern "C" {
 
 int ssplat_abi_version(void) { return SSPLAT_ABI_VERSION; }
 
-const char* ssplat_last_error(void) { return g_last_error.c_str(); }
+const char* ssplat_last_error(void) { return last_error(); }
 
 int ssplat_kernel_timing_enable(int enable) {
   KernelTimer& t = kernel_timer();

@@ -28,7 +28,8 @@ from torch.autograd.function import once_differentiable
 
 from . import render as HR
 from .gaussians import GaussianField
-from .metrics import SSIMTargetStats, psnr_tensor, ssim
+from .loss import l1_ssim_loss
+from .metrics import psnr_tensor
 
 
 @dataclass
@@ -115,13 +116,10 @@ def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> to
                                 float(cfg.sigma_cutoff))
 
 
-def loss_fn(img: torch.Tensor, target: torch.Tensor, stats: SSIMTargetStats | None,
-            w: float) -> torch.Tensor:
-    """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1")."""
-    l1 = (img - target).abs().mean()
-    if w == 0.0:
-        return l1
-    return (1.0 - w) * l1 + w * (1.0 - ssim(img, stats=stats))
+def loss_fn(img: torch.Tensor, target: torch.Tensor, w: float) -> torch.Tensor:
+    """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1"), computed with its
+    gradient by the fused HIP loss kernels."""
+    return l1_ssim_loss(img, target, w)
 
 
 def pair_count(field: GaussianField, H: int, W: int, cfg: FitConfig, tile_size: int = 16) -> int:
@@ -167,7 +165,6 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     dev = target.device
     field.trainable()
     opt = make_optimizer(field, cfg)
-    stats = SSIMTargetStats(target) if cfg.ssim_weight != 0.0 else None
     lo, hi = math.log(0.35), math.log(max(H, W))
     plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev)
     state = {"loss": None}
@@ -175,7 +172,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     def step():
         opt.zero_grad(set_to_none=False)
         img = planned_render(field, plan, cfg)
-        loss = loss_fn(img, target, stats, cfg.ssim_weight)
+        loss = loss_fn(img, target, cfg.ssim_weight)
         loss.backward()
         opt.step()
         with torch.no_grad():

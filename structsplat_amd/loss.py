@@ -1,0 +1,103 @@
+"""The fit step's image loss on the GPU: (1 - w) * L1 + w * (1 - SSIM) through ``libssplat_hip``.
+
+Same definition as the reference's fit loss with ``pixel_loss="l1"`` (fit.py:800-823) and its
+SSIM (metrics.py:88-151): L1 is the mean absolute difference over the 3 H W values, SSIM the mean
+of the SSIM map with the 11-tap, sigma-1.5 Gaussian window applied per channel as two zero-padded
+1-D passes.  The kernels compute the loss and its gradient with respect to the image in one call
+(the loss is a scalar, so the backward only scales the stored gradient); a batch's loss is the sum
+of its images' losses.  ``metrics.ssim`` is the torch statement of the same SSIM.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
+
+from . import _native
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def loss_workspace(pixels: int, tiles: int, device) -> torch.Tensor:
+    n = ctypes.c_size_t(0)
+    _native.check(_native.load().ssplat_loss_workspace_bytes(int(pixels), int(tiles),
+                                                               ctypes.byref(n)),
+                  "ssplat_loss_workspace_bytes")
+    return torch.empty(max(int(n.value), 256), dtype=torch.uint8, device=device)
+
+
+def _check_pair(img, target, what):
+    if not (img.is_cuda and target.is_cuda):
+        raise RuntimeError(f"{what} requires CUDA tensors")
+    if img.dtype != torch.float32 or target.dtype != torch.float32:
+        raise ValueError(f"{what}: image and target must be float32")
+    if img.shape != target.shape or img.shape[-1] != 3:
+        raise ValueError(f"{what}: image and target must have the same (..., 3) shape")
+
+
+def _check_weight(w: float) -> float:
+    w = float(w)
+    if not 0.0 <= w <= 1.0:
+        raise ValueError(f"ssim_weight must be in [0, 1], got {w}")
+    return w
+
+
+class _L1SSIM(Function):
+    @staticmethod
+    def forward(ctx, img, target, ssim_weight, batch):
+        img = img.contiguous()
+        target = target.contiguous()
+        dev = img.device
+        loss = torch.empty((), dtype=torch.float32, device=dev)
+        grad = torch.empty_like(img) if ctx.needs_input_grad[0] else None
+        lib = _native.load()
+        if batch is None:
+            H, W = int(img.shape[0]), int(img.shape[1])
+            tiles = ((H + 15) // 16) * ((W + 15) // 16)
+            ws = loss_workspace(H * W, tiles, dev)
+            _native.check(lib.ssplat_l1_ssim_loss(
+                _ptr(img), _ptr(target), H, W, float(ssim_weight), _ptr(loss), _ptr(grad),
+                _ptr(ws), ws.numel(), _stream(dev)), "ssplat_l1_ssim_loss")
+        else:
+            ws = loss_workspace(batch.total_pixels, batch.total_tiles, dev)
+            _native.check(lib.ssplat_batch_l1_ssim_loss(
+                _ptr(batch.fields_dev), batch.num_fields, batch.total_tiles, batch.total_pixels,
+                _ptr(img), _ptr(target), float(ssim_weight), _ptr(loss), _ptr(grad), _ptr(ws),
+                ws.numel(), _stream(dev)), "ssplat_batch_l1_ssim_loss")
+        ctx.save_for_backward(grad)
+        return loss
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_loss):
+        (grad,) = ctx.saved_tensors
+        return grad * grad_loss, None, None, None
+
+
+def l1_ssim_loss(img: torch.Tensor, target: torch.Tensor, ssim_weight: float = 0.2):
+    """(1 - w) * L1 + w * (1 - SSIM) of an (H, W, 3) image against its target (differentiable in
+    ``img``)."""
+    _check_pair(img, target, "l1_ssim_loss")
+    if img.dim() != 3:
+        raise ValueError("l1_ssim_loss: image must be (H, W, 3)")
+    return _L1SSIM.apply(img, target, _check_weight(ssim_weight), None)
+
+
+def batch_l1_ssim_loss(batch, img_flat: torch.Tensor, target_flat: torch.Tensor,
+                       ssim_weight: float = 0.2):
+    """Sum over the fields of ``batch`` (a ``batch.Batch`` planned with 16-pixel tiles) of each
+    field's loss, for concatenated ``(total_pixels, 3)`` images."""
+    _check_pair(img_flat, target_flat, "batch_l1_ssim_loss")
+    if batch.tile_size != 16:
+        raise ValueError("batch_l1_ssim_loss needs a batch planned with tile_size 16")
+    if img_flat.shape != (batch.total_pixels, 3):
+        raise ValueError("batch_l1_ssim_loss: images must be (total_pixels, 3)")
+    return _L1SSIM.apply(img_flat, target_flat, _check_weight(ssim_weight), batch)

@@ -29,14 +29,13 @@ def test_graph_replays_match_eager_steps(cuda):
     H, W = target.shape[:2]
     field.trainable()
     opt = F.make_optimizer(field, cfg)
-    stats = F.SSIMTargetStats(target)
     plan = F.RenderPlan(field.n, H, W, 4 * F.pair_count(field, H, W, cfg) + 4096, cuda)
     losses = {}
 
     def step():
         opt.zero_grad(set_to_none=False)
         img = F.planned_render(field, plan, cfg)
-        loss = F.loss_fn(img, target, stats, cfg.ssim_weight)
+        loss = F.loss_fn(img, target, cfg.ssim_weight)
         loss.backward()
         opt.step()
         losses["last"] = loss.detach()

@@ -233,6 +233,33 @@ int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_
                               double ssim_weight, float* loss, float* grad_image,
                               void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Fit optimizer: one Adam step over up to 8 parameter groups in one launch (the reference's
+ * torch.optim.Adam, fit.py:672-691, default betas (0.9, 0.999), eps 1e-8), with the step count
+ * in device memory (*step is incremented first, so the call can be captured in a HIP graph):
+ *   m = m + (1 - b1)(g - m),  v = b2 v + (1 - b2) g^2,
+ *   p = p - lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps).
+ * A group with clamp set is then clamped to [lo, hi], or to [lo, hi_rows[i / row_width]] when
+ * hi_rows is given (the log-scale clamp of fit.py:3033, :3359-3375).  No reference native
+ * counterpart.
+ */
+typedef struct ssplat_adam_group {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  int64_t count;          /* elements of param (and grad, exp_avg, exp_avg_sq) */
+  double lr;
+  int32_t clamp;          /* 0: no clamp */
+  double lo;
+  double hi;              /* used when hi_rows is NULL */
+  const float* hi_rows;   /* per-row upper bound, or NULL */
+  int64_t row_width;      /* elements per row of param for hi_rows */
+} ssplat_adam_group;
+
+int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* step, double beta1,
+                     double beta2, double eps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

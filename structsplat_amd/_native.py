@@ -56,6 +56,7 @@ SIGNATURES = {
                                      _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_l1_ssim_loss": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_ptr, _c_ptr,
                                            _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_adam_step": (_c_int, [_c_ptr, _c_int, _c_ptr, _c_dbl, _c_dbl, _c_dbl, _c_ptr]),
     "ssplat_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_size)]),
@@ -90,6 +91,24 @@ class Field(ctypes.Structure):
 
 _LIB = None
 _LOCK = threading.Lock()
+
+
+class AdamGroup(ctypes.Structure):
+    """ssplat_adam_group of include/ssplat.h (one parameter group of an Adam step)."""
+
+    _fields_ = [
+        ("param", ctypes.c_void_p),
+        ("grad", ctypes.c_void_p),
+        ("exp_avg", ctypes.c_void_p),
+        ("exp_avg_sq", ctypes.c_void_p),
+        ("count", ctypes.c_int64),
+        ("lr", ctypes.c_double),
+        ("clamp", ctypes.c_int32),
+        ("lo", ctypes.c_double),
+        ("hi", ctypes.c_double),
+        ("hi_rows", ctypes.c_void_p),
+        ("row_width", ctypes.c_int64),
+    ]
 
 
 class NativeLibraryMissing(RuntimeError):

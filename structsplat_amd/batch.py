@@ -204,9 +204,9 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     big = concat_fields(fields).trainable()
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
     opt = F.make_optimizer(big, cfg)
-    lo = math.log(0.35)
-    hi = torch.cat([torch.full((f.n, 1), math.log(max(h, w)), device=dev)
+    hi = torch.cat([torch.full((f.n,), math.log(max(h, w)), device=dev)
                     for f, (h, w) in zip(fields, sizes)], 0)
+    opt.set_clamp(big.log_scales, math.log(0.35), hi)
     w = cfg.ssim_weight
 
     def count():
@@ -226,10 +226,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                                  float(cfg.sigma_cutoff))
         loss = batch_l1_ssim_loss(batch, img, targets_flat, w)
         loss.backward()
-        opt.step()
-        with torch.no_grad():
-            big.log_scales.clamp_(min=lo)
-            torch.minimum(big.log_scales, hi, out=big.log_scales)
+        opt.step()  # Adam and the per-field log-scale clamp
 
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()

@@ -83,21 +83,55 @@ __device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
 }
 
 // Stages rows y0-5 .. y0+20, columns x0-5 .. x0+20 of the 3-channel map `src` into
-// dst[ch][row][col], zero outside the image.  Each staged row is 78 contiguous floats.
-__device__ __forceinline__ void stage_halo(const float* __restrict__ src, const TileGeo& g,
-                                           float (*dst)[kSpan][kSpan + 1]) {
-  for (int i = threadIdx.x; i < kSpan * kLoadRow; i += kLossThreads) {
-    const int row = i / kLoadRow;
-    const int c3 = i - row * kLoadRow;
-    const int col = c3 / 3;
-    const int ch = c3 - col * 3;
-    const int gx = g.x0 - kHalo + col;
-    const int gy = g.y0 - kHalo + row;
-    float v = 0.0f;
-    if (gx >= 0 && gx < g.W && gy >= 0 && gy < g.H) {
-      v = src[(g.first_pixel + int64_t(gy) * g.W + gx) * 3 + ch];
+// dst[ch][row][col], zero outside the image.  Each staged row is 78 contiguous floats.  All of a
+// thread's loads are issued before its LDS stores (one memory latency per staging); a position
+// outside the image loads element 0 of the map, which always exists, and stores 0.
+constexpr int kHaloItems = kSpan * kLoadRow;
+constexpr int kHaloPer = (kHaloItems + kLossThreads - 1) / kLossThreads;
+
+struct HaloSlot {
+  int lds;   // offset in dst (floats), -1 when the slot is past the staged rows
+  bool ok;   // inside the image
+  int64_t at;
+};
+
+__device__ __forceinline__ HaloSlot halo_slot(const TileGeo& g, int k) {
+  HaloSlot h;
+  const int i = static_cast<int>(threadIdx.x) + k * kLossThreads;
+  const int row = i / kLoadRow;
+  const int c3 = i - row * kLoadRow;
+  const int col = c3 / 3;
+  const int ch = c3 - col * 3;
+  const int gx = g.x0 - kHalo + col;
+  const int gy = g.y0 - kHalo + row;
+  h.lds = i < kHaloItems ? (ch * kSpan + row) * (kSpan + 1) + col : -1;
+  h.ok = i < kHaloItems && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H;
+  h.at = h.ok ? (g.first_pixel + int64_t(gy) * g.W + gx) * 3 + ch : 0;
+  return h;
+}
+
+template <int kMaps>
+__device__ __forceinline__ void stage_halo(const float* const (&src)[kMaps], const TileGeo& g,
+                                           float (*const (&dst)[kMaps])[kSpan][kSpan + 1]) {
+  HaloSlot slot[kHaloPer];
+#pragma unroll
+  for (int k = 0; k < kHaloPer; ++k) slot[k] = halo_slot(g, k);
+  float v[kMaps][kHaloPer];
+#pragma unroll
+  for (int m = 0; m < kMaps; ++m) {
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+      const float x = src[m][slot[k].at];
+      v[m][k] = slot[k].ok ? x : 0.0f;
     }
-    dst[ch][row][col] = v;
+  }
+#pragma unroll
+  for (int m = 0; m < kMaps; ++m) {
+    float* base = &dst[m][0][0][0];
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+      if (slot[k].lds >= 0) base[slot[k].lds] = v[m][k];
+    }
   }
 }
 
@@ -143,8 +177,11 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
   __shared__ float s_p[3][kSpan][kSpan + 1];
   __shared__ float s_t[3][kSpan][kSpan + 1];
   __shared__ float s_h[5][3][kSpan][kLT + 1];  // horizontally blurred p, t, p^2, t^2, p t
-  stage_halo(a.img, g, s_p);
-  stage_halo(a.target, g, s_t);
+  {
+    const float* const src[2] = {a.img, a.target};
+    float (*const dst[2])[kSpan][kSpan + 1] = {s_p, s_t};
+    stage_halo<2>(src, g, dst);
+  }
   __syncthreads();
 
   // Horizontal pass: task = (channel, staged row, 8-column segment).
@@ -244,9 +281,11 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
   const TileGeo g = tile_geo(a, tile);
   __shared__ float s_m[3][3][kSpan][kSpan + 1];  // a, b, c staged with the halo
   __shared__ float s_h[3][3][kSpan][kLT + 1];
-  stage_halo(a.abc, g, s_m[0]);
-  stage_halo(a.abc + a.pixels * 3, g, s_m[1]);
-  stage_halo(a.abc + 2 * a.pixels * 3, g, s_m[2]);
+  {
+    const float* const src[3] = {a.abc, a.abc + a.pixels * 3, a.abc + 2 * a.pixels * 3};
+    float (*const dst[3])[kSpan][kSpan + 1] = {s_m[0], s_m[1], s_m[2]};
+    stage_halo<3>(src, g, dst);
+  }
   __syncthreads();
   if (threadIdx.x < kHTasks) {
     const int ch = threadIdx.x / (kSpan * 2);

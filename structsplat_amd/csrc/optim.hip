@@ -1,0 +1,136 @@
+// optim.hip - the fit step's optimizer on gfx950: Adam over every parameter group of a field in one
+// launch, with the log-scale clamp fused into the update.
+//
+// The update is torch.optim.Adam's (the reference's optimizer, fit.py:672-691, with its default
+// betas (0.9, 0.999) and eps 1e-8), with the step count kept on the device so the launch can be
+// captured in a HIP graph:
+//   t += 1
+//   m = m + (1 - b1) (g - m)          v = b2 v + (1 - b2) g^2
+//   p = p - (lr / (1 - b1^t)) m / (sqrt(v) / sqrt(1 - b2^t) + eps)
+// followed, for a clamped group, by p = min(max(p, lo), hi) with hi one value or one per row
+// (fit.py:3033, :3359-3375: log-scales clamped to [ln 0.35, ln max(H, W)] of their image).
+
+#include <cmath>
+#include <cstdint>
+
+#include "common.h"
+
+namespace {
+
+using namespace ssplat_detail;
+
+constexpr int kMaxGroups = 8;
+constexpr int kAdamThreads = 256;
+
+struct AdamGroupArgs {
+  float* param;
+  const float* grad;
+  float* m;
+  float* v;
+  int64_t count;
+  const float* hi_rows;  // per-row upper bound (clamp with rows) or NULL
+  int64_t row_width;
+  float lr;
+  float lo, hi;
+  int clamp;
+};
+
+struct AdamArgs {
+  AdamGroupArgs g[kMaxGroups];
+  int64_t start[kMaxGroups + 1];  // prefix of the group counts
+  int num_groups;
+  const float* step;
+  float beta1, beta2, eps;
+};
+
+__global__ void k_adam_tick(float* step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1.0f;
+}
+
+__global__ __launch_bounds__(kAdamThreads) void k_adam(AdamArgs a) {
+  const float t = *a.step;
+  const float bc1 = 1.0f - powf(a.beta1, t);
+  const float bc2_sqrt = sqrtf(1.0f - powf(a.beta2, t));
+  const int64_t total = a.start[a.num_groups];
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+    int k = 0;
+    while (k + 1 < a.num_groups && i >= a.start[k + 1]) ++k;
+    const AdamGroupArgs& g = a.g[k];
+    const int64_t j = i - a.start[k];
+    const float grad = g.grad[j];
+    float m = g.m[j];
+    float v = g.v[j];
+    m = fmaf(1.0f - a.beta1, grad - m, m);
+    v = fmaf(1.0f - a.beta2, grad * grad, a.beta2 * v);
+    const float step_size = g.lr / bc1;
+    float p = g.param[j] - step_size * m / (sqrtf(v) / bc2_sqrt + a.eps);
+    if (g.clamp) {
+      const float hi = g.hi_rows != nullptr ? g.hi_rows[j / g.row_width] : g.hi;
+      p = fminf(fmaxf(p, g.lo), hi);
+    }
+    g.m[j] = m;
+    g.v[j] = v;
+    g.param[j] = p;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* step, double beta1,
+                     double beta2, double eps, void* stream) {
+  if (groups == nullptr || num_groups <= 0 || num_groups > kMaxGroups) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "num_groups must be in [1, %d]", kMaxGroups);
+  }
+  if (step == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "step is NULL");
+  if (!(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0 && eps >= 0.0)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "betas must be in [0, 1) and eps >= 0");
+  }
+  AdamArgs a{};
+  a.num_groups = num_groups;
+  a.step = step;
+  a.beta1 = static_cast<float>(beta1);
+  a.beta2 = static_cast<float>(beta2);
+  a.eps = static_cast<float>(eps);
+  int64_t total = 0;
+  for (int k = 0; k < num_groups; ++k) {
+    const ssplat_adam_group& s = groups[k];
+    if (s.count < 0) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: count < 0", k);
+    if (s.count > 0 && (s.param == nullptr || s.grad == nullptr || s.exp_avg == nullptr ||
+                        s.exp_avg_sq == nullptr)) {
+      return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: NULL tensor", k);
+    }
+    if (s.clamp && s.hi_rows != nullptr && s.row_width <= 0) {
+      return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: row_width must be > 0", k);
+    }
+    AdamGroupArgs& g = a.g[k];
+    g.param = s.param;
+    g.grad = s.grad;
+    g.m = s.exp_avg;
+    g.v = s.exp_avg_sq;
+    g.count = s.count;
+    g.lr = static_cast<float>(s.lr);
+    g.clamp = s.clamp != 0;
+    g.lo = static_cast<float>(s.lo);
+    g.hi = static_cast<float>(s.hi);
+    g.hi_rows = s.hi_rows;
+    g.row_width = s.row_width > 0 ? s.row_width : 1;
+    a.start[k] = total;
+    total += s.count;
+  }
+  a.start[num_groups] = total;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  k_adam_tick<<<1, 64, 0, st>>>(step);
+  SSPLAT_LAUNCH_CHECK("k_adam_tick");
+  if (total > 0) {
+    const int64_t blocks64 = (total + kAdamThreads - 1) / kAdamThreads;
+    const int blocks = static_cast<int>(blocks64 < 8192 ? blocks64 : 8192);
+    k_adam<<<blocks, kAdamThreads, 0, st>>>(a);
+    SSPLAT_LAUNCH_CHECK("k_adam");
+  }
+  return success();
+}
+
+}  // extern "C"

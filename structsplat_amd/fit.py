@@ -29,6 +29,7 @@ from torch.autograd.function import once_differentiable
 from . import render as HR
 from .gaussians import GaussianField
 from .loss import l1_ssim_loss
+from .optim import FusedAdam
 from .metrics import psnr_tensor
 
 
@@ -132,30 +133,26 @@ def pair_count(field: GaussianField, H: int, W: int, cfg: FitConfig, tile_size: 
     return int(offsets[-1].item())
 
 
-def make_optimizer(field: GaussianField, cfg: FitConfig) -> torch.optim.Adam:
+def make_optimizer(field: GaussianField, cfg: FitConfig) -> FusedAdam:
+    """Adam over the field's parameter groups (gaussians.py:203-214) in one HIP launch."""
     groups = field.parameter_groups(cfg.lr_means, cfg.lr_scales, cfg.lr_rot, cfg.lr_color,
                                     cfg.lr_opacity)
-    return torch.optim.Adam(groups, capturable=True)
+    return FusedAdam(groups)
 
 
 class _Snapshot:
-    def __init__(self, field: GaussianField, opt: torch.optim.Adam):
-        self.params = [p.detach().clone() for p in field.parameters()]
-        self.state = {id(p): {k: (v.clone() if torch.is_tensor(v) else v)
-                              for k, v in opt.state[p].items()} for p in field.parameters()
-                      if p in opt.state}
+    """Parameters and optimizer state at the start of a window (rolled back on overflow)."""
 
-    def restore(self, field: GaussianField, opt: torch.optim.Adam) -> None:
+    def __init__(self, field: GaussianField, opt: FusedAdam):
+        self.params = [p.detach().clone() for p in field.parameters()]
+        self.state = [t.clone() for t in opt.state_tensors()]
+
+    def restore(self, field: GaussianField, opt: FusedAdam) -> None:
         with torch.no_grad():
             for p, saved in zip(field.parameters(), self.params):
                 p.copy_(saved)
-                st = self.state.get(id(p))
-                if st is None:
-                    opt.state.pop(p, None)  # snapshot predates Adam's lazy state
-                    continue
-                for k, v in st.items():
-                    if torch.is_tensor(v):
-                        opt.state[p][k].copy_(v)
+            for t, saved in zip(opt.state_tensors(), self.state):
+                t.copy_(saved)
 
 
 def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool = False,
@@ -165,7 +162,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     dev = target.device
     field.trainable()
     opt = make_optimizer(field, cfg)
-    lo, hi = math.log(0.35), math.log(max(H, W))
+    opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))
     plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev)
     state = {"loss": None}
 
@@ -174,9 +171,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         img = planned_render(field, plan, cfg)
         loss = loss_fn(img, target, cfg.ssim_weight)
         loss.backward()
-        opt.step()
-        with torch.no_grad():
-            field.log_scales.clamp_(min=lo, max=hi)
+        opt.step()  # Adam and the log-scale clamp
         state["loss"] = loss.detach()
 
     done = 0

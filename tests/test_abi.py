@@ -74,3 +74,11 @@ def test_kernel_timing_rejects_unknown_kernels():
     assert lib.ssplat_kernel_timing_read(7, None, None) == _native.ERR_INVALID_ARGUMENT
     n, ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)  # nothing recorded: no GPU call
     assert (n, ms) == (0, 0.0)
+
+
+def test_adam_step_rejects_bad_groups():
+    lib = _native.load()
+    assert lib.ssplat_adam_step(None, 0, None, 0.9, 0.999, 1e-8, None) == _native.ERR_INVALID_ARGUMENT
+    groups = (_native.AdamGroup * 9)()
+    assert lib.ssplat_adam_step(groups, 9, None, 0.9, 0.999, 1e-8, None) == \
+        _native.ERR_INVALID_ARGUMENT

@@ -82,16 +82,19 @@ __device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
   return g;
 }
 
-// Stages rows y0-5 .. y0+20, columns x0-5 .. x0+20 of the 3-channel map `src` into
-// dst[ch][row][col], zero outside the image.  Each staged row is 78 contiguous floats.  All of a
-// thread's loads are issued before its LDS stores (one memory latency per staging); a position
-// outside the image loads element 0 of the map, which always exists, and stores 0.
+// Staging of rows y0-5 .. y0+20, columns x0-5 .. x0+20 of 3-channel maps into LDS as
+// [map][ch][row][col] (row stride kSpan + 1), zero outside the image.  Each staged row is 78
+// contiguous floats.  All of a thread's loads are issued before its LDS stores (one memory latency
+// per staging); a position outside the image loads element 0 of the map, which always exists,
+// and stores 0.
 constexpr int kHaloItems = kSpan * kLoadRow;
 constexpr int kHaloPer = (kHaloItems + kLossThreads - 1) / kLossThreads;
+constexpr int kMapFloats = 3 * kSpan * (kSpan + 1);  // one staged 3-channel map
 
 struct HaloSlot {
-  int lds;   // offset in dst (floats), -1 when the slot is past the staged rows
-  bool ok;   // inside the image
+  int lds;      // offset in a staged map (floats), -1 when the slot is past the staged rows
+  bool ok;      // inside the image
+  bool center;  // one of the tile's own pixels (inside the image)
   int64_t at;
 };
 
@@ -106,13 +109,16 @@ __device__ __forceinline__ HaloSlot halo_slot(const TileGeo& g, int k) {
   const int gy = g.y0 - kHalo + row;
   h.lds = i < kHaloItems ? (ch * kSpan + row) * (kSpan + 1) + col : -1;
   h.ok = i < kHaloItems && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H;
+  h.center = h.ok && row >= kHalo && row < kHalo + kLT && col >= kHalo && col < kHalo + kLT;
   h.at = h.ok ? (g.first_pixel + int64_t(gy) * g.W + gx) * 3 + ch : 0;
   return h;
 }
 
-template <int kMaps>
-__device__ __forceinline__ void stage_halo(const float* const (&src)[kMaps], const TileGeo& g,
-                                           float (*const (&dst)[kMaps])[kSpan][kSpan + 1]) {
+// Stages kMaps maps into lds[m * kMapFloats + ...]; returns sum |map0 - map1| over the tile's own
+// pixels when kL1 (the L1 term of the loss).
+template <int kMaps, bool kL1>
+__device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], const TileGeo& g,
+                                            float* lds) {
   HaloSlot slot[kHaloPer];
 #pragma unroll
   for (int k = 0; k < kHaloPer; ++k) slot[k] = halo_slot(g, k);
@@ -125,14 +131,15 @@ __device__ __forceinline__ void stage_halo(const float* const (&src)[kMaps], con
       v[m][k] = slot[k].ok ? x : 0.0f;
     }
   }
+  float l1 = 0.0f;
 #pragma unroll
-  for (int m = 0; m < kMaps; ++m) {
-    float* base = &dst[m][0][0][0];
+  for (int k = 0; k < kHaloPer; ++k) {
+    if (slot[k].lds < 0) continue;
 #pragma unroll
-    for (int k = 0; k < kHaloPer; ++k) {
-      if (slot[k].lds >= 0) base[slot[k].lds] = v[m][k];
-    }
+    for (int m = 0; m < kMaps; ++m) lds[m * kMapFloats + slot[k].lds] = v[m][k];
+    if (kL1 && slot[k].center) l1 += fabsf(v[0][k] - v[1][k]);
   }
+  return l1;
 }
 
 // Sum of `vals` over the workgroup (returned to thread 0).
@@ -174,47 +181,59 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     return;
   }
 
-  __shared__ float s_p[3][kSpan][kSpan + 1];
-  __shared__ float s_t[3][kSpan][kSpan + 1];
-  __shared__ float s_h[5][3][kSpan][kLT + 1];  // horizontally blurred p, t, p^2, t^2, p t
+  // LDS: the staged p and t, then (reusing the same space) the horizontally blurred
+  // p, t, p^2, t^2, p t.
+  constexpr int kStaged = 2 * kMapFloats;
+  constexpr int kBlurred = 5 * 3 * kSpan * (kLT + 1);
+  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
   {
     const float* const src[2] = {a.img, a.target};
-    float (*const dst[2])[kSpan][kSpan + 1] = {s_p, s_t};
-    stage_halo<2>(src, g, dst);
+    l1 = stage_halo<2, true>(src, g, s_buf);
   }
   __syncthreads();
 
-  // Horizontal pass: task = (channel, staged row, 8-column segment).
-  if (threadIdx.x < kHTasks) {
-    const int ch = threadIdx.x / (kSpan * 2);
-    const int rem = threadIdx.x - ch * (kSpan * 2);
-    const int row = rem >> 1;
-    const int c0 = (rem & 1) * kHSeg;
-    float pv[kHSeg + kWin - 1], tv[kHSeg + kWin - 1];
+  // Horizontal pass into registers: task = (channel, staged row, 8-column segment); input-major
+  // so each product p^2, t^2, p t is formed once.
+  const float (*s_p)[kSpan][kSpan + 1] = reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf);
+  const float (*s_t)[kSpan][kSpan + 1] =
+      reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + kMapFloats);
+  float h[5][kHSeg];
+  const bool h_task = threadIdx.x < kHTasks;
+  const int h_ch = threadIdx.x / (kSpan * 2);
+  const int h_rem = threadIdx.x - h_ch * (kSpan * 2);
+  const int h_row = h_rem >> 1;
+  const int h_c0 = (h_rem & 1) * kHSeg;
+  if (h_task) {
 #pragma unroll
-    for (int i = 0; i < kHSeg + kWin - 1; ++i) {
-      pv[i] = s_p[ch][row][c0 + i];
-      tv[i] = s_t[ch][row][c0 + i];
+    for (int q = 0; q < 5; ++q) {
+#pragma unroll
+      for (int o = 0; o < kHSeg; ++o) h[q][o] = 0.0f;
     }
 #pragma unroll
-    for (int o = 0; o < kHSeg; ++o) {
-      float hp = 0.0f, ht = 0.0f, hpp = 0.0f, htt = 0.0f, hpt = 0.0f;
+    for (int i = 0; i < kHSeg + kWin - 1; ++i) {
+      const float p = s_p[h_ch][h_row][h_c0 + i];
+      const float t = s_t[h_ch][h_row][h_c0 + i];
+      const float pp = p * p, tt = t * t, pt = p * t;
 #pragma unroll
-      for (int k = 0; k < kWin; ++k) {
+      for (int o = 0; o < kHSeg; ++o) {
+        const int k = i - o;
+        if (k < 0 || k >= kWin) continue;
         const float w = a.win.w[k];
-        const float p = pv[o + k], t = tv[o + k];
-        const float wp = w * p, wt = w * t;
-        hp += wp;
-        ht += wt;
-        hpp = fmaf(wp, p, hpp);
-        htt = fmaf(wt, t, htt);
-        hpt = fmaf(wp, t, hpt);
+        h[0][o] = fmaf(w, p, h[0][o]);
+        h[1][o] = fmaf(w, t, h[1][o]);
+        h[2][o] = fmaf(w, pp, h[2][o]);
+        h[3][o] = fmaf(w, tt, h[3][o]);
+        h[4][o] = fmaf(w, pt, h[4][o]);
       }
-      s_h[0][ch][row][c0 + o] = hp;
-      s_h[1][ch][row][c0 + o] = ht;
-      s_h[2][ch][row][c0 + o] = hpp;
-      s_h[3][ch][row][c0 + o] = htt;
-      s_h[4][ch][row][c0 + o] = hpt;
+    }
+  }
+  __syncthreads();  // every read of the staged p and t is done
+  float (*s_h)[3][kSpan][kLT + 1] = reinterpret_cast<float (*)[3][kSpan][kLT + 1]>(s_buf);
+  if (h_task) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+#pragma unroll
+      for (int o = 0; o < kHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
     }
   }
   __syncthreads();
@@ -252,19 +271,18 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
       const float A2 = 2.0f * s_pt + kC2;
       const float B1 = mu_p2 + mu_t2 + kC1;
       const float B2 = s_pp + s_tt + kC2;
-      const float inv_d = 1.0f / (B1 * B2);
+      // B1 >= C1 > 0 and B2 is C2 plus two variances: the hardware reciprocal (1 ulp) is safe.
+      const float inv_d = __builtin_amdgcn_rcpf(B1 * B2);
       const float S = A1 * A2 * inv_d;
       const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
       ssim_sum += S;
-      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
       if (a.grad) {
+        const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
         const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
         a.abc[at] = da;
         a.abc[a.pixels * 3 + at] = -S * inv_b2;
         a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
       }
-      const float d = s_p[ch][kHalo + r0 + o][kHalo + col] - s_t[ch][kHalo + r0 + o][kHalo + col];
-      l1 += fabsf(d);
     }
   }
   const float l1_tot = block_sum(l1);
@@ -279,31 +297,45 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
 __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileGeo g = tile_geo(a, tile);
-  __shared__ float s_m[3][3][kSpan][kSpan + 1];  // a, b, c staged with the halo
-  __shared__ float s_h[3][3][kSpan][kLT + 1];
+  // LDS: the staged a, b, c, then (reusing the same space) their horizontal blurs.
+  constexpr int kStaged = 3 * kMapFloats;
+  constexpr int kBlurred = 3 * 3 * kSpan * (kLT + 1);
+  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
   {
     const float* const src[3] = {a.abc, a.abc + a.pixels * 3, a.abc + 2 * a.pixels * 3};
-    float (*const dst[3])[kSpan][kSpan + 1] = {s_m[0], s_m[1], s_m[2]};
-    stage_halo<3>(src, g, dst);
+    stage_halo<3, false>(src, g, s_buf);
   }
   __syncthreads();
-  if (threadIdx.x < kHTasks) {
-    const int ch = threadIdx.x / (kSpan * 2);
-    const int rem = threadIdx.x - ch * (kSpan * 2);
-    const int row = rem >> 1;
-    const int c0 = (rem & 1) * kHSeg;
+  float h[3][kHSeg];
+  const bool h_task = threadIdx.x < kHTasks;
+  const int h_ch = threadIdx.x / (kSpan * 2);
+  const int h_rem = threadIdx.x - h_ch * (kSpan * 2);
+  const int h_row = h_rem >> 1;
+  const int h_c0 = (h_rem & 1) * kHSeg;
+  if (h_task) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
+      const float (*s_m)[kSpan][kSpan + 1] =
+          reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + q * kMapFloats);
       float v[kHSeg + kWin - 1];
 #pragma unroll
-      for (int i = 0; i < kHSeg + kWin - 1; ++i) v[i] = s_m[q][ch][row][c0 + i];
+      for (int i = 0; i < kHSeg + kWin - 1; ++i) v[i] = s_m[h_ch][h_row][h_c0 + i];
 #pragma unroll
       for (int o = 0; o < kHSeg; ++o) {
         float acc = 0.0f;
 #pragma unroll
         for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
-        s_h[q][ch][row][c0 + o] = acc;
+        h[q][o] = acc;
       }
+    }
+  }
+  __syncthreads();  // every read of the staged maps is done
+  float (*s_h)[3][kSpan][kLT + 1] = reinterpret_cast<float (*)[3][kSpan][kLT + 1]>(s_buf);
+  if (h_task) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+      for (int o = 0; o < kHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
     }
   }
   __syncthreads();

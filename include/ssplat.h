@@ -260,6 +260,29 @@ typedef struct ssplat_adam_group {
 int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* step, double beta1,
                      double beta2, double eps, void* stream);
 
+/*
+ * Renderer inputs from a field (the reference's GaussianField.conics / radii / opacity_values,
+ * gaussians.py:270-321, in one launch): conics (n,3), int64 radii (n,2) and opacities (n,) =
+ * sigmoid(opacity_logits) from log_scales (n,2), rotations (n,) and optional opacity_logits and
+ * filter_variance (n,).  Any output may be NULL.  The radii follow the reference's float32
+ * operation order, so they equal the torch expression's bit for bit.
+ */
+int ssplat_produce_field(const float* log_scales, const float* rotations,
+                         const float* opacity_logits, const float* filter_variance, int64_t n,
+                         double dilation, double sigma_cutoff, float* conics, int64_t* radii,
+                         float* opacities, void* stream);
+
+/*
+ * Backward of ssplat_produce_field: gradients of the conics (and opacities) to the log_scales,
+ * rotations (and opacity logits; opacities are the forward's sigmoid values).  Outputs are
+ * overwritten.  The radii carry no gradient (gaussians.py:306).
+ */
+int ssplat_produce_field_backward(const float* log_scales, const float* rotations,
+                                  const float* opacities, const float* filter_variance,
+                                  int64_t n, double dilation, const float* grad_conics,
+                                  const float* grad_opacities, float* grad_log_scales,
+                                  float* grad_rotations, float* grad_logits, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -218,10 +218,9 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
 
     def step():
         opt.zero_grad(set_to_none=False)
-        op = big.opacity_values()
+        conics, radii, op = big.produce(cfg.sigma_cutoff)
         op = big.means.new_empty(0) if op is None else op
-        img = _BatchRender.apply(big.means, big.conics(), big.colors,
-                                 big.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
+        img = _BatchRender.apply(big.means, conics, big.colors, radii, op, plan, cfg.normalize,
                                  float(cfg.normalization_eps), _fade_alpha(cfg.support_fade),
                                  float(cfg.sigma_cutoff))
         loss = batch_l1_ssim_loss(batch, img, targets_flat, w)

@@ -108,11 +108,12 @@ class _PlannedRender(Function):
 
 
 def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> torch.Tensor:
-    op = field.opacity_values()
+    """Renders the field with the plan's buffers: conics, radii and opacities from the fused
+    producer (gaussians.py:270-321), then binning, forward and (in backward) the gradients."""
+    conics, radii, op = field.produce(cfg.sigma_cutoff)
     if op is None:
         op = field.means.new_empty(0)
-    return _PlannedRender.apply(field.means, field.conics(), field.colors,
-                                field.radii(cfg.sigma_cutoff), op, plan, cfg.normalize,
+    return _PlannedRender.apply(field.means, conics, field.colors, radii, op, plan, cfg.normalize,
                                 float(cfg.normalization_eps), HR._fade_alpha(cfg.support_fade),
                                 float(cfg.sigma_cutoff))
 

@@ -8,8 +8,12 @@ integer radii, and therefore the tile lists, are identical to the reference's.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
+from torch.autograd import Function
+from torch.autograd.function import once_differentiable
 
 
 class GaussianField:
@@ -120,6 +124,23 @@ class GaussianField:
             r = sigma_cutoff * torch.sqrt(torch.stack([var_x, var_y], dim=1))
             return torch.clamp(r.ceil().long(), min=1)
 
+    def produce(self, sigma_cutoff: float, dilation: float = 0.0):
+        """``(conics(dilation), radii(sigma_cutoff, dilation), opacity_values())`` of a CUDA field in
+        one HIP launch (and one for the backward of the conics and opacities).  The radii equal
+        ``radii()``'s bit for bit; the conics and opacities agree with the torch expressions to
+        float32 rounding.  Opacities are ``None`` when the field has none."""
+        if dilation < 0.0:
+            raise ValueError(f"dilation must be >= 0, got {dilation}")
+        if not self.means.is_cuda:
+            raise RuntimeError("GaussianField.produce requires CUDA tensors")
+        logits = self.opacities if self.opacities is not None else self.log_scales.new_empty(0)
+        fv = self.filter_variance
+        if fv is not None:
+            fv = fv.to(device=self.log_scales.device, dtype=torch.float32).reshape(-1)
+        conics, radii, op = _Produce.apply(self.log_scales, self.rotations, logits, fv,
+                                           float(sigma_cutoff), float(dilation))
+        return conics, radii, (op if self.opacities is not None else None)
+
     @torch.no_grad()
     def save(self, path: str) -> None:
         data = {
@@ -144,3 +165,54 @@ class GaussianField:
         return cls(t("means"), t("log_scales"), t("rotations"), t("colors"),
                    t("opacities") if "opacities" in z.files else None,
                    t("filter_variance") if "filter_variance" in z.files else None)
+
+
+def _ptr(t):
+    return None if t is None or t.numel() == 0 else ctypes.c_void_p(t.data_ptr())
+
+
+class _Produce(Function):
+    """Autograd node of ssplat_produce_field (conics and opacities differentiable, radii not)."""
+
+    @staticmethod
+    def forward(ctx, log_scales, rotations, logits, filter_variance, sigma_cutoff, dilation):
+        from . import _native
+
+        log_scales = log_scales.detach().contiguous()
+        rotations = rotations.detach().contiguous()
+        logits = logits.detach().contiguous()
+        n = log_scales.shape[0]
+        dev = log_scales.device
+        conics = torch.empty(n, 3, dtype=torch.float32, device=dev)
+        radii = torch.empty(n, 2, dtype=torch.int64, device=dev)
+        op = torch.empty(n if logits.numel() else 0, dtype=torch.float32, device=dev)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _native.check(_native.load().ssplat_produce_field(
+            _ptr(log_scales), _ptr(rotations), _ptr(logits), _ptr(filter_variance), n,
+            dilation, sigma_cutoff, _ptr(conics), _ptr(radii), _ptr(op), stream),
+            "ssplat_produce_field")
+        ctx.mark_non_differentiable(radii)
+        ctx.save_for_backward(log_scales, rotations, op, filter_variance)
+        ctx.dilation = dilation
+        return conics, radii, op
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g_conics, g_radii, g_op):
+        from . import _native
+
+        log_scales, rotations, op, fv = ctx.saved_tensors
+        n = log_scales.shape[0]
+        g_ls = torch.empty_like(log_scales)
+        g_rot = torch.empty_like(rotations)
+        has_op = op.numel() > 0
+        g_logits = torch.empty_like(op) if has_op else None
+        if g_conics is None:
+            g_conics = torch.zeros(n, 3, dtype=torch.float32, device=log_scales.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(log_scales.device).cuda_stream)
+        _native.check(_native.load().ssplat_produce_field_backward(
+            _ptr(log_scales), _ptr(rotations), _ptr(op) if has_op else None, _ptr(fv), n,
+            ctx.dilation, _ptr(g_conics.contiguous()),
+            _ptr(g_op.contiguous()) if (has_op and g_op is not None) else None,
+            _ptr(g_ls), _ptr(g_rot), _ptr(g_logits), stream), "ssplat_produce_field_backward")
+        return g_ls, g_rot, g_logits, None, None, None

@@ -226,12 +226,36 @@ int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int
                         double ssim_weight, float* loss, float* grad_image, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/*
+ * The same loss in two calls for autograd: _forward writes *loss and keeps the SSIM derivative
+ * maps in the workspace; _backward (same image, target, weight and workspace, nothing else using
+ * the workspace in between) writes grad_image = (*grad_loss) * dloss/dimage, with grad_loss a
+ * device float (NULL: 1).
+ */
+int ssplat_l1_ssim_loss_forward(const float* image, const float* target, int height, int width,
+                                double ssim_weight, float* loss, void* workspace,
+                                size_t workspace_bytes, void* stream);
+int ssplat_l1_ssim_loss_backward(const float* image, const float* target, int height, int width,
+                                 double ssim_weight, const float* grad_loss, float* grad_image,
+                                 void* workspace, size_t workspace_bytes, void* stream);
+
 /* Batched loss: the sum of every field's loss over concatenated images (a field table planned
-   with tile_size 16 by ssplat_batch_plan, copied to the device). */
+   with tile_size 16 by ssplat_batch_plan, copied to the device), and its two-call form. */
 int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
                               int64_t total_pixels, const float* image, const float* target,
                               double ssim_weight, float* loss, float* grad_image,
                               void* workspace, size_t workspace_bytes, void* stream);
+int ssplat_batch_l1_ssim_loss_forward(const ssplat_field* fields, int num_fields,
+                                      int64_t total_tiles, int64_t total_pixels,
+                                      const float* image, const float* target, double ssim_weight,
+                                      float* loss, void* workspace, size_t workspace_bytes,
+                                      void* stream);
+int ssplat_batch_l1_ssim_loss_backward(const ssplat_field* fields, int num_fields,
+                                       int64_t total_tiles, int64_t total_pixels,
+                                       const float* image, const float* target,
+                                       double ssim_weight, const float* grad_loss,
+                                       float* grad_image, void* workspace,
+                                       size_t workspace_bytes, void* stream);
 
 /*
  * Fit optimizer: one Adam step over up to 8 parameter groups in one launch (the reference's

@@ -54,7 +54,9 @@ struct LossArgs {
   FieldTable fields;
   const float* img;     // (pixels, 3)
   const float* target;  // (pixels, 3)
-  float* grad;          // (pixels, 3) or NULL (loss only)
+  float* grad;          // (pixels, 3): the gradient (backward)
+  const float* scale;   // device scalar dL/dloss multiplying the gradient, or NULL (1)
+  bool keep;            // forward: write a, b, c for a later backward
   float* abc;           // 3 maps of (pixels, 3): dS/dmu_p, dS/dm2, dS/dmpt
   float* partial;       // one float per tile
   int64_t pixels;       // pixels over all fields
@@ -167,14 +169,7 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     if (x < g.W && y < g.H) {
       const int64_t base = (g.first_pixel + int64_t(y) * g.W + x) * 3;
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float d = a.img[base + ch] - a.target[base + ch];
-        l1 += fabsf(d);
-        if (a.grad) {
-          const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-          a.grad[base + ch] = a.w_l1 * g.inv_n * sg;
-        }
-      }
+      for (int ch = 0; ch < 3; ++ch) l1 += fabsf(a.img[base + ch] - a.target[base + ch]);
     }
     const float t = block_sum(l1);
     if (threadIdx.x == 0) a.partial[tile] = a.w_l1 * g.inv_n * t;
@@ -276,7 +271,7 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
       const float S = A1 * A2 * inv_d;
       const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
       ssim_sum += S;
-      if (a.grad) {
+      if (a.keep) {
         const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
         const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
         a.abc[at] = da;
@@ -293,10 +288,27 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
   }
 }
 
-// dL/dp = w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n.
+// dL/dp = scale (w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n); without SSIM
+// only the L1 term, one pixel per thread.
+template <bool kSSIM>
 __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileGeo g = tile_geo(a, tile);
+  const float scale = a.scale != nullptr ? *a.scale : 1.0f;
+  if (!kSSIM) {
+    const int x = g.x0 + (threadIdx.x & (kLT - 1));
+    const int y = g.y0 + (threadIdx.x >> 4);
+    if (x < g.W && y < g.H) {
+      const int64_t base = (g.first_pixel + int64_t(y) * g.W + x) * 3;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float d = a.img[base + ch] - a.target[base + ch];
+        const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+        a.grad[base + ch] = scale * (a.w_l1 * g.inv_n * sg);
+      }
+    }
+    return;
+  }
   // LDS: the staged a, b, c, then (reusing the same space) their horizontal blurs.
   constexpr int kStaged = 3 * kMapFloats;
   constexpr int kBlurred = 3 * 3 * kSpan * (kLT + 1);
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
       const float d = p - t;
       const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
       const float dssim = bl[0][o] + 2.0f * p * bl[1][o] + t * bl[2][o];
-      a.grad[at] = g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim);
+      a.grad[at] = scale * (g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim));
     }
   }
 }
@@ -426,11 +438,12 @@ Window gaussian_window() {
   return w;
 }
 
-int run_loss(const FieldTable& table, int num_fields, int64_t tiles, int64_t pixels,
-             const float* img, const float* target, double ssim_weight, float* loss, float* grad,
-             void* ws, size_t ws_bytes, void* stream) {
-  if (img == nullptr || target == nullptr || loss == nullptr) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "image, target and loss must not be NULL");
+// Validates the common arguments and fills `a` (everything but the per-call outputs).
+int prepare_loss(const FieldTable& table, int64_t tiles, int64_t pixels, const float* img,
+                 const float* target, double ssim_weight, void* ws, size_t ws_bytes,
+                 LossArgs* a) {
+  if (img == nullptr || target == nullptr) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "image and target must not be NULL");
   }
   if (!(std::isfinite(ssim_weight) && ssim_weight >= 0.0 && ssim_weight <= 1.0)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "ssim_weight must be in [0, 1]");
@@ -446,33 +459,73 @@ int run_loss(const FieldTable& table, int num_fields, int64_t tiles, int64_t pix
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace has %zu bytes, needs %zu", ws_bytes,
                      l.bytes);
   }
-  LossArgs a;
-  a.fields = table;
-  a.img = img;
-  a.target = target;
-  a.grad = grad;
-  a.abc = l.abc;
-  a.partial = l.partial;
-  a.pixels = pixels;
-  a.w_ssim = static_cast<float>(ssim_weight);
-  a.w_l1 = static_cast<float>(1.0 - ssim_weight);
-  a.win = gaussian_window();
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  *a = LossArgs{};
+  a->fields = table;
+  a->img = img;
+  a->target = target;
+  a->abc = l.abc;
+  a->partial = l.partial;
+  a->pixels = pixels;
+  a->w_ssim = static_cast<float>(ssim_weight);
+  a->w_l1 = static_cast<float>(1.0 - ssim_weight);
+  a->win = gaussian_window();
+  return SSPLAT_OK;
+}
+
+// Loss value into *loss; with keep, also the SSIM derivative maps for run_loss_backward.
+int run_loss_forward(LossArgs a, int num_fields, int64_t tiles, double ssim_weight, float* loss,
+                     bool keep, hipStream_t s) {
+  if (loss == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "loss is NULL");
+  a.keep = keep;
   const int grid = static_cast<int>(tiles);
   if (ssim_weight > 0.0) {
     k_loss_stats<true><<<grid, kLossThreads, 0, s>>>(a);
-    SSPLAT_LAUNCH_CHECK("k_loss_stats");
-    if (grad != nullptr) {
-      k_loss_grad<<<grid, kLossThreads, 0, s>>>(a);
-      SSPLAT_LAUNCH_CHECK("k_loss_grad");
-    }
   } else {
     k_loss_stats<false><<<grid, kLossThreads, 0, s>>>(a);
-    SSPLAT_LAUNCH_CHECK("k_loss_stats");
   }
-  k_loss_total<<<1, 1024, 0, s>>>(l.partial, tiles, ssim_weight * num_fields, loss);
+  SSPLAT_LAUNCH_CHECK("k_loss_stats");
+  k_loss_total<<<1, 1024, 0, s>>>(a.partial, tiles, ssim_weight * num_fields, loss);
   SSPLAT_LAUNCH_CHECK("k_loss_total");
-  return success();
+  return SSPLAT_OK;
+}
+
+int run_loss_backward(LossArgs a, int64_t tiles, double ssim_weight, const float* scale,
+                      float* grad, hipStream_t s) {
+  if (grad == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_image is NULL");
+  a.grad = grad;
+  a.scale = scale;
+  const int grid = static_cast<int>(tiles);
+  if (ssim_weight > 0.0) {
+    k_loss_grad<true><<<grid, kLossThreads, 0, s>>>(a);
+  } else {
+    k_loss_grad<false><<<grid, kLossThreads, 0, s>>>(a);
+  }
+  SSPLAT_LAUNCH_CHECK("k_loss_grad");
+  return SSPLAT_OK;
+}
+
+FieldTable single_image(int height, int width, int64_t* tiles) {
+  ssplat_field f{};
+  f.height = height;
+  f.width = width;
+  f.tiles_x = (width + kLT - 1) / kLT;
+  f.num_tiles = f.tiles_x * ((height + kLT - 1) / kLT);
+  *tiles = int64_t(f.tiles_x) * ((height + kLT - 1) / kLT);
+  return FieldTable{nullptr, 0, f};
+}
+
+int check_size(int height, int width) {
+  if (height <= 0 || width <= 0 || height > 65535 || width > 65535) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be in [1, 65535]");
+  }
+  return SSPLAT_OK;
+}
+
+int check_table(const ssplat_field* fields, int num_fields) {
+  if (fields == nullptr || num_fields <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "fields must be a non-empty device table");
+  }
+  return SSPLAT_OK;
 }
 
 }  // namespace
@@ -491,29 +544,124 @@ int ssplat_loss_workspace_bytes(int64_t total_pixels, int64_t total_tiles, size_
 int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int width,
                         double ssim_weight, float* loss, float* grad_image, void* workspace,
                         size_t workspace_bytes, void* stream) {
-  if (height <= 0 || width <= 0 || height > 65535 || width > 65535) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "height and width must be in [1, 65535]");
+  int rc;
+  if ((rc = check_size(height, width))) return rc;
+  int64_t tiles;
+  const FieldTable t = single_image(height, width, &tiles);
+  LossArgs a;
+  if ((rc = prepare_loss(t, tiles, int64_t(height) * width, image, target, ssim_weight,
+                         workspace, workspace_bytes, &a))) {
+    return rc;
   }
-  ssplat_field f{};
-  f.height = height;
-  f.width = width;
-  f.tiles_x = (width + kLT - 1) / kLT;
-  f.num_tiles = f.tiles_x * ((height + kLT - 1) / kLT);
-  const int64_t tiles = int64_t(f.tiles_x) * ((height + kLT - 1) / kLT);
-  return run_loss(FieldTable{nullptr, 0, f}, 1, tiles, int64_t(height) * width, image, target,
-                  ssim_weight, loss, grad_image, workspace, workspace_bytes, stream);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = run_loss_forward(a, 1, tiles, ssim_weight, loss, grad_image != nullptr, s))) return rc;
+  if (grad_image && (rc = run_loss_backward(a, tiles, ssim_weight, nullptr, grad_image, s))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_l1_ssim_loss_forward(const float* image, const float* target, int height, int width,
+                                double ssim_weight, float* loss, void* workspace,
+                                size_t workspace_bytes, void* stream) {
+  int rc;
+  if ((rc = check_size(height, width))) return rc;
+  int64_t tiles;
+  const FieldTable t = single_image(height, width, &tiles);
+  LossArgs a;
+  if ((rc = prepare_loss(t, tiles, int64_t(height) * width, image, target, ssim_weight,
+                         workspace, workspace_bytes, &a))) {
+    return rc;
+  }
+  if ((rc = run_loss_forward(a, 1, tiles, ssim_weight, loss, true,
+                             static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_l1_ssim_loss_backward(const float* image, const float* target, int height, int width,
+                                 double ssim_weight, const float* grad_loss, float* grad_image,
+                                 void* workspace, size_t workspace_bytes, void* stream) {
+  int rc;
+  if ((rc = check_size(height, width))) return rc;
+  int64_t tiles;
+  const FieldTable t = single_image(height, width, &tiles);
+  LossArgs a;
+  if ((rc = prepare_loss(t, tiles, int64_t(height) * width, image, target, ssim_weight,
+                         workspace, workspace_bytes, &a))) {
+    return rc;
+  }
+  if ((rc = run_loss_backward(a, tiles, ssim_weight, grad_loss, grad_image,
+                              static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
 }
 
 int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
                               int64_t total_pixels, const float* image, const float* target,
                               double ssim_weight, float* loss, float* grad_image,
                               void* workspace, size_t workspace_bytes, void* stream) {
-  if (fields == nullptr || num_fields <= 0) {
-    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "fields must be a non-empty device table");
+  int rc;
+  if ((rc = check_table(fields, num_fields))) return rc;
+  LossArgs a;
+  if ((rc = prepare_loss(FieldTable{fields, num_fields, ssplat_field{}}, total_tiles,
+                         total_pixels, image, target, ssim_weight, workspace, workspace_bytes,
+                         &a))) {
+    return rc;
   }
-  return run_loss(FieldTable{fields, num_fields, ssplat_field{}}, num_fields, total_tiles,
-                  total_pixels, image, target, ssim_weight, loss, grad_image, workspace,
-                  workspace_bytes, stream);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = run_loss_forward(a, num_fields, total_tiles, ssim_weight, loss,
+                             grad_image != nullptr, s))) {
+    return rc;
+  }
+  if (grad_image &&
+      (rc = run_loss_backward(a, total_tiles, ssim_weight, nullptr, grad_image, s))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_batch_l1_ssim_loss_forward(const ssplat_field* fields, int num_fields,
+                                      int64_t total_tiles, int64_t total_pixels,
+                                      const float* image, const float* target, double ssim_weight,
+                                      float* loss, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  int rc;
+  if ((rc = check_table(fields, num_fields))) return rc;
+  LossArgs a;
+  if ((rc = prepare_loss(FieldTable{fields, num_fields, ssplat_field{}}, total_tiles,
+                         total_pixels, image, target, ssim_weight, workspace, workspace_bytes,
+                         &a))) {
+    return rc;
+  }
+  if ((rc = run_loss_forward(a, num_fields, total_tiles, ssim_weight, loss, true,
+                             static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_batch_l1_ssim_loss_backward(const ssplat_field* fields, int num_fields,
+                                       int64_t total_tiles, int64_t total_pixels,
+                                       const float* image, const float* target,
+                                       double ssim_weight, const float* grad_loss,
+                                       float* grad_image, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  int rc;
+  if ((rc = check_table(fields, num_fields))) return rc;
+  LossArgs a;
+  if ((rc = prepare_loss(FieldTable{fields, num_fields, ssplat_field{}}, total_tiles,
+                         total_pixels, image, target, ssim_weight, workspace, workspace_bytes,
+                         &a))) {
+    return rc;
+  }
+  if ((rc = run_loss_backward(a, total_tiles, ssim_weight, grad_loss, grad_image,
+                              static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
 }
 
 }  // extern "C"

@@ -3,9 +3,9 @@
 Same definition as the reference's fit loss with ``pixel_loss="l1"`` (fit.py:800-823) and its
 SSIM (metrics.py:88-151): L1 is the mean absolute difference over the 3 H W values, SSIM the mean
 of the SSIM map with the 11-tap, sigma-1.5 Gaussian window applied per channel as two zero-padded
-1-D passes.  The kernels compute the loss and its gradient with respect to the image in one call
-(the loss is a scalar, so the backward only scales the stored gradient); a batch's loss is the sum
-of its images' losses.  ``metrics.ssim`` is the torch statement of the same SSIM.
+1-D passes.  The forward computes the loss and keeps the SSIM derivative maps; the backward turns
+them into the image gradient, scaled on the device by the incoming gradient.  A batch's loss is
+the sum of its images' losses.  ``metrics.ssim`` is the torch statement of the same SSIM.
 """
 from __future__ import annotations
 
@@ -51,35 +51,54 @@ def _check_weight(w: float) -> float:
 
 
 class _L1SSIM(Function):
+    """Forward: the loss (and the SSIM derivative maps kept in a workspace); backward: the image
+    gradient scaled by the incoming gradient, read on the device."""
+
     @staticmethod
     def forward(ctx, img, target, ssim_weight, batch):
         img = img.contiguous()
         target = target.contiguous()
         dev = img.device
         loss = torch.empty((), dtype=torch.float32, device=dev)
-        grad = torch.empty_like(img) if ctx.needs_input_grad[0] else None
         lib = _native.load()
         if batch is None:
             H, W = int(img.shape[0]), int(img.shape[1])
             tiles = ((H + 15) // 16) * ((W + 15) // 16)
             ws = loss_workspace(H * W, tiles, dev)
-            _native.check(lib.ssplat_l1_ssim_loss(
-                _ptr(img), _ptr(target), H, W, float(ssim_weight), _ptr(loss), _ptr(grad),
-                _ptr(ws), ws.numel(), _stream(dev)), "ssplat_l1_ssim_loss")
+            _native.check(lib.ssplat_l1_ssim_loss_forward(
+                _ptr(img), _ptr(target), H, W, float(ssim_weight), _ptr(loss), _ptr(ws),
+                ws.numel(), _stream(dev)), "ssplat_l1_ssim_loss_forward")
         else:
             ws = loss_workspace(batch.total_pixels, batch.total_tiles, dev)
-            _native.check(lib.ssplat_batch_l1_ssim_loss(
+            _native.check(lib.ssplat_batch_l1_ssim_loss_forward(
                 _ptr(batch.fields_dev), batch.num_fields, batch.total_tiles, batch.total_pixels,
-                _ptr(img), _ptr(target), float(ssim_weight), _ptr(loss), _ptr(grad), _ptr(ws),
-                ws.numel(), _stream(dev)), "ssplat_batch_l1_ssim_loss")
-        ctx.save_for_backward(grad)
+                _ptr(img), _ptr(target), float(ssim_weight), _ptr(loss), _ptr(ws), ws.numel(),
+                _stream(dev)), "ssplat_batch_l1_ssim_loss_forward")
+        ctx.save_for_backward(img, target, ws)
+        ctx.ssim_weight = float(ssim_weight)
+        ctx.batch = batch
         return loss
 
     @staticmethod
     @once_differentiable
     def backward(ctx, grad_loss):
-        (grad,) = ctx.saved_tensors
-        return grad * grad_loss, None, None, None
+        img, target, ws = ctx.saved_tensors
+        dev = img.device
+        grad = torch.empty_like(img)
+        g = grad_loss.to(dtype=torch.float32).contiguous()
+        lib = _native.load()
+        b = ctx.batch
+        if b is None:
+            _native.check(lib.ssplat_l1_ssim_loss_backward(
+                _ptr(img), _ptr(target), int(img.shape[0]), int(img.shape[1]), ctx.ssim_weight,
+                _ptr(g), _ptr(grad), _ptr(ws), ws.numel(), _stream(dev)),
+                "ssplat_l1_ssim_loss_backward")
+        else:
+            _native.check(lib.ssplat_batch_l1_ssim_loss_backward(
+                _ptr(b.fields_dev), b.num_fields, b.total_tiles, b.total_pixels, _ptr(img),
+                _ptr(target), ctx.ssim_weight, _ptr(g), _ptr(grad), _ptr(ws), ws.numel(),
+                _stream(dev)), "ssplat_batch_l1_ssim_loss_backward")
+        return grad, None, None, None
 
 
 def l1_ssim_loss(img: torch.Tensor, target: torch.Tensor, ssim_weight: float = 0.2):

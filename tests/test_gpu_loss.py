@@ -54,6 +54,16 @@ def test_loss_and_gradient_match_float64(cuda, H, W, w):
     assert err <= 2e-5
 
 
+def test_backward_scales_by_the_incoming_gradient(cuda):
+    p, t = _pair(5, 33, 40, cuda)
+    x = p.clone().requires_grad_(True)
+    l1_ssim_loss(x, t, 0.2).backward()
+    y = p.clone().requires_grad_(True)
+    (2.5 * l1_ssim_loss(y, t, 0.2)).backward()
+    torch.cuda.synchronize()
+    assert torch.allclose(y.grad, 2.5 * x.grad, rtol=1e-6, atol=0.0)
+
+
 def test_loss_only_matches_loss_with_gradient(cuda):
     p, t = _pair(3, 45, 61, cuda)
     with torch.no_grad():

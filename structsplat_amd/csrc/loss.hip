@@ -86,60 +86,57 @@ __device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
 
 // Staging of rows y0-5 .. y0+20, columns x0-5 .. x0+20 of 3-channel maps into LDS as
 // [map][ch][row][col] (row stride kSpan + 1), zero outside the image.  Each staged row is 78
-// contiguous floats.  All of a thread's loads are issued before its LDS stores (one memory latency
-// per staging); a position outside the image loads element 0 of the map, which always exists,
-// and stores 0.
-constexpr int kHaloItems = kSpan * kLoadRow;
-constexpr int kHaloPer = (kHaloItems + kLossThreads - 1) / kLossThreads;
+// contiguous floats; thread t < 234 owns float t % 78 of rows t / 78, t / 78 + 3, ... (9 passes),
+// so the divisions happen once per thread.  All loads are issued before the LDS stores (one
+// memory latency per staging); a position outside the image loads element 0 of the map, which
+// always exists, and stores 0.
+constexpr int kStageRowsPerPass = kLossThreads / kLoadRow;                      // 3
+constexpr int kStagePasses = (kSpan + kStageRowsPerPass - 1) / kStageRowsPerPass;  // 9
 constexpr int kMapFloats = 3 * kSpan * (kSpan + 1);  // one staged 3-channel map
-
-struct HaloSlot {
-  int lds;      // offset in a staged map (floats), -1 when the slot is past the staged rows
-  bool ok;      // inside the image
-  bool center;  // one of the tile's own pixels (inside the image)
-  int64_t at;
-};
-
-__device__ __forceinline__ HaloSlot halo_slot(const TileGeo& g, int k) {
-  HaloSlot h;
-  const int i = static_cast<int>(threadIdx.x) + k * kLossThreads;
-  const int row = i / kLoadRow;
-  const int c3 = i - row * kLoadRow;
-  const int col = c3 / 3;
-  const int ch = c3 - col * 3;
-  const int gx = g.x0 - kHalo + col;
-  const int gy = g.y0 - kHalo + row;
-  h.lds = i < kHaloItems ? (ch * kSpan + row) * (kSpan + 1) + col : -1;
-  h.ok = i < kHaloItems && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H;
-  h.center = h.ok && row >= kHalo && row < kHalo + kLT && col >= kHalo && col < kHalo + kLT;
-  h.at = h.ok ? (g.first_pixel + int64_t(gy) * g.W + gx) * 3 + ch : 0;
-  return h;
-}
 
 // Stages kMaps maps into lds[m * kMapFloats + ...]; returns sum |map0 - map1| over the tile's own
 // pixels when kL1 (the L1 term of the loss).
 template <int kMaps, bool kL1>
 __device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], const TileGeo& g,
                                             float* lds) {
-  HaloSlot slot[kHaloPer];
+  const int t = threadIdx.x;
+  const bool active = t < kStageRowsPerPass * kLoadRow;
+  const int r0 = t / kLoadRow;
+  const int c3 = t - r0 * kLoadRow;
+  const int col = c3 / 3;
+  const int ch = c3 - col * 3;
+  const int gx = g.x0 - kHalo + col;
+  const bool x_ok = active && gx >= 0 && gx < g.W;
+  const bool x_center = col >= kHalo && col < kHalo + kLT;
+  // Row stride in the map: W pixels of 3 floats.
+  const int64_t row_stride = int64_t(g.W) * 3;
+  const int64_t at0 = (g.first_pixel + int64_t(g.y0 - kHalo + r0) * g.W + gx) * 3 + ch;
+  const int lds0 = (ch * kSpan + r0) * (kSpan + 1) + col;
+  float v[kMaps][kStagePasses];
+  bool ok[kStagePasses];
 #pragma unroll
-  for (int k = 0; k < kHaloPer; ++k) slot[k] = halo_slot(g, k);
-  float v[kMaps][kHaloPer];
+  for (int k = 0; k < kStagePasses; ++k) {
+    const int row = r0 + kStageRowsPerPass * k;
+    const int gy = g.y0 - kHalo + row;
+    ok[k] = x_ok && row < kSpan && gy >= 0 && gy < g.H;
+    const int64_t at = ok[k] ? at0 + int64_t(kStageRowsPerPass * k) * row_stride : 0;
 #pragma unroll
-  for (int m = 0; m < kMaps; ++m) {
-#pragma unroll
-    for (int k = 0; k < kHaloPer; ++k) {
-      const float x = src[m][slot[k].at];
-      v[m][k] = slot[k].ok ? x : 0.0f;
+    for (int m = 0; m < kMaps; ++m) {
+      const float x = src[m][at];
+      v[m][k] = ok[k] ? x : 0.0f;
     }
   }
   float l1 = 0.0f;
 #pragma unroll
-  for (int k = 0; k < kHaloPer; ++k) {
-    if (slot[k].lds < 0) continue;
+  for (int k = 0; k < kStagePasses; ++k) {
+    const int row = r0 + kStageRowsPerPass * k;
+    if (!active || row >= kSpan) continue;
+    const int off = lds0 + kStageRowsPerPass * k * (kSpan + 1);
 #pragma unroll
-    for (int m = 0; m < kMaps; ++m) lds[m * kMapFloats + slot[k].lds] = v[m][k];
-    if (kL1 && slot[k].center) l1 += fabsf(v[0][k] - v[1][k]);
+    for (int m = 0; m < kMaps; ++m) lds[m * kMapFloats + off] = v[m][k];
+    if (kL1 && ok[k] && x_center && row >= kHalo && row < kHalo + kLT) {
+      l1 += fabsf(v[0][k] - v[1][k]);
+    }
   }
   return l1;
 }

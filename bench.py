@@ -1,9 +1,12 @@
 """Benchmark of the HIP rasterizer's hot path: one fwd+bwd of a 2048x2048 field of 200k Gaussians.
 
 A step is one pass of the renderer's hot path over the resident field: tile binning (no host
-synchronization), tiled forward (image and coverage), tiled backward from the saved image and
-coverage, against a fixed synthetic dL/dout.  Prints one JSON line (metric, value, roofline of
-the dominant kernel, CPU baseline).  `value` is the algorithmic HBM throughput of the whole step:
+synchronization), tiled forward, and the tiled backward of the default autograd path, which
+recomputes the per-pixel normalizer (a denominator-only sweep of the tile lists) instead of
+storing it and reads the forward's image, against a fixed synthetic dL/dout.  The variant that
+stores the normalizer (the reference's saved den/out) is timed beside it.  Prints one JSON line
+(metric, value, roofline of the dominant kernel, CPU baseline).  `value` is the algorithmic HBM
+throughput of the whole step:
 
     B_step = B_bin + B_fwd + B_bwd
     B_bin  = 52 N + 4 P + 4 (T+1)                     (means, conics, radii in; lists, offsets out)
@@ -12,9 +15,8 @@ the dominant kernel, CPU baseline).  `value` is the algorithmic HBM throughput o
                                                         atomic set per pair; gradients)
 with R = 44 bytes of Gaussian parameters per pair visit (mean 8, conic 12, color 12, opacity 4,
 radii 8), N Gaussians, P (tile, Gaussian) pairs and T tiles.  This is the byte model of a backward
-that recomputes the normalizer.  The implementation reads the forward's saved coverage and image
-instead (16 H W more in the forward and backward, one list sweep less), which counts 37 MB more
-at the metric workload; the smaller model is reported.
+that recomputes the normalizer with a second list sweep; the default backward does so and also
+reads the forward's image (12 H W, not counted).
 
 The roofline is reported for k_backward16, the step's dominant kernel: its algorithmic bytes per
 launch (B_bwd) over its average duration, measured by HIP events that the library records on the
@@ -215,15 +217,20 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20,
                     help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched-vs-plain fit (--batched-steps steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
-    if args.copy_probe > 0:
-        return copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the renderer's fields are independent units and this "
               "build runs on a single MI355X; run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.copy_probe > 0:
+        return copy_probe(args.copy_probe)
+    if args.batched_only:
+        print(json.dumps({"batched": batched_fit(max(args.batched_steps, 1))}), flush=True)
+        return 0
 
     import torch
 
@@ -258,7 +265,8 @@ def main() -> int:
     ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(2)] for k in ("b", "f", "g")}
     phase_ms = {"bin": [], "fwd": [], "bwd": []}
 
-    def step(record: bool):
+    def step(record: bool, saved: bool = False):
+        """One fwd+bwd; `saved`: the forward stores den and the backward reads it."""
         if record:
             ev["b"][0].record(stream)
         HR.build_tile_index_capacity(means, conics, radii, H, W, capacity,
@@ -267,12 +275,12 @@ def main() -> int:
             ev["b"][1].record(stream)
             ev["f"][0].record(stream)
         HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8, 16,
-                         False, 3.0, out=out, den=den, ws=ws)
+                         False, 3.0, out=out, den=den if saved else None, ws=ws)
         if record:
             ev["f"][1].record(stream)
             ev["g"][0].record(stream)
-        HR.backward_tiled(grad_out, means, conics, colors, radii, op, den, out, gids, offsets, H,
-                          W, True, 1e-8, 16, False, 3.0, grads=grads, ws=ws)
+        HR.backward_tiled(grad_out, means, conics, colors, radii, op, den if saved else None, out,
+                          gids, offsets, H, W, True, 1e-8, 16, False, 3.0, grads=grads, ws=ws)
         if record:
             ev["g"][1].record(stream)
 
@@ -304,19 +312,32 @@ def main() -> int:
     # Kernel-only durations: the library's event pairs around k_forward16 / k_backward16.
     from structsplat_amd import _native
 
-    _native.kernel_timing_read(_native.KERNEL_FORWARD)
-    _native.kernel_timing_read(_native.KERNEL_BACKWARD)
-    _native.kernel_timing(True)
-    try:
-        for _ in range(args.steps):
-            step(False)
-        torch.cuda.synchronize()
-    finally:
-        _native.kernel_timing(False)
-    nf, fwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)
-    nb, bwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_BACKWARD)
-    kernel_ms = {"k_forward16": fwd_kernel_ms / max(nf, 1),
-                 "k_backward16": bwd_kernel_ms / max(nb, 1)}
+    def kernel_times(saved: bool) -> dict[str, float]:
+        _native.kernel_timing_read(_native.KERNEL_FORWARD)
+        _native.kernel_timing_read(_native.KERNEL_BACKWARD)
+        _native.kernel_timing(True)
+        try:
+            for _ in range(args.steps):
+                step(False, saved)
+            torch.cuda.synchronize()
+        finally:
+            _native.kernel_timing(False)
+        nf, fwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)
+        nb, bwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_BACKWARD)
+        return {"k_forward16": fwd_kernel_ms / max(nf, 1),
+                "k_backward16": bwd_kernel_ms / max(nb, 1)}
+
+    kernel_ms = kernel_times(False)
+    # The stored-normalizer variant: whole-step wall time and kernel times.
+    for _ in range(args.warmup):
+        step(False, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False, True)
+    torch.cuda.synchronize()
+    saved_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    saved_kernel_ms = kernel_times(True)
 
     P_now = int(offsets[-1].item())
     b = algorithmic_bytes(N, P_now, T, H * W)
@@ -340,6 +361,11 @@ def main() -> int:
             "H": H, "W": W, "n_gaussians": N, "tile_pairs": P_now, "tiles": T,
             "phase_ms": {k: round(v, 4) for k, v in phase.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+            "normalizer": "recomputed in the backward (den not stored)",
+            "stored_normalizer_variant": {
+                "ms_per_step": round(saved_ms, 4),
+                "kernel_ms": {k: round(v, 4) for k, v in saved_kernel_ms.items()},
+            },
             "algorithmic_bytes": b,
         },
         "roofline": {

@@ -133,12 +133,16 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /*
- * Tiled backward.  Given grad_out (H,W,3) = dL/dout and the forward's den (H,W) and out (H,W,3)
- * of the same inputs, writes dL/dmeans (N,2), dL/dconics (N,3), dL/dcolors (N,3) and, when
- * opacities and grad_opacities are both non-NULL, dL/dopacities (N,).  den and out are read only
- * when normalize is set (the per-pixel adjoints are dL/dN_k = G_k/(den+eps) and
- * dL/dden = -sum_k G_k out_k/(den+eps)); an additive call may pass NULL for both.  Gradient
- * outputs are overwritten (the library clears them first).
+ * Tiled backward.  Given grad_out (H,W,3) = dL/dout, writes dL/dmeans (N,2), dL/dconics (N,3),
+ * dL/dcolors (N,3) and, when opacities and grad_opacities are both non-NULL, dL/dopacities (N,).
+ * A normalized backward needs the per-pixel adjoints dL/dN_k = G_k/(den+eps) and
+ * dL/dden = -sum_k G_k out_k/(den+eps) of the forward's den (H,W) and out (H,W,3):
+ *   den and out given   : both read (the reference's saved tensors);
+ *   out given, den NULL : den, the normalizer, is recomputed by a denominator-only sweep of
+ *                         the tile lists instead of being stored;
+ *   both NULL           : den and the numerator are recomputed, out = N/(den+eps).
+ * The recomputed values equal the forward's bit for bit, so all three give the same gradients.
+ * An additive call reads neither.  Gradient outputs are overwritten (the library clears them).
  * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231).
  */
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,

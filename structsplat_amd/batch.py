@@ -134,14 +134,16 @@ class Batch:
 class BatchPlan:
     """Pair buffer, offsets and sticky status of a batch (capturable binning)."""
 
-    def __init__(self, batch: Batch, capacity: int):
+    def __init__(self, batch: Batch, capacity: int, recompute_normalizer: bool = True):
         self.batch = batch
         self.capacity = max(int(capacity), 1)
+        self.recompute_normalizer = bool(recompute_normalizer)
         dev = batch.device
         self.gids = torch.empty(self.capacity, dtype=torch.int32, device=dev)
         self.offsets = torch.empty(batch.total_tiles + 1, dtype=torch.int32, device=dev)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.den = torch.empty(batch.total_pixels, dtype=torch.float32, device=dev)
+        self.den = (None if self.recompute_normalizer
+                    else torch.empty(batch.total_pixels, dtype=torch.float32, device=dev))
 
 
 class _BatchRender(Function):
@@ -214,7 +216,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
             return batch.pair_count(big.means, big.conics(), big.radii(cfg.sigma_cutoff),
                                     _fade_alpha(cfg.support_fade) == 1.0, cfg.sigma_cutoff)
 
-    plan = BatchPlan(batch, F._capacity(count(), cfg))
+    plan = BatchPlan(batch, F._capacity(count(), cfg), cfg.recompute_normalizer)
 
     def step():
         opt.zero_grad(set_to_none=False)
@@ -253,13 +255,14 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                 step()
         if int(plan.status.item()) != 0:
             snap.restore(big, opt)
-            plan = BatchPlan(batch, max(2 * plan.capacity, F._capacity(count(), cfg)))
+            plan = BatchPlan(batch, max(2 * plan.capacity, F._capacity(count(), cfg)),
+                             cfg.recompute_normalizer)
             g = None
             continue
         done += n
         p = count()
         if p * 1.15 > plan.capacity:
-            plan = BatchPlan(batch, F._headroom(p, cfg))
+            plan = BatchPlan(batch, F._headroom(p, cfg), cfg.recompute_normalizer)
             g = None
     torch.cuda.synchronize(dev)
     seconds = time.perf_counter() - t0

@@ -872,18 +872,19 @@ struct StageLds {
 // Coalesced staging of up to kStage records of the tile list into LDS (kChunks 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
-template <int kChunks>
+template <int kLoad, int kChunks>
 __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks>& st,
                                             int64_t batch, int m) {
+  static_assert(kLoad <= kChunks, "staging more chunks than the buffer holds");
   if (static_cast<int>(threadIdx.x) < m) {
     const int g = a.gids[batch + threadIdx.x];
     if (g >= 0 && g < a.n) {
       const float4* src = reinterpret_cast<const float4*>(a.recs + g);
 #pragma unroll
-      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = src[c];
+      for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = src[c];
     } else {
 #pragma unroll
-      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
 }
@@ -908,8 +909,10 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // the wave first computes (one record per lane) the mask of quadrant pixels inside its rectangle
 // into its LDS slice; a ballot keeps the records that meet the quadrant, records covering all of
 // it skip the pixel test, and for the others each lane reads its bit of the mask.
-template <bool kFade>
-__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<3>& st,
+// kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
+// recomputation); it is the same sum, in the same order, as the forward's D.
+template <bool kFade, bool kDenOnly, int kChunks>
+__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks>& st,
                                                     uint32_t* s_mask, int tile, int qx0, int qy0,
                                                     int px, int py, float tail, float4& acc) {
   const int lane = threadIdx.x & 63;
@@ -920,7 +923,7 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   float nr = 0.0f, ng = 0.0f, nb = 0.0f, d = 0.0f;
   for (int64_t batch = start; batch < end; batch += kStage) {
     const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
-    stage_batch(a, st, batch, m);
+    stage_batch<3>(a, st, batch, m);
     __syncthreads();
     uint64_t hit[kStage / 64], full[kStage / 64];
 #pragma unroll
@@ -960,9 +963,11 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
           const float e = fmaf(t, dx, v1.x * dy * dy);
           const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
           const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
-          nr = fmaf(base, v2.x, nr);
-          ng = fmaf(base, v2.y, ng);
-          nb = fmaf(base, v2.z, nb);
+          if (!kDenOnly) {
+            nr = fmaf(base, v2.x, nr);
+            ng = fmaf(base, v2.y, ng);
+            nb = fmaf(base, v2.z, nb);
+          }
           d = fmaf(base, v2.w, d);
         }
       }
@@ -991,7 +996,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
-  accumulate_quadrant<kFade>(a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
+  accumulate_quadrant<kFade, false>(a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
     const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
     if (den) den[flat] = acc.w;
@@ -1018,10 +1023,19 @@ struct GradArgs {
   float* grad_opacities;  // NULL when the field has no opacities
 };
 
-template <bool kFade, bool kNormalize>
+// Where the normalized backward gets the forward's D (den) and out:
+constexpr int kNormSaved = 0;      // both read from the forward's saved outputs
+constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-only sweep
+constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
+
+template <bool kFade, bool kNormalize, int kNorm>
 __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, GradArgs g) {
+  constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
   __shared__ StageLds<4> st;
-  __shared__ float4 s_adj[kTile * kTile];
+  // The per-pixel adjoints; when recomputing, the same space first holds the sweep's per-wave
+  // quadrant masks (two uint32 per staged record and wave).
+  __shared__ float4 s_scratch[kRecompute ? (kRenderThreads / 64) * 2 * kStage / 4 : kTile * kTile];
+  float4* s_adj = s_scratch;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
@@ -1035,9 +1049,42 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   const int ty0 = tile_y * kTile;
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
 
-  // Per-pixel adjoints from the forward's saved outputs (render_ext.cu:496-510):
+  // Per-pixel adjoints (render_ext.cu:496-510):
   //   dL/dN_k = G_k/(D+eps), dL/dD = -sum_k G_k out_k/(D+eps)  (additive: G_k and 0).
-  {
+  if (kRecompute) {
+    // D (and N) recomputed for one pixel per lane by the forward's own sweep and order, so they
+    // equal the forward's values bit for bit.
+    const int qx0 = tx0 + (wave & 1) * 8;
+    const int qy0 = ty0 + (wave >> 1) * 8;
+    const int px = qx0 + lx;
+    const int py = qy0 + ly;
+    uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kStage);
+    float4 acc;
+    accumulate_quadrant<kFade, kNorm == kNormRecompD>(a, st, s_mask, tile, qx0, qy0, px, py,
+                                                      tail, acc);
+    // Every wave is past the sweep's last barrier: the mask space can take the adjoints.
+    float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (px < f.width && py < f.height) {
+      const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
+      const float gr = g.grad_out[flat * 3 + 0];
+      const float gg = g.grad_out[flat * 3 + 1];
+      const float gb = g.grad_out[flat * 3 + 2];
+      const float denom = acc.w + a.eps;
+      float o_r, o_g, o_b;
+      if (kNorm == kNormRecompD) {
+        o_r = g.out[flat * 3 + 0];
+        o_g = g.out[flat * 3 + 1];
+        o_b = g.out[flat * 3 + 2];
+      } else {
+        o_r = acc.x / denom;
+        o_g = acc.y / denom;
+        o_b = acc.z / denom;
+      }
+      const float dot = gr * o_r + gg * o_g + gb * o_b;
+      adj = make_float4(gr / denom, gg / denom, gb / denom, -dot / denom);
+    }
+    s_adj[(py - ty0) * kTile + (px - tx0)] = adj;
+  } else {
     const int px = tx0 + (threadIdx.x & (kTile - 1));
     const int py = ty0 + (threadIdx.x >> 4);
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1079,7 +1126,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kStage) {
     const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
-    stage_batch(a, st, batch, m);
+    stage_batch<4>(a, st, batch, m);
     __syncthreads();
     for (int j = wave; j < m; j += kRenderThreads / 64) {
       const float4 v0 = st.c[0][j];
@@ -1507,10 +1554,12 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
     return rc;
   }
   if (grad_out == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_out is NULL");
-  if (normalize && (den == nullptr || out == nullptr)) {
+  if (normalize && den != nullptr && out == nullptr) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
-                     "normalized backward needs the forward's den and out");
+                     "a normalized backward given den also needs the forward's out");
   }
+  // Normalized: saved den and out, saved out with D recomputed, or both recomputed.
+  const int norm = den != nullptr ? kNormSaved : (out != nullptr ? kNormRecompD : kNormRecompAll);
   if (p.n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
   }
@@ -1528,14 +1577,29 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   GradArgs ga{grad_out, den, out, grad_means, grad_conics, grad_colors, gop};
   hipEvent_t t0;
   const bool timed = timer_begin(1, s, &t0);
-  if (support_fade && normalize) {
-    k_backward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+  const dim3 grid(p.num_tiles), block(kRenderThreads);
+  if (!normalize) {
+    if (support_fade) {
+      k_backward16<true, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    } else {
+      k_backward16<false, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    }
   } else if (support_fade) {
-    k_backward16<true, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
-  } else if (normalize) {
-    k_backward16<false, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+    if (norm == kNormSaved) {
+      k_backward16<true, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    } else if (norm == kNormRecompD) {
+      k_backward16<true, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
+    } else {
+      k_backward16<true, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
+    }
   } else {
-    k_backward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, ga);
+    if (norm == kNormSaved) {
+      k_backward16<false, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    } else if (norm == kNormRecompD) {
+      k_backward16<false, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
+    } else {
+      k_backward16<false, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
+    }
   }
   SSPLAT_LAUNCH_CHECK("k_backward16");
   if (timed) timer_end(1, s, t0);

@@ -42,6 +42,7 @@ class FitConfig:
     lr_color: float = 5e-3
     lr_opacity: float = 5e-3
     ssim_weight: float = 0.2       # loss = (1 - w) * L1 + w * (1 - SSIM)
+    recompute_normalizer: bool = True  # backward recomputes den instead of storing it
     sigma_cutoff: float = 3.0
     support_fade: bool | float = False  # bool, or a fade strength in [0, 1]
     normalization_eps: float = 1e-8
@@ -54,15 +55,18 @@ class FitConfig:
 class RenderPlan:
     """Preallocated pair buffer, offsets, sticky status and workspace for one field shape."""
 
-    def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16):
+    def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16,
+                 recompute_normalizer: bool = True):
         self.n, self.H, self.W, self.tile_size = int(n), int(H), int(W), int(tile_size)
+        self.recompute_normalizer = bool(recompute_normalizer)
         self.capacity = max(int(capacity), 1)
         T = HR.num_tiles(H, W, tile_size)
         self.gids = torch.empty(self.capacity, dtype=torch.int32, device=device)
         self.offsets = torch.empty(T + 1, dtype=torch.int32, device=device)
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
         self.ws = HR.workspace(n, H, W, tile_size, device)
-        self.den = torch.empty(H, W, dtype=torch.float32, device=device)
+        self.den = (None if self.recompute_normalizer
+                    else torch.empty(H, W, dtype=torch.float32, device=device))
 
     def buffers(self):
         return self.gids, self.offsets, self.status, self.ws
@@ -164,7 +168,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     field.trainable()
     opt = make_optimizer(field, cfg)
     opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))
-    plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev)
+    plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev,
+                      recompute_normalizer=cfg.recompute_normalizer)
     state = {"loss": None}
 
     def step():
@@ -204,7 +209,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
             snap.restore(field, opt)
             plan = RenderPlan(field.n, H, W,
                               max(2 * plan.capacity, _capacity(pair_count(field, H, W, cfg), cfg)),
-                              dev)
+                              dev, recompute_normalizer=cfg.recompute_normalizer)
             graph_obj = None
             regrows += 1
             continue
@@ -212,7 +217,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         # Keep headroom: regrow (and re-capture) when the pair count approaches capacity.
         p = pair_count(field, H, W, cfg)
         if p * 1.15 > plan.capacity:
-            plan = RenderPlan(field.n, H, W, _headroom(p, cfg), dev)
+            plan = RenderPlan(field.n, H, W, _headroom(p, cfg), dev,
+                              recompute_normalizer=cfg.recompute_normalizer)
             graph_obj = None
             regrows += 1
         if verbose:

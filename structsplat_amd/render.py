@@ -13,9 +13,11 @@ phrases and routes every mode to the gfx950 kernels of ``libssplat_hip.so`` thro
 * ``render_hip_exact_with_coverage(...)`` - ``render_cuda_exact_with_coverage``
   (cuda_render.py:229-253): normalized RGB plus raw coverage, no autograd.
 
-As in the reference, the backward reads the forward's saved ``den``/``out``.  Differences by
-design: tile lists are built without a radix sort (ascending Gaussian id per tile, like the
-reference's stable sort); the forward is bitwise deterministic.
+By default the backward recomputes the per-pixel normalizer ``den`` instead of storing it (it
+reads only the saved image); ``recompute_normalizer=False`` saves ``den`` and ``out`` as the
+reference does.  Other differences by design: tile lists are built without a radix sort
+(ascending Gaussian id per tile, like the reference's stable sort); the forward is bitwise
+deterministic.
 """
 from __future__ import annotations
 
@@ -248,11 +250,15 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, 
 
 class _ExactRenderHip(Function):
     """Autograd boundary of the HIP renderer (the role of ``_ExactRenderCuda``,
-    cuda_render.py:52-174): gradients for means, conics, colors and opacities only."""
+    cuda_render.py:52-174): gradients for means, conics, colors and opacities only.
+
+    With ``recompute_normalizer`` the per-pixel normalizer ``den`` is not stored: the backward
+    recomputes it from the tile lists (bit-identical to the forward's) and reads only the saved
+    image.  Otherwise ``den`` and ``out`` are saved, as the reference does."""
 
     @staticmethod
     def forward(ctx, means, conics, colors, radii, opacities, height, width, normalize, eps,
-                tile_size, support_fade, sigma_cutoff, tile_ellipse_cull):
+                tile_size, support_fade, sigma_cutoff, tile_ellipse_cull, recompute_normalizer):
         means = means.contiguous()
         conics = conics.contiguous()
         colors = colors.contiguous()
@@ -265,7 +271,9 @@ class _ExactRenderHip(Function):
         gids, offsets = build_tile_index(means, conics, radii, int(height), int(width),
                                          int(tile_size), cull, float(sigma_cutoff))
         ws = workspace(means.shape[0], height, width, tile_size, means.device)
-        den = torch.empty(int(height), int(width), dtype=torch.float32, device=means.device)
+        keep_den = bool(normalize) and not recompute_normalizer
+        den = (torch.empty(int(height), int(width), dtype=torch.float32, device=means.device)
+               if keep_den else None)
         out, _ = forward_tiled(means, conics, colors, radii, opacities, gids, offsets,
                                int(height), int(width), bool(normalize), float(eps),
                                int(tile_size), alpha, float(sigma_cutoff), den=den, ws=ws)
@@ -284,7 +292,7 @@ class _ExactRenderHip(Function):
     def backward(ctx, grad_out):
         means, conics, colors, radii, opacities, den, out, gids, offsets = ctx.saved_tensors
         need_means, need_conics, need_colors, _, need_opac = ctx.needs_input_grad[:5]
-        none_tail = (None,) * 8
+        none_tail = (None,) * 9
         if not (need_means or need_conics or need_colors or need_opac):
             return (None,) * 5 + none_tail
         g_means, g_conics, g_colors, g_opac = backward_tiled(
@@ -306,15 +314,17 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
                      normalize: bool = True, eps: float = 1e-8, tiled: bool = True,
                      tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool | float = False,
                      sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
-                     tile_index_backend: str = "hip", tile_ellipse_cull: bool = True):
+                     tile_index_backend: str = "hip", tile_ellipse_cull: bool = True,
+                     recompute_normalizer: bool = True):
     """Exact normalized/additive render on the GPU (signature of ``render_cuda_exact``,
-    cuda_render.py:256-294).
+    cuda_render.py:256-294, plus ``recompute_normalizer``).
 
     ``support_fade`` is the reference's bool (full fade) or a fade strength in [0, 1]
     (render.py:94-107).
 
     ``tiled`` and ``backward_variant`` are accepted for drop-in compatibility: the HIP path is
-    always tiled and has one backward.  ``tile_index_backend`` accepts ``"hip"`` (or the
+    always tiled and has one backward.  ``recompute_normalizer`` (default) keeps only the image
+    for the backward and recomputes the per-pixel normalizer there; ``False`` saves it too.  ``tile_index_backend`` accepts ``"hip"`` (or the
     reference's ``"cuda"``); the reference's torch-op builder exists here only as the oracle.
     """
     _check_field(means, conics, colors, radii, opacities)
@@ -332,7 +342,7 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
     return _ExactRenderHip.apply(means, conics, colors, radii, opacities, int(H), int(W),
                                  bool(normalize), float(eps), int(tile_size),
                                  _fade_alpha(support_fade), float(sigma_cutoff),
-                                 bool(tile_ellipse_cull))
+                                 bool(tile_ellipse_cull), bool(recompute_normalizer))
 
 
 @torch.no_grad()

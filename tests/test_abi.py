@@ -56,17 +56,16 @@ def test_null_output_pointer_is_rejected():
     assert isinstance(ctypes.c_size_t(0).value, int)
 
 
-def test_normalized_backward_requires_saved_forward_outputs():
-    """Argument checks run on the host before any device work: a normalized backward without the
-    forward's den/out is rejected (the reference's backward_tiled requires both,
-    render_ext.cpp:288-293)."""
+def test_normalized_backward_given_den_needs_out():
+    """Argument checks run on the host before any device work: den without out is rejected (den
+    alone cannot give the adjoints; out alone, or neither, selects the recomputing backward)."""
     lib = _native.load()
     dummy = ctypes.create_string_buffer(64)
     p = ctypes.cast(dummy, ctypes.c_void_p)
-    rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, None, None, 1, p, p, 1, 16, 16, 16, 1,
+    rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, p, None, 1, p, p, 1, 16, 16, 16, 1,
                                    1e-8, 0.0, 3.0, p, p, p, None, p, 64, None)
     assert rc == _native.ERR_INVALID_ARGUMENT
-    assert "den and out" in _native.last_error()
+    assert "also needs the forward's out" in _native.last_error()
 
 
 def test_kernel_timing_rejects_unknown_kernels():

@@ -110,6 +110,35 @@ def test_backward_matches_fp64(cuda, normalize, fade):
         assert err <= 1e-4, name
 
 
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+def test_backward_normalizer_modes_agree(cuda, fade):
+    """Saved den and out, recomputed den, and recomputed numerator and den: same gradients
+    (the recomputed values equal the forward's bit for bit; only atomics reorder sums)."""
+    _print_case(test="backward_modes", fade=fade)
+    N, H, W = 300, 66, 83
+    f = random_field(19, N, H, W, scale_lo=0.8, scale_hi=6.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    alpha = HR._fade_alpha(fade)
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, 16, alpha == 1.0, 3.0)
+    den = torch.empty(H, W, device=cuda)
+    out, _ = HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8,
+                              16, alpha, 3.0, den=den)
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(4))
+    ref = exact64.backward(to_np(g), to_np(means), to_np(conics), to_np(colors), to_np(radii),
+                           to_np(op), H, W, True, 1e-8, fade, 3.0)
+    results = []
+    for d, o in ((den, out), (None, out), (None, None)):
+        grads = HR.backward_tiled(g, means, conics, colors, radii, op, d, o, gids, offsets, H, W,
+                                  True, 1e-8, 16, alpha, 3.0)
+        torch.cuda.synchronize()
+        results.append([to_np(x) for x in grads])
+        for got, exp in zip(results[-1], ref):
+            assert scaled_err(got, exp) <= 1e-4
+    for other in results[1:]:
+        for a, b in zip(results[0], other):
+            assert scaled_err(b, a) <= 1e-6
+
+
 def test_empty_field_renders_zero(cuda):
     H, W = 8, 9
     z2 = torch.zeros(0, 2, device=cuda)

@@ -18,13 +18,19 @@ radii 8), N Gaussians, P (tile, Gaussian) pairs and T tiles.  This is the byte m
 that recomputes the normalizer with a second list sweep; the default backward does so and also
 reads the forward's image (12 H W, not counted).
 
-The roofline is reported for k_backward16, the step's dominant kernel: its algorithmic bytes per
-launch (B_bwd) over its average duration, measured by HIP events that the library records on the
-launch stream around that kernel alone.  `traffic` is the kernel's measured HBM bytes per launch
+P in `value` is counted at the reference's 16-pixel tiling whatever tile the renderer runs: the
+workload's unit of work stays fixed, so a tiling with fewer pairs shows up as a shorter step, not
+as fewer bytes.  The render tile (16 or 32 pixels) is the autotuner's choice on this field
+(`--tile-size` overrides it); every candidate's fwd+bwd time is reported.
+
+The roofline is reported for k_backward, the step's dominant kernel, at the chosen tile: its
+algorithmic bytes per launch (B_bwd with that tile's P) over its average duration, measured by
+HIP events that the library records on the launch stream around that kernel alone.  `traffic` is the kernel's measured HBM bytes per launch
 (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy; `--copy-probe` is that copy)
 when profiles/hbm_traffic.json holds a measurement of this workload, else null.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|single|stress]
+                       [--tile-size auto|16|32]
        python bench.py --copy-probe MIB     (one device copy of MIB MiB, for counter calibration)
 """
 from __future__ import annotations
@@ -48,7 +54,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 WORKLOADS = {
     "metric": ("fwd+bwd 2048x2048 RGB fp32, 200,000 Gaussians: scales log-U[0.5,12] px, aspect "
                "log-U[1,10] (major s, minor s/a), smoothed random angle field, opacity U[0.1,1], "
-               "normalized, sigma_cutoff 3, tile 16, seed 0", 2048, 2048, 200_000),
+               "normalized, sigma_cutoff 3, seed 0", 2048, 2048, 200_000),
     "single": ("fwd+bwd 1024x1024 RGB fp32, 50,000 Gaussians, same recipe, seed 0",
                1024, 1024, 50_000),
     "stress": ("fwd+bwd 4096x4096 RGB fp32, 1,000,000 Gaussians: scales log-normal(ln 2, 1) "
@@ -215,6 +221,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32"],
+                    help="render tile edge; auto = the autotuner's choice on this field")
     ap.add_argument("--batched-steps", type=int, default=20,
                     help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
@@ -235,6 +243,7 @@ def main() -> int:
     import torch
 
     from structsplat_amd import render as HR
+    from structsplat_amd import tune
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
@@ -244,18 +253,27 @@ def main() -> int:
     torch.cuda.set_device(dev)
     means, conics, colors, radii, op = make_field(args.workload, dev)
     N = means.shape[0]
-    T = HR.num_tiles(H, W, 16)
+    # Tile shape: the autotuner's choice (fwd+bwd of this field timed at each candidate).
+    best, tile_ms = tune.tune_tile_size([(H, W)], [N], means, conics, colors, radii, op,
+                                        normalize=True, eps=1e-8, support_fade=False,
+                                        sigma_cutoff=3.0, recompute_normalizer=True)
+    ts = best if args.tile_size == "auto" else int(args.tile_size)
+    T = HR.num_tiles(H, W, ts)
+    T16 = HR.num_tiles(H, W, 16)
     gen = torch.Generator(device=dev).manual_seed(0)
     grad_out = (torch.rand(H, W, 3, device=dev, generator=gen) - 0.5) / (3 * H * W)
 
+    # Pair count at the reference's 16-pixel tiling (the byte model's P).
+    _, offsets16 = HR.build_tile_index(means, conics, radii, H, W, 16)
+    P16 = int(offsets16[-1].item())
     # Size the pair buffer once (outside the timed region), then bin without host syncs.
-    gids0, offsets0 = HR.build_tile_index(means, conics, radii, H, W)
+    _, offsets0 = HR.build_tile_index(means, conics, radii, H, W, ts)
     P = int(offsets0[-1].item())
     capacity = P + P // 8 + 1024
     gids = torch.empty(capacity, dtype=torch.int32, device=dev)
     offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
-    ws = HR.workspace(N, H, W, 16, dev)
+    ws = HR.workspace(N, H, W, ts, dev)
     out = torch.empty(H, W, 3, device=dev)
     den = torch.empty(H, W, device=dev)
     grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
@@ -269,18 +287,18 @@ def main() -> int:
         """One fwd+bwd; `saved`: the forward stores den and the backward reads it."""
         if record:
             ev["b"][0].record(stream)
-        HR.build_tile_index_capacity(means, conics, radii, H, W, capacity,
+        HR.build_tile_index_capacity(means, conics, radii, H, W, capacity, ts,
                                      out=(gids, offsets, status, ws))
         if record:
             ev["b"][1].record(stream)
             ev["f"][0].record(stream)
-        HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8, 16,
+        HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True, 1e-8, ts,
                          False, 3.0, out=out, den=den if saved else None, ws=ws)
         if record:
             ev["f"][1].record(stream)
             ev["g"][0].record(stream)
         HR.backward_tiled(grad_out, means, conics, colors, radii, op, den if saved else None, out,
-                          gids, offsets, H, W, True, 1e-8, 16, False, 3.0, grads=grads, ws=ws)
+                          gids, offsets, H, W, True, 1e-8, ts, False, 3.0, grads=grads, ws=ws)
         if record:
             ev["g"][1].record(stream)
 
@@ -309,7 +327,7 @@ def main() -> int:
         raise RuntimeError("pair capacity exceeded")
     phase = {k: statistics.mean(v) for k, v in phase_ms.items()}
 
-    # Kernel-only durations: the library's event pairs around k_forward16 / k_backward16.
+    # Kernel-only durations: the library's event pairs around k_forward / k_backward.
     from structsplat_amd import _native
 
     def kernel_times(saved: bool) -> dict[str, float]:
@@ -324,8 +342,8 @@ def main() -> int:
             _native.kernel_timing(False)
         nf, fwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_FORWARD)
         nb, bwd_kernel_ms = _native.kernel_timing_read(_native.KERNEL_BACKWARD)
-        return {"k_forward16": fwd_kernel_ms / max(nf, 1),
-                "k_backward16": bwd_kernel_ms / max(nb, 1)}
+        return {"k_forward": fwd_kernel_ms / max(nf, 1),
+                "k_backward": bwd_kernel_ms / max(nb, 1)}
 
     kernel_ms = kernel_times(False)
     # The stored-normalizer variant: whole-step wall time and kernel times.
@@ -340,9 +358,12 @@ def main() -> int:
     saved_kernel_ms = kernel_times(True)
 
     P_now = int(offsets[-1].item())
-    b = algorithmic_bytes(N, P_now, T, H * W)
+    if P_now != P:
+        raise RuntimeError(f"pair count changed between binnings: {P} -> {P_now}")
+    b = algorithmic_bytes(N, P16, T16, H * W)       # the workload's work (reference tiling)
+    b_tile = algorithmic_bytes(N, P, T, H * W)      # what the chosen tiling must move
     value = b["step"] / (ms_per_step * 1e-3) / 1e9
-    bwd_gbs = b["bwd"] / (kernel_ms["k_backward16"] * 1e-3) / 1e9
+    bwd_gbs = b_tile["bwd"] / (kernel_ms["k_backward"] * 1e-3) / 1e9
     line = {
         "metric": "fwd+bwd ms & HBM GB/s at 2048^2/200k Gaussians; batched fit steps/s vs plain",
         "value": round(value, 3),
@@ -358,7 +379,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "H": H, "W": W, "n_gaussians": N, "tile_pairs": P_now, "tiles": T,
+            "H": H, "W": W, "n_gaussians": N,
+            "tile_size": ts, "tile_choice": args.tile_size,
+            "autotune_fwd_bwd_ms": {str(k): round(v, 4) for k, v in tile_ms.items()},
+            "tile_pairs": P, "tiles": T, "tile_pairs_at_16": P16,
             "phase_ms": {k: round(v, 4) for k, v in phase.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "normalizer": "recomputed in the backward (den not stored)",
@@ -367,15 +391,16 @@ def main() -> int:
                 "kernel_ms": {k: round(v, 4) for k, v in saved_kernel_ms.items()},
             },
             "algorithmic_bytes": b,
+            "algorithmic_bytes_at_tile": b_tile,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_backward16",
+            "kernel": f"k_backward<{ts}>",
             "achieved": round(bwd_gbs, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(args.workload, "k_backward16"),
+            "traffic": measured_traffic(args.workload, f"k_backward<{ts}>"),
         },
     }
     if args.batched_steps > 0:

@@ -50,16 +50,22 @@ def _visit_chunks(x0, x1, y0, y1, valid):
         start = end
 
 
-def _prepare(means, conics, colors, radii, opacities, H, W, rects):
-    means = np.asarray(means, dtype=np.float32).reshape(-1, 2)
+def _prepare(means, conics, colors, radii, opacities, H, W, rects, float64_params=False):
+    """Parameters as float64 arrays: rounded to float32 first (the renderer's inputs), or taken
+    as given with ``float64_params`` (finite differences with steps below float32's spacing;
+    the support rectangles must then be passed in)."""
+    if float64_params and rects is None:
+        raise ValueError("float64_params needs explicit support rectangles")
+    inp = np.float64 if float64_params else np.float32
+    means = np.asarray(means, dtype=inp).reshape(-1, 2)
     rect = support_rects(means, radii, H, W) if rects is None else rects
     m = means.astype(np.float64)
-    q = np.asarray(conics, dtype=np.float32).reshape(-1, 3).astype(np.float64)
-    col = np.asarray(colors, dtype=np.float32).reshape(-1, 3).astype(np.float64)
+    q = np.asarray(conics, dtype=inp).reshape(-1, 3).astype(np.float64)
+    col = np.asarray(colors, dtype=inp).reshape(-1, 3).astype(np.float64)
     if opacities is None or np.size(opacities) == 0:
         op = np.ones(m.shape[0])
     else:
-        op = np.asarray(opacities, dtype=np.float32).reshape(-1).astype(np.float64)
+        op = np.asarray(opacities, dtype=inp).reshape(-1).astype(np.float64)
     return m, q, col, op, rect
 
 
@@ -82,9 +88,10 @@ def _weights(m, q, op, gid, px, py, fade, tail):
 
 
 def accumulate(means, conics, colors, radii, opacities, H: int, W: int, support_fade: bool,
-               sigma_cutoff: float, rects=None):
+               sigma_cutoff: float, rects=None, float64_params: bool = False):
     """Float64 numerator (H*W, 3) and denominator (H*W,) sums."""
-    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects,
+                                   float64_params)
     fade, tail = _tail(support_fade, sigma_cutoff)
     num = np.zeros((H * W, 3))
     den = np.zeros(H * W)
@@ -99,24 +106,25 @@ def accumulate(means, conics, colors, radii, opacities, H: int, W: int, support_
 
 def forward(means, conics, colors, radii, opacities, H: int, W: int, normalize: bool = True,
             eps: float = 1e-8, support_fade: bool = False, sigma_cutoff: float = 3.0,
-            rects=None):
+            rects=None, float64_params: bool = False):
     """Float64 image (H, W, 3) and raw coverage (H, W)."""
     num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
-                          sigma_cutoff, rects)
+                          sigma_cutoff, rects, float64_params)
     out = num / (den[:, None] + eps) if normalize else num
     return out.reshape(H, W, 3), den.reshape(H, W)
 
 
 def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
              normalize: bool = True, eps: float = 1e-8, support_fade: bool = False,
-             sigma_cutoff: float = 3.0, rects=None):
+             sigma_cutoff: float = 3.0, rects=None, float64_params: bool = False):
     """Analytic float64 gradients (g_means (N,2), g_conics (N,3), g_colors (N,3), g_op (N,))."""
-    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects)
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects,
+                                   float64_params)
     fade, tail = _tail(support_fade, sigma_cutoff)
     G = np.asarray(grad_out, dtype=np.float64).reshape(H * W, 3)
     if normalize:
         num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
-                              sigma_cutoff, rect)
+                              sigma_cutoff, rect, float64_params)
         denom = den + eps
         out = num / denom[:, None]
         dN = G / denom[:, None]

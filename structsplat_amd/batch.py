@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes
 import math
 import time
+import dataclasses
 from dataclasses import dataclass
 
 import torch
@@ -28,38 +29,56 @@ from . import fit as F
 from .gaussians import GaussianField
 from .loss import batch_l1_ssim_loss
 from .metrics import psnr_tensor
+from .tune import tune_field, tune_tile_size
 from .render import _fade_alpha, _ptr, _stream
 
 
+def _plan(sizes, counts, tile_size):
+    """Completed host field table and (gaussians, tiles, pixels, workspace bytes)."""
+    lib = _native.load()
+    nf = len(sizes)
+    table = (_native.Field * nf)()
+    for i, ((h, w), n) in enumerate(zip(sizes, counts)):
+        table[i].height = int(h)
+        table[i].width = int(w)
+        table[i].num_gaussians = int(n)
+    g, t, px = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    wsb = ctypes.c_size_t(0)
+    _native.check(lib.ssplat_batch_plan(ctypes.addressof(table), nf, int(tile_size),
+                                        ctypes.byref(g), ctypes.byref(t), ctypes.byref(px),
+                                        ctypes.byref(wsb)), "ssplat_batch_plan")
+    return table, int(g.value), int(t.value), int(px.value), int(wsb.value)
+
+
+def _device_table(table, device):
+    raw = bytes(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table)))
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+
+
 class Batch:
-    """Planned layout of a batch of fields of sizes `sizes` [(H, W)] with `counts` Gaussians."""
+    """Planned layout of a batch of fields of sizes `sizes` [(H, W)] with `counts` Gaussians,
+    rendered with `tile_size`-pixel tiles (16 or 32).  The loss always tiles by 16 pixels
+    (``loss_fields_dev`` / ``loss_total_tiles``)."""
 
     def __init__(self, sizes, counts, device, tile_size: int = 16):
-        lib = _native.load()
         nf = len(sizes)
         if nf == 0 or len(counts) != nf:
             raise ValueError("a batch needs one Gaussian count per field size")
-        table = (_native.Field * nf)()
-        for i, ((h, w), n) in enumerate(zip(sizes, counts)):
-            table[i].height = int(h)
-            table[i].width = int(w)
-            table[i].num_gaussians = int(n)
-        g, t, px = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
-        wsb = ctypes.c_size_t(0)
-        _native.check(lib.ssplat_batch_plan(ctypes.addressof(table), nf, int(tile_size),
-                                            ctypes.byref(g), ctypes.byref(t), ctypes.byref(px),
-                                            ctypes.byref(wsb)), "ssplat_batch_plan")
+        table, g, t, px, wsb = _plan(sizes, counts, tile_size)
         self.device = torch.device(device)
         self.tile_size = int(tile_size)
         self.num_fields = nf
         self.table = table
-        self.total_gaussians = int(g.value)
-        self.total_tiles = int(t.value)
-        self.total_pixels = int(px.value)
-        raw = bytes(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table)))
-        self.fields_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
-        self.workspace = torch.empty(max(int(wsb.value), 256), dtype=torch.uint8,
-                                     device=self.device)
+        self.total_gaussians = g
+        self.total_tiles = t
+        self.total_pixels = px
+        self.fields_dev = _device_table(table, self.device)
+        self.workspace = torch.empty(max(wsb, 256), dtype=torch.uint8, device=self.device)
+        if self.tile_size == 16:
+            self.loss_fields_dev, self.loss_total_tiles = self.fields_dev, t
+        else:
+            table16, _, t16, _, _ = _plan(sizes, counts, 16)
+            self.loss_fields_dev, self.loss_total_tiles = _device_table(table16, self.device), t16
         self.sizes = [(int(h), int(w)) for h, w in sizes]
         self.gauss_slices = [slice(table[i].first_gaussian,
                                    table[i].first_gaussian + table[i].num_gaussians)
@@ -202,8 +221,18 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     copied back into `fields`."""
     dev = targets[0].device
     sizes = [tuple(t.shape[:2]) for t in targets]
-    batch = Batch(sizes, [f.n for f in fields], dev)
+    counts = [f.n for f in fields]
     big = concat_fields(fields).trainable()
+
+    def tune():
+        conics, radii, op = big.produce(cfg.sigma_cutoff)
+        return tune_tile_size(sizes, counts, big.means.detach(), conics.detach(),
+                              big.colors.detach(), radii, None if op is None else op.detach(),
+                              normalize=cfg.normalize, eps=cfg.normalization_eps,
+                              support_fade=cfg.support_fade, sigma_cutoff=cfg.sigma_cutoff,
+                              recompute_normalizer=cfg.recompute_normalizer)[0]
+
+    batch = Batch(sizes, counts, dev, tile_size=F.resolve_tile_size(cfg, tune))
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
     opt = F.make_optimizer(big, cfg)
     hi = torch.cat([torch.full((f.n,), math.log(max(h, w)), device=dev)
@@ -289,10 +318,14 @@ def fit_each(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.Fi
              ) -> BatchResult:
     """The plain per-image loop: each field fitted on its own, eagerly, one after the other."""
     dev = targets[0].device
+    # Tile sizes are chosen before the clock starts, as the batched fit's is.
+    cfgs = [dataclasses.replace(cfg, tile_size=F.resolve_tile_size(
+        cfg, lambda f=f, t=t: tune_field(f, int(t.shape[0]), int(t.shape[1]), cfg)[0]))
+        for f, t in zip(fields, targets)]
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     psnrs = []
-    for f, t in zip(fields, targets):
-        psnrs.append(F.fit(f, t, cfg, graph=False)["psnr"])
+    for f, t, c in zip(fields, targets, cfgs):
+        psnrs.append(F.fit(f, t, c, graph=False)["psnr"])
     torch.cuda.synchronize(dev)
     return BatchResult(psnrs, time.perf_counter() - t0, cfg.iters * len(fields))

@@ -121,10 +121,9 @@ constexpr int64_t kSingleScanItems = int64_t(1) << 16;
 constexpr int kSortThreads = 256;
 constexpr int kSortLds = 4096;      // lists up to this length sort in the block's LDS
 constexpr int kSortWaveLds = 1024;  // lists up to this length sort in one wave's LDS slice
-constexpr int kRenderThreads = 256;
 constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
-constexpr int kTile = 16;    // render tile edge (pixels)
-static_assert(kRenderThreads == kTile * kTile, "one render thread per tile pixel");
+// The render kernels run one workgroup per tile, one thread per pixel (tile edge 16: 256 threads,
+// 4 waves; tile edge 32: 1024 threads, 16 waves), one wave per 8x8 quadrant.
 // -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
 constexpr float kNegHalfLog2e = -0.72134752044448170368f;
 constexpr float kInvNegHalfLog2e = -1.38629436111989061883f;  // 1 / kNegHalfLog2e = -2 ln 2
@@ -977,12 +976,12 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   acc = make_float4(nr, ng, nb, d);
 }
 
-template <bool kFade, bool kNormalize>
-__global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
-                                                              float* __restrict__ out,
-                                                              float* __restrict__ den) {
+template <int kT, bool kFade, bool kNormalize>
+__global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __restrict__ out,
+                                                     float* __restrict__ den) {
+  constexpr int kQ = kT / 8;  // quadrants per tile row
   __shared__ StageLds<3> st;
-  __shared__ uint32_t s_mask[kRenderThreads / 64][2 * kStage];  // per-wave quadrant masks
+  __shared__ uint32_t s_mask[kQ * kQ][2 * kStage];  // per-wave quadrant masks
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
@@ -990,8 +989,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_forward16(RenderArgs a,
   const int tile_y = local / f.tiles_x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int qx0 = tile_x * kTile + (wave & 1) * 8;
-  const int qy0 = tile_y * kTile + (wave >> 1) * 8;
+  const int qx0 = tile_x * kT + (wave % kQ) * 8;
+  const int qy0 = tile_y * kT + (wave / kQ) * 8;
   const int px = qx0 + (lane & 7);
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
@@ -1028,13 +1027,16 @@ constexpr int kNormSaved = 0;      // both read from the forward's saved outputs
 constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-only sweep
 constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
 
-template <bool kFade, bool kNormalize, int kNorm>
-__global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, GradArgs g) {
+template <int kT, bool kFade, bool kNormalize, int kNorm>
+__global__ __launch_bounds__(kT * kT) void k_backward(RenderArgs a, GradArgs g) {
   constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
+  constexpr int kQ = kT / 8;  // quadrants per tile row
+  constexpr int kWaves = kQ * kQ;
+  constexpr int kMaskF4 = kWaves * 2 * kStage / 4;  // the sweep's masks, in float4 units
   __shared__ StageLds<4> st;
   // The per-pixel adjoints; when recomputing, the same space first holds the sweep's per-wave
   // quadrant masks (two uint32 per staged record and wave).
-  __shared__ float4 s_scratch[kRecompute ? (kRenderThreads / 64) * 2 * kStage / 4 : kTile * kTile];
+  __shared__ float4 s_scratch[kRecompute && kMaskF4 > kT * kT ? kMaskF4 : kT * kT];
   float4* s_adj = s_scratch;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
@@ -1045,8 +1047,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   const int lane = threadIdx.x & 63;
   const int lx = lane & 7;
   const int ly = lane >> 3;
-  const int tx0 = tile_x * kTile;
-  const int ty0 = tile_y * kTile;
+  const int tx0 = tile_x * kT;
+  const int ty0 = tile_y * kT;
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
 
   // Per-pixel adjoints (render_ext.cu:496-510):
@@ -1054,8 +1056,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   if (kRecompute) {
     // D (and N) recomputed for one pixel per lane by the forward's own sweep and order, so they
     // equal the forward's values bit for bit.
-    const int qx0 = tx0 + (wave & 1) * 8;
-    const int qy0 = ty0 + (wave >> 1) * 8;
+    const int qx0 = tx0 + (wave % kQ) * 8;
+    const int qy0 = ty0 + (wave / kQ) * 8;
     const int px = qx0 + lx;
     const int py = qy0 + ly;
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kStage);
@@ -1083,10 +1085,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
       const float dot = gr * o_r + gg * o_g + gb * o_b;
       adj = make_float4(gr / denom, gg / denom, gb / denom, -dot / denom);
     }
-    s_adj[(py - ty0) * kTile + (px - tx0)] = adj;
+    s_adj[(py - ty0) * kT + (px - tx0)] = adj;
   } else {
-    const int px = tx0 + (threadIdx.x & (kTile - 1));
-    const int py = ty0 + (threadIdx.x >> 4);
+    const int px = tx0 + static_cast<int>(threadIdx.x % kT);
+    const int py = ty0 + static_cast<int>(threadIdx.x / kT);
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (px < f.width && py < f.height) {
       const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
@@ -1106,11 +1108,14 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
   }
   __syncthreads();
 
-  // Phase 2: each lane owns the pixel (lx, ly) of all four quadrants.
+  // Gradient sweep: each lane owns the pixel (lx, ly) of every quadrant; with 16-pixel tiles
+  // the four quadrants' adjoints stay in registers.
   float4 adj[4];
+  if (kT == 16) {
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    adj[s] = s_adj[((s >> 1) * 8 + ly) * kTile + (s & 1) * 8 + lx];
+    for (int s = 0; s < 4; ++s) {
+      adj[s] = s_adj[((s >> 1) * 8 + ly) * kT + (s & 1) * 8 + lx];
+    }
   }
   const bool has_opacity = g.grad_opacities != nullptr;
   // Lane roles of the transposed reduction: lane 8k+7 owns gradient component k
@@ -1128,7 +1133,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
     const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
     stage_batch<4>(a, st, batch, m);
     __syncthreads();
-    for (int j = wave; j < m; j += kRenderThreads / 64) {
+    for (int j = wave; j < m; j += kWaves) {
       const float4 v0 = st.c[0][j];
       const float4 v1 = st.c[1][j];
       const float4 v2 = st.c[2][j];
@@ -1140,41 +1145,60 @@ __global__ __launch_bounds__(kRenderThreads) void k_backward16(RenderArgs a, Gra
       const float cr = v3.x, cg = v3.y, cbl = v3.z;
       float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
       float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
-      // One straight-line block per quadrant: the per-lane rectangle test alone decides which
-      // lanes accumulate (a quadrant the rectangle misses has no lane inside and is skipped by
-      // the exec-mask branch), so the accumulators stay in the same registers throughout.
+      // One visit: pixel (px, py) inside the rectangle, with its adjoint.
+      auto visit = [&](int px, int py, const float4& ad) {
+        const float dx = static_cast<float>(px) - mx;
+        const float dy = static_cast<float>(py) - my;
+        const float t = fmaf(qa, dx, qb * dy);
+        const float e = fmaf(t, dx, qc * dy * dy);
+        const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
+        const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
+        float base, dqr;
+        if (kFade) {
+          base = fmaxf(raw - tail, 0.0f);
+          dqr = raw > tail ? dw * raw : 0.0f;
+        } else {
+          base = raw;
+          dqr = dw * raw;
+        }
+        gcr = fmaf(ad.x, base, gcr);
+        gcg = fmaf(ad.y, base, gcg);
+        gcb = fmaf(ad.z, base, gcb);
+        gop = fmaf(dw, base, gop);
+        sx = fmaf(dqr, dx, sx);
+        sy = fmaf(dqr, dy, sy);
+        const float dqx = dqr * dx;
+        sxx = fmaf(dqx, dx, sxx);
+        sxy = fmaf(dqx, dy, sxy);
+        syy = fmaf(dqr * dy, dy, syy);
+      };
+      const unsigned rw = static_cast<unsigned>(x1 - x0), rh = static_cast<unsigned>(y1 - y0);
+      if constexpr (kT == 16) {
+        // One straight-line block per quadrant: the per-lane rectangle test alone decides which
+        // lanes accumulate (a quadrant the rectangle misses has no lane inside and is skipped by
+        // the exec-mask branch), so the accumulators stay in the same registers throughout.
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int px = tx0 + (s & 1) * 8 + lx;
-        const int py = ty0 + (s >> 1) * 8 + ly;
-        const bool inside = (static_cast<unsigned>(px - x0) < static_cast<unsigned>(x1 - x0)) &
-                            (static_cast<unsigned>(py - y0) < static_cast<unsigned>(y1 - y0));
-        if (inside) {
-          const float dx = static_cast<float>(px) - mx;
-          const float dy = static_cast<float>(py) - my;
-          const float t = fmaf(qa, dx, qb * dy);
-          const float e = fmaf(t, dx, qc * dy * dy);
-          const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
-          const float4 ad = adj[s];
-          const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
-          float base, dqr;
-          if (kFade) {
-            base = fmaxf(raw - tail, 0.0f);
-            dqr = raw > tail ? dw * raw : 0.0f;
-          } else {
-            base = raw;
-            dqr = dw * raw;
+        for (int s = 0; s < 4; ++s) {
+          const int px = tx0 + (s & 1) * 8 + lx;
+          const int py = ty0 + (s >> 1) * 8 + ly;
+          if ((static_cast<unsigned>(px - x0) < rw) & (static_cast<unsigned>(py - y0) < rh)) {
+            visit(px, py, adj[s]);
           }
-          gcr = fmaf(ad.x, base, gcr);
-          gcg = fmaf(ad.y, base, gcg);
-          gcb = fmaf(ad.z, base, gcb);
-          gop = fmaf(dw, base, gop);
-          sx = fmaf(dqr, dx, sx);
-          sy = fmaf(dqr, dy, sy);
-          const float dqx = dqr * dx;
-          sxx = fmaf(dqx, dx, sxx);
-          sxy = fmaf(dqx, dy, sxy);
-          syy = fmaf(dqr * dy, dy, syy);
+        }
+      } else {
+        // Only the quadrants the rectangle meets (uniform bounds), adjoints read from LDS.
+        const int cx0 = max(x0, tx0) - tx0, cx1 = min(x1, tx0 + kT) - tx0;
+        const int cy0 = max(y0, ty0) - ty0, cy1 = min(y1, ty0 + kT) - ty0;
+        if (cx1 > cx0 && cy1 > cy0) {
+          for (int qy = cy0 >> 3; qy <= (cy1 - 1) >> 3; ++qy) {
+            for (int qx = cx0 >> 3; qx <= (cx1 - 1) >> 3; ++qx) {
+              const int px = tx0 + qx * 8 + lx;
+              const int py = ty0 + qy * 8 + ly;
+              if ((static_cast<unsigned>(px - x0) < rw) & (static_cast<unsigned>(py - y0) < rh)) {
+                visit(px, py, s_adj[(qy * 8 + ly) * kT + qx * 8 + lx]);
+              }
+            }
+          }
         }
       }
       // Reduce the eight partials to lanes 8k+7 (k = component), the opacity partial to lane 63.
@@ -1456,9 +1480,9 @@ int check_render_inputs(const float* means, const float* conics, const float* co
   if (num_gids < 0 || (num_gids > 0 && gids == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or num_gids negative");
   }
-  if (p.ts != kTile) {
-    return set_error(SSPLAT_ERR_UNSUPPORTED, "the render kernels are built for tile_size %d, got %d",
-                     kTile, p.ts);
+  if (p.ts != 16 && p.ts != 32) {
+    return set_error(SSPLAT_ERR_UNSUPPORTED,
+                     "the render kernels are built for tile_size 16 and 32, got %d", p.ts);
   }
   if (normalize && !(std::isfinite(eps) && eps > 0.0)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "normalization_eps must be finite and > 0");
@@ -1500,6 +1524,53 @@ int run_prepare(const float* means, const float* conics, const float* colors,
   return SSPLAT_OK;
 }
 
+// Kernel selection for one tile edge: support fade, normalization and the backward's normalizer
+// source are template parameters (no per-pixel branches on them).
+template <int kT>
+void launch_forward(bool fade, bool normalize, int tiles, const RenderArgs& ra, float* out,
+                    float* den, hipStream_t s) {
+  const dim3 grid(tiles), block(kT * kT);
+  if (fade && normalize) {
+    k_forward<kT, true, true><<<grid, block, 0, s>>>(ra, out, den);
+  } else if (fade) {
+    k_forward<kT, true, false><<<grid, block, 0, s>>>(ra, out, den);
+  } else if (normalize) {
+    k_forward<kT, false, true><<<grid, block, 0, s>>>(ra, out, den);
+  } else {
+    k_forward<kT, false, false><<<grid, block, 0, s>>>(ra, out, den);
+  }
+}
+
+template <int kT, bool kFade>
+void launch_backward_normalized(int norm, int tiles, const RenderArgs& ra, const GradArgs& ga,
+                                hipStream_t s) {
+  const dim3 grid(tiles), block(kT * kT);
+  if (norm == kNormSaved) {
+    k_backward<kT, kFade, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+  } else if (norm == kNormRecompD) {
+    k_backward<kT, kFade, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
+  } else {
+    k_backward<kT, kFade, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
+  }
+}
+
+template <int kT>
+void launch_backward(bool fade, bool normalize, int norm, int tiles, const RenderArgs& ra,
+                     const GradArgs& ga, hipStream_t s) {
+  const dim3 grid(tiles), block(kT * kT);
+  if (!normalize) {
+    if (fade) {
+      k_backward<kT, true, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    } else {
+      k_backward<kT, false, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    }
+  } else if (fade) {
+    launch_backward_normalized<kT, true>(norm, tiles, ra, ga, s);
+  } else {
+    launch_backward_normalized<kT, false>(norm, tiles, ra, ga, s);
+  }
+}
+
 int forward_tiled(const Plan& p, const float* means, const float* conics, const float* colors,
                   const int64_t* radii, const float* opacities, const int32_t* gids,
                   const int32_t* offsets, int64_t num_gids, int normalize, double eps,
@@ -1525,16 +1596,12 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
                                         sigma_cutoff);
   hipEvent_t t0;
   const bool timed = timer_begin(0, s, &t0);
-  if (support_fade && normalize) {
-    k_forward16<true, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  } else if (support_fade) {
-    k_forward16<true, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
-  } else if (normalize) {
-    k_forward16<false, true><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+  if (p.ts == 16) {
+    launch_forward<16>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
   } else {
-    k_forward16<false, false><<<p.num_tiles, kRenderThreads, 0, s>>>(ra, out, den);
+    launch_forward<32>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
   }
-  SSPLAT_LAUNCH_CHECK("k_forward16");
+  SSPLAT_LAUNCH_CHECK("k_forward");
   if (timed) timer_end(0, s, t0);
   return success();
 }
@@ -1577,31 +1644,12 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   GradArgs ga{grad_out, den, out, grad_means, grad_conics, grad_colors, gop};
   hipEvent_t t0;
   const bool timed = timer_begin(1, s, &t0);
-  const dim3 grid(p.num_tiles), block(kRenderThreads);
-  if (!normalize) {
-    if (support_fade) {
-      k_backward16<true, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
-    } else {
-      k_backward16<false, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
-    }
-  } else if (support_fade) {
-    if (norm == kNormSaved) {
-      k_backward16<true, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
-    } else if (norm == kNormRecompD) {
-      k_backward16<true, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
-    } else {
-      k_backward16<true, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
-    }
+  if (p.ts == 16) {
+    launch_backward<16>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
   } else {
-    if (norm == kNormSaved) {
-      k_backward16<false, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
-    } else if (norm == kNormRecompD) {
-      k_backward16<false, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
-    } else {
-      k_backward16<false, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
-    }
+    launch_backward<32>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
   }
-  SSPLAT_LAUNCH_CHECK("k_backward16");
+  SSPLAT_LAUNCH_CHECK("k_backward");
   if (timed) timer_end(1, s, t0);
   return success();
 }

@@ -30,6 +30,7 @@ from . import render as HR
 from .gaussians import GaussianField
 from .loss import l1_ssim_loss
 from .optim import FusedAdam
+from .tune import tune_field
 from .metrics import psnr_tensor
 
 
@@ -50,6 +51,7 @@ class FitConfig:
     window: int = 25               # steps between capacity checks
     capacity_slack: float = 1.5    # pair-buffer capacity / current pair count
     capacity_floor: int = 4096     # minimum pair-buffer capacity
+    tile_size: int | str = "auto"  # render tile edge (16 or 32), or "auto": the autotuner's choice
 
 
 class RenderPlan:
@@ -168,8 +170,13 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     field.trainable()
     opt = make_optimizer(field, cfg)
     opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))
-    plan = RenderPlan(field.n, H, W, _capacity(pair_count(field, H, W, cfg), cfg), dev,
-                      recompute_normalizer=cfg.recompute_normalizer)
+    ts = resolve_tile_size(cfg, lambda: tune_field(field, H, W, cfg)[0])
+
+    def new_plan(capacity):
+        return RenderPlan(field.n, H, W, capacity, dev, tile_size=ts,
+                          recompute_normalizer=cfg.recompute_normalizer)
+
+    plan = new_plan(_capacity(pair_count(field, H, W, cfg, ts), cfg))
     state = {"loss": None}
 
     def step():
@@ -207,18 +214,16 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
                 step()
         if int(plan.status.item()) != 0:
             snap.restore(field, opt)
-            plan = RenderPlan(field.n, H, W,
-                              max(2 * plan.capacity, _capacity(pair_count(field, H, W, cfg), cfg)),
-                              dev, recompute_normalizer=cfg.recompute_normalizer)
+            plan = new_plan(max(2 * plan.capacity,
+                                _capacity(pair_count(field, H, W, cfg, ts), cfg)))
             graph_obj = None
             regrows += 1
             continue
         done += n
         # Keep headroom: regrow (and re-capture) when the pair count approaches capacity.
-        p = pair_count(field, H, W, cfg)
+        p = pair_count(field, H, W, cfg, ts)
         if p * 1.15 > plan.capacity:
-            plan = RenderPlan(field.n, H, W, _headroom(p, cfg), dev,
-                              recompute_normalizer=cfg.recompute_normalizer)
+            plan = new_plan(_headroom(p, cfg))
             graph_obj = None
             regrows += 1
         if verbose:
@@ -231,7 +236,17 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
                                   sigma_cutoff=cfg.sigma_cutoff)
         final_psnr = float(psnr_tensor(img, target))
     return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
-            "capacity": plan.capacity}
+            "capacity": plan.capacity, "tile_size": ts}
+
+
+def resolve_tile_size(cfg: FitConfig, tune) -> int:
+    """cfg.tile_size, or the autotuner's choice (`tune()`) for "auto"."""
+    if cfg.tile_size == "auto":
+        return int(tune())
+    ts = int(cfg.tile_size)
+    if ts not in (16, 32):
+        raise ValueError(f"tile_size must be 16, 32 or 'auto', got {cfg.tile_size!r}")
+    return ts
 
 
 def _capacity(pairs: int, cfg: FitConfig) -> int:

@@ -69,9 +69,10 @@ class _L1SSIM(Function):
                 _ptr(img), _ptr(target), H, W, float(ssim_weight), _ptr(loss), _ptr(ws),
                 ws.numel(), _stream(dev)), "ssplat_l1_ssim_loss_forward")
         else:
-            ws = loss_workspace(batch.total_pixels, batch.total_tiles, dev)
+            ws = loss_workspace(batch.total_pixels, batch.loss_total_tiles, dev)
             _native.check(lib.ssplat_batch_l1_ssim_loss_forward(
-                _ptr(batch.fields_dev), batch.num_fields, batch.total_tiles, batch.total_pixels,
+                _ptr(batch.loss_fields_dev), batch.num_fields, batch.loss_total_tiles,
+                batch.total_pixels,
                 _ptr(img), _ptr(target), float(ssim_weight), _ptr(loss), _ptr(ws), ws.numel(),
                 _stream(dev)), "ssplat_batch_l1_ssim_loss_forward")
         ctx.save_for_backward(img, target, ws)
@@ -95,7 +96,8 @@ class _L1SSIM(Function):
                 "ssplat_l1_ssim_loss_backward")
         else:
             _native.check(lib.ssplat_batch_l1_ssim_loss_backward(
-                _ptr(b.fields_dev), b.num_fields, b.total_tiles, b.total_pixels, _ptr(img),
+                _ptr(b.loss_fields_dev), b.num_fields, b.loss_total_tiles, b.total_pixels,
+                _ptr(img),
                 _ptr(target), ctx.ssim_weight, _ptr(g), _ptr(grad), _ptr(ws), ws.numel(),
                 _stream(dev)), "ssplat_batch_l1_ssim_loss_backward")
         return grad, None, None, None
@@ -112,11 +114,9 @@ def l1_ssim_loss(img: torch.Tensor, target: torch.Tensor, ssim_weight: float = 0
 
 def batch_l1_ssim_loss(batch, img_flat: torch.Tensor, target_flat: torch.Tensor,
                        ssim_weight: float = 0.2):
-    """Sum over the fields of ``batch`` (a ``batch.Batch`` planned with 16-pixel tiles) of each
-    field's loss, for concatenated ``(total_pixels, 3)`` images."""
+    """Sum over the fields of ``batch`` (a ``batch.Batch``) of each field's loss, for concatenated
+    ``(total_pixels, 3)`` images."""
     _check_pair(img_flat, target_flat, "batch_l1_ssim_loss")
-    if batch.tile_size != 16:
-        raise ValueError("batch_l1_ssim_loss needs a batch planned with tile_size 16")
     if img_flat.shape != (batch.total_pixels, 3):
         raise ValueError("batch_l1_ssim_loss: images must be (total_pixels, 3)")
     return _L1SSIM.apply(img_flat, target_flat, _check_weight(ssim_weight), batch)

@@ -1,0 +1,95 @@
+"""Autotuner of the render path's tile shape.
+
+The render kernels run one workgroup per square tile, one thread per pixel and one wave per 8x8
+quadrant, for 16-pixel tiles (4 waves) or 32-pixel tiles (16 waves).  Larger tiles give fewer
+(tile, Gaussian) pairs: less binning work and fewer per-pair gradient reductions in the backward.
+They also give longer lists that every wave of a tile filters.  Which wins depends on the Gaussian
+sizes, so the tuner times the fit step's render work on the actual field: binning, forward,
+backward in the fit's normalizer mode.  It does this for each candidate and keeps the fastest.
+Results are cached per batch layout and mode for the process.
+"""
+from __future__ import annotations
+
+import statistics
+
+import torch
+
+from . import render as HR
+
+CANDIDATE_TILE_SIZES = (16, 32)
+_CACHE: dict = {}
+
+
+def _time_ms(fn, reps: int) -> float:
+    stream = torch.cuda.current_stream()
+    fn()  # warm-up
+    times = []
+    for _ in range(reps):
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        fn()
+        stop.record(stream)
+        stop.synchronize()
+        times.append(start.elapsed_time(stop))
+    return statistics.median(times)
+
+
+@torch.no_grad()
+def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, normalize=True,
+                   eps=1e-8, support_fade=False, sigma_cutoff=3.0, recompute_normalizer=True,
+                   candidates=CANDIDATE_TILE_SIZES, reps=3):
+    """Fastest tile size for rendering (fwd+bwd) the fields `sizes` [(H, W)] with `counts`
+    Gaussians each (rows of `means`, ... concatenated in field order).
+
+    Returns ``(tile_size, {tile_size: ms})``; cached by layout and mode.
+    """
+    from .batch import Batch
+
+    key = (tuple((int(h), int(w)) for h, w in sizes), tuple(int(c) for c in counts),
+           bool(normalize), HR._fade_alpha(support_fade), float(sigma_cutoff),
+           bool(recompute_normalizer), tuple(candidates))
+    hit = _CACHE.get(key)
+    if hit is not None:
+        return hit
+    dev = means.device
+    alpha = HR._fade_alpha(support_fade)
+    cull = alpha == 1.0
+    op = opacities if opacities is not None else means.new_empty(0)
+    means, conics, colors = means.contiguous(), conics.contiguous(), colors.contiguous()
+    radii = HR._radii64(radii)
+    timings = {}
+    for ts in candidates:
+        b = Batch(sizes, counts, dev, tile_size=ts)
+        pairs = b.pair_count(means, conics, radii, cull, sigma_cutoff)
+        gids, offsets, status = b.build_tile_index(means, conics, radii, pairs + 1, cull,
+                                                   sigma_cutoff)
+        den = (torch.empty(b.total_pixels, device=dev)
+               if normalize and not recompute_normalizer else None)
+        grad = torch.rand(b.total_pixels, 3, device=dev,
+                          generator=torch.Generator(device=dev).manual_seed(0))
+        out = torch.empty(b.total_pixels, 3, device=dev)
+        bufs = (gids, offsets, status)
+
+        def step():
+            b.build_tile_index(means, conics, radii, pairs + 1, cull, sigma_cutoff, out=bufs)
+            b.forward(means, conics, colors, radii, op, gids, offsets, normalize, eps, alpha,
+                      sigma_cutoff, out=out, den=den)
+            b.backward(grad, means, conics, colors, radii, op, den, out, gids, offsets,
+                       normalize, eps, alpha, sigma_cutoff)
+
+        timings[ts] = _time_ms(step, reps)
+    best = min(timings, key=timings.get)
+    _CACHE[key] = (best, timings)
+    return best, timings
+
+
+def tune_field(field, H: int, W: int, cfg):
+    """Tile size for fitting one field (the fit step's render configuration)."""
+    conics, radii, op = field.produce(cfg.sigma_cutoff)
+    return tune_tile_size([(H, W)], [field.n], field.means.detach(), conics.detach(),
+                          field.colors.detach(), radii,
+                          None if op is None else op.detach(), normalize=cfg.normalize,
+                          eps=cfg.normalization_eps, support_fade=cfg.support_fade,
+                          sigma_cutoff=cfg.sigma_cutoff,
+                          recompute_normalizer=cfg.recompute_normalizer)

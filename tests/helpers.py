@@ -7,6 +7,7 @@ import numpy as np
 import torch
 
 from oracle import reference_render as R
+from oracle.tile_index import support_rects
 
 
 def random_field(seed: int, N: int, H: int, W: int, scale_lo=0.7, scale_hi=4.0, margin=2.0,
@@ -63,3 +64,28 @@ def scaled_err(got, ref):
     ref = np.asarray(ref, dtype=np.float64)
     scale = float(np.max(np.abs(ref))) if ref.size else 0.0
     return float(np.max(np.abs(got - ref))) / max(scale, 1e-30) if ref.size else 0.0
+
+
+def window_field(means, conics, colors, radii, opacities, H: int, W: int, x0: int, x1: int,
+                 y0: int, y1: int):
+    """The Gaussians that render into the pixel window [x0, x1) x [y0, y1) of an H x W image,
+    moved into window coordinates (for the float64 oracle with ``float64_params=True``).
+
+    A pixel sums exactly the Gaussians whose clipped support rectangle holds it, so rendering
+    these Gaussians over the window, with their image rectangles clipped to the window and
+    shifted (``rects``), gives the image's pixels there.  Means are shifted in float64, which
+    holds every float32 mean minus an integer exactly.  Returns ``(ids, means, conics, colors,
+    radii, opacities, rects)``; ``ids`` are the Gaussians' indices in the field, ascending.
+    """
+    m32 = np.asarray(means, dtype=np.float32).reshape(-1, 2)
+    rx0, rx1, ry0, ry1, valid = support_rects(m32, radii, H, W)
+    hit = valid & (rx0 < x1) & (rx1 > x0) & (ry0 < y1) & (ry1 > y0)
+    ids = np.nonzero(hit)[0]
+    rects = (np.clip(rx0[ids], x0, x1) - x0, np.clip(rx1[ids], x0, x1) - x0,
+             np.clip(ry0[ids], y0, y1) - y0, np.clip(ry1[ids], y0, y1) - y0,
+             np.ones(ids.size, dtype=bool))
+    shift = np.array([x0, y0], dtype=np.float64)
+    f64 = lambda a, k: np.asarray(a, dtype=np.float32).reshape(-1, k)[ids].astype(np.float64)
+    op = None if opacities is None or np.size(opacities) == 0 else f64(opacities, 1)[:, 0]
+    return (ids, m32[ids].astype(np.float64) - shift, f64(conics, 3), f64(colors, 3),
+            np.asarray(radii).reshape(-1, 2)[ids], op, rects)

@@ -31,11 +31,12 @@ def _cat(parts, i):
     return torch.cat([p[i] for p in parts], 0).contiguous()
 
 
+@pytest.mark.parametrize("ts", [16, 32])
 @pytest.mark.parametrize("fade", [False, True])
-def test_batched_index_and_forward_match_per_field(cuda, fade):
-    print(f"case: batch forward, fade={fade}", flush=True)
+def test_batched_index_and_forward_match_per_field(cuda, fade, ts):
+    print(f"case: batch forward, fade={fade}, tile {ts}", flush=True)
     parts = _fields(cuda)
-    b = B.Batch(SIZES, COUNTS, cuda)
+    b = B.Batch(SIZES, COUNTS, cuda, tile_size=ts)
     means, conics, colors, radii, op = (_cat(parts, i) for i in range(5))
     gids, offsets, status = b.build_tile_index(means, conics, radii, 1 << 16, fade, 3.0)
     out = b.forward(means, conics, colors, radii, op, gids, offsets, True, 1e-8, fade, 3.0)
@@ -44,7 +45,7 @@ def test_batched_index_and_forward_match_per_field(cuda, fade):
     offs = offsets.cpu().numpy()
     g_all = gids.cpu().numpy()
     for k, (p, (h, w)) in enumerate(zip(parts, SIZES)):
-        fg, fo = HR.build_tile_index(p[0], p[1], p[3], h, w, 16, fade, 3.0)
+        fg, fo = HR.build_tile_index(p[0], p[1], p[3], h, w, ts, fade, 3.0)
         t0 = b.table[k].first_tile
         g0 = b.table[k].first_gaussian
         seg = offs[t0:t0 + b.table[k].num_tiles + 1]
@@ -54,10 +55,11 @@ def test_batched_index_and_forward_match_per_field(cuda, fade):
         assert torch.equal(b.images(out)[k], single), k
 
 
-def test_batched_backward_matches_per_field(cuda):
-    print("case: batch backward", flush=True)
+@pytest.mark.parametrize("ts", [16, 32])
+def test_batched_backward_matches_per_field(cuda, ts):
+    print(f"case: batch backward, tile {ts}", flush=True)
     parts = _fields(cuda)
-    b = B.Batch(SIZES, COUNTS, cuda)
+    b = B.Batch(SIZES, COUNTS, cuda, tile_size=ts)
     means, conics, colors, radii, op = (_cat(parts, i) for i in range(5))
     gen = torch.Generator(device=cuda).manual_seed(5)
     grad = torch.rand(b.total_pixels, 3, device=cuda, generator=gen)
@@ -69,11 +71,11 @@ def test_batched_backward_matches_per_field(cuda):
     for k, (p, (h, w)) in enumerate(zip(parts, SIZES)):
         if COUNTS[k] == 0:
             continue
-        fg, fo = HR.build_tile_index(p[0], p[1], p[3], h, w)
+        fg, fo = HR.build_tile_index(p[0], p[1], p[3], h, w, ts)
         fden = torch.empty(h, w, device=cuda)
-        fout, _ = HR.forward_tiled(*p[:5], fg, fo, h, w, True, 1e-8, 16, False, 3.0, den=fden)
+        fout, _ = HR.forward_tiled(*p[:5], fg, fo, h, w, True, 1e-8, ts, False, 3.0, den=fden)
         ref = HR.backward_tiled(grad[b.pixel_slices[k]].view(h, w, 3).contiguous(), *p[:5], fden,
-                                fout, fg, fo, h, w, True, 1e-8, 16, False, 3.0)
+                                fout, fg, fo, h, w, True, 1e-8, ts, False, 3.0)
         sl = b.gauss_slices[k]
         for got, exp in zip((gm[sl], gq[sl], gc[sl], go[sl]), ref):
             scale = exp.abs().max().clamp_min(1e-30)

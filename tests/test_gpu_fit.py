@@ -90,3 +90,35 @@ def test_capacity_overflow_rolls_back_and_matches(cuda):
     assert tight["regrows"] >= 1
     assert abs(ok["psnr"] - tight["psnr"]) < 0.01
     assert np.isfinite(tight["loss"])
+
+
+def test_tile32_fit_matches_tile16(cuda):
+    """The fit at 32-pixel tiles renders the same images as at 16 (gradients differ only in
+    atomic summation order), so both fits reach the same PSNR."""
+    print("case: fit tile 16 vs 32", flush=True)
+    a, target = _problem(cuda, seed=9)
+    b, _ = _problem(cuda, seed=9)
+    ra = F.fit(a, target, F.FitConfig(iters=30, window=10, tile_size=16), graph=True)
+    rb = F.fit(b, target, F.FitConfig(iters=30, window=10, tile_size=32), graph=True)
+    print(ra, rb)
+    assert (ra["tile_size"], rb["tile_size"]) == (16, 32)
+    assert abs(ra["psnr"] - rb["psnr"]) < 0.01
+
+
+def test_autotuned_tile_size(cuda):
+    """The tuner times every candidate, picks the fastest, and the fit uses its choice."""
+    from structsplat_amd import tune
+
+    print("case: tile autotuner", flush=True)
+    field, target = _problem(cuda, seed=11)
+    H, W = target.shape[:2]
+    cfg = F.FitConfig(iters=4, window=2)
+    best, timings = tune.tune_field(field, H, W, cfg)
+    assert set(timings) == set(tune.CANDIDATE_TILE_SIZES)
+    assert all(t > 0 for t in timings.values())
+    assert best == min(timings, key=timings.get)
+    assert tune.tune_field(field, H, W, cfg) == (best, timings)  # cached per layout and mode
+    r = F.fit(field, target, cfg, graph=False)
+    assert r["tile_size"] == best
+    with pytest.raises(ValueError):
+        F.resolve_tile_size(F.FitConfig(tile_size=24), lambda: 16)

@@ -139,6 +139,63 @@ def test_backward_normalizer_modes_agree(cuda, fade):
             assert scaled_err(b, a) <= 1e-6
 
 
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_tile32_forward_equals_tile16_and_gradients_match_fp64(cuda, fade, normalize):
+    """Every pixel accumulates the Gaussians whose rectangle holds it in ascending id at any tile
+    size, so the 32-pixel-tile forward equals the 16-pixel-tile forward bit for bit; the
+    32-pixel-tile backward (all normalizer modes) matches the float64 oracle."""
+    _print_case(test="tile32", fade=fade, normalize=normalize)
+    N, H, W = 700, 90, 111
+    f = random_field(23, N, H, W, scale_lo=0.8, scale_hi=9.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    alpha = HR._fade_alpha(fade)
+    outs = {}
+    for ts in (16, 32):
+        gids, offsets = HR.build_tile_index(means, conics, radii, H, W, ts, alpha == 1.0, 3.0)
+        den = torch.empty(H, W, device=cuda)
+        out, _ = HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W,
+                                  normalize, 1e-8, ts, alpha, 3.0, den=den)
+        outs[ts] = (out, den, gids, offsets)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[16][0], outs[32][0])
+    assert torch.equal(outs[16][1], outs[32][1])
+    out, den, gids, offsets = outs[32]
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(8))
+    ref = exact64.backward(to_np(g), to_np(means), to_np(conics), to_np(colors), to_np(radii),
+                           to_np(op), H, W, normalize, 1e-8, fade, 3.0)
+    modes = ((den, out), (None, out), (None, None)) if normalize else ((None, None),)
+    for d, o in modes:
+        grads = HR.backward_tiled(g, means, conics, colors, radii, op, d, o, gids, offsets, H, W,
+                                  normalize, 1e-8, 32, alpha, 3.0)
+        torch.cuda.synchronize()
+        for got, exp in zip(grads, ref):
+            assert scaled_err(to_np(got), exp) <= 1e-4
+
+
+def test_tile32_long_lists(cuda):
+    """32-pixel tiles whose lists span several 256-record batches."""
+    _print_case(test="tile32 long lists")
+    rng = np.random.default_rng(99)
+    N, H, W = 1500, 40, 40
+    f = {"means": rng.uniform(0, 40, (N, 2)).astype(np.float32),
+         "scales": np.exp(rng.uniform(np.log(1.0), np.log(8.0), (N, 2))).astype(np.float32),
+         "angles": rng.uniform(0, np.pi, N).astype(np.float32),
+         "colors": rng.random((N, 3)).astype(np.float32),
+         "opacity": rng.uniform(0.1, 1.0, N).astype(np.float32)}
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    out16 = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op, tile_size=16)
+    x = means.clone().requires_grad_(True)
+    out32 = HR.render_hip_exact(x, conics, colors, radii, H, W, opacities=op, tile_size=32)
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(9))
+    (out32 * g).sum().backward()
+    torch.cuda.synchronize()
+    assert torch.equal(out16, out32)
+    ref = exact64.backward(to_np(g), to_np(means), to_np(conics), to_np(colors), to_np(radii),
+                           to_np(op), H, W, True, 1e-8, False, 3.0)
+    assert scaled_err(to_np(x.grad), ref[0]) <= 1e-4
+
+
 def test_empty_field_renders_zero(cuda):
     H, W = 8, 9
     z2 = torch.zeros(0, 2, device=cuda)

@@ -88,3 +88,29 @@ def _loss_f64(m, q, c, o, rects, G, H, W, normalize, fade):
         den[y0[i]:y1[i], x0[i]:x1[i]] += w
     out = num / (den[..., None] + 1e-8) if normalize else num
     return float((out * G).sum())
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True])
+def test_float64_params_forward_matches_per_gaussian_loop(normalize, fade):
+    """exact64.forward(float64_params=True) keeps parameters that float32 cannot hold: it equals
+    the per-Gaussian float64 loop above on such parameters, and differs from the default
+    (float32-rounded) forward."""
+    H, W = 21, 18
+    f = random_field(14, 15, H, W, scale_lo=0.8, scale_hi=3.0, opacity=True, margin=0.0)
+    means, conics, colors, radii, op = (to_np(t).astype(np.float64) if t is not None else None
+                                        for t in field_tensors(f, "cpu"))
+    radii = radii.astype(np.int64)
+    rects = support_rects(means.astype(np.float32), radii, H, W)
+    jitter = np.random.default_rng(5)
+    params = [p + jitter.uniform(-1e-9, 1e-9, p.shape) for p in (means, conics, colors, op)]
+    G = jitter.random((H, W, 3))
+    out, _ = exact64.forward(*params[:3], radii, params[3], H, W, normalize, 1e-8, fade, 3.0,
+                             rects=rects, float64_params=True)
+    ref = _loss_f64(*params, rects, G, H, W, normalize, fade)
+    assert abs(float((out * G).sum()) - ref) <= 1e-12 * abs(ref)
+    rounded, _ = exact64.forward(*params[:3], radii, params[3], H, W, normalize, 1e-8, fade, 3.0,
+                                 rects=rects)
+    assert not np.array_equal(out, rounded)
+    with pytest.raises(ValueError):
+        exact64.forward(*params[:3], radii, params[3], H, W, float64_params=True)

@@ -36,6 +36,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|si
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -158,9 +159,10 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
         return [GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
                                          opacities=f["opacity"], device=dev) for f in inits]
 
-    cfg = F.FitConfig(iters=steps, window=max(steps // 2, 1))
+    cfg = F.FitConfig(iters=steps)
     batched = B.fit_batch(fields(), targets, cfg, graph=True)
-    plain = B.fit_each(fields(), targets, cfg)
+    # The plain loop at the batched fit's (autotuned) tile size: the same arithmetic per field.
+    plain = B.fit_each(fields(), targets, dataclasses.replace(cfg, tile_size=batched.tile_size))
     diff = float(np.max(np.abs(np.array(batched.psnr) - np.array(plain.psnr))))
     return {
         "fields": num_fields,
@@ -172,7 +174,45 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
         "speedup": round(plain.seconds / batched.seconds, 3),
         "psnr_max_abs_diff_db": round(diff, 5),
         "mean_psnr_db": round(float(np.mean(batched.psnr)), 3),
+        "tile_size": batched.tile_size,
         "note": "a step advances every field by one Adam step",
+    }
+
+
+def fit_loop(steps: int, seed: int = 0) -> dict:
+    """The full fit loop: a 2048x2048 target (32 random sinusoids + 200 random step edges),
+    200,000 Gaussians initialised uniformly (isotropic s = 0.5 sqrt(HW/N), angle 0, colors from
+    the target, opacity 0.5), Adam (lr 1e-2 means, 5e-3 others), L1 + 0.2 (1 - SSIM).  The
+    graph-captured step against the eager per-step loop, same steps, same initial field."""
+    import torch
+
+    from structsplat_amd import fit as F
+    from structsplat_amd import synth
+    from structsplat_amd.gaussians import GaussianField
+
+    dev = torch.device("cuda:0")
+    t = synth.fit_target(seed, 2048, 2048)
+    target = torch.tensor(t, device=dev)
+    init = synth.fit_init_field(t, 200_000, seed)
+
+    def field():
+        return GaussianField.from_numpy(init["means"], init["scales"], init["angles"],
+                                        init["colors"], opacities=init["opacity"], device=dev)
+
+    cfg = F.FitConfig(iters=steps)
+    graphed = F.fit(field(), target, cfg, graph=True)
+    eager = F.fit(field(), target, cfg, graph=False)
+    return {
+        "steps": steps,
+        "graph_steps_per_s": round(steps / graphed["seconds"], 3),
+        "graph_ms_per_step": round(graphed["seconds"] * 1e3 / steps, 4),
+        "eager_steps_per_s": round(steps / eager["seconds"], 3),
+        "eager_ms_per_step": round(eager["seconds"] * 1e3 / steps, 4),
+        "graph_psnr_db": round(graphed["psnr"], 4),
+        "eager_psnr_db": round(eager["psnr"], 4),
+        "tile_size": graphed["tile_size"],
+        "regrows": graphed["regrows"],
+        "note": "loop time includes the per-window capacity checks",
     }
 
 
@@ -223,8 +263,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32"],
                     help="render tile edge; auto = the autotuner's choice on this field")
-    ap.add_argument("--batched-steps", type=int, default=20,
+    ap.add_argument("--batched-steps", type=int, default=500,
                     help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
+    ap.add_argument("--fit-steps", type=int, default=1000,
+                    help="Adam steps of the 2048x2048/200k fit loop (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched-vs-plain fit (--batched-steps steps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -274,6 +316,7 @@ def main() -> int:
     offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = HR.workspace(N, H, W, ts, dev)
+    slots = HR.grad_slots(capacity, dev)  # deterministic backward: one slot per pair
     out = torch.empty(H, W, 3, device=dev)
     den = torch.empty(H, W, device=dev)
     grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
@@ -298,7 +341,8 @@ def main() -> int:
             ev["f"][1].record(stream)
             ev["g"][0].record(stream)
         HR.backward_tiled(grad_out, means, conics, colors, radii, op, den if saved else None, out,
-                          gids, offsets, H, W, True, 1e-8, ts, False, 3.0, grads=grads, ws=ws)
+                          gids, offsets, H, W, True, 1e-8, ts, False, 3.0, grads=grads, ws=ws,
+                          slots=slots, slot_capacity=capacity, status=status)
         if record:
             ev["g"][1].record(stream)
 
@@ -405,6 +449,8 @@ def main() -> int:
     }
     if args.batched_steps > 0:
         line["batched"] = batched_fit(args.batched_steps)
+    if args.fit_steps > 0:
+        line["fit"] = fit_loop(args.fit_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, H, W, b["step"])
     print(json.dumps(line), flush=True)

@@ -33,12 +33,12 @@
 extern "C" {
 #endif
 
-#define SSPLAT_ABI_VERSION 1
+#define SSPLAT_ABI_VERSION 2
 
 enum ssplat_status {
   SSPLAT_OK = 0,
   SSPLAT_ERR_INVALID_ARGUMENT = 1, /* bad shape, size, tile size or pointer */
-  SSPLAT_ERR_CAPACITY = 2,         /* tile_gids capacity smaller than the pair count */
+  SSPLAT_ERR_CAPACITY = 2,         /* tile_gids or gradient slot capacity too small */
   SSPLAT_ERR_HIP = 3,              /* a HIP runtime call or kernel launch failed */
   SSPLAT_ERR_UNSUPPORTED = 4       /* valid request this build does not implement */
 };
@@ -52,7 +52,7 @@ const char* ssplat_last_error(void);
 /*
  * Kernel timing, a measurement hook with no reference counterpart.  While enabled, every
  * forward / backward call records a HIP event pair on its stream around its render-kernel
- * launch (k_forward16 / k_backward16, not the record packing).  ssplat_kernel_timing_read
+ * launch (k_forward / k_backward, not the record packing or the gradient reduction).  ssplat_kernel_timing_read
  * waits for the recorded pairs of one kernel, returns their count and summed milliseconds, and
  * releases them.  Not for use while a stream is being captured into a HIP graph.
  */
@@ -143,8 +143,24 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
  *                         the tile lists instead of being stored;
  *   both NULL           : den and the numerator are recomputed, out = N/(den+eps).
  * The recomputed values equal the forward's bit for bit, so all three give the same gradients.
- * An additive call reads neither.  Gradient outputs are overwritten (the library clears them).
- * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231).
+ * An additive call reads neither.  Every gradient output is written.
+ *
+ * The gradients are deterministic: they do not depend on scheduling, so repeated calls on the
+ * same inputs give the same bits.  Gaussian g owns one "slot" per tile of its clipped support
+ * rectangle, R(g) = tw * th slots (tile bounds x0 / tile_size .. (x1 - 1) / tile_size, likewise
+ * in y; 0 for an empty rectangle), numbered consecutively over the Gaussians.  Every
+ * (tile, Gaussian) pair of the lists stores its partial gradients in its slot of grad_slots, and
+ * each Gaussian's slots are then summed in a fixed order; a slot the lists do not reach (a pair
+ * left out by the ellipse cull) counts as zero.  grad_slots is a 256-byte aligned buffer of
+ * ssplat_grad_slots_bytes(slot_capacity) bytes, ZERO-FILLED ONCE when allocated (its header
+ * counts the calls that used it, and tags each slot with the call that wrote it); it may then
+ * be reused by any number of calls on one stream.  slot_capacity must be at least the total
+ * slot count sum_g R(g): that equals the pair count P of lists built without ellipse cull, and
+ * ssplat_rect_pair_count computes it in general.  With too small a capacity the slots beyond it
+ * are lost and *status (a device int32, may be NULL) is set to SSPLAT_ERR_CAPACITY (sticky,
+ * as in ssplat_tile_index_fill).
+ * Replaces backward_tiled (render_ext.cpp:269-301, render_ext.cu:696-847, :1169-1231), whose
+ * float atomics make the gradients depend on scheduling.
  */
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
                           const float* colors, const int64_t* radii, const float* opacities,
@@ -152,9 +168,23 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
                           double eps, double support_fade_alpha, double sigma_cutoff,
-                          float* grad_means,
-                          float* grad_conics, float* grad_colors, float* grad_opacities,
-                          void* workspace, size_t workspace_bytes, void* stream);
+                          float* grad_means, float* grad_conics, float* grad_colors,
+                          float* grad_opacities, void* grad_slots, int64_t slot_capacity,
+                          int32_t* status, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/* Bytes of a gradient slot buffer holding slot_capacity slots (40 bytes each plus a header).
+   No reference counterpart (the reference accumulates with float atomics). */
+int ssplat_grad_slots_bytes(int64_t slot_capacity, size_t* bytes);
+
+/*
+ * Total slot count sum_g R(g) of a field (see ssplat_backward_tiled) into *total, a device
+ * int64 (-1 if it exceeds int32), without host synchronization.  tile_size is 16 or 32.
+ * No reference counterpart.
+ */
+int ssplat_rect_pair_count(const float* means, const int64_t* radii, int64_t n, int height,
+                           int width, int tile_size, int64_t* total, void* workspace,
+                           size_t workspace_bytes, void* stream);
 
 /*
  * Batches: many independent fields (images of their own size, each with its own Gaussians)
@@ -204,7 +234,9 @@ int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,
                                float* den, void* workspace,
                                size_t workspace_bytes, void* stream);
 
-/* Batched ssplat_backward_tiled; den and out are the batched forward's concatenated outputs. */
+/* Batched ssplat_backward_tiled; den and out are the batched forward's concatenated outputs.
+   Slots are numbered over the concatenated Gaussians; each Gaussian's sum is the same as in a
+   single-field call (the order depends on the Gaussian alone). */
 int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 int64_t total_gaussians, int64_t total_tiles, int tile_size,
                                 const float* grad_out, const float* means, const float* conics,
@@ -214,7 +246,8 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
                                 double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,
-                                float* grad_opacities, void* workspace, size_t workspace_bytes,
+                                float* grad_opacities, void* grad_slots, int64_t slot_capacity,
+                                int32_t* status, void* workspace, size_t workspace_bytes,
                                 void* stream);
 
 /*

@@ -45,9 +45,13 @@ SIGNATURES = {
                                       _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                       _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                       _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int,
-                                       _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                       _c_ptr, _c_size, _c_ptr]),
+                                       _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int,
+                                       _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr,
+                                       _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size,
+                                       _c_ptr]),
+    "ssplat_grad_slots_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_rect_pair_count": (_c_int, [_c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_ptr,
+                                        _c_ptr, _c_size, _c_ptr]),
     "ssplat_kernel_timing_enable": (_c_int, [_c_int]),
     "ssplat_kernel_timing_read": (_c_int, [_c_int, ctypes.POINTER(_c_i64),
                                            ctypes.POINTER(_c_dbl)]),
@@ -83,9 +87,9 @@ SIGNATURES = {
                                             _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_backward_tiled": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_int, _c_ptr,
                                              _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                             _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_dbl, _c_dbl, _c_dbl,
-                                             _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
-                                             _c_ptr]),
+                                             _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_dbl,
+                                             _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                             _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size, _c_ptr]),
 }
 
 
@@ -165,7 +169,7 @@ def check(rc: int, what: str) -> None:
     if rc == ERR_UNSUPPORTED:
         raise ValueError(f"{what}: unsupported: {msg}")
     if rc == ERR_CAPACITY:
-        raise RuntimeError(f"{what}: tile index capacity exceeded: {msg}")
+        raise RuntimeError(f"{what}: capacity exceeded: {msg}")
     raise RuntimeError(f"{what}: HIP error: {msg}")
 
 
@@ -173,6 +177,13 @@ def workspace_bytes(n: int, height: int, width: int, tile_size: int) -> int:
     out = _c_size(0)
     check(load().ssplat_workspace_bytes(int(n), int(height), int(width), int(tile_size),
                                         ctypes.byref(out)), "ssplat_workspace_bytes")
+    return int(out.value)
+
+
+def grad_slots_bytes(slot_capacity: int) -> int:
+    out = _c_size(0)
+    check(load().ssplat_grad_slots_bytes(int(slot_capacity), ctypes.byref(out)),
+          "ssplat_grad_slots_bytes")
     return int(out.value)
 
 

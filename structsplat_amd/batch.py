@@ -15,9 +15,9 @@ kernel which image a tile or a Gaussian belongs to (``ssplat_batch_*`` in includ
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 import time
-import dataclasses
 from dataclasses import dataclass
 
 import torch
@@ -30,7 +30,7 @@ from .gaussians import GaussianField
 from .loss import batch_l1_ssim_loss
 from .metrics import psnr_tensor
 from .tune import tune_field, tune_tile_size
-from .render import _fade_alpha, _ptr, _stream
+from .render import _fade_alpha, _ptr, _stream, grad_slots
 
 
 def _plan(sizes, counts, tile_size):
@@ -131,27 +131,39 @@ class Batch:
         return out
 
     def backward(self, grad_out, means, conics, colors, radii, opacities, den, out, gids,
-                 offsets, normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0):
-        """Batched backward; ``den``/``out`` are the batched forward's outputs (``None`` allowed
-        for additive renders)."""
+                 offsets, normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0,
+                 slots=None, slot_capacity=None, status=None):
+        """Batched deterministic backward; ``den``/``out`` are the batched forward's outputs
+        (``None`` allowed for additive renders).  ``slots`` holds ``slot_capacity`` gradient
+        slots and ``status`` records an overflow (both from a ``BatchPlan`` in the fit loop).
+        Without them, the slots get the pair buffer's length and an overflow raises (one host
+        synchronization)."""
         lib = _native.load()
         has_op = opacities is not None and opacities.numel() > 0
         g_m = torch.empty_like(means)
         g_q = torch.empty_like(conics)
         g_c = torch.empty_like(colors)
         g_o = torch.empty_like(opacities) if has_op else None
+        check_here = slots is None
+        if check_here:
+            slot_capacity = gids.numel()
+            slots = grad_slots(slot_capacity, self.device)
+            status = torch.zeros(1, dtype=torch.int32, device=self.device)
         _native.check(lib.ssplat_batch_backward_tiled(
             *self._head(), _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
             _ptr(opacities), _ptr(den), _ptr(out), _ptr(gids), _ptr(offsets), gids.numel(),
             int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
-            _ptr(g_m), _ptr(g_q),
-            _ptr(g_c), _ptr(g_o), _ptr(self.workspace), self.workspace.numel(),
-            _stream(self.device)), "ssplat_batch_backward_tiled")
+            _ptr(g_m), _ptr(g_q), _ptr(g_c), _ptr(g_o), _ptr(slots), int(slot_capacity),
+            _ptr(status), _ptr(self.workspace), self.workspace.numel(), _stream(self.device)),
+            "ssplat_batch_backward_tiled")
+        if check_here and int(status.item()) != 0:
+            raise RuntimeError("batched backward: gradient slot capacity exceeded")
         return g_m, g_q, g_c, g_o
 
 
 class BatchPlan:
-    """Pair buffer, offsets and sticky status of a batch (capturable binning)."""
+    """Pair buffer, offsets, sticky status and gradient slots of a batch (capturable binning
+    and backward); the slot buffer has the pair buffer's capacity."""
 
     def __init__(self, batch: Batch, capacity: int, recompute_normalizer: bool = True):
         self.batch = batch
@@ -161,6 +173,7 @@ class BatchPlan:
         self.gids = torch.empty(self.capacity, dtype=torch.int32, device=dev)
         self.offsets = torch.empty(batch.total_tiles + 1, dtype=torch.int32, device=dev)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.slots = grad_slots(self.capacity, dev)
         self.den = (None if self.recompute_normalizer
                     else torch.empty(batch.total_pixels, dtype=torch.float32, device=dev))
 
@@ -189,7 +202,9 @@ class _BatchRender(Function):
         plan = ctx.plan
         g_m, g_q, g_c, g_o = plan.batch.backward(grad_out.contiguous(), means, conics, colors,
                                                  radii, opacities, den, out, plan.gids,
-                                                 plan.offsets, *ctx.args)
+                                                 plan.offsets, *ctx.args, slots=plan.slots,
+                                                 slot_capacity=plan.capacity,
+                                                 status=plan.status)
         if opacities.numel() == 0:
             g_o = None
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None
@@ -213,6 +228,7 @@ class BatchResult:
     psnr: list
     seconds: float
     steps: int
+    tile_size: int | None = None  # the batched fit's render tile (per field for the plain loop)
 
 
 def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig,
@@ -311,7 +327,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                 src = getattr(big, name)
                 if src is not None:
                     setattr(f, name, src.detach()[sl].clone())
-    return BatchResult(psnrs, seconds, cfg.iters)
+    return BatchResult(psnrs, seconds, cfg.iters, batch.tile_size)
 
 
 def fit_each(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig

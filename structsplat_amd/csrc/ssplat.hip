@@ -122,6 +122,18 @@ constexpr int kSortThreads = 256;
 constexpr int kSortLds = 4096;      // lists up to this length sort in the block's LDS
 constexpr int kSortWaveLds = 1024;  // lists up to this length sort in one wave's LDS slice
 constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
+// The 16-pixel backward stages fewer records per batch: with the recomputing sweep's masks its
+// LDS then stays within 20 KB, so eight workgroups (eight waves per SIMD) fit on a CU.
+constexpr int kBwdStage16 = 128;
+// Deterministic gradient accumulation.  Gaussian g owns R(g) slots, one per tile of its clipped
+// support rectangle (row-major); slots of consecutive Gaussians are consecutive.  The backward
+// writes each (tile, Gaussian) pair's nine partial gradients and the call's tag to the pair's
+// slot, and the reduction sums every Gaussian's slots in a fixed order, so the gradients do not
+// depend on scheduling.  A slot whose tag is not the current call's (a tile the lists do not
+// hold, e.g. culled) counts as zero.
+constexpr int kSlotWords = 10;            // 9 partial gradients + the call's tag
+constexpr size_t kSlotHeaderBytes = 256;  // slot buffer header: the call counter (uint32)
+constexpr int kLaneSumMax = 64;           // Gaussians with up to this many slots: one lane each
 // The render kernels run one workgroup per tile, one thread per pixel (tile edge 16: 256 threads,
 // 4 waves; tile edge 32: 1024 threads, 16 waves), one wave per 8x8 quadrant.
 // -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
@@ -200,6 +212,27 @@ __device__ __forceinline__ float tile_min_q(float mx, float my, float a, float b
   dy = y_hi - my;
   dx = fminf(fmaxf(slope_x * dy, x_lo - mx), x_hi - mx);
   return fminf(best, conic_q_exact(dx, dy, a, b, c));
+}
+
+// Tile range of a clipped support rectangle packed as xr = x0 | x1 << 16, yr = y0 | y1 << 16,
+// in a grid of tile edge 1 << lg; count = tw * th is the number of slots the Gaussian owns (0 for
+// an empty rectangle).  Same tile bounds as the binning's enumeration.
+struct RectTiles {
+  int tx0, ty0, tw, th, count;
+};
+
+__device__ __forceinline__ RectTiles rect_tiles(uint32_t xr, uint32_t yr, int lg) {
+  const int x0 = static_cast<int>(xr & 0xffff), x1 = static_cast<int>(xr >> 16);
+  const int y0 = static_cast<int>(yr & 0xffff), y1 = static_cast<int>(yr >> 16);
+  RectTiles r{0, 0, 0, 0, 0};
+  if (x1 > x0 && y1 > y0) {
+    r.tx0 = x0 >> lg;
+    r.ty0 = y0 >> lg;
+    r.tw = ((x1 - 1) >> lg) - r.tx0 + 1;
+    r.th = ((y1 - 1) >> lg) - r.ty0 + 1;
+    r.count = r.tw * r.th;
+  }
+  return r;
 }
 
 // Hardware 2^x (v_exp_f32).  The same instruction evaluates both the per-pixel weight and the
@@ -781,7 +814,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict
 //   c0 = (mx, my, qa, qb)       conic in the exponent's units: the visit computes
 //   c1 = (qc, xr, yr, gid)        e = qa dx^2 + qb dx dy + qc dy^2 and weights with exp2(e)
 //   c2 = (o r, o g, o b, o)     opacity-premultiplied color and opacity (accumulation)
-//   c3 = (r, g, b, 0)           plain color (gradient sweep)
+//   c3 = (r, g, b, slot)        plain color (gradient sweep); first slot relative to the wave
 // Without support fade the record holds (qa, qb, qc) = -log2(e)/2 * (a, 2b, c) (the fold saves
 // a multiply per visit); with fade it holds (a, 2b, c) and the kernels scale the quadratic so
 // that a pixel at exactly sigma_cutoff evaluates the same 2^x as the fade tail.
@@ -791,41 +824,53 @@ struct __align__(16) GaussRec {
 };
 static_assert(sizeof(GaussRec) == 64, "GaussRec must be 64 bytes");
 
-// Gradient buffers the backward's record packing clears (all NULL in the forward).
-struct GradClear {
-  float* means;
-  float* conics;
-  float* colors;
-  float* opacities;
+// Per-wave slot bookkeeping written by the record packing: rect_wave[w] = slots owned by the 64
+// Gaussians of wave w (the scan input; rect_wave[num_waves] = 0), each record's c3.w = the
+// Gaussian's first slot relative to its wave.  `epoch` (the backward's slot buffer header, NULL in
+// the forward) is advanced once per call: the call's tag.
+struct SlotPrep {
+  int32_t* rect_wave;
+  int num_waves;
+  int lg;
+  uint32_t* epoch;
+  int32_t* rect_status;
 };
 
 template <bool kFold>
-__global__ void k_prepare(const float* __restrict__ means, const float* __restrict__ conics,
-                          const float* __restrict__ colors, const int64_t* __restrict__ radii,
-                          const float* __restrict__ opacities, int n, FieldTable fields,
-                          GaussRec* __restrict__ recs, GradClear clear) {
+__global__ __launch_bounds__(256) void k_prepare(
+    const float* __restrict__ means, const float* __restrict__ conics,
+    const float* __restrict__ colors, const int64_t* __restrict__ radii,
+    const float* __restrict__ opacities, int n, FieldTable fields, GaussRec* __restrict__ recs,
+    SlotPrep sp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (clear.means != nullptr) {
-    clear.means[2 * i + 0] = 0.0f;
-    clear.means[2 * i + 1] = 0.0f;
-    clear.conics[3 * i + 0] = 0.0f;
-    clear.conics[3 * i + 1] = 0.0f;
-    clear.conics[3 * i + 2] = 0.0f;
-    clear.colors[3 * i + 0] = 0.0f;
-    clear.colors[3 * i + 1] = 0.0f;
-    clear.colors[3 * i + 2] = 0.0f;
-    if (clear.opacities != nullptr) clear.opacities[i] = 0.0f;
+  const int lane = threadIdx.x & 63;
+  if (i == 0) {
+    sp.rect_wave[sp.num_waves] = 0;
+    *sp.rect_status = SSPLAT_OK;
+    if (sp.epoch != nullptr) *sp.epoch += 1u;
   }
-  const ssplat_field f = field_of_gaussian(fields, i);
+  uint32_t xr = 0, yr = 0;
+  if (i < n) {
+    const ssplat_field f = field_of_gaussian(fields, i);
+    const Rect r = support_rect(means[2 * i + 0], means[2 * i + 1], radii[2 * i + 0],
+                                radii[2 * i + 1], f.height, f.width);
+    if (r.x1 > r.x0 && r.y1 > r.y0) {
+      xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
+      yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
+    }
+  }
+  // Slots of this wave's Gaussians: inclusive prefix of R over the 64 lanes.
+  const int slots = rect_tiles(xr, yr, sp.lg).count;
+  int incl = slots;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63 && (i >> 6) < sp.num_waves) sp.rect_wave[i >> 6] = incl;
+  if (i >= n) return;
   const float mx = means[2 * i + 0];
   const float my = means[2 * i + 1];
-  const Rect r = support_rect(mx, my, radii[2 * i + 0], radii[2 * i + 1], f.height, f.width);
-  uint32_t xr = 0, yr = 0;
-  if (r.x1 > r.x0 && r.y1 > r.y0) {
-    xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
-    yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
-  }
   const float scale = kFold ? kNegHalfLog2e : 1.0f;
   const float op = opacities ? opacities[i] : 1.0f;
   const float cr = colors[3 * i + 0], cg = colors[3 * i + 1], cb = colors[3 * i + 2];
@@ -834,12 +879,13 @@ __global__ void k_prepare(const float* __restrict__ means, const float* __restri
   g.c1 = make_float4(scale * conics[3 * i + 2], __uint_as_float(xr), __uint_as_float(yr),
                      __int_as_float(i));
   g.c2 = make_float4(op * cr, op * cg, op * cb, op);
-  g.c3 = make_float4(cr, cg, cb, 0.0f);
+  g.c3 = make_float4(cr, cg, cb, __int_as_float(incl - slots));
   recs[i] = g;
 }
 
 struct RenderArgs {
   const GaussRec* recs;
+  const int32_t* rect_off;  // first slot of each wave of 64 Gaussians (backward only)
   const int32_t* gids;
   const int32_t* offsets;
   int64_t num_gids;
@@ -863,16 +909,17 @@ __device__ __forceinline__ void tile_range(const RenderArgs& a, int tile, int64_
 
 // The first kChunks 16-byte chunks of up to kStage records: the forward stages chunks 0-2, the
 // backward all four.
-template <int kChunks>
+template <int kChunks, int kS = kStage>
 struct StageLds {
-  float4 c[kChunks][kStage];
+  static constexpr int kSize = kS;
+  float4 c[kChunks][kS];
 };
 
 // Coalesced staging of up to kStage records of the tile list into LDS (kChunks 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
-template <int kLoad, int kChunks>
-__device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks>& st,
+template <int kLoad, int kChunks, int kS>
+__device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                             int64_t batch, int m) {
   static_assert(kLoad <= kChunks, "staging more chunks than the buffer holds");
   if (static_cast<int>(threadIdx.x) < m) {
@@ -881,6 +928,10 @@ __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunk
       const float4* src = reinterpret_cast<const float4*>(a.recs + g);
 #pragma unroll
       for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = src[c];
+      if constexpr (kLoad == 4) {
+        // The gradient sweep's records carry the Gaussian's absolute first slot.
+        st.c[3][threadIdx.x].w = __int_as_float(__float_as_int(src[3].w) + a.rect_off[g >> 6]);
+      }
     } else {
 #pragma unroll
       for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -910,8 +961,8 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // it skip the pixel test, and for the others each lane reads its bit of the mask.
 // kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
 // recomputation); it is the same sum, in the same order, as the forward's D.
-template <bool kFade, bool kDenOnly, int kChunks>
-__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks>& st,
+template <bool kFade, bool kDenOnly, int kChunks, int kS>
+__device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                                     uint32_t* s_mask, int tile, int qx0, int qy0,
                                                     int px, int py, float tail, float4& acc) {
   const int lane = threadIdx.x & 63;
@@ -920,13 +971,13 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   int64_t start, end;
   tile_range(a, tile, &start, &end);
   float nr = 0.0f, ng = 0.0f, nb = 0.0f, d = 0.0f;
-  for (int64_t batch = start; batch < end; batch += kStage) {
-    const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
+  for (int64_t batch = start; batch < end; batch += kS) {
+    const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
     stage_batch<3>(a, st, batch, m);
     __syncthreads();
-    uint64_t hit[kStage / 64], full[kStage / 64];
+    uint64_t hit[kS / 64], full[kS / 64];
 #pragma unroll
-    for (int k = 0; k < kStage / 64; ++k) {
+    for (int k = 0; k < kS / 64; ++k) {
       const int j = lane + 64 * k;
       uint64_t qm = 0;
       if (j < m) {
@@ -942,7 +993,7 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
     const uint32_t lane_bit_index = lane & 31;
     const uint32_t* my_half = s_mask + (lane >> 5);
 #pragma unroll
-    for (int k = 0; k < kStage / 64; ++k) {
+    for (int k = 0; k < kS / 64; ++k) {
       uint64_t mask = hit[k];
       while (mask) {
         const int b = __ffsll(static_cast<unsigned long long>(mask)) - 1;
@@ -1016,10 +1067,10 @@ struct GradArgs {
   const float* grad_out;
   const float* den;  // forward coverage (read when normalizing)
   const float* out;  // forward image (read when normalizing)
-  float* grad_means;
-  float* grad_conics;
-  float* grad_colors;
-  float* grad_opacities;  // NULL when the field has no opacities
+  float* slots;      // kSlotWords floats per slot
+  int64_t slot_capacity;
+  const uint32_t* epoch;  // the call's tag
+  bool has_opacity;
 };
 
 // Where the normalized backward gets the forward's D (den) and out:
@@ -1028,12 +1079,16 @@ constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-on
 constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
 
 template <int kT, bool kFade, bool kNormalize, int kNorm>
-__global__ __launch_bounds__(kT * kT) void k_backward(RenderArgs a, GradArgs g) {
+// Eight waves per SIMD (at most 64 VGPRs): with 16-pixel tiles that is eight resident
+// workgroups per CU, whose different phases (list sweeps, reductions) overlap.
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_backward(
+    RenderArgs a, GradArgs g) {
   constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
   constexpr int kQ = kT / 8;  // quadrants per tile row
   constexpr int kWaves = kQ * kQ;
-  constexpr int kMaskF4 = kWaves * 2 * kStage / 4;  // the sweep's masks, in float4 units
-  __shared__ StageLds<4> st;
+  constexpr int kS = kT == 16 ? kBwdStage16 : kStage;  // records per staged batch
+  constexpr int kMaskF4 = kWaves * 2 * kS / 4;          // the sweep's masks, in float4 units
+  __shared__ StageLds<4, kS> st;
   // The per-pixel adjoints; when recomputing, the same space first holds the sweep's per-wave
   // quadrant masks (two uint32 per staged record and wave).
   __shared__ float4 s_scratch[kRecompute && kMaskF4 > kT * kT ? kMaskF4 : kT * kT];
@@ -1060,7 +1115,7 @@ __global__ __launch_bounds__(kT * kT) void k_backward(RenderArgs a, GradArgs g) 
     const int qy0 = ty0 + (wave / kQ) * 8;
     const int px = qx0 + lx;
     const int py = qy0 + ly;
-    uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kStage);
+    uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kS);
     float4 acc;
     accumulate_quadrant<kFade, kNorm == kNormRecompD>(a, st, s_mask, tile, qx0, qy0, px, py,
                                                       tail, acc);
@@ -1117,20 +1172,19 @@ __global__ __launch_bounds__(kT * kT) void k_backward(RenderArgs a, GradArgs g) 
       adj[s] = s_adj[((s >> 1) * 8 + ly) * kT + (s & 1) * 8 + lx];
     }
   }
-  const bool has_opacity = g.grad_opacities != nullptr;
+  const bool has_opacity = g.has_opacity;
+  const float tag = __uint_as_float(*g.epoch);
+  constexpr int kLg = kT == 16 ? 4 : 5;
   // Lane roles of the transposed reduction: lane 8k+7 owns gradient component k
-  // (0-1 mean x/y, 2-4 conic a/b/c, 5-7 color r/g/b).
+  // (0-1 mean x/y, 2-4 conic a/b/c, 5-7 color r/g/b); lane 63 also the opacity and the tag.
   const bool hi8 = (lane & 8) != 0;
   const int comp = lane >> 3;
   const bool result_lane = (lane & 7) == 7;
-  float* const comp_base = comp < 2 ? g.grad_means : (comp < 5 ? g.grad_conics : g.grad_colors);
-  const int comp_stride = comp < 2 ? 2 : 3;
-  const int comp_offset = comp < 2 ? comp : (comp < 5 ? comp - 2 : comp - 5);
   const float comp_scale = (comp == 2 || comp == 4) ? -0.5f : (comp == 3 ? -1.0f : 1.0f);
   int64_t start, end;
   tile_range(a, tile, &start, &end);
-  for (int64_t batch = start; batch < end; batch += kStage) {
-    const int m = static_cast<int>(end - batch < kStage ? end - batch : kStage);
+  for (int64_t batch = start; batch < end; batch += kS) {
+    const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
     stage_batch<4>(a, st, batch, m);
     __syncthreads();
     for (int j = wave; j < m; j += kWaves) {
@@ -1217,14 +1271,151 @@ __global__ __launch_bounds__(kT * kT) void k_backward(RenderArgs a, GradArgs g) 
       const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
       const float cross = comp < 2 ? partner * cb : 0.0f;
       const float gval = op * fmaf(red, own_scale, cross);
-      if (result_lane && gval != 0.0f) {
-        unsafeAtomicAdd(comp_base + comp_stride * int64_t(gid) + comp_offset, gval);
-      }
-      if (has_opacity && lane == 63 && gop_sum != 0.0f) {
-        unsafeAtomicAdd(g.grad_opacities + gid, gop_sum);
+      // The pair's slot: the Gaussian's first slot plus this tile's row-major index in its
+      // rectangle (uniform values).  A list entry outside the rectangle has no pixel inside
+      // and no slot.
+      const RectTiles rt = rect_tiles(xr, yr, kLg);
+      const int kx = tile_x - rt.tx0, ky = tile_y - rt.ty0;
+      if (kx >= 0 && kx < rt.tw && ky >= 0 && ky < rt.th) {
+        const int64_t slot = int64_t(__builtin_amdgcn_readfirstlane(__float_as_int(v3.w))) +
+                             ky * rt.tw + kx;
+        if (slot < g.slot_capacity) {
+          float* dst = g.slots + slot * kSlotWords;
+          if (result_lane) dst[comp] = gval;
+          if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
+        }
       }
     }
     __syncthreads();
+  }
+}
+
+struct ReduceArgs {
+  const GaussRec* recs;
+  int n;
+  int lg;
+  int num_waves;
+  const int32_t* rect_off;
+  const int32_t* rect_status;
+  const float* slots;
+  int64_t slot_capacity;
+  const uint32_t* epoch;
+  float* grad_means;
+  float* grad_conics;
+  float* grad_colors;
+  float* grad_opacities;  // NULL when the field has no opacities
+  int32_t* status;        // may be NULL
+};
+
+// Adds one slot's nine partial gradients when its tag is the call's (a slot the call did not
+// write counts as zero; the select keeps stale values out of the sum).
+__device__ __forceinline__ void add_slot(const float* s, uint32_t tag, float (&acc)[9]) {
+  const float2 p01 = *reinterpret_cast<const float2*>(s + 0);
+  const float2 p23 = *reinterpret_cast<const float2*>(s + 2);
+  const float2 p45 = *reinterpret_cast<const float2*>(s + 4);
+  const float2 p67 = *reinterpret_cast<const float2*>(s + 6);
+  const float2 p89 = *reinterpret_cast<const float2*>(s + 8);
+  const bool live = __float_as_uint(p89.y) == tag;
+  acc[0] += live ? p01.x : 0.0f;
+  acc[1] += live ? p01.y : 0.0f;
+  acc[2] += live ? p23.x : 0.0f;
+  acc[3] += live ? p23.y : 0.0f;
+  acc[4] += live ? p45.x : 0.0f;
+  acc[5] += live ? p45.y : 0.0f;
+  acc[6] += live ? p67.x : 0.0f;
+  acc[7] += live ? p67.y : 0.0f;
+  acc[8] += live ? p89.x : 0.0f;
+}
+
+// Gradients of every Gaussian from its slots, in an order fixed by the Gaussian alone: up to
+// kLaneSumMax slots are summed by one lane in ascending slot order; more are summed by the whole
+// wave, lane l taking slots l, l + 64, ... in order, then a fixed DPP tree.  Every Gaussian's
+// outputs are written (zero without slots).
+__global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const uint32_t tag = *a.epoch;
+  int count = 0;
+  int64_t base = 0;
+  if (i < a.n) {
+    const GaussRec& r = a.recs[i];
+    const float4 c1 = r.c1;
+    count = rect_tiles(__float_as_uint(c1.y), __float_as_uint(c1.z), a.lg).count;
+    base = int64_t(a.rect_off[i >> 6]) + __float_as_int(r.c3.w);
+    // Slots beyond the capacity were never written (the status reports the overflow).
+    const int64_t room = a.slot_capacity - base;
+    count = room <= 0 ? 0 : (room < count ? static_cast<int>(room) : count);
+  }
+  float acc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (count <= kLaneSumMax) {
+    for (int k = 0; k < count; ++k) add_slot(a.slots + (base + k) * kSlotWords, tag, acc);
+  }
+  uint64_t big = __ballot(count > kLaneSumMax);
+  while (big) {
+    const int b = __ffsll(static_cast<unsigned long long>(big)) - 1;
+    big &= big - 1;
+    const int cb = __shfl(count, b, 64);
+    const int64_t bb = __shfl(base, b, 64);
+    float part[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = lane; k < cb; k += 64) add_slot(a.slots + (bb + k) * kSlotWords, tag, part);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const float total = __shfl(wave_sum_lane63(part[c]), 63, 64);
+      if (lane == b) acc[c] = total;
+    }
+  }
+  if (i < a.n) {
+    a.grad_means[2 * i + 0] = acc[0];
+    a.grad_means[2 * i + 1] = acc[1];
+    a.grad_conics[3 * i + 0] = acc[2];
+    a.grad_conics[3 * i + 1] = acc[3];
+    a.grad_conics[3 * i + 2] = acc[4];
+    a.grad_colors[3 * i + 0] = acc[5];
+    a.grad_colors[3 * i + 1] = acc[6];
+    a.grad_colors[3 * i + 2] = acc[7];
+    if (a.grad_opacities != nullptr) a.grad_opacities[i] = acc[8];
+  }
+  if (i == 0 && a.status != nullptr &&
+      (*a.rect_status != SSPLAT_OK || int64_t(a.rect_off[a.num_waves]) > a.slot_capacity)) {
+    *a.status = SSPLAT_ERR_CAPACITY;
+  }
+}
+
+// Number of slots (support-rectangle tiles) of every Gaussian into rect_wave, per wave of 64,
+// for a slot count without record packing.
+__global__ __launch_bounds__(256) void k_rect_count(const float* __restrict__ means,
+                                                    const int64_t* __restrict__ radii, int n,
+                                                    FieldTable fields, SlotPrep sp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (i == 0) {
+    sp.rect_wave[sp.num_waves] = 0;
+    *sp.rect_status = SSPLAT_OK;
+  }
+  uint32_t xr = 0, yr = 0;
+  if (i < n) {
+    const ssplat_field f = field_of_gaussian(fields, i);
+    const Rect r = support_rect(means[2 * i + 0], means[2 * i + 1], radii[2 * i + 0],
+                                radii[2 * i + 1], f.height, f.width);
+    if (r.x1 > r.x0 && r.y1 > r.y0) {
+      xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
+      yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
+    }
+  }
+  int incl = rect_tiles(xr, yr, sp.lg).count;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63 && (i >> 6) < sp.num_waves) sp.rect_wave[i >> 6] = incl;
+}
+
+// *total = the slot count after the scan (int64), or -1 when it overflowed int32.
+__global__ void k_rect_total(const int32_t* __restrict__ rect_off, int num_waves,
+                             const int32_t* __restrict__ rect_status, int64_t* __restrict__ total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *total = *rect_status == SSPLAT_OK ? int64_t(rect_off[num_waves]) : int64_t(-1);
   }
 }
 
@@ -1306,9 +1497,14 @@ int make_batch_plan(const ssplat_field* fields, int num_fields, int64_t total_ga
 
 struct Workspace {
   GaussRec* recs;
-  int32_t* counts;   // num_tiles + 1 per-tile counters (binning), reused as scatter cursors
-  int64_t* bsums;    // first-level scan block sums
-  int32_t* status;   // internal status word of the last binning
+  int32_t* counts;       // num_tiles + 1 per-tile counters (binning), reused as scatter cursors
+  int64_t* bsums;        // first-level scan block sums
+  int32_t* status;       // internal status word of the last binning
+  int32_t* rect_wave;    // slots per wave of 64 Gaussians (num_waves + 1, the last 0)
+  int32_t* rect_off;     // their exclusive scan: each wave's first slot; [num_waves] = total
+  int64_t* rect_bsums;   // first-level block sums of that scan
+  int32_t* rect_status;  // int32 overflow of that scan
+  int num_waves;
   size_t bytes;
 };
 
@@ -1324,11 +1520,25 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   off = align_up(off + blocks * sizeof(int64_t), 256);
   const size_t status = off;
   off = align_up(off + sizeof(int32_t), 256);
+  const size_t waves = (size_t(n) + 63) / 64;
+  const size_t rect_wave = off;
+  off = align_up(off + (waves + 1) * sizeof(int32_t), 256);
+  const size_t rect_off = off;
+  off = align_up(off + (waves + 1) * sizeof(int32_t), 256);
+  const size_t rect_bsums = off;
+  off = align_up(off + ((waves + 1 + kScanBlock - 1) / kScanBlock) * sizeof(int64_t), 256);
+  const size_t rect_status = off;
+  off = align_up(off + sizeof(int32_t), 256);
   char* b = static_cast<char*>(base);
   w.recs = reinterpret_cast<GaussRec*>(b + recs);
   w.counts = reinterpret_cast<int32_t*>(b + counts);
   w.bsums = reinterpret_cast<int64_t*>(b + bsums);
   w.status = reinterpret_cast<int32_t*>(b + status);
+  w.rect_wave = reinterpret_cast<int32_t*>(b + rect_wave);
+  w.rect_off = reinterpret_cast<int32_t*>(b + rect_off);
+  w.rect_bsums = reinterpret_cast<int64_t*>(b + rect_bsums);
+  w.rect_status = reinterpret_cast<int32_t*>(b + rect_status);
+  w.num_waves = static_cast<int>(waves);
   w.bytes = off;
   return w;
 }
@@ -1370,6 +1580,29 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
   return SSPLAT_OK;
 }
 
+// Exclusive int32 scan of counts[0, items) into offsets (saturating; *status flags an overflow):
+// one launch up to kSingleScanItems items, three beyond.
+int run_scan(const int32_t* counts, int64_t items, int32_t* offsets, int64_t* bsums,
+             int32_t* status, hipStream_t stream) {
+  if (items > kMaxScanItems) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "scan of %lld items is out of range",
+                     static_cast<long long>(items));
+  }
+  if (items <= kSingleScanItems) {
+    k_scan_single<<<1, kScanThreads, 0, stream>>>(counts, items, offsets, status);
+    SSPLAT_LAUNCH_CHECK("k_scan_single");
+    return SSPLAT_OK;
+  }
+  const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
+  k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums);
+  SSPLAT_LAUNCH_CHECK("k_scan_partials");
+  k_scan_top<<<1, kScanThreads, 0, stream>>>(bsums, blocks);
+  SSPLAT_LAUNCH_CHECK("k_scan_top");
+  k_scan_apply<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums, offsets, status);
+  SSPLAT_LAUNCH_CHECK("k_scan_apply");
+  return SSPLAT_OK;
+}
+
 // Counts and scans: offsets[0..T] written, the counters left holding the per-tile counts (the
 // scatter's cursors).  Three launches (reset, count, one-workgroup scan) up to kSingleScanItems
 // counters, five beyond.
@@ -1386,19 +1619,7 @@ int run_count(const BinArgs& a, const Plan& p, const Workspace& w, int32_t* offs
     k_bin_count<<<(a.n + kBinThreads - 1) / kBinThreads, kBinThreads, 0, stream>>>(a, w.counts);
     SSPLAT_LAUNCH_CHECK("k_bin_count");
   }
-  if (items <= kSingleScanItems) {
-    k_scan_single<<<1, kScanThreads, 0, stream>>>(w.counts, items, offsets, w.status);
-    SSPLAT_LAUNCH_CHECK("k_scan_single");
-    return SSPLAT_OK;
-  }
-  const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
-  k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(w.counts, items, w.bsums);
-  SSPLAT_LAUNCH_CHECK("k_scan_partials");
-  k_scan_top<<<1, kScanThreads, 0, stream>>>(w.bsums, blocks);
-  SSPLAT_LAUNCH_CHECK("k_scan_top");
-  k_scan_apply<<<blocks, kScanThreads, 0, stream>>>(w.counts, items, w.bsums, offsets, w.status);
-  SSPLAT_LAUNCH_CHECK("k_scan_apply");
-  return SSPLAT_OK;
+  return run_scan(w.counts, items, offsets, w.bsums, w.status, stream);
 }
 
 // Scatter + per-tile sort (which also merges the status).  rebuild_cursors: the counters do not
@@ -1495,6 +1716,7 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
                             double fade_alpha, double sigma_cutoff) {
   RenderArgs r;
   r.recs = w.recs;
+  r.rect_off = nullptr;
   r.gids = gids;
   r.offsets = offsets;
   r.num_gids = num_gids;
@@ -1506,19 +1728,26 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
   return r;
 }
 
-// Packs the records (and clears the gradient buffers named in `clear`); without support fade
-// the conic is folded with -log2(e)/2.
+int tile_lg(int ts) { return ts == 32 ? 5 : 4; }
+
+SlotPrep make_slot_prep(const Plan& p, const Workspace& w, uint32_t* epoch) {
+  return SlotPrep{w.rect_wave, w.num_waves, tile_lg(p.ts), epoch, w.rect_status};
+}
+
+// Packs the records with each Gaussian's slot bookkeeping; without support fade the conic is
+// folded with -log2(e)/2.  `epoch` (the backward's slot header) is advanced; NULL in the forward.
 int run_prepare(const float* means, const float* conics, const float* colors,
                 const int64_t* radii, const float* opacities, const Plan& p, const Workspace& w,
-                bool support_fade, GradClear clear, hipStream_t stream) {
+                bool support_fade, uint32_t* epoch, hipStream_t stream) {
   if (p.n == 0) return SSPLAT_OK;
   const int blocks = static_cast<int>((p.n + 255) / 256);
+  const SlotPrep sp = make_slot_prep(p, w, epoch);
   if (support_fade) {
     k_prepare<false><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
-                                                 static_cast<int>(p.n), p.table, w.recs, clear);
+                                                 static_cast<int>(p.n), p.table, w.recs, sp);
   } else {
     k_prepare<true><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
-                                                static_cast<int>(p.n), p.table, w.recs, clear);
+                                                static_cast<int>(p.n), p.table, w.recs, sp);
   }
   SSPLAT_LAUNCH_CHECK("k_prepare");
   return SSPLAT_OK;
@@ -1588,8 +1817,8 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   Workspace w;
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade,
-                        GradClear{nullptr, nullptr, nullptr, nullptr}, s))) {
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, nullptr,
+                        s))) {
     return rc;
   }
   const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha,
@@ -1606,12 +1835,23 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   return success();
 }
 
+int check_slots(void* grad_slots, int64_t slot_capacity) {
+  if (slot_capacity < 0) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "slot_capacity < 0");
+  if (grad_slots == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_slots is NULL");
+  if (reinterpret_cast<uintptr_t>(grad_slots) % 256 != 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "grad_slots must be 256-byte aligned");
+  }
+  return SSPLAT_OK;
+}
+
 int backward_tiled(const Plan& p, const float* grad_out, const float* means, const float* conics,
                    const float* colors, const int64_t* radii, const float* opacities,
-                   const float* den, const float* out, const int32_t* gids, const int32_t* offsets, int64_t num_gids, int normalize,
-                   double eps, double support_fade_alpha, double sigma_cutoff,
-                   float* grad_means, float* grad_conics, float* grad_colors,
-                   float* grad_opacities, void* ws, size_t ws_bytes, void* stream) {
+                   const float* den, const float* out, const int32_t* gids,
+                   const int32_t* offsets, int64_t num_gids, int normalize, double eps,
+                   double support_fade_alpha, double sigma_cutoff, float* grad_means,
+                   float* grad_conics, float* grad_colors, float* grad_opacities,
+                   void* grad_slots, int64_t slot_capacity, int32_t* status, void* ws,
+                   size_t ws_bytes, void* stream) {
   int rc;
   double fade_alpha;
   if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
@@ -1630,18 +1870,24 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   if (p.n > 0 && (grad_means == nullptr || grad_conics == nullptr || grad_colors == nullptr)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "gradient outputs must not be NULL");
   }
+  if ((rc = check_slots(grad_slots, slot_capacity))) return rc;
   Workspace w;
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (p.n == 0) return success();
   float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
-  const GradClear clear{grad_means, grad_conics, grad_colors, gop};
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, clear, s))) {
+  uint32_t* epoch = static_cast<uint32_t*>(grad_slots);
+  float* slots = reinterpret_cast<float*>(static_cast<char*>(grad_slots) + kSlotHeaderBytes);
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, epoch, s))) {
     return rc;
   }
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha,
-                                        sigma_cutoff);
-  GradArgs ga{grad_out, den, out, grad_means, grad_conics, grad_colors, gop};
+  if ((rc = run_scan(w.rect_wave, int64_t(w.num_waves) + 1, w.rect_off, w.rect_bsums,
+                     w.rect_status, s))) {
+    return rc;
+  }
+  RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
+  ra.rect_off = w.rect_off;
+  const GradArgs ga{grad_out, den, out, slots, slot_capacity, epoch, gop != nullptr};
   hipEvent_t t0;
   const bool timed = timer_begin(1, s, &t0);
   if (p.ts == 16) {
@@ -1651,6 +1897,39 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   }
   SSPLAT_LAUNCH_CHECK("k_backward");
   if (timed) timer_end(1, s, t0);
+  const ReduceArgs red{w.recs,      static_cast<int>(p.n), tile_lg(p.ts), w.num_waves,
+                       w.rect_off,  w.rect_status,         slots,         slot_capacity,
+                       epoch,       grad_means,            grad_conics,   grad_colors,
+                       gop,         status};
+  k_grad_reduce<<<static_cast<int>((p.n + 255) / 256), 256, 0, s>>>(red);
+  SSPLAT_LAUNCH_CHECK("k_grad_reduce");
+  return success();
+}
+
+int rect_pair_count(const Plan& p, const float* means, const int64_t* radii, int64_t* total,
+                    void* ws, size_t ws_bytes, void* stream) {
+  int rc;
+  if (total == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "total is NULL");
+  if (p.ts != 16 && p.ts != 32) {
+    return set_error(SSPLAT_ERR_UNSUPPORTED,
+                     "the render kernels are built for tile_size 16 and 32, got %d", p.ts);
+  }
+  if (p.n > 0 && (means == nullptr || radii == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/radii must not be NULL");
+  }
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int blocks = static_cast<int>((p.n + 255) / 256);
+  k_rect_count<<<blocks > 0 ? blocks : 1, 256, 0, s>>>(means, radii, static_cast<int>(p.n),
+                                                       p.table, make_slot_prep(p, w, nullptr));
+  SSPLAT_LAUNCH_CHECK("k_rect_count");
+  if ((rc = run_scan(w.rect_wave, int64_t(w.num_waves) + 1, w.rect_off, w.rect_bsums,
+                     w.rect_status, s))) {
+    return rc;
+  }
+  k_rect_total<<<1, 64, 0, s>>>(w.rect_off, w.num_waves, w.rect_status, total);
+  SSPLAT_LAUNCH_CHECK("k_rect_total");
   return success();
 }
 
@@ -1770,19 +2049,39 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
 
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,
                           const float* colors, const int64_t* radii, const float* opacities,
-                          const float* den, const float* out, int64_t n, const int32_t* tile_gids, const int32_t* tile_offsets,
+                          const float* den, const float* out, int64_t n,
+                          const int32_t* tile_gids, const int32_t* tile_offsets,
                           int64_t num_gids, int height, int width, int tile_size, int normalize,
                           double eps, double support_fade_alpha, double sigma_cutoff,
-                          float* grad_means,
-                          float* grad_conics, float* grad_colors, float* grad_opacities,
-                          void* workspace, size_t workspace_bytes, void* stream) {
+                          float* grad_means, float* grad_conics, float* grad_colors,
+                          float* grad_opacities, void* grad_slots, int64_t slot_capacity,
+                          int32_t* status, void* workspace, size_t workspace_bytes,
+                          void* stream) {
   Plan p;
   const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
   return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
-                        grad_means, grad_conics, grad_colors, grad_opacities, workspace,
-                        workspace_bytes, stream);
+                        grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
+                        slot_capacity, status, workspace, workspace_bytes, stream);
+}
+
+int ssplat_grad_slots_bytes(int64_t slot_capacity, size_t* bytes) {
+  if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  if (slot_capacity < 0 || slot_capacity > (int64_t(1) << 40)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "slot_capacity out of range");
+  }
+  *bytes = align_up(kSlotHeaderBytes + size_t(slot_capacity) * kSlotWords * sizeof(float), 256);
+  return success();
+}
+
+int ssplat_rect_pair_count(const float* means, const int64_t* radii, int64_t n, int height,
+                           int width, int tile_size, int64_t* total, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  return rect_pair_count(p, means, radii, total, workspace, workspace_bytes, stream);
 }
 
 int ssplat_batch_plan(ssplat_field* fields, int num_fields, int tile_size,
@@ -1862,15 +2161,16 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 const int32_t* tile_offsets, int64_t num_gids, int normalize,
                                 double eps, double support_fade_alpha, double sigma_cutoff,
                                 float* grad_means, float* grad_conics, float* grad_colors,
-                                float* grad_opacities, void* workspace, size_t workspace_bytes,
+                                float* grad_opacities, void* grad_slots, int64_t slot_capacity,
+                                int32_t* status, void* workspace, size_t workspace_bytes,
                                 void* stream) {
   Plan p;
   const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
   if (rc) return rc;
   return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
-                        grad_means, grad_conics, grad_colors, grad_opacities, workspace,
-                        workspace_bytes, stream);
+                        grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
+                        slot_capacity, status, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -20,6 +20,7 @@ capacity, so the result equals a run with unlimited capacity.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass
 
 import torch
@@ -55,7 +56,9 @@ class FitConfig:
 
 
 class RenderPlan:
-    """Preallocated pair buffer, offsets, sticky status and workspace for one field shape."""
+    """Preallocated pair buffer, offsets, sticky status, workspace and gradient slots for one
+    field shape.  The slot buffer has the pair buffer's capacity (the slot count equals the pair
+    count without ellipse cull); an overflow of either sets the status word."""
 
     def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16,
                  recompute_normalizer: bool = True):
@@ -67,6 +70,7 @@ class RenderPlan:
         self.offsets = torch.empty(T + 1, dtype=torch.int32, device=device)
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
         self.ws = HR.workspace(n, H, W, tile_size, device)
+        self.slots = HR.grad_slots(self.capacity, device)
         self.den = (None if self.recompute_normalizer
                     else torch.empty(H, W, dtype=torch.float32, device=device))
 
@@ -107,7 +111,7 @@ class _PlannedRender(Function):
         g_m, g_q, g_c, g_o = HR.backward_tiled(
             grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, plan.gids,
             plan.offsets, plan.H, plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
-            ws=plan.ws)
+            ws=plan.ws, slots=plan.slots, slot_capacity=plan.capacity, status=plan.status)
         if opacities.numel() == 0:
             g_o = None
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None
@@ -164,7 +168,11 @@ class _Snapshot:
 
 def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool = False,
         verbose: bool = False) -> dict:
-    """Fit `field` to `target` (H, W, 3) for cfg.iters Adam steps; returns history and PSNR."""
+    """Fit `field` to `target` (H, W, 3) for cfg.iters Adam steps.
+
+    Returns the final PSNR and loss, the pair-buffer regrows and capacity, the tile size, and
+    ``seconds``: the wall time of the step loop (window checks and regrows included; the tile
+    tuning and first plan before it are not)."""
     H, W = target.shape[:2]
     dev = target.device
     field.trainable()
@@ -190,6 +198,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     done = 0
     graph_obj = None
     regrows = 0
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
     while done < cfg.iters:
         n = min(cfg.window, cfg.iters - done)
         snap = _Snapshot(field, opt)
@@ -228,6 +238,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
             regrows += 1
         if verbose:
             print(f"step {done}: loss {float(state['loss']):.6f} pairs {p}", flush=True)
+    torch.cuda.synchronize(dev)
+    seconds = time.perf_counter() - t0
     with torch.no_grad():
         img = HR.render_hip_exact(field.means, field.conics(), field.colors,
                                   field.radii(cfg.sigma_cutoff), H, W,
@@ -236,7 +248,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
                                   sigma_cutoff=cfg.sigma_cutoff)
         final_psnr = float(psnr_tensor(img, target))
     return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
-            "capacity": plan.capacity, "tile_size": ts}
+            "capacity": plan.capacity, "tile_size": ts, "seconds": seconds}
 
 
 def resolve_tile_size(cfg: FitConfig, tune) -> int:

@@ -129,6 +129,33 @@ def workspace(n: int, height: int, width: int, tile_size: int, device) -> torch.
     return torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
 
 
+def grad_slots(slot_capacity: int, device) -> torch.Tensor:
+    """Zero-filled gradient slot buffer for deterministic backwards with up to `slot_capacity`
+    slots (one per support-rectangle tile of each Gaussian); reusable across calls."""
+    nbytes = _native.grad_slots_bytes(max(int(slot_capacity), 0))
+    return torch.zeros(nbytes, dtype=torch.uint8, device=device)
+
+
+@torch.no_grad()
+def rect_pair_count(means, radii, H: int, W: int, tile_size: int, ws=None) -> int:
+    """Total slot count of a field: support-rectangle tiles summed over the Gaussians (equal to
+    the pair count of lists built without ellipse cull).  One host synchronization."""
+    dev = means.device
+    means = means.detach().contiguous()
+    radii = _radii64(radii)
+    n = means.shape[0]
+    if ws is None:
+        ws = workspace(n, H, W, tile_size, dev)
+    total = torch.empty(1, dtype=torch.int64, device=dev)
+    _native.check(_native.load().ssplat_rect_pair_count(
+        _ptr(means), _ptr(radii), n, int(H), int(W), int(tile_size), _ptr(total), _ptr(ws),
+        ws.numel(), _stream(dev)), "ssplat_rect_pair_count")
+    value = int(total.item())
+    if value < 0:
+        raise RuntimeError("gradient slot count exceeds int32; reduce N or image size")
+    return value
+
+
 @torch.no_grad()
 def build_tile_index(means, conics, radii, H: int, W: int, tile_size: int = DEFAULT_TILE_SIZE,
                      ellipse_cull: bool = False, sigma_cutoff: float = 3.0):
@@ -223,11 +250,15 @@ def forward_tiled(means, conics, colors, radii, opacities, gids, offsets, H: int
 
 def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, gids, offsets,
                    H: int, W: int, normalize: bool, eps: float, tile_size: int,
-                   support_fade: bool, sigma_cutoff: float, grads=None, ws=None):
+                   support_fade: bool, sigma_cutoff: float, grads=None, ws=None, slots=None,
+                   slot_capacity=None, status=None):
     """One tiled backward; returns ``(g_means, g_conics, g_colors, g_opacities or None)``.
 
     ``den`` and ``out`` are the forward's coverage and image for the same inputs (the
     reference's saved tensors, cuda_render.py:111-113); additive renders may pass ``None``.
+    The gradients are deterministic.  ``slots`` (from ``grad_slots``) holds ``slot_capacity``
+    gradient slots; without it a buffer is made for the field's exact slot count.  ``status``
+    (device int32) is set to 2 if the slot capacity was too small.
     """
     lib = _native.load()
     dev = means.device
@@ -239,12 +270,19 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, 
     g_means, g_conics, g_colors, g_opac = grads
     if ws is None:
         ws = workspace(n, H, W, tile_size, dev)
+    if slots is None:
+        if slot_capacity is None:
+            slot_capacity = rect_pair_count(means, radii, H, W, tile_size, ws)
+        slots = grad_slots(slot_capacity, dev)
+    elif slot_capacity is None:
+        raise ValueError("slot_capacity is required with a slot buffer")
     _native.check(lib.ssplat_backward_tiled(
         _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
-        _ptr(den), _ptr(out), n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W), int(tile_size),
-        int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
-        _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac), _ptr(ws), ws.numel(),
-        _stream(dev)), "ssplat_backward_tiled")
+        _ptr(den), _ptr(out), n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W),
+        int(tile_size), int(bool(normalize)), float(eps), _fade_alpha(support_fade),
+        float(sigma_cutoff), _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac),
+        _ptr(slots), int(slot_capacity), _ptr(status), _ptr(ws), ws.numel(), _stream(dev)),
+        "ssplat_backward_tiled")
     return g_means, g_conics, g_colors, g_opac
 
 
@@ -271,6 +309,9 @@ class _ExactRenderHip(Function):
         gids, offsets = build_tile_index(means, conics, radii, int(height), int(width),
                                          int(tile_size), cull, float(sigma_cutoff))
         ws = workspace(means.shape[0], height, width, tile_size, means.device)
+        # Gradient slots: one per support-rectangle tile, i.e. the pair count unless culled.
+        ctx.slot_capacity = (rect_pair_count(means, radii, int(height), int(width),
+                                             int(tile_size), ws) if cull else gids.numel())
         keep_den = bool(normalize) and not recompute_normalizer
         den = (torch.empty(int(height), int(width), dtype=torch.float32, device=means.device)
                if keep_den else None)
@@ -298,7 +339,7 @@ class _ExactRenderHip(Function):
         g_means, g_conics, g_colors, g_opac = backward_tiled(
             grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, gids,
             offsets, ctx.height, ctx.width, ctx.normalize, ctx.eps, ctx.tile_size, ctx.support_fade,
-            ctx.sigma_cutoff)
+            ctx.sigma_cutoff, slot_capacity=ctx.slot_capacity)
         if opacities.numel() == 0 or not need_opac:
             g_opac = None
         return (

@@ -70,13 +70,15 @@ def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, no
                           generator=torch.Generator(device=dev).manual_seed(0))
         out = torch.empty(b.total_pixels, 3, device=dev)
         bufs = (gids, offsets, status)
+        slots = HR.grad_slots(pairs + 1, dev)
 
         def step():
             b.build_tile_index(means, conics, radii, pairs + 1, cull, sigma_cutoff, out=bufs)
             b.forward(means, conics, colors, radii, op, gids, offsets, normalize, eps, alpha,
                       sigma_cutoff, out=out, den=den)
             b.backward(grad, means, conics, colors, radii, op, den, out, gids, offsets,
-                       normalize, eps, alpha, sigma_cutoff)
+                       normalize, eps, alpha, sigma_cutoff, slots=slots, slot_capacity=pairs + 1,
+                       status=status)
 
         timings[ts] = _time_ms(step, reps)
     best = min(timings, key=timings.get)

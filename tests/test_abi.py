@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_workspace_query():
     lib = _native.load()
-    assert lib.ssplat_abi_version() == 1
+    assert lib.ssplat_abi_version() == 2
     nbytes = _native.workspace_bytes(1000, 256, 256, 16)
     # 1000 records of 64 B + (256 tiles + 1) counters + scan sums + status, 256-byte aligned.
     assert nbytes >= 1000 * 64 + 257 * 4
@@ -63,9 +63,31 @@ def test_normalized_backward_given_den_needs_out():
     dummy = ctypes.create_string_buffer(64)
     p = ctypes.cast(dummy, ctypes.c_void_p)
     rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, p, None, 1, p, p, 1, 16, 16, 16, 1,
-                                   1e-8, 0.0, 3.0, p, p, p, None, p, 64, None)
+                                   1e-8, 0.0, 3.0, p, p, p, None, p, 1, None, p, 64, None)
     assert rc == _native.ERR_INVALID_ARGUMENT
     assert "also needs the forward's out" in _native.last_error()
+
+
+def test_backward_needs_a_gradient_slot_buffer():
+    lib = _native.load()
+    dummy = ctypes.create_string_buffer(64)
+    p = ctypes.cast(dummy, ctypes.c_void_p)
+    rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, p, p, 1, p, p, 1, 16, 16, 16, 1,
+                                   1e-8, 0.0, 3.0, p, p, p, None, None, 1, None, p, 64, None)
+    assert rc == _native.ERR_INVALID_ARGUMENT
+    assert "grad_slots is NULL" in _native.last_error()
+    rc = lib.ssplat_backward_tiled(p, p, p, p, p, None, p, p, 1, p, p, 1, 16, 16, 16, 1,
+                                   1e-8, 0.0, 3.0, p, p, p, None, p, -1, None, p, 64, None)
+    assert rc == _native.ERR_INVALID_ARGUMENT
+
+
+def test_grad_slot_buffer_size():
+    """Header (the call counter) plus 10 words per slot, rounded to 256 bytes."""
+    assert _native.grad_slots_bytes(0) == 256
+    assert _native.grad_slots_bytes(1) == 512
+    assert _native.grad_slots_bytes(1000) == 256 + ((1000 * 40 + 255) // 256) * 256
+    with pytest.raises(ValueError):
+        _native.grad_slots_bytes(-1)
 
 
 def test_kernel_timing_rejects_unknown_kernels():

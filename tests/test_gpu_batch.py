@@ -1,7 +1,8 @@
 """GPU tests of the ragged batch path against per-field renders and the plain per-image loop."""
 from __future__ import annotations
 
-import numpy as np
+import dataclasses
+
 import pytest
 import torch
 
@@ -99,7 +100,8 @@ def test_batched_fit_matches_plain_loop(cuda):
                                          opacities=f["opacity"], device=cuda) for f in init]
 
     cfg = F.FitConfig(iters=30, window=10)
-    plain = B.fit_each(fields(), targets, cfg)
     batched = B.fit_batch(fields(), targets, cfg, graph=True)
+    plain = B.fit_each(fields(), targets, dataclasses.replace(cfg, tile_size=batched.tile_size))
     print("plain", plain.psnr, plain.seconds, "batched", batched.psnr, batched.seconds)
-    assert np.max(np.abs(np.array(plain.psnr) - np.array(batched.psnr))) < 0.01
+    # Deterministic gradients and the same tile size: the same fits, bit for bit.
+    assert plain.psnr == batched.psnr

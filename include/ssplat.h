@@ -173,6 +173,16 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           int32_t* status, void* workspace, size_t workspace_bytes,
                           void* stream);
 
+/*
+ * Per-Gaussian activity: activity[i] = sum over the pixels of Gaussian i's clipped support
+ * rectangle of its weight exp(-q/2) (support fade applied; opacity not included), the pruning
+ * statistic of gaussian_activity (render.py:417-442, a torch loop over pixel slices in the
+ * reference).  Each Gaussian's sum is formed in a fixed order (bitwise repeatable).
+ */
+int ssplat_gaussian_activity(const float* means, const float* conics, const int64_t* radii,
+                             int64_t n, int height, int width, double support_fade_alpha,
+                             double sigma_cutoff, float* activity, void* stream);
+
 /* Bytes of a gradient slot buffer holding slot_capacity slots (40 bytes each plus a header).
    No reference counterpart (the reference accumulates with float atomics). */
 int ssplat_grad_slots_bytes(int64_t slot_capacity, size_t* bytes);

@@ -5,6 +5,7 @@ Public entry points mirror the reference's render path:
 * :func:`render_field` - mode dispatcher (render.py:319-414).
 * :func:`render_hip_exact` - the ``render_cuda_exact`` entry (cuda_render.py:256-294).
 * :func:`build_tile_index` - per-tile Gaussian lists (render_ext.cpp:139-167).
+* :func:`gaussian_activity` - per-Gaussian summed weight, the pruning statistic (render.py:417-442).
 * :class:`GaussianField` - field layout and conic/radii/opacity producer (gaussians.py).
 
 The compute runs in hand-written HIP kernels in ``libssplat_hip.so`` behind a C ABI
@@ -14,6 +15,7 @@ from .gaussians import GaussianField
 from .render import (
     build_tile_index,
     build_tile_index_capacity,
+    gaussian_activity,
     render_field,
     render_hip_exact,
     render_hip_exact_with_coverage,
@@ -23,6 +25,7 @@ __all__ = [
     "GaussianField",
     "build_tile_index",
     "build_tile_index_capacity",
+    "gaussian_activity",
     "render_field",
     "render_hip_exact",
     "render_hip_exact_with_coverage",

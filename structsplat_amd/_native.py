@@ -50,6 +50,8 @@ SIGNATURES = {
                                        _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size,
                                        _c_ptr]),
     "ssplat_grad_slots_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_gaussian_activity": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int,
+                                          _c_dbl, _c_dbl, _c_ptr, _c_ptr]),
     "ssplat_rect_pair_count": (_c_int, [_c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_ptr,
                                         _c_ptr, _c_size, _c_ptr]),
     "ssplat_kernel_timing_enable": (_c_int, [_c_int]),

@@ -134,6 +134,7 @@ constexpr int kBwdStage16 = 128;
 constexpr int kSlotWords = 10;            // 9 partial gradients + the call's tag
 constexpr size_t kSlotHeaderBytes = 256;  // slot buffer header: the call counter (uint32)
 constexpr int kLaneSumMax = 64;           // Gaussians with up to this many slots: one lane each
+constexpr int kFieldWaves = 4;            // waves per 256-thread workgroup of per-Gaussian sweeps
 // The render kernels run one workgroup per tile, one thread per pixel (tile edge 16: 256 threads,
 // 4 waves; tile edge 32: 1024 threads, 16 waves), one wave per 8x8 quadrant.
 // -0.5 * log2(e): exp(-q/2) == exp2(q * kNegHalfLog2e).
@@ -346,7 +347,13 @@ struct BinArgs {
 
 // Enumerates every live (tile, Gaussian) pair of this workgroup's 256 Gaussians.  Each wave owns
 // 64 consecutive Gaussians; their pairs are numbered by a wave-level prefix sum and dealt out
-// to the lanes round-robin, so one large Gaussian does not serialize its wave.
+// to the lanes round-robin, so one large Gaussian does not serialize its wave.  Each lane takes
+// kEnumUnroll pairs per step (pair k, k + 64, ...): their owner searches (a fixed six-step
+// binary search over the wave's 64 prefix sums in LDS) are independent and overlap, and the
+// step's pairs go to `emit` together so that its memory operations overlap too.
+// emit(tile[kEnumUnroll], gid[kEnumUnroll], live[kEnumUnroll]).
+constexpr int kEnumUnroll = 4;
+
 template <class Emit>
 __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
   __shared__ int64_t s_incl[kBinThreads];
@@ -401,36 +408,55 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
     s_b[threadIdx.x] = cb;
   }
   __syncthreads();
-  for (int64_t k = lane; k < total; k += 64) {
-    int lo = 0, hi = 63;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_incl[wbase + mid] > k) {
-        hi = mid;
-      } else {
-        lo = mid + 1;
+  for (int64_t base = 0; base < total; base += 64 * kEnumUnroll) {
+    int64_t kk[kEnumUnroll];
+    int lo[kEnumUnroll];
+#pragma unroll
+    for (int u = 0; u < kEnumUnroll; ++u) {
+      const int64_t k = base + 64 * u + lane;
+      kk[u] = k < total ? k : total - 1;
+      lo[u] = 0;
+    }
+    // Owner = the first of the 64 Gaussians whose inclusive prefix exceeds k.
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+#pragma unroll
+      for (int u = 0; u < kEnumUnroll; ++u) {
+        if (s_incl[wbase + lo[u] + step - 1] <= kk[u]) lo[u] += step;
       }
     }
-    const int o = wbase + lo;
-    const int64_t excl = lo > 0 ? s_incl[o - 1] : 0;
-    const int64_t local = k - excl;
-    const int4 tl = s_tile[o];
-    const int tile_x = tl.x + static_cast<int>(local % tl.z);
-    const int tile_y = tl.y + static_cast<int>(local / tl.z);
-    if (a.cull) {
-      const int4 rc = s_rect[o];
-      const float4 g = s_geo[o];
-      const int rx0 = max(rc.x, tile_x * a.ts);
-      const int rx1 = min(rc.y - 1, (tile_x + 1) * a.ts - 1);
-      const int ry0 = max(rc.z, tile_y * a.ts);
-      const int ry1 = min(rc.w - 1, (tile_y + 1) * a.ts - 1);
-      const float mq = tile_min_q(g.x, g.y, g.z, s_b[o], g.w, static_cast<float>(rx0),
-                                  static_cast<float>(rx1), static_cast<float>(ry0),
-                                  static_cast<float>(ry1));
-      if (mq > a.cull_thr) continue;
+    int tiles[kEnumUnroll], gidv[kEnumUnroll];
+    bool live[kEnumUnroll];
+#pragma unroll
+    for (int u = 0; u < kEnumUnroll; ++u) {
+      const int o = wbase + lo[u];
+      const int64_t excl = lo[u] > 0 ? s_incl[o - 1] : 0;
+      // A pair's index within its Gaussian is below tw * th <= 65535^2, so 32-bit unsigned.
+      const uint32_t local = static_cast<uint32_t>(kk[u] - excl);
+      const int4 tl = s_tile[o];
+      const uint32_t w = static_cast<uint32_t>(tl.z);
+      const uint32_t row = local / w;
+      const int tile_x = tl.x + static_cast<int>(local - row * w);
+      const int tile_y = tl.y + static_cast<int>(row);
+      bool keep = base + 64 * u + lane < total;
+      if (a.cull) {
+        const int4 rc = s_rect[o];
+        const float4 g = s_geo[o];
+        const int rx0 = max(rc.x, tile_x * a.ts);
+        const int rx1 = min(rc.y - 1, (tile_x + 1) * a.ts - 1);
+        const int ry0 = max(rc.z, tile_y * a.ts);
+        const int ry1 = min(rc.w - 1, (tile_y + 1) * a.ts - 1);
+        const float mq = tile_min_q(g.x, g.y, g.z, s_b[o], g.w, static_cast<float>(rx0),
+                                    static_cast<float>(rx1), static_cast<float>(ry0),
+                                    static_cast<float>(ry1));
+        keep = keep && !(mq > a.cull_thr);
+      }
+      const int2 grid = s_grid[o];
+      tiles[u] = grid.x + tile_y * grid.y + tile_x;
+      gidv[u] = tl.w;
+      live[u] = keep;
     }
-    const int2 grid = s_grid[o];
-    emit(grid.x + tile_y * grid.y + tile_x, tl.w);
+    emit(tiles, gidv, live);
   }
 }
 
@@ -477,7 +503,13 @@ __global__ void k_bin_reset(int32_t* __restrict__ counts, int64_t items,
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_bin_count(BinArgs a, int32_t* __restrict__ counts) {
-  enumerate_pairs(a, [&](int tile, int) { atomicAdd(&counts[tile], 1); });
+  enumerate_pairs(a, [&](const int (&tiles)[kEnumUnroll], const int (&)[kEnumUnroll],
+                         const bool (&live)[kEnumUnroll]) {
+#pragma unroll
+    for (int u = 0; u < kEnumUnroll; ++u) {
+      if (live[u]) atomicAdd(&counts[tiles[u]], 1);
+    }
+  });
 }
 
 // One-workgroup exclusive scan of counts[0, items) into offsets, for grids of up to
@@ -560,12 +592,24 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(BinArgs a,
   // Each tile's cursor holds its pair count and is counted down to zero: the pairs land in
   // [offsets[t], offsets[t+1]) in arbitrary order (the per-tile sort orders them) and the counters
   // end at zero.
-  enumerate_pairs(a, [&](int tile, int gid) {
-    const int64_t pos = int64_t(offsets[tile]) + atomicSub(&cursors[tile], 1) - 1;
-    if (pos >= 0 && pos < capacity) {
-      gids[pos] = gid;
-    } else {
-      *status = SSPLAT_ERR_CAPACITY;
+  enumerate_pairs(a, [&](const int (&tiles)[kEnumUnroll], const int (&gid)[kEnumUnroll],
+                         const bool (&live)[kEnumUnroll]) {
+    // All of the step's cursor updates are issued before their results are used.
+    int32_t slot[kEnumUnroll], first[kEnumUnroll];
+#pragma unroll
+    for (int u = 0; u < kEnumUnroll; ++u) {
+      slot[u] = live[u] ? atomicSub(&cursors[tiles[u]], 1) : 0;
+      first[u] = live[u] ? offsets[tiles[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kEnumUnroll; ++u) {
+      if (!live[u]) continue;
+      const int64_t pos = int64_t(first[u]) + slot[u] - 1;
+      if (pos >= 0 && pos < capacity) {
+        gids[pos] = gid[u];
+      } else {
+        *status = SSPLAT_ERR_CAPACITY;
+      }
     }
   });
 }
@@ -849,11 +893,24 @@ __global__ __launch_bounds__(256) void k_prepare(
     *sp.rect_status = SSPLAT_OK;
     if (sp.epoch != nullptr) *sp.epoch += 1u;
   }
+  // Every input is loaded up front (one memory round trip per thread).
+  const bool live = i < n;
+  const int j = live ? i : 0;
+  const float mx = live ? means[2 * j + 0] : 0.0f;
+  const float my = live ? means[2 * j + 1] : 0.0f;
+  const int64_t rx = live ? radii[2 * j + 0] : 0;
+  const int64_t ry = live ? radii[2 * j + 1] : 0;
+  const float qa = live ? conics[3 * j + 0] : 0.0f;
+  const float qb = live ? conics[3 * j + 1] : 0.0f;
+  const float qc = live ? conics[3 * j + 2] : 0.0f;
+  const float cr = live ? colors[3 * j + 0] : 0.0f;
+  const float cg = live ? colors[3 * j + 1] : 0.0f;
+  const float cb = live ? colors[3 * j + 2] : 0.0f;
+  const float op = (live && opacities) ? opacities[j] : 1.0f;
   uint32_t xr = 0, yr = 0;
-  if (i < n) {
+  if (live) {
     const ssplat_field f = field_of_gaussian(fields, i);
-    const Rect r = support_rect(means[2 * i + 0], means[2 * i + 1], radii[2 * i + 0],
-                                radii[2 * i + 1], f.height, f.width);
+    const Rect r = support_rect(mx, my, rx, ry, f.height, f.width);
     if (r.x1 > r.x0 && r.y1 > r.y0) {
       xr = uint32_t(r.x0) | (uint32_t(r.x1) << 16);
       yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
@@ -868,16 +925,11 @@ __global__ __launch_bounds__(256) void k_prepare(
     if (lane >= d) incl += y;
   }
   if (lane == 63 && (i >> 6) < sp.num_waves) sp.rect_wave[i >> 6] = incl;
-  if (i >= n) return;
-  const float mx = means[2 * i + 0];
-  const float my = means[2 * i + 1];
+  if (!live) return;
   const float scale = kFold ? kNegHalfLog2e : 1.0f;
-  const float op = opacities ? opacities[i] : 1.0f;
-  const float cr = colors[3 * i + 0], cg = colors[3 * i + 1], cb = colors[3 * i + 2];
   GaussRec g;
-  g.c0 = make_float4(mx, my, scale * conics[3 * i + 0], scale * (2.0f * conics[3 * i + 1]));
-  g.c1 = make_float4(scale * conics[3 * i + 2], __uint_as_float(xr), __uint_as_float(yr),
-                     __int_as_float(i));
+  g.c0 = make_float4(mx, my, scale * qa, scale * (2.0f * qb));
+  g.c1 = make_float4(scale * qc, __uint_as_float(xr), __uint_as_float(yr), __int_as_float(i));
   g.c2 = make_float4(op * cr, op * cg, op * cb, op);
   g.c3 = make_float4(cr, cg, cb, __int_as_float(incl - slots));
   recs[i] = g;
@@ -1309,22 +1361,25 @@ struct ReduceArgs {
 
 // Adds one slot's nine partial gradients when its tag is the call's (a slot the call did not
 // write counts as zero; the select keeps stale values out of the sum).
+__device__ __forceinline__ void add_slot_values(const float2 (&v)[5], uint32_t tag,
+                                                float (&acc)[9]) {
+  const bool live = __float_as_uint(v[4].y) == tag;
+  acc[0] += live ? v[0].x : 0.0f;
+  acc[1] += live ? v[0].y : 0.0f;
+  acc[2] += live ? v[1].x : 0.0f;
+  acc[3] += live ? v[1].y : 0.0f;
+  acc[4] += live ? v[2].x : 0.0f;
+  acc[5] += live ? v[2].y : 0.0f;
+  acc[6] += live ? v[3].x : 0.0f;
+  acc[7] += live ? v[3].y : 0.0f;
+  acc[8] += live ? v[4].x : 0.0f;
+}
+
 __device__ __forceinline__ void add_slot(const float* s, uint32_t tag, float (&acc)[9]) {
-  const float2 p01 = *reinterpret_cast<const float2*>(s + 0);
-  const float2 p23 = *reinterpret_cast<const float2*>(s + 2);
-  const float2 p45 = *reinterpret_cast<const float2*>(s + 4);
-  const float2 p67 = *reinterpret_cast<const float2*>(s + 6);
-  const float2 p89 = *reinterpret_cast<const float2*>(s + 8);
-  const bool live = __float_as_uint(p89.y) == tag;
-  acc[0] += live ? p01.x : 0.0f;
-  acc[1] += live ? p01.y : 0.0f;
-  acc[2] += live ? p23.x : 0.0f;
-  acc[3] += live ? p23.y : 0.0f;
-  acc[4] += live ? p45.x : 0.0f;
-  acc[5] += live ? p45.y : 0.0f;
-  acc[6] += live ? p67.x : 0.0f;
-  acc[7] += live ? p67.y : 0.0f;
-  acc[8] += live ? p89.x : 0.0f;
+  float2 v[5];
+#pragma unroll
+  for (int w = 0; w < 5; ++w) v[w] = *reinterpret_cast<const float2*>(s + 2 * w);
+  add_slot_values(v, tag, acc);
 }
 
 // Gradients of every Gaussian from its slots, in an order fixed by the Gaussian alone: up to
@@ -1348,7 +1403,22 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   }
   float acc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (count <= kLaneSumMax) {
-    for (int k = 0; k < count; ++k) add_slot(a.slots + (base + k) * kSlotWords, tag, acc);
+    // Four slots' loads in flight per step; the additions stay in ascending slot order.
+    const float* sl = a.slots + base * kSlotWords;
+    int k = 0;
+    for (; k + 4 <= count; k += 4) {
+      float2 v[4][5];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int w = 0; w < 5; ++w) {
+          v[u][w] = *reinterpret_cast<const float2*>(sl + (k + u) * kSlotWords + 2 * w);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) add_slot_values(v[u], tag, acc);
+    }
+    for (; k < count; ++k) add_slot(sl + k * kSlotWords, tag, acc);
   }
   uint64_t big = __ballot(count > kLaneSumMax);
   while (big) {
@@ -1416,6 +1486,53 @@ __global__ void k_rect_total(const int32_t* __restrict__ rect_off, int num_waves
                              const int32_t* __restrict__ rect_status, int64_t* __restrict__ total) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *total = *rect_status == SSPLAT_OK ? int64_t(rect_off[num_waves]) : int64_t(-1);
+  }
+}
+
+// Per Gaussian: the sum of its weight exp(-q/2) (with the support fade applied) over the pixels
+// of its clipped support rectangle, without opacity (render.py:417-442).  Each wave takes one
+// Gaussian at a time (grid-stride over the Gaussians) and sweeps its rectangle in 8x8 pixel
+// blocks, one pixel per lane; every lane sums its pixels in sweep order and a fixed DPP tree
+// adds the lanes, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_activity(const float* __restrict__ means,
+                                                  const float* __restrict__ conics,
+                                                  const int64_t* __restrict__ radii, int n,
+                                                  FieldTable fields, float fade_alpha,
+                                                  float tail_exp, float* __restrict__ activity) {
+  const int lane = threadIdx.x & 63;
+  const int lx = lane & 7, ly = lane >> 3;
+  const int waves = static_cast<int>(gridDim.x) * (kFieldWaves);
+  const bool fade = fade_alpha > 0.0f;
+  const float tail = fade ? fade_alpha * exp2_hw(tail_exp) : 0.0f;
+  const int first = __builtin_amdgcn_readfirstlane(
+      static_cast<int>(blockIdx.x) * kFieldWaves + static_cast<int>(threadIdx.x >> 6));
+  for (int g = first; g < n; g += waves) {
+    const ssplat_field f = field_of_gaussian(fields, g);
+    const float mx = means[2 * g + 0];
+    const float my = means[2 * g + 1];
+    const Rect r = support_rect(mx, my, radii[2 * g + 0], radii[2 * g + 1], f.height, f.width);
+    // The quadratic in exp2 units, as the render kernels evaluate it: (a, 2b, c) scaled by
+    // -log2(e)/2 (with fade, the scale is applied after so that a pixel at exactly the cutoff
+    // gives the tail's value).
+    const float scale = fade ? 1.0f : kNegHalfLog2e;
+    const float qa = scale * conics[3 * g + 0];
+    const float qb = scale * (2.0f * conics[3 * g + 1]);
+    const float qc = scale * conics[3 * g + 2];
+    float sum = 0.0f;
+    for (int by = r.y0; by < r.y1; by += 8) {
+      for (int bx = r.x0; bx < r.x1; bx += 8) {
+        const int px = bx + lx, py = by + ly;
+        if (px < r.x1 && py < r.y1) {
+          const float dx = static_cast<float>(px) - mx;
+          const float dy = static_cast<float>(py) - my;
+          const float e = fmaf(fmaf(qa, dx, qb * dy), dx, qc * dy * dy);
+          const float raw = exp2_hw(fade ? e * kNegHalfLog2e : e);
+          sum += fade ? fmaxf(raw - tail, 0.0f) : raw;
+        }
+      }
+    }
+    sum = wave_sum_lane63(sum);
+    if (lane == 63) activity[g] = sum;
   }
 }
 
@@ -1906,6 +2023,29 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   return success();
 }
 
+int gaussian_activity(const Plan& p, const float* means, const float* conics,
+                      const int64_t* radii, double support_fade_alpha, double sigma_cutoff,
+                      float* activity, void* stream) {
+  int rc;
+  double fade_alpha;
+  if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
+  if (!std::isfinite(sigma_cutoff)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "sigma_cutoff must be finite");
+  }
+  if (p.n > 0 && (means == nullptr || conics == nullptr || radii == nullptr ||
+                  activity == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "means/conics/radii/activity must not be NULL");
+  }
+  if (p.n == 0) return success();
+  const int64_t blocks64 = (p.n + kFieldWaves - 1) / kFieldWaves;
+  const int blocks = static_cast<int>(blocks64 < 2048 ? blocks64 : 2048);
+  k_activity<<<blocks, 64 * kFieldWaves, 0, static_cast<hipStream_t>(stream)>>>(
+      means, conics, radii, static_cast<int>(p.n), p.table, static_cast<float>(fade_alpha),
+      static_cast<float>(sigma_cutoff * sigma_cutoff) * kNegHalfLog2e, activity);
+  SSPLAT_LAUNCH_CHECK("k_activity");
+  return success();
+}
+
 int rect_pair_count(const Plan& p, const float* means, const int64_t* radii, int64_t* total,
                     void* ws, size_t ws_bytes, void* stream) {
   int rc;
@@ -2073,6 +2213,16 @@ int ssplat_grad_slots_bytes(int64_t slot_capacity, size_t* bytes) {
   }
   *bytes = align_up(kSlotHeaderBytes + size_t(slot_capacity) * kSlotWords * sizeof(float), 256);
   return success();
+}
+
+int ssplat_gaussian_activity(const float* means, const float* conics, const int64_t* radii,
+                             int64_t n, int height, int width, double support_fade_alpha,
+                             double sigma_cutoff, float* activity, void* stream) {
+  Plan p;
+  const int rc = make_single_plan(n, height, width, 16, &p);
+  if (rc) return rc;
+  return gaussian_activity(p, means, conics, radii, support_fade_alpha, sigma_cutoff, activity,
+                           stream);
 }
 
 int ssplat_rect_pair_count(const float* means, const int64_t* radii, int64_t n, int height,

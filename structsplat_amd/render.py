@@ -415,6 +415,30 @@ def render_hip_exact_with_coverage(means, conics, colors, radii, H: int, W: int,
     return out, den
 
 
+@torch.no_grad()
+def gaussian_activity(means, conics, radii, H: int, W: int, chunk: int = 4096,
+                      support_fade: bool = False, sigma_cutoff: float = 3.0,
+                      support_fade_alpha: float | None = None) -> torch.Tensor:
+    """Each Gaussian's summed unnormalized, opacity-free weight over the image (the reference's
+    pruning statistic, render.py:417-442), from one HIP launch.  ``chunk`` bounds the reference
+    loop's memory and has no effect here.  Each sum is formed in a fixed order."""
+    del chunk
+    if not means.is_cuda:
+        raise RuntimeError(_DEVICE_MESSAGE)
+    if means.dtype != torch.float32 or conics.dtype != torch.float32:
+        raise RuntimeError("gaussian_activity requires float32 means and conics")
+    fade_alpha = _resolve_support_fade_alpha(support_fade, support_fade_alpha)
+    means = means.detach().contiguous()
+    conics = conics.detach().contiguous()
+    radii = _radii64(radii)
+    n = means.shape[0]
+    activity = torch.empty(n, dtype=torch.float32, device=means.device)
+    _native.check(_native.load().ssplat_gaussian_activity(
+        _ptr(means), _ptr(conics), _ptr(radii), n, int(H), int(W), float(fade_alpha),
+        float(sigma_cutoff), _ptr(activity), _stream(means.device)), "ssplat_gaussian_activity")
+    return activity
+
+
 def render_field(means, conics, colors, radii, H: int, W: int,
                  chunk: int = 4096, mode: str = "normalized", opacities=None,
                  scales=None, rotations=None, support_fade: bool = False,

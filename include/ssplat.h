@@ -261,6 +261,34 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 void* stream);
 
 /*
+ * SSPL1 decoding (the reference's codec.decode, codec.py:333-391).  The host parses the
+ * container (magic, JSON header, framed streams) and inflates the streams; this decodes the
+ * raw streams, copied to the device, into the field arrays: means (n,2), log_scales (n,2),
+ * rotations (n,), colors (n,3) and, with has_opacity, opacity logits (n,), all float32 and
+ * bit-identical to the reference's NumPy decode (float64 dequantization rounded to float32; the
+ * logits' final log is correctly rounded).  Streams: means 2n values (x then y; byte-planar when
+ * means_byte_planar), log-scales 2n, rotations n, colors 3n (r, g, b), opacities n; each value
+ * uint8/16/32 for bit widths <= 8/16/32.  Morton-coded means and delta-coded colors are prefix
+ * sums mod 2^bits.  The workspace holds ssplat_sspl1_workspace_bytes(n) bytes, 256-aligned.
+ */
+typedef struct ssplat_sspl1 {
+  int64_t n;
+  int32_t bits_means, bits_scales, bits_rot, bits_colors, bits_opacity;
+  int32_t morton, color_delta, means_byte_planar, rot_circular, has_opacity;
+  double means_lo[2], means_hi[2];
+  double scale_lo[2], scale_hi[2];
+  double color_lo[3], color_hi[3];
+} ssplat_sspl1;
+
+int ssplat_sspl1_workspace_bytes(int64_t n, size_t* bytes);
+int ssplat_sspl1_decode(const ssplat_sspl1* layout, const uint8_t* means_raw,
+                        const uint8_t* scales_raw, const uint8_t* rot_raw,
+                        const uint8_t* colors_raw, const uint8_t* opacity_raw, float* means,
+                        float* log_scales, float* rotations, float* colors,
+                        float* opacity_logits, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/*
  * Fit loss: (1 - w) * L1 + w * (1 - SSIM) of an RGB image against its target, and its gradient
  * with respect to the image (the reference computes it with torch in fit.py:800-823 and
  * metrics.py:88-151; no native counterpart).  L1 is the mean absolute difference over the 3 H W

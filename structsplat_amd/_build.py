@@ -12,7 +12,8 @@ import sys
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
-SOURCES = [os.path.join(PKG_DIR, "csrc", name) for name in ("ssplat.hip", "loss.hip", "optim.hip", "field.hip")]
+SOURCES = [os.path.join(PKG_DIR, "csrc", name)
+           for name in ("ssplat.hip", "loss.hip", "optim.hip", "field.hip", "codec.hip")]
 INCLUDE_DIR = os.path.join(REPO_DIR, "include")
 LIB_PATH = os.path.join(PKG_DIR, "libssplat_hip.so")
 ARCH = "gfx950"

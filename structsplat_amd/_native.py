@@ -50,6 +50,8 @@ SIGNATURES = {
                                        _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size,
                                        _c_ptr]),
     "ssplat_grad_slots_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_sspl1_workspace_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_sspl1_decode": (_c_int, [_c_ptr] * 11 + [_c_ptr, _c_size, _c_ptr]),
     "ssplat_gaussian_activity": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int,
                                           _c_dbl, _c_dbl, _c_ptr, _c_ptr]),
     "ssplat_rect_pair_count": (_c_int, [_c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_ptr,
@@ -128,6 +130,30 @@ class AdamGroup(ctypes.Structure):
         ("hi", ctypes.c_double),
         ("hi_rows", ctypes.c_void_p),
         ("row_width", ctypes.c_int64),
+    ]
+
+
+class SSPL1Layout(ctypes.Structure):
+    """ssplat_sspl1 of include/ssplat.h (an SSPL1 header's decoding parameters)."""
+
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("bits_means", ctypes.c_int32),
+        ("bits_scales", ctypes.c_int32),
+        ("bits_rot", ctypes.c_int32),
+        ("bits_colors", ctypes.c_int32),
+        ("bits_opacity", ctypes.c_int32),
+        ("morton", ctypes.c_int32),
+        ("color_delta", ctypes.c_int32),
+        ("means_byte_planar", ctypes.c_int32),
+        ("rot_circular", ctypes.c_int32),
+        ("has_opacity", ctypes.c_int32),
+        ("means_lo", ctypes.c_double * 2),
+        ("means_hi", ctypes.c_double * 2),
+        ("scale_lo", ctypes.c_double * 2),
+        ("scale_hi", ctypes.c_double * 2),
+        ("color_lo", ctypes.c_double * 3),
+        ("color_hi", ctypes.c_double * 3),
     ]
 
 

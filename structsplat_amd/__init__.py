@@ -7,6 +7,7 @@ Public entry points mirror the reference's render path:
 * :func:`build_tile_index` - per-tile Gaussian lists (render_ext.cpp:139-167).
 * :func:`gaussian_activity` - per-Gaussian summed weight, the pruning statistic (render.py:417-442).
 * :class:`GaussianField` - field layout and conic/radii/opacity producer (gaussians.py).
+* :mod:`structsplat_amd.codec` - SSPL1 decode and decode-and-render (codec.py:333-500).
 
 The compute runs in hand-written HIP kernels in ``libssplat_hip.so`` behind a C ABI
 (``include/ssplat.h``); build it with ``python -m structsplat_amd._build``.

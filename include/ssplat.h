@@ -174,6 +174,54 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           void* stream);
 
 /*
+ * Affine (gradient) colors: Gaussian i's color at pixel p is
+ *   c_i + G_i0 * lx + G_i1 * ly,   lx = ( cos t dx + sin t dy) / max(sx, 1e-6),
+ *                                  ly = (-sin t dx + cos t dy) / max(sy, 1e-6),
+ * (dx, dy) = p - mean_i, t = rotations[i], (sx, sy) = scales[i] (linear, not log), G_i the
+ * (2,3) row of color_grads (N,2,3).  The weights, normalization and fade are those of
+ * ssplat_forward_tiled.  Replaces the color_grads path of render_normalized_weighted_sum /
+ * render_additive_weighted_sum (render.py:110-123, :190-197, :300-307; the reference has no
+ * kernel for it and runs it as a torch loop over pixel chunks).  tile_size 16 only
+ * (SSPLAT_ERR_UNSUPPORTED otherwise); the workspace holds
+ * ssplat_affine_workspace_bytes(n, height, width, tile_size) bytes.
+ */
+int ssplat_forward_tiled_affine(const float* means, const float* conics, const float* colors,
+                                const int64_t* radii, const float* opacities,
+                                const float* color_grads, const float* scales,
+                                const float* rotations, int64_t n, const int32_t* tile_gids,
+                                const int32_t* tile_offsets, int64_t num_gids, int height,
+                                int width, int tile_size, int normalize, double eps,
+                                double support_fade_alpha, double sigma_cutoff, float* out,
+                                float* den, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
+/*
+ * Backward of ssplat_forward_tiled_affine: as ssplat_backward_tiled, and also writes
+ * dL/dcolor_grads (N,2,3), dL/dscales (N,2) and dL/drotations (N,).  A scale below 1e-6
+ * (clamped in the forward) gets gradient 0.  grad_slots holds
+ * ssplat_affine_grad_slots_bytes(slot_capacity) bytes (80 bytes a slot), zero-filled once and
+ * not shared with the plain backward's slot buffers.  Deterministic like ssplat_backward_tiled.
+ */
+int ssplat_backward_tiled_affine(
+    const float* grad_out, const float* means, const float* conics, const float* colors,
+    const int64_t* radii, const float* opacities, const float* color_grads, const float* scales,
+    const float* rotations, const float* den, const float* out, int64_t n,
+    const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids, int height,
+    int width, int tile_size, int normalize, double eps, double support_fade_alpha,
+    double sigma_cutoff, float* grad_means, float* grad_conics, float* grad_colors,
+    float* grad_opacities, float* grad_color_grads, float* grad_scales, float* grad_rotations,
+    void* grad_slots, int64_t slot_capacity, int32_t* status, void* workspace,
+    size_t workspace_bytes, void* stream);
+
+/* Workspace bytes of the affine entry points (the plain layout plus 48 bytes a Gaussian). */
+int ssplat_affine_workspace_bytes(int64_t n, int height, int width, int tile_size,
+                                  size_t* bytes);
+
+/* Bytes of an affine gradient slot buffer holding slot_capacity slots (80 bytes each plus a
+   header).  No reference counterpart. */
+int ssplat_affine_grad_slots_bytes(int64_t slot_capacity, size_t* bytes);
+
+/*
  * Per-Gaussian activity: activity[i] = sum over the pixels of Gaussian i's clipped support
  * rectangle of its weight exp(-q/2) (support fade applied; opacity not included), the pruning
  * statistic of gaussian_activity (render.py:417-442, a torch loop over pixel slices in the

@@ -99,11 +99,32 @@ def support_weight(q, sigma_cutoff: float, support_fade: bool,
     return w
 
 
+def pixel_colors(colors, color_grads, gid, dx, dy, scales, rotations):
+    """render.py:110-123: the Gaussian's color, plus with affine colors
+    G[0] * local_x + G[1] * local_y in its own frame (local = R(-theta) d / scale)."""
+    base = colors[gid]
+    if color_grads is None:
+        return base
+    if scales is None or rotations is None:
+        raise ValueError("affine color rendering requires scales and rotations")
+    theta = rotations[gid]
+    c, s = torch.cos(theta), torch.sin(theta)
+    sx = scales[gid, 0].clamp_min(1e-6)
+    sy = scales[gid, 1].clamp_min(1e-6)
+    local_x = (c * dx + s * dy) / sx
+    local_y = (-s * dx + c * dy) / sy
+    grads = color_grads[gid]
+    return base + grads[:, 0, :] * local_x[:, None] + grads[:, 1, :] * local_y[:, None]
+
+
 def accumulate(means, conics, colors, radii, H: int, W: int, chunk: int, opacities,
                normalize: bool, support_fade: bool = False, sigma_cutoff: float = 3.0,
-               support_fade_alpha: float | None = None, normalization_eps: float = EPS):
+               support_fade_alpha: float | None = None, normalization_eps: float = EPS,
+               color_grads=None, scales=None, rotations=None):
     """render.py:180-235 without the optional per-slice checkpointing (a memory knob that
-    leaves values and gradients unchanged, test_render.py:53-81)."""
+    leaves values and gradients unchanged, test_render.py:53-81).  With ``color_grads``
+    (N, 2, 3), ``scales`` (N, 2) and ``rotations`` (N,) the colors are affine in each
+    Gaussian's local frame (render.py:110-123)."""
     dev, dt = means.device, means.dtype
     num = torch.zeros(H * W, 3, device=dev, dtype=dt)
     den = torch.zeros(H * W, 1, device=dev, dtype=dt) if normalize else None
@@ -118,7 +139,8 @@ def accumulate(means, conics, colors, radii, H: int, W: int, chunk: int, opaciti
         if opacities is not None:
             w = w * opacities[gid]
         flat = py * W + px
-        num = num.index_add(0, flat, w[:, None] * colors[gid])
+        col = pixel_colors(colors, color_grads, gid, dx, dy, scales, rotations)
+        num = num.index_add(0, flat, w[:, None] * col)
         if normalize:
             den = den.index_add(0, flat, w[:, None])
     if normalize:
@@ -128,19 +150,22 @@ def accumulate(means, conics, colors, radii, H: int, W: int, chunk: int, opaciti
 
 def render(means, conics, colors, radii, H: int, W: int, chunk: int = 4096, opacities=None,
            support_fade: bool = False, sigma_cutoff: float = 3.0,
-           support_fade_alpha: float | None = None, normalization_eps: float = EPS):
+           support_fade_alpha: float | None = None, normalization_eps: float = EPS,
+           color_grads=None, scales=None, rotations=None):
     """Normalized weighted sum (render.py:238-249)."""
     normalization_eps = validate_normalization_eps(normalization_eps)
     return accumulate(means, conics, colors, radii, H, W, chunk, opacities, True, support_fade,
-                      sigma_cutoff, support_fade_alpha, normalization_eps)
+                      sigma_cutoff, support_fade_alpha, normalization_eps, color_grads, scales,
+                      rotations)
 
 
 def render_additive(means, conics, colors, radii, H: int, W: int, chunk: int = 4096,
                     opacities=None, support_fade: bool = False, sigma_cutoff: float = 3.0,
-                    support_fade_alpha: float | None = None):
+                    support_fade_alpha: float | None = None, color_grads=None, scales=None,
+                    rotations=None):
     """Additive accumulation (render.py:252-263)."""
     return accumulate(means, conics, colors, radii, H, W, chunk, opacities, False, support_fade,
-                      sigma_cutoff, support_fade_alpha)
+                      sigma_cutoff, support_fade_alpha, EPS, color_grads, scales, rotations)
 
 
 @torch.no_grad()

@@ -50,6 +50,16 @@ SIGNATURES = {
                                        _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size,
                                        _c_ptr]),
     "ssplat_grad_slots_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_forward_tiled_affine": (_c_int, [_c_ptr] * 8 + [_c_i64, _c_ptr, _c_ptr, _c_i64,
+                                             _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_dbl,
+                                             _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_backward_tiled_affine": (_c_int, [_c_ptr] * 11 + [_c_i64, _c_ptr, _c_ptr, _c_i64,
+                                              _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_dbl,
+                                              _c_dbl] + [_c_ptr] * 8 + [_c_i64, _c_ptr, _c_ptr,
+                                                                        _c_size, _c_ptr]),
+    "ssplat_affine_workspace_bytes": (_c_int, [_c_i64, _c_int, _c_int, _c_int,
+                                               ctypes.POINTER(_c_size)]),
+    "ssplat_affine_grad_slots_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
     "ssplat_sspl1_workspace_bytes": (_c_int, [_c_i64, ctypes.POINTER(_c_size)]),
     "ssplat_sspl1_decode": (_c_int, [_c_ptr] * 11 + [_c_ptr, _c_size, _c_ptr]),
     "ssplat_gaussian_activity": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int,
@@ -212,6 +222,20 @@ def grad_slots_bytes(slot_capacity: int) -> int:
     out = _c_size(0)
     check(load().ssplat_grad_slots_bytes(int(slot_capacity), ctypes.byref(out)),
           "ssplat_grad_slots_bytes")
+    return int(out.value)
+
+
+def affine_workspace_bytes(n: int, height: int, width: int, tile_size: int) -> int:
+    out = _c_size(0)
+    check(load().ssplat_affine_workspace_bytes(int(n), int(height), int(width), int(tile_size),
+                                               ctypes.byref(out)), "ssplat_affine_workspace_bytes")
+    return int(out.value)
+
+
+def affine_grad_slots_bytes(slot_capacity: int) -> int:
+    out = _c_size(0)
+    check(load().ssplat_affine_grad_slots_bytes(int(slot_capacity), ctypes.byref(out)),
+          "ssplat_affine_grad_slots_bytes")
     return int(out.value)
 
 

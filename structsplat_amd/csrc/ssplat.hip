@@ -132,6 +132,9 @@ constexpr int kBwdStage16 = 128;
 // depend on scheduling.  A slot whose tag is not the current call's (a tile the lists do not
 // hold, e.g. culled) counts as zero.
 constexpr int kSlotWords = 10;            // 9 partial gradients + the call's tag
+// Affine colors add 6 color-gradient, 2 scale and 1 rotation partials: 18 values + padding +
+// the tag (word 19).
+constexpr int kSlotWordsAffine = 20;
 constexpr size_t kSlotHeaderBytes = 256;  // slot buffer header: the call counter (uint32)
 constexpr int kLaneSumMax = 64;           // Gaussians with up to this many slots: one lane each
 constexpr int kFieldWaves = 4;            // waves per 256-thread workgroup of per-Gaussian sweeps
@@ -935,8 +938,17 @@ __global__ __launch_bounds__(256) void k_prepare(
   recs[i] = g;
 }
 
+// Affine colors (render.py:110-123): per Gaussian three 16-byte chunks
+//   a0 = (cos t / sx, sin t / sx, cos t / sy, sin t / sy)   local x = a0.x dx + a0.y dy,
+//                                                          local y = a0.z dy - a0.w dx
+//   a1 = (G00, G01, G02, G10), a2 = (G11, G12, 1/sx, 1/sy)  color_k = c_k + G0k x + G1k y
+// with sx, sy clamped below at 1e-6 like the reference; a2.z (a2.w) is stored negated where
+// the clamp holds, whose scale then gets no gradient.
+constexpr int kAffineChunks = 3;
+
 struct RenderArgs {
   const GaussRec* recs;
+  const float4* arecs;      // affine records (kAffineChunks per Gaussian), or NULL
   const int32_t* rect_off;  // first slot of each wave of 64 Gaussians (backward only)
   const int32_t* gids;
   const int32_t* offsets;
@@ -970,23 +982,31 @@ struct StageLds {
 // Coalesced staging of up to kStage records of the tile list into LDS (kChunks 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
-template <int kLoad, int kChunks, int kS>
+// kAffine: the affine chunks follow in st.c[4 .. 4 + kAffineChunks).  kSlot: the gradient
+// sweep's records carry the Gaussian's absolute first slot in c3.w.
+template <int kLoad, int kChunks, int kS, bool kAffine = false, bool kSlot = false>
 __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                             int64_t batch, int m) {
   static_assert(kLoad <= kChunks, "staging more chunks than the buffer holds");
+  static_assert(!kAffine || (kLoad == 4 && kChunks == 4 + kAffineChunks), "affine layout");
   if (static_cast<int>(threadIdx.x) < m) {
     const int g = a.gids[batch + threadIdx.x];
     if (g >= 0 && g < a.n) {
       const float4* src = reinterpret_cast<const float4*>(a.recs + g);
 #pragma unroll
       for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = src[c];
-      if constexpr (kLoad == 4) {
-        // The gradient sweep's records carry the Gaussian's absolute first slot.
+      if constexpr (kSlot) {
         st.c[3][threadIdx.x].w = __int_as_float(__float_as_int(src[3].w) + a.rect_off[g >> 6]);
+      }
+      if constexpr (kAffine) {
+#pragma unroll
+        for (int c = 0; c < kAffineChunks; ++c) {
+          st.c[4 + c][threadIdx.x] = a.arecs[int64_t(g) * kAffineChunks + c];
+        }
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
 }
@@ -1013,7 +1033,7 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // it skip the pixel test, and for the others each lane reads its bit of the mask.
 // kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
 // recomputation); it is the same sum, in the same order, as the forward's D.
-template <bool kFade, bool kDenOnly, int kChunks, int kS>
+template <bool kFade, bool kDenOnly, int kChunks, int kS, bool kAffine = false>
 __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                                     uint32_t* s_mask, int tile, int qx0, int qy0,
                                                     int px, int py, float tail, float4& acc) {
@@ -1025,7 +1045,11 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   float nr = 0.0f, ng = 0.0f, nb = 0.0f, d = 0.0f;
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    stage_batch<3>(a, st, batch, m);
+    if constexpr (kAffine && !kDenOnly) {
+      stage_batch<4, kChunks, kS, true>(a, st, batch, m);
+    } else {
+      stage_batch<3>(a, st, batch, m);
+    }
     __syncthreads();
     uint64_t hit[kS / 64], full[kS / 64];
 #pragma unroll
@@ -1065,7 +1089,19 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
           const float e = fmaf(t, dx, v1.x * dy * dy);
           const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
           const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
-          if (!kDenOnly) {
+          if constexpr (kAffine && !kDenOnly) {
+            // w (c_k + G0k x + G1k y) with the pixel's local coordinates x, y.
+            const float4 v3 = st.c[3][j];
+            const float4 a0 = st.c[4][j];
+            const float4 a1 = st.c[5][j];
+            const float4 a2 = st.c[6][j];
+            const float lxc = fmaf(a0.x, dx, a0.y * dy);
+            const float lyc = fmaf(a0.z, dy, -(a0.w * dx));
+            const float w = base * v2.w;
+            nr = fmaf(w, fmaf(a1.x, lxc, fmaf(a1.w, lyc, v3.x)), nr);
+            ng = fmaf(w, fmaf(a1.y, lxc, fmaf(a2.x, lyc, v3.y)), ng);
+            nb = fmaf(w, fmaf(a1.z, lxc, fmaf(a2.y, lyc, v3.z)), nb);
+          } else if constexpr (!kDenOnly) {
             nr = fmaf(base, v2.x, nr);
             ng = fmaf(base, v2.y, ng);
             nb = fmaf(base, v2.z, nb);
@@ -1079,11 +1115,11 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   acc = make_float4(nr, ng, nb, d);
 }
 
-template <int kT, bool kFade, bool kNormalize>
+template <int kT, bool kFade, bool kNormalize, bool kAffine = false>
 __global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __restrict__ out,
                                                      float* __restrict__ den) {
   constexpr int kQ = kT / 8;  // quadrants per tile row
-  __shared__ StageLds<3> st;
+  __shared__ StageLds<kAffine ? 4 + kAffineChunks : 3> st;
   __shared__ uint32_t s_mask[kQ * kQ][2 * kStage];  // per-wave quadrant masks
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
@@ -1098,7 +1134,8 @@ __global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __rest
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
-  accumulate_quadrant<kFade, false>(a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
+  accumulate_quadrant<kFade, false, kAffine ? 4 + kAffineChunks : 3, kStage, kAffine>(
+      a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
     const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
     if (den) den[flat] = acc.w;
@@ -1130,7 +1167,7 @@ constexpr int kNormSaved = 0;      // both read from the forward's saved outputs
 constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-only sweep
 constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
 
-template <int kT, bool kFade, bool kNormalize, int kNorm>
+template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine = false>
 // Eight waves per SIMD (at most 64 VGPRs): with 16-pixel tiles that is eight resident
 // workgroups per CU, whose different phases (list sweeps, reductions) overlap.
 __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_backward(
@@ -1140,7 +1177,9 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
   constexpr int kWaves = kQ * kQ;
   constexpr int kS = kT == 16 ? kBwdStage16 : kStage;  // records per staged batch
   constexpr int kMaskF4 = kWaves * 2 * kS / 4;          // the sweep's masks, in float4 units
-  __shared__ StageLds<4, kS> st;
+  constexpr int kChunks = kAffine ? 4 + kAffineChunks : 4;
+  constexpr int kWords = kAffine ? kSlotWordsAffine : kSlotWords;
+  __shared__ StageLds<kChunks, kS> st;
   // The per-pixel adjoints; when recomputing, the same space first holds the sweep's per-wave
   // quadrant masks (two uint32 per staged record and wave).
   __shared__ float4 s_scratch[kRecompute && kMaskF4 > kT * kT ? kMaskF4 : kT * kT];
@@ -1169,7 +1208,8 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
     const int py = qy0 + ly;
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kS);
     float4 acc;
-    accumulate_quadrant<kFade, kNorm == kNormRecompD>(a, st, s_mask, tile, qx0, qy0, px, py,
+    accumulate_quadrant<kFade, kNorm == kNormRecompD, kChunks, kS, kAffine>(
+        a, st, s_mask, tile, qx0, qy0, px, py,
                                                       tail, acc);
     // Every wave is past the sweep's last barrier: the mask space can take the adjoints.
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1237,7 +1277,7 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    stage_batch<4>(a, st, batch, m);
+    stage_batch<4, kChunks, kS, kAffine, true>(a, st, batch, m);
     __syncthreads();
     for (int j = wave; j < m; j += kWaves) {
       const float4 v0 = st.c[0][j];
@@ -1251,6 +1291,15 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
       const float cr = v3.x, cg = v3.y, cbl = v3.z;
       float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
       float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
+      // Affine colors: the frame and color gradients, and their partials (unused otherwise).
+      float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0, a2 = a0;
+      if constexpr (kAffine) {
+        a0 = st.c[4][j];
+        a1 = st.c[5][j];
+        a2 = st.c[6][j];
+      }
+      float g00 = 0.0f, g01 = 0.0f, g02 = 0.0f, g10 = 0.0f, g11 = 0.0f, g12 = 0.0f;
+      float sa = 0.0f, sb = 0.0f, cxx = 0.0f, cyy = 0.0f, cxy = 0.0f, cyx = 0.0f;
       // One visit: pixel (px, py) inside the rectangle, with its adjoint.
       auto visit = [&](int px, int py, const float4& ad) {
         const float dx = static_cast<float>(px) - mx;
@@ -1258,7 +1307,16 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
         const float t = fmaf(qa, dx, qb * dy);
         const float e = fmaf(t, dx, qc * dy * dy);
         const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
-        const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
+        float lxc = 0.0f, lyc = 0.0f;
+        float pr = cr, pg = cg, pb = cbl;  // the pixel's color
+        if constexpr (kAffine) {
+          lxc = fmaf(a0.x, dx, a0.y * dy);
+          lyc = fmaf(a0.z, dy, -(a0.w * dx));
+          pr = fmaf(a1.x, lxc, fmaf(a1.w, lyc, cr));
+          pg = fmaf(a1.y, lxc, fmaf(a2.x, lyc, cg));
+          pb = fmaf(a1.z, lxc, fmaf(a2.y, lyc, cbl));
+        }
+        const float dw = fmaf(ad.x, pr, fmaf(ad.y, pg, fmaf(ad.z, pb, ad.w)));
         float base, dqr;
         if (kFade) {
           base = fmaxf(raw - tail, 0.0f);
@@ -1277,6 +1335,26 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
         sxx = fmaf(dqx, dx, sxx);
         sxy = fmaf(dqx, dy, sxy);
         syy = fmaf(dqr * dy, dy, syy);
+        if constexpr (kAffine) {
+          // Color-gradient partials sum base * local * dL/dN; the local-coordinate adjoints
+          // base * (dL/dN . G0) and base * (dL/dN . G1) feed the mean, scale and rotation
+          // gradients (times the opacity, applied per pair).
+          const float bl = base * lxc, bly = base * lyc;
+          g00 = fmaf(ad.x, bl, g00);
+          g01 = fmaf(ad.y, bl, g01);
+          g02 = fmaf(ad.z, bl, g02);
+          g10 = fmaf(ad.x, bly, g10);
+          g11 = fmaf(ad.y, bly, g11);
+          g12 = fmaf(ad.z, bly, g12);
+          const float ax = base * fmaf(ad.x, a1.x, fmaf(ad.y, a1.y, ad.z * a1.z));
+          const float ay = base * fmaf(ad.x, a1.w, fmaf(ad.y, a2.x, ad.z * a2.y));
+          sa += ax;
+          sb += ay;
+          cxx = fmaf(ax, lxc, cxx);
+          cyy = fmaf(ay, lyc, cyy);
+          cxy = fmaf(ax, lyc, cxy);
+          cyx = fmaf(ay, lxc, cyx);
+        }
       };
       const unsigned rw = static_cast<unsigned>(x1 - x0), rh = static_cast<unsigned>(y1 - y0);
       if constexpr (kT == 16) {
@@ -1311,7 +1389,17 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
       const float vals[8] = {sx, sy, sxx, sxy, syy, gcr, gcg, gcb};
       const float red = wave_sum8_transposed(vals, hi8);
       const float partner = dpp_move<0x128>(red);  // lane 7 <-> lane 15: Sx and Sy meet
-      const float gop_sum = has_opacity ? wave_sum_lane63(gop) : 0.0f;
+      float gop_sum, red2 = 0.0f, red3 = 0.0f;
+      if constexpr (kAffine) {
+        const float vals2[8] = {g00, g01, g02, g10, g11, g12, sa, sb};
+        const float vals3[8] = {cxx, cyy, cxy, cyx, gop, 0.0f, 0.0f, 0.0f};
+        red2 = wave_sum8_transposed(vals2, hi8);
+        red3 = wave_sum8_transposed(vals3, hi8);
+        gop_sum = has_opacity ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 39))
+                              : 0.0f;
+      } else {
+        gop_sum = has_opacity ? wave_sum_lane63(gop) : 0.0f;
+      }
       // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries:
       //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
       //   g_c = -op/2 Syy, g_color = op C.
@@ -1322,7 +1410,28 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
       const float ca = qa * unfold, cc = qc * unfold, cb = (0.5f * unfold) * qb;
       const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
       const float cross = comp < 2 ? partner * cb : 0.0f;
-      const float gval = op * fmaf(red, own_scale, cross);
+      float gval = op * fmaf(red, own_scale, cross);
+      float extra[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // affine: scale x, scale y, rotation
+      if constexpr (kAffine) {
+        // Local coordinates x = (c dx + s dy) isx, y = (c dy - s dx) isy (isx = 1/sx), d = p - mean,
+        // with A, B the sums of the local-coordinate adjoints and Cuv = sum (adjoint u) * v:
+        //   dL/dmean = -op (A (a0.x, a0.y) + B (-a0.w, a0.z));
+        //   dL/dsx = -op Cxx isx (zero where the 1e-6 clamp holds), likewise sy;
+        //   dL/dtheta = op (Cxy isx / isy - Cyx isy / isx).
+        const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red2), 55));
+        const float B = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red2), 63));
+        const float Cxx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 7));
+        const float Cyy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 15));
+        const float Cxy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 23));
+        const float Cyx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 31));
+        if (comp == 0) gval -= op * fmaf(a0.x, A, -(a0.w * B));
+        if (comp == 1) gval -= op * fmaf(a0.y, A, a0.z * B);
+        // a2.z / a2.w carry 1/s, negated where the clamp holds (s < 1e-6).
+        const float isx = fabsf(a2.z), isy = fabsf(a2.w);
+        extra[0] = a2.z < 0.0f ? 0.0f : -op * Cxx * isx;
+        extra[1] = a2.w < 0.0f ? 0.0f : -op * Cyy * isy;
+        extra[2] = op * (Cxy * (isx / isy) - Cyx * (isy / isx));
+      }
       // The pair's slot: the Gaussian's first slot plus this tile's row-major index in its
       // rectangle (uniform values).  A list entry outside the rectangle has no pixel inside
       // and no slot.
@@ -1332,9 +1441,20 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
         const int64_t slot = int64_t(__builtin_amdgcn_readfirstlane(__float_as_int(v3.w))) +
                              ky * rt.tw + kx;
         if (slot < g.slot_capacity) {
-          float* dst = g.slots + slot * kSlotWords;
+          float* dst = g.slots + slot * kWords;
           if (result_lane) dst[comp] = gval;
-          if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
+          if constexpr (kAffine) {
+            if (result_lane && comp < 6) dst[9 + comp] = op * red2;
+            if (lane == 63) {
+              dst[8] = gop_sum;
+              dst[15] = extra[0];
+              dst[16] = extra[1];
+              dst[17] = extra[2];
+              dst[kWords - 1] = tag;
+            }
+          } else {
+            if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
+          }
         }
       }
     }
@@ -1355,37 +1475,40 @@ struct ReduceArgs {
   float* grad_means;
   float* grad_conics;
   float* grad_colors;
-  float* grad_opacities;  // NULL when the field has no opacities
-  int32_t* status;        // may be NULL
+  float* grad_opacities;    // NULL when the field has no opacities
+  float* grad_color_grads;  // affine only: (n, 2, 3)
+  float* grad_scales;       // affine only: (n, 2)
+  float* grad_rotations;    // affine only: (n,)
+  int32_t* status;          // may be NULL
 };
 
-// Adds one slot's nine partial gradients when its tag is the call's (a slot the call did not
-// write counts as zero; the select keeps stale values out of the sum).
-__device__ __forceinline__ void add_slot_values(const float2 (&v)[5], uint32_t tag,
-                                                float (&acc)[9]) {
-  const bool live = __float_as_uint(v[4].y) == tag;
-  acc[0] += live ? v[0].x : 0.0f;
-  acc[1] += live ? v[0].y : 0.0f;
-  acc[2] += live ? v[1].x : 0.0f;
-  acc[3] += live ? v[1].y : 0.0f;
-  acc[4] += live ? v[2].x : 0.0f;
-  acc[5] += live ? v[2].y : 0.0f;
-  acc[6] += live ? v[3].x : 0.0f;
-  acc[7] += live ? v[3].y : 0.0f;
-  acc[8] += live ? v[4].x : 0.0f;
+// Adds one slot's kVals partial gradients when its tag (word kWords - 1) is the call's: a slot
+// the call did not write counts as zero, and the select keeps stale values out of the sum.
+template <int kWords, int kVals>
+__device__ __forceinline__ void add_slot_values(const float2 (&v)[kWords / 2], uint32_t tag,
+                                                float (&acc)[kVals]) {
+  const bool live = __float_as_uint(v[kWords / 2 - 1].y) == tag;
+#pragma unroll
+  for (int c = 0; c < kVals; ++c) {
+    const float x = (c & 1) ? v[c >> 1].y : v[c >> 1].x;
+    acc[c] += live ? x : 0.0f;
+  }
 }
 
-__device__ __forceinline__ void add_slot(const float* s, uint32_t tag, float (&acc)[9]) {
-  float2 v[5];
+template <int kWords, int kVals>
+__device__ __forceinline__ void add_slot(const float* s, uint32_t tag, float (&acc)[kVals]) {
+  float2 v[kWords / 2];
 #pragma unroll
-  for (int w = 0; w < 5; ++w) v[w] = *reinterpret_cast<const float2*>(s + 2 * w);
-  add_slot_values(v, tag, acc);
+  for (int w = 0; w < kWords / 2; ++w) v[w] = *reinterpret_cast<const float2*>(s + 2 * w);
+  add_slot_values<kWords, kVals>(v, tag, acc);
 }
 
 // Gradients of every Gaussian from its slots, in an order fixed by the Gaussian alone: up to
 // kLaneSumMax slots are summed by one lane in ascending slot order; more are summed by the whole
 // wave, lane l taking slots l, l + 64, ... in order, then a fixed DPP tree.  Every Gaussian's
-// outputs are written (zero without slots).
+// outputs are written (zero without slots).  kVals = 9 (means, conics, colors, opacity) or 18
+// (plus color gradients, scales and rotation with affine colors).
+template <int kWords, int kVals>
 __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -1401,24 +1524,26 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     const int64_t room = a.slot_capacity - base;
     count = room <= 0 ? 0 : (room < count ? static_cast<int>(room) : count);
   }
-  float acc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  float acc[kVals];
+#pragma unroll
+  for (int c = 0; c < kVals; ++c) acc[c] = 0.0f;
   if (count <= kLaneSumMax) {
     // Four slots' loads in flight per step; the additions stay in ascending slot order.
-    const float* sl = a.slots + base * kSlotWords;
+    const float* sl = a.slots + base * kWords;
     int k = 0;
     for (; k + 4 <= count; k += 4) {
-      float2 v[4][5];
+      float2 v[4][kWords / 2];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
 #pragma unroll
-        for (int w = 0; w < 5; ++w) {
-          v[u][w] = *reinterpret_cast<const float2*>(sl + (k + u) * kSlotWords + 2 * w);
+        for (int w = 0; w < kWords / 2; ++w) {
+          v[u][w] = *reinterpret_cast<const float2*>(sl + (k + u) * kWords + 2 * w);
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) add_slot_values(v[u], tag, acc);
+      for (int u = 0; u < 4; ++u) add_slot_values<kWords, kVals>(v[u], tag, acc);
     }
-    for (; k < count; ++k) add_slot(sl + k * kSlotWords, tag, acc);
+    for (; k < count; ++k) add_slot<kWords, kVals>(sl + k * kWords, tag, acc);
   }
   uint64_t big = __ballot(count > kLaneSumMax);
   while (big) {
@@ -1426,10 +1551,14 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     big &= big - 1;
     const int cb = __shfl(count, b, 64);
     const int64_t bb = __shfl(base, b, 64);
-    float part[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k = lane; k < cb; k += 64) add_slot(a.slots + (bb + k) * kSlotWords, tag, part);
+    float part[kVals];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) {
+    for (int c = 0; c < kVals; ++c) part[c] = 0.0f;
+    for (int k = lane; k < cb; k += 64) {
+      add_slot<kWords, kVals>(a.slots + (bb + k) * kWords, tag, part);
+    }
+#pragma unroll
+    for (int c = 0; c < kVals; ++c) {
       const float total = __shfl(wave_sum_lane63(part[c]), 63, 64);
       if (lane == b) acc[c] = total;
     }
@@ -1444,11 +1573,38 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     a.grad_colors[3 * i + 1] = acc[6];
     a.grad_colors[3 * i + 2] = acc[7];
     if (a.grad_opacities != nullptr) a.grad_opacities[i] = acc[8];
+    if constexpr (kVals > 9) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.grad_color_grads[6 * i + c] = acc[9 + c];
+      a.grad_scales[2 * i + 0] = acc[15];
+      a.grad_scales[2 * i + 1] = acc[16];
+      a.grad_rotations[i] = acc[17];
+    }
   }
   if (i == 0 && a.status != nullptr &&
       (*a.rect_status != SSPLAT_OK || int64_t(a.rect_off[a.num_waves]) > a.slot_capacity)) {
     *a.status = SSPLAT_ERR_CAPACITY;
   }
+}
+
+// Affine records (see RenderArgs) from scales (n,2), rotations (n,) and color gradients
+// (n,2,3), in float32 like the reference's torch expression (render.py:110-123).
+__global__ __launch_bounds__(256) void k_prepare_affine(const float* __restrict__ scales,
+                                                        const float* __restrict__ rotations,
+                                                        const float* __restrict__ color_grads,
+                                                        int n, float4* __restrict__ arecs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float sx = scales[2 * i + 0], sy = scales[2 * i + 1];
+  const float isx = 1.0f / fmaxf(sx, 1e-6f);
+  const float isy = 1.0f / fmaxf(sy, 1e-6f);
+  const float t = rotations[i];
+  const float ct = cosf(t), st = sinf(t);
+  const float* G = color_grads + 6 * int64_t(i);
+  arecs[3 * int64_t(i) + 0] = make_float4(ct * isx, st * isx, ct * isy, st * isy);
+  arecs[3 * int64_t(i) + 1] = make_float4(G[0], G[1], G[2], G[3]);
+  arecs[3 * int64_t(i) + 2] = make_float4(G[4], G[5], sx < 1e-6f ? -isx : isx,
+                                          sy < 1e-6f ? -isy : isy);
 }
 
 // Number of slots (support-rectangle tiles) of every Gaussian into rect_wave, per wave of 64,
@@ -1833,6 +1989,7 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
                             double fade_alpha, double sigma_cutoff) {
   RenderArgs r;
   r.recs = w.recs;
+  r.arecs = nullptr;
   r.rect_off = nullptr;
   r.gids = gids;
   r.offsets = offsets;
@@ -1872,56 +2029,101 @@ int run_prepare(const float* means, const float* conics, const float* colors,
 
 // Kernel selection for one tile edge: support fade, normalization and the backward's normalizer
 // source are template parameters (no per-pixel branches on them).
-template <int kT>
+template <int kT, bool kAffine = false>
 void launch_forward(bool fade, bool normalize, int tiles, const RenderArgs& ra, float* out,
                     float* den, hipStream_t s) {
   const dim3 grid(tiles), block(kT * kT);
   if (fade && normalize) {
-    k_forward<kT, true, true><<<grid, block, 0, s>>>(ra, out, den);
+    k_forward<kT, true, true, kAffine><<<grid, block, 0, s>>>(ra, out, den);
   } else if (fade) {
-    k_forward<kT, true, false><<<grid, block, 0, s>>>(ra, out, den);
+    k_forward<kT, true, false, kAffine><<<grid, block, 0, s>>>(ra, out, den);
   } else if (normalize) {
-    k_forward<kT, false, true><<<grid, block, 0, s>>>(ra, out, den);
+    k_forward<kT, false, true, kAffine><<<grid, block, 0, s>>>(ra, out, den);
   } else {
-    k_forward<kT, false, false><<<grid, block, 0, s>>>(ra, out, den);
+    k_forward<kT, false, false, kAffine><<<grid, block, 0, s>>>(ra, out, den);
   }
 }
 
-template <int kT, bool kFade>
+template <int kT, bool kFade, bool kAffine>
 void launch_backward_normalized(int norm, int tiles, const RenderArgs& ra, const GradArgs& ga,
                                 hipStream_t s) {
   const dim3 grid(tiles), block(kT * kT);
   if (norm == kNormSaved) {
-    k_backward<kT, kFade, true, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+    k_backward<kT, kFade, true, kNormSaved, kAffine><<<grid, block, 0, s>>>(ra, ga);
   } else if (norm == kNormRecompD) {
-    k_backward<kT, kFade, true, kNormRecompD><<<grid, block, 0, s>>>(ra, ga);
+    k_backward<kT, kFade, true, kNormRecompD, kAffine><<<grid, block, 0, s>>>(ra, ga);
   } else {
-    k_backward<kT, kFade, true, kNormRecompAll><<<grid, block, 0, s>>>(ra, ga);
+    k_backward<kT, kFade, true, kNormRecompAll, kAffine><<<grid, block, 0, s>>>(ra, ga);
   }
 }
 
-template <int kT>
+template <int kT, bool kAffine = false>
 void launch_backward(bool fade, bool normalize, int norm, int tiles, const RenderArgs& ra,
                      const GradArgs& ga, hipStream_t s) {
   const dim3 grid(tiles), block(kT * kT);
   if (!normalize) {
     if (fade) {
-      k_backward<kT, true, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+      k_backward<kT, true, false, kNormSaved, kAffine><<<grid, block, 0, s>>>(ra, ga);
     } else {
-      k_backward<kT, false, false, kNormSaved><<<grid, block, 0, s>>>(ra, ga);
+      k_backward<kT, false, false, kNormSaved, kAffine><<<grid, block, 0, s>>>(ra, ga);
     }
   } else if (fade) {
-    launch_backward_normalized<kT, true>(norm, tiles, ra, ga, s);
+    launch_backward_normalized<kT, true, kAffine>(norm, tiles, ra, ga, s);
   } else {
-    launch_backward_normalized<kT, false>(norm, tiles, ra, ga, s);
+    launch_backward_normalized<kT, false, kAffine>(norm, tiles, ra, ga, s);
   }
+}
+
+// Affine-color inputs (render.py:110-123): color gradients (n,2,3), scales (n,2), rotations (n,)
+// and, in the backward, their gradient outputs.  The affine records live in the workspace
+// after the plain layout (ssplat_affine_workspace_bytes).
+struct Affine {
+  const float* color_grads;
+  const float* scales;
+  const float* rotations;
+  float* grad_color_grads;
+  float* grad_scales;
+  float* grad_rotations;
+};
+
+size_t affine_record_bytes(int64_t n) {
+  return align_up(size_t(n) * kAffineChunks * sizeof(float4), 256);
+}
+
+// Checks the affine inputs and packs their records; *arecs gets their address.
+int prepare_affine(const Plan& p, const Affine& af, void* ws, size_t ws_bytes,
+                   const Workspace& w, bool backward, const float4** arecs, hipStream_t s) {
+  if (p.ts != 16) {
+    return set_error(SSPLAT_ERR_UNSUPPORTED, "affine colors render with tile_size 16 only");
+  }
+  if (p.n > 0 && (af.color_grads == nullptr || af.scales == nullptr ||
+                  af.rotations == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
+                     "affine color rendering requires scales and rotations");
+  }
+  if (backward && p.n > 0 && (af.grad_color_grads == nullptr || af.grad_scales == nullptr ||
+                              af.grad_rotations == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "affine gradient outputs must not be NULL");
+  }
+  if (ws_bytes < w.bytes + affine_record_bytes(p.n)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace has %zu bytes, needs %zu", ws_bytes,
+                     w.bytes + affine_record_bytes(p.n));
+  }
+  float4* a = reinterpret_cast<float4*>(static_cast<char*>(ws) + w.bytes);
+  if (p.n > 0) {
+    k_prepare_affine<<<static_cast<int>((p.n + 255) / 256), 256, 0, s>>>(
+        af.scales, af.rotations, af.color_grads, static_cast<int>(p.n), a);
+    SSPLAT_LAUNCH_CHECK("k_prepare_affine");
+  }
+  *arecs = a;
+  return SSPLAT_OK;
 }
 
 int forward_tiled(const Plan& p, const float* means, const float* conics, const float* colors,
                   const int64_t* radii, const float* opacities, const int32_t* gids,
                   const int32_t* offsets, int64_t num_gids, int normalize, double eps,
                   double support_fade_alpha, double sigma_cutoff, float* out, float* den,
-                  void* ws, size_t ws_bytes, void* stream) {
+                  void* ws, size_t ws_bytes, void* stream, const Affine* af = nullptr) {
   int rc;
   double fade_alpha;
   if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
@@ -1934,15 +2136,21 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   Workspace w;
   if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const float4* arecs = nullptr;
+  if (af != nullptr && (rc = prepare_affine(p, *af, ws, ws_bytes, w, false, &arecs, s))) {
+    return rc;
+  }
   if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, nullptr,
                         s))) {
     return rc;
   }
-  const RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha,
-                                        sigma_cutoff);
+  RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
+  ra.arecs = arecs;
   hipEvent_t t0;
   const bool timed = timer_begin(0, s, &t0);
-  if (p.ts == 16) {
+  if (af != nullptr) {
+    launch_forward<16, true>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
+  } else if (p.ts == 16) {
     launch_forward<16>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
   } else {
     launch_forward<32>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
@@ -1968,7 +2176,7 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
                    double support_fade_alpha, double sigma_cutoff, float* grad_means,
                    float* grad_conics, float* grad_colors, float* grad_opacities,
                    void* grad_slots, int64_t slot_capacity, int32_t* status, void* ws,
-                   size_t ws_bytes, void* stream) {
+                   size_t ws_bytes, void* stream, const Affine* af = nullptr) {
   int rc;
   double fade_alpha;
   if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
@@ -1995,6 +2203,10 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   float* gop = (opacities != nullptr) ? grad_opacities : nullptr;
   uint32_t* epoch = static_cast<uint32_t*>(grad_slots);
   float* slots = reinterpret_cast<float*>(static_cast<char*>(grad_slots) + kSlotHeaderBytes);
+  const float4* arecs = nullptr;
+  if (af != nullptr && (rc = prepare_affine(p, *af, ws, ws_bytes, w, true, &arecs, s))) {
+    return rc;
+  }
   if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, epoch, s))) {
     return rc;
   }
@@ -2004,10 +2216,13 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   }
   RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
   ra.rect_off = w.rect_off;
+  ra.arecs = arecs;
   const GradArgs ga{grad_out, den, out, slots, slot_capacity, epoch, gop != nullptr};
   hipEvent_t t0;
   const bool timed = timer_begin(1, s, &t0);
-  if (p.ts == 16) {
+  if (af != nullptr) {
+    launch_backward<16, true>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
+  } else if (p.ts == 16) {
     launch_backward<16>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
   } else {
     launch_backward<32>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
@@ -2017,8 +2232,15 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   const ReduceArgs red{w.recs,      static_cast<int>(p.n), tile_lg(p.ts), w.num_waves,
                        w.rect_off,  w.rect_status,         slots,         slot_capacity,
                        epoch,       grad_means,            grad_conics,   grad_colors,
-                       gop,         status};
-  k_grad_reduce<<<static_cast<int>((p.n + 255) / 256), 256, 0, s>>>(red);
+                       gop,         af ? af->grad_color_grads : nullptr,
+                       af ? af->grad_scales : nullptr,      af ? af->grad_rotations : nullptr,
+                       status};
+  const int blocks = static_cast<int>((p.n + 255) / 256);
+  if (af != nullptr) {
+    k_grad_reduce<kSlotWordsAffine, 18><<<blocks, 256, 0, s>>>(red);
+  } else {
+    k_grad_reduce<kSlotWords, 9><<<blocks, 256, 0, s>>>(red);
+  }
   SSPLAT_LAUNCH_CHECK("k_grad_reduce");
   return success();
 }
@@ -2204,6 +2426,64 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
                         slot_capacity, status, workspace, workspace_bytes, stream);
+}
+
+int ssplat_forward_tiled_affine(const float* means, const float* conics, const float* colors,
+                                const int64_t* radii, const float* opacities,
+                                const float* color_grads, const float* scales,
+                                const float* rotations, int64_t n, const int32_t* tile_gids,
+                                const int32_t* tile_offsets, int64_t num_gids, int height,
+                                int width, int tile_size, int normalize, double eps,
+                                double support_fade_alpha, double sigma_cutoff, float* out,
+                                float* den, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  const Affine af{color_grads, scales, rotations, nullptr, nullptr, nullptr};
+  return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
+                       num_gids, normalize, eps, support_fade_alpha, sigma_cutoff, out, den,
+                       workspace, workspace_bytes, stream, &af);
+}
+
+int ssplat_backward_tiled_affine(
+    const float* grad_out, const float* means, const float* conics, const float* colors,
+    const int64_t* radii, const float* opacities, const float* color_grads, const float* scales,
+    const float* rotations, const float* den, const float* out, int64_t n,
+    const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids, int height,
+    int width, int tile_size, int normalize, double eps, double support_fade_alpha,
+    double sigma_cutoff, float* grad_means, float* grad_conics, float* grad_colors,
+    float* grad_opacities, float* grad_color_grads, float* grad_scales, float* grad_rotations,
+    void* grad_slots, int64_t slot_capacity, int32_t* status, void* workspace,
+    size_t workspace_bytes, void* stream) {
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  const Affine af{color_grads, scales, rotations, grad_color_grads, grad_scales, grad_rotations};
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
+                        tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
+                        grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
+                        slot_capacity, status, workspace, workspace_bytes, stream, &af);
+}
+
+int ssplat_affine_workspace_bytes(int64_t n, int height, int width, int tile_size,
+                                  size_t* bytes) {
+  if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  *bytes = layout(nullptr, p.n, p.num_tiles).bytes + affine_record_bytes(p.n);
+  return success();
+}
+
+int ssplat_affine_grad_slots_bytes(int64_t slot_capacity, size_t* bytes) {
+  if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  if (slot_capacity < 0 || slot_capacity > (int64_t(1) << 40)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "slot_capacity out of range");
+  }
+  *bytes = align_up(kSlotHeaderBytes + size_t(slot_capacity) * kSlotWordsAffine * sizeof(float),
+                    256);
+  return success();
 }
 
 int ssplat_grad_slots_bytes(int64_t slot_capacity, size_t* bytes) {

@@ -129,6 +129,18 @@ def workspace(n: int, height: int, width: int, tile_size: int, device) -> torch.
     return torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
 
 
+def affine_workspace(n: int, height: int, width: int, device) -> torch.Tensor:
+    """Scratch of the affine-color entry points (tile size 16)."""
+    nbytes = _native.affine_workspace_bytes(n, height, width, DEFAULT_TILE_SIZE)
+    return torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+
+
+def affine_grad_slots(slot_capacity: int, device) -> torch.Tensor:
+    """Zero-filled gradient slot buffer of the affine-color backward."""
+    nbytes = _native.affine_grad_slots_bytes(max(int(slot_capacity), 0))
+    return torch.zeros(nbytes, dtype=torch.uint8, device=device)
+
+
 def grad_slots(slot_capacity: int, device) -> torch.Tensor:
     """Zero-filled gradient slot buffer for deterministic backwards with up to `slot_capacity`
     slots (one per support-rectangle tile of each Gaussian); reusable across calls."""
@@ -286,6 +298,53 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, 
     return g_means, g_conics, g_colors, g_opac
 
 
+def forward_tiled_affine(means, conics, colors, radii, opacities, color_grads, scales, rotations,
+                         gids, offsets, H: int, W: int, normalize: bool, eps: float,
+                         support_fade: bool, sigma_cutoff: float, den=None, ws=None):
+    """Tiled forward with affine colors (tile size 16); returns ``(out (H,W,3), den or None)``.
+    ``ws`` comes from ``affine_workspace``."""
+    lib = _native.load()
+    dev = means.device
+    n = means.shape[0]
+    out = torch.empty(int(H), int(W), 3, dtype=torch.float32, device=dev)
+    if ws is None:
+        ws = affine_workspace(n, H, W, dev)
+    _native.check(lib.ssplat_forward_tiled_affine(
+        _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities), _ptr(color_grads),
+        _ptr(scales), _ptr(rotations), n, _ptr(gids), _ptr(offsets), gids.numel(), int(H),
+        int(W), DEFAULT_TILE_SIZE, int(bool(normalize)), float(eps), _fade_alpha(support_fade),
+        float(sigma_cutoff), _ptr(out), _ptr(den), _ptr(ws), ws.numel(), _stream(dev)),
+        "ssplat_forward_tiled_affine")
+    return out, den
+
+
+def backward_tiled_affine(grad_out, means, conics, colors, radii, opacities, color_grads, scales,
+                          rotations, den, out, gids, offsets, H: int, W: int, normalize: bool,
+                          eps: float, support_fade: bool, sigma_cutoff: float, slot_capacity: int,
+                          ws=None, slots=None, status=None):
+    """Backward of ``forward_tiled_affine``; returns the gradients of means, conics, colors,
+    opacities (or None), color_grads, scales and rotations.  Deterministic."""
+    lib = _native.load()
+    dev = means.device
+    n = means.shape[0]
+    has_opacity = opacities is not None and opacities.numel() > 0
+    grads = (torch.empty_like(means), torch.empty_like(conics), torch.empty_like(colors),
+             torch.empty_like(opacities) if has_opacity else None, torch.empty_like(color_grads),
+             torch.empty_like(scales), torch.empty_like(rotations))
+    if ws is None:
+        ws = affine_workspace(n, H, W, dev)
+    if slots is None:
+        slots = affine_grad_slots(slot_capacity, dev)
+    _native.check(lib.ssplat_backward_tiled_affine(
+        _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
+        _ptr(color_grads), _ptr(scales), _ptr(rotations), _ptr(den), _ptr(out), n, _ptr(gids),
+        _ptr(offsets), gids.numel(), int(H), int(W), DEFAULT_TILE_SIZE, int(bool(normalize)),
+        float(eps), _fade_alpha(support_fade), float(sigma_cutoff), *(_ptr(g) for g in grads),
+        _ptr(slots), int(slot_capacity), _ptr(status), _ptr(ws), ws.numel(), _stream(dev)),
+        "ssplat_backward_tiled_affine")
+    return grads
+
+
 class _ExactRenderHip(Function):
     """Autograd boundary of the HIP renderer (the role of ``_ExactRenderCuda``,
     cuda_render.py:52-174): gradients for means, conics, colors and opacities only.
@@ -349,6 +408,81 @@ class _ExactRenderHip(Function):
             None,
             g_opac,
         ) + none_tail
+
+
+class _AffineRenderHip(Function):
+    """Autograd boundary of the affine-color render (render.py:110-123 inside the reference's
+    torch renderer): gradients for means, conics, colors, opacities, color_grads, scales and
+    rotations.  The normalizer is recomputed in the backward like ``_ExactRenderHip``'s."""
+
+    @staticmethod
+    def forward(ctx, means, conics, colors, radii, opacities, color_grads, scales, rotations,
+                height, width, normalize, eps, support_fade, sigma_cutoff):
+        means, conics, colors = means.contiguous(), conics.contiguous(), colors.contiguous()
+        color_grads = color_grads.contiguous()
+        scales, rotations = scales.contiguous(), rotations.contiguous()
+        radii = _radii64(radii)
+        opacities = opacities.contiguous()
+        alpha = _fade_alpha(support_fade)
+        cull = alpha == 1.0
+        H, W = int(height), int(width)
+        gids, offsets = build_tile_index(means, conics, radii, H, W, DEFAULT_TILE_SIZE, cull,
+                                         float(sigma_cutoff))
+        ws = affine_workspace(means.shape[0], H, W, means.device)
+        ctx.slot_capacity = (rect_pair_count(means, radii, H, W, DEFAULT_TILE_SIZE, ws)
+                             if cull else gids.numel())
+        out, _ = forward_tiled_affine(means, conics, colors, radii, opacities, color_grads,
+                                      scales, rotations, gids, offsets, H, W, bool(normalize),
+                                      float(eps), alpha, float(sigma_cutoff), ws=ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, color_grads, scales,
+                              rotations, out, gids, offsets)
+        ctx.args = (H, W, bool(normalize), float(eps), alpha, float(sigma_cutoff))
+        return out
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_out):
+        (means, conics, colors, radii, opacities, color_grads, scales, rotations, out, gids,
+         offsets) = ctx.saved_tensors
+        needs = ctx.needs_input_grad[:8]
+        none_tail = (None,) * 6
+        if not any(needs):
+            return (None,) * 8 + none_tail
+        H, W, normalize, eps, alpha, sigma = ctx.args
+        g = backward_tiled_affine(grad_out.contiguous(), means, conics, colors, radii, opacities,
+                                  color_grads, scales, rotations, None,
+                                  out if normalize else None, gids, offsets, H, W, normalize,
+                                  eps, alpha, sigma, ctx.slot_capacity)
+        g_means, g_conics, g_colors, g_opac, g_cg, g_scales, g_rot = g
+        if opacities.numel() == 0:
+            g_opac = None
+        grads = (g_means, g_conics, g_colors, None, g_opac, g_cg, g_scales, g_rot)
+        return tuple(x if need else None for x, need in zip(grads, needs)) + none_tail
+
+
+def render_hip_affine(means, conics, colors, radii, H: int, W: int, color_grads, scales,
+                      rotations, opacities=None, normalize: bool = True, eps: float = 1e-8,
+                      support_fade: bool | float = False, sigma_cutoff: float = 3.0):
+    """Normalized or additive render with affine colors c_i + G_i0 x + G_i1 y in each Gaussian's
+    scaled, rotated frame (render.py:110-123); ``scales`` (N,2) are linear scales, ``rotations``
+    (N,) angles, ``color_grads`` (N,2,3)."""
+    _check_field(means, conics, colors, radii, opacities)
+    if scales is None or rotations is None:
+        raise ValueError("affine color rendering requires scales and rotations")
+    n = means.shape[0]
+    if color_grads.shape != (n, 2, 3) or scales.shape != (n, 2) or rotations.shape != (n,):
+        raise ValueError("color_grads must be (N, 2, 3), scales (N, 2) and rotations (N,)")
+    if any(t.dtype != torch.float32 or t.device != means.device
+           for t in (color_grads, scales, rotations)):
+        raise RuntimeError("color_grads, scales and rotations must be float32 on the means' device")
+    _check_image(H, W, DEFAULT_TILE_SIZE)
+    if normalize:
+        eps = _validate_normalization_eps(eps)
+    if opacities is None:
+        opacities = means.new_empty(0)
+    return _AffineRenderHip.apply(means, conics, colors, radii, opacities, color_grads, scales,
+                                  rotations, int(H), int(W), bool(normalize), float(eps),
+                                  _fade_alpha(support_fade), float(sigma_cutoff))
 
 
 def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=None,
@@ -450,8 +584,10 @@ def render_field(means, conics, colors, radii, H: int, W: int,
 
     ``chunk`` and ``checkpoint_chunks`` bound the reference renderer's memory and have no effect
     here.  Fractional fade strengths run in the HIP kernels (the reference falls back to torch for
-    them, render.py:341-358).  Affine colors (``color_grads``) are not implemented by the HIP
-    kernels and raise ``ValueError`` instead of silently changing the renderer.
+    them, render.py:341-358), and so do affine colors (``color_grads`` with ``scales`` and
+    ``rotations``), which the reference renders with its torch loop for the normalized modes and
+    ``additive`` (render.py:359-374); the fixed-fade additive kernel modes reject them as the
+    reference does.
     """
     del chunk, checkpoint_chunks
     fade_alpha = _resolve_support_fade_alpha(support_fade, support_fade_alpha)
@@ -470,9 +606,16 @@ def render_field(means, conics, colors, radii, H: int, W: int,
     if not means.is_cuda:
         raise RuntimeError(_DEVICE_MESSAGE)
     if color_grads is not None:
+        if mode in _NORMALIZED_MODES:
+            eps = _validate_normalization_eps(normalization_eps)
+            return render_hip_affine(means, conics, colors, radii, H, W, color_grads, scales,
+                                     rotations, opacities, True, eps, fade_alpha, sigma_cutoff)
+        if mode == "additive" or fade_alpha not in (0.0, 1.0):
+            return render_hip_affine(means, conics, colors, radii, H, W, color_grads, scales,
+                                     rotations, opacities, False, _EPS, fade_alpha, sigma_cutoff)
         raise ValueError(
-            "affine color rendering (color_grads) is not implemented by the HIP renderer; "
-            f"got renderer={mode!r}"
+            "affine color rendering is currently supported by the reference normalized/additive "
+            f"renderers only; got renderer={mode!r}"
         )
     if mode in _NORMALIZED_MODES:
         eps = _validate_normalization_eps(normalization_eps)

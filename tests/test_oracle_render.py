@@ -246,3 +246,63 @@ def test_radii_positive_and_bound_the_cutoff_ellipse():
         cov = np.linalg.inv(_explicit_inv_cov(scales[i], angles[i]))
         assert radii[i, 0] >= 3.0 * np.sqrt(cov[0, 0]) - 1e-3
         assert radii[i, 1] >= 3.0 * np.sqrt(cov[1, 1]) - 1e-3
+
+
+def test_affine_colors_zero_gradients_equal_plain_render():
+    f = random_field(41, 30, 20, 24, scale_lo=0.8, scale_hi=4.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, "cpu")
+    scales = torch.tensor(f["scales"])
+    rot = torch.tensor(f["angles"])
+    plain = R.render(means, conics, colors, radii, 20, 24, opacities=op)
+    affine = R.render(means, conics, colors, radii, 20, 24, opacities=op,
+                      color_grads=torch.zeros(30, 2, 3), scales=scales, rotations=rot)
+    assert torch.equal(plain, affine)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.7, 2.4])
+def test_affine_colors_single_gaussian_closed_form(theta):
+    """One Gaussian, normalized: every pixel shows its affine color
+    c + G0 (cos t dx + sin t dy) / sx + G1 (-sin t dx + cos t dy) / sy (render.py:110-123),
+    up to the eps term; the additive render shows weight times that color."""
+    sx, sy = 3.0, 1.5
+    mx, my = 9.3, 7.6
+    c = np.array([0.2, 0.5, 0.9])
+    G = np.array([[0.3, -0.2, 0.1], [-0.4, 0.25, 0.05]])
+    log_scales = torch.tensor([[np.log(sx), np.log(sy)]], dtype=torch.float64)
+    rot = torch.tensor([theta], dtype=torch.float64)
+    conics = R.conics_from_rs(log_scales, rot)
+    radii = R.radii_from_rs(log_scales.float(), rot.float(), 3.0)
+    args = (torch.tensor([[mx, my]], dtype=torch.float64), conics,
+            torch.tensor([c], dtype=torch.float64), radii, 18, 20)
+    kw = dict(color_grads=torch.tensor([G], dtype=torch.float64),
+              scales=torch.exp(log_scales), rotations=rot)
+    out = R.render(*args, **kw).numpy()
+    add = R.render_additive(*args, **kw).numpy()
+    y, x = np.mgrid[0:18, 0:20].astype(np.float64)
+    dx, dy = x - mx, y - my
+    lx = (np.cos(theta) * dx + np.sin(theta) * dy) / sx
+    ly = (-np.sin(theta) * dx + np.cos(theta) * dy) / sy
+    color = c + lx[..., None] * G[0] + ly[..., None] * G[1]
+    a, b, cc = (float(v) for v in conics[0])
+    w = np.exp(-0.5 * (a * dx * dx + 2 * b * dx * dy + cc * dy * dy))
+    x0, y0, Tx, n = R.tile_bounds(args[0], radii, 18, 20)
+    inside = np.zeros((18, 20), bool)
+    inside[int(y0[0]):int(y0[0]) + int(n[0]) // int(Tx[0]), int(x0[0]):int(x0[0]) + int(Tx[0])] = True
+    near = inside & (w > 1e-3)
+    assert np.abs(out[near] - color[near]).max() < 1e-4
+    assert np.allclose(add[inside], (w[..., None] * color)[inside], rtol=1e-12, atol=1e-14)
+    assert (add[~inside] == 0).all()
+
+
+def test_affine_colors_require_scales_and_rotations_and_clamp_scales():
+    f = random_field(43, 5, 12, 12, opacity=False)
+    means, conics, colors, radii, _ = field_tensors(f, "cpu")
+    G = torch.full((5, 2, 3), 0.1)
+    with pytest.raises(ValueError, match="requires scales and rotations"):
+        R.render(means, conics, colors, radii, 12, 12, color_grads=G)
+    rot = torch.zeros(5)
+    tiny = R.render(means, conics, colors, radii, 12, 12, color_grads=G,
+                    scales=torch.full((5, 2), 1e-9), rotations=rot)
+    clamped = R.render(means, conics, colors, radii, 12, 12, color_grads=G,
+                       scales=torch.full((5, 2), 1e-6), rotations=rot)
+    assert torch.equal(tiny, clamped)

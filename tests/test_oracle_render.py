@@ -294,6 +294,41 @@ def test_affine_colors_single_gaussian_closed_form(theta):
     assert (add[~inside] == 0).all()
 
 
+@pytest.mark.parametrize("theta", [0.0, 0.4, 1.9, -2.8])
+def test_affine_local_frame_is_the_conic_frame(theta):
+    """The affine colors' local coordinates are the Gaussian's own whitened frame: with color
+    gradients G0 = (1, 0, 0), G1 = (0, 1, 0) and c = 0, one additive Gaussian shows
+    w * (x, y, 0), and at every pixel x^2 + y^2 equals the conic quadratic d^T Q d (Q from the
+    producer formulas, pinned separately); x > 0 along the rotation's first axis
+    (cos t, sin t) and y > 0 along (-sin t, cos t).  A dropped or flipped sine, swapped scales
+    or a transposed rotation breaks one of these."""
+    sx, sy = 3.4, 1.3
+    log_scales = torch.tensor([[np.log(sx), np.log(sy)]], dtype=torch.float64)
+    rot = torch.tensor([theta], dtype=torch.float64)
+    conics = R.conics_from_rs(log_scales, rot)
+    radii = R.radii_from_rs(log_scales.float(), rot.float(), 3.0)
+    means = torch.tensor([[10.37, 9.61]], dtype=torch.float64)
+    H, W = 21, 22
+    G = torch.tensor([[[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]], dtype=torch.float64)
+    kw = dict(scales=torch.exp(log_scales), rotations=rot)
+    affine = R.render_additive(means, conics, torch.zeros(1, 3, dtype=torch.float64), radii, H,
+                               W, color_grads=G, **kw).numpy()
+    weight = R.render_additive(means, conics, torch.ones(1, 3, dtype=torch.float64), radii, H,
+                               W).numpy()[..., 0]
+    live = weight > 1e-6
+    lx, ly = affine[..., 0][live] / weight[live], affine[..., 1][live] / weight[live]
+    assert (affine[..., 2] == 0).all() and live.sum() > 100
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    dx, dy = (x - 10.37)[live], (y - 9.61)[live]
+    a, b, c = (float(v) for v in conics[0])
+    q = a * dx * dx + 2 * b * dx * dy + c * dy * dy
+    assert np.allclose(lx * lx + ly * ly, q, rtol=1e-9, atol=1e-12)
+    along1 = np.cos(theta) * dx + np.sin(theta) * dy
+    along2 = -np.sin(theta) * dx + np.cos(theta) * dy
+    assert (np.sign(lx) == np.sign(along1))[np.abs(along1) > 1e-9].all()
+    assert (np.sign(ly) == np.sign(along2))[np.abs(along2) > 1e-9].all()
+
+
 def test_affine_colors_require_scales_and_rotations_and_clamp_scales():
     f = random_field(43, 5, 12, 12, opacity=False)
     means, conics, colors, radii, _ = field_tensors(f, "cpu")

@@ -6,6 +6,8 @@ Public entry points mirror the reference's render path:
 * :func:`render_hip_exact` - the ``render_cuda_exact`` entry (cuda_render.py:256-294).
 * :func:`build_tile_index` - per-tile Gaussian lists (render_ext.cpp:139-167).
 * :func:`gaussian_activity` - per-Gaussian summed weight, the pruning statistic (render.py:417-442).
+* :func:`raw_weight_map` - differentiable raw coverage sum_i o_i G_i (fit.py:338-367).
+* :func:`render_hip_affine` - affine-color render (render.py:110-123).
 * :class:`GaussianField` - field layout and conic/radii/opacity producer (gaussians.py).
 * :mod:`structsplat_amd.codec` - SSPL1 decode and decode-and-render (codec.py:333-500).
 
@@ -17,7 +19,9 @@ from .render import (
     build_tile_index,
     build_tile_index_capacity,
     gaussian_activity,
+    raw_weight_map,
     render_field,
+    render_hip_affine,
     render_hip_exact,
     render_hip_exact_with_coverage,
 )
@@ -27,7 +31,9 @@ __all__ = [
     "build_tile_index",
     "build_tile_index_capacity",
     "gaussian_activity",
+    "raw_weight_map",
     "render_field",
+    "render_hip_affine",
     "render_hip_exact",
     "render_hip_exact_with_coverage",
 ]

@@ -1169,8 +1169,10 @@ constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep,
 
 template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine = false>
 // Eight waves per SIMD (at most 64 VGPRs): with 16-pixel tiles that is eight resident
-// workgroups per CU, whose different phases (list sweeps, reductions) overlap.
-__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_backward(
+// workgroups per CU, whose different phases (list sweeps, reductions) overlap.  The affine
+// sweep's 21 accumulators need more registers: four waves per SIMD (up to 128 VGPRs).
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
+    kAffine ? 4 : 8, kAffine ? 4 : 8))) void k_backward(
     RenderArgs a, GradArgs g) {
   constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
   constexpr int kQ = kT / 8;  // quadrants per tile row

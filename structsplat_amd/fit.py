@@ -128,6 +128,22 @@ def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> to
                                 float(cfg.sigma_cutoff))
 
 
+def raw_weight_map_field(field: GaussianField, cfg: FitConfig, H: int, W: int,
+                         support_fade_alpha: float | None = None,
+                         detach_opacities: bool = False) -> torch.Tensor:
+    """Differentiable (H*W,) raw opacity-weighted weight sum sum_i o_i G_i(p) of a field
+    (``_raw_weight_map_field``, fit.py:338-367): the coverage behind the reference's mask and
+    coverage-matching penalties.  ``support_fade_alpha`` overrides ``cfg.support_fade``;
+    ``detach_opacities`` cuts only the opacity gradient path."""
+    dilation = float(getattr(cfg, "aa_dilation", 0.0))
+    fade = (HR._fade_alpha(cfg.support_fade) if support_fade_alpha is None
+            else max(0.0, min(1.0, float(support_fade_alpha))))
+    return HR.raw_weight_map(field.means, field.conics(dilation),
+                             field.radii(cfg.sigma_cutoff, dilation), H, W,
+                             opacities=field.opacity_values(), support_fade=fade,
+                             sigma_cutoff=cfg.sigma_cutoff, detach_opacities=detach_opacities)
+
+
 def loss_fn(img: torch.Tensor, target: torch.Tensor, w: float) -> torch.Tensor:
     """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1"), computed with its
     gradient by the fused HIP loss kernels."""

@@ -573,6 +573,22 @@ def gaussian_activity(means, conics, radii, H: int, W: int, chunk: int = 4096,
     return activity
 
 
+def raw_weight_map(means, conics, radii, H: int, W: int, opacities=None,
+                   support_fade: bool | float = False, sigma_cutoff: float = 3.0,
+                   detach_opacities: bool = False) -> torch.Tensor:
+    """Differentiable (H*W,) raw coverage sum_i o_i G_i(p), the accumulation of the reference's
+    ``_raw_weight_map_field`` (fit.py:338-367), on the HIP kernels: an additive render of unit
+    colors, whose first channel is the coverage bit for bit (w * 1 is exact).  Gradients reach
+    the means, conics and (unless ``detach_opacities``) the opacities."""
+    _check_field(means, conics, conics, radii, opacities, need_colors=False)
+    if opacities is not None and detach_opacities:
+        opacities = opacities.detach()
+    ones = torch.ones(means.shape[0], 3, dtype=torch.float32, device=means.device)
+    out = render_hip_exact(means, conics, ones, radii, H, W, opacities=opacities,
+                           normalize=False, support_fade=support_fade, sigma_cutoff=sigma_cutoff)
+    return out[..., 0].reshape(-1)
+
+
 def render_field(means, conics, colors, radii, H: int, W: int,
                  chunk: int = 4096, mode: str = "normalized", opacities=None,
                  scales=None, rotations=None, support_fade: bool = False,

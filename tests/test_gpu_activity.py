@@ -8,7 +8,7 @@ import torch
 from oracle import reference_render as R
 from structsplat_amd import render as HR
 from structsplat_amd import synth
-from tests.helpers import field_tensors, random_field, rel_err, to_np
+from tests.helpers import field_tensors, random_field, rel_err, scaled_err, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +74,44 @@ def test_activity_full_size_sample_and_repeatable(cuda):
     err = rel_err(to_np(a)[idx.numpy()], exp, 1e-3)
     print("rel err", err)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+def test_raw_weight_map_matches_fp64_and_coverage(cuda, fade):
+    """raw_weight_map (fit.py:338-367's accumulation): values within 1e-5 of the float64
+    coverage, bit-identical to the coverage output of the normalized forward, gradients within
+    1e-4 of the float64 backward of an additive render with unit colors and adjoint (g, 0, 0);
+    detach_opacities cuts only the opacity gradient."""
+    from oracle import exact64
+    from structsplat_amd import render as HR
+
+    print(f"case: raw_weight_map fade={fade}", flush=True)
+    N, H, W = 300, 57, 71
+    f = random_field(17, N, H, W, scale_lo=0.7, scale_hi=5.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    leaves = [t.clone().requires_grad_(True) for t in (means, conics, op)]
+    den = HR.raw_weight_map(leaves[0], leaves[1], radii, H, W, opacities=leaves[2],
+                            support_fade=fade)
+    g = torch.rand(H * W, device=cuda, generator=torch.Generator(device=cuda).manual_seed(4))
+    (den * g).sum().backward()
+    _, cov = HR.render_hip_exact_with_coverage(means, conics, colors, radii, H, W, opacities=op,
+                                                support_fade=fade)
+    torch.cuda.synchronize()
+    assert torch.equal(den.detach().reshape(H, W), cov)
+    ones = np.ones((N, 3), np.float32)
+    ref, _ = exact64.forward(to_np(means), to_np(conics), ones, to_np(radii), to_np(op), H, W,
+                             False, 1e-8, fade, 3.0)
+    assert scaled_err(to_np(den).reshape(H, W), ref[..., 0]) <= 1e-5
+    go = np.zeros((H, W, 3), np.float32)
+    go[..., 0] = to_np(g).reshape(H, W)
+    rg = exact64.backward(go, to_np(means), to_np(conics), ones, to_np(radii), to_np(op), H, W,
+                          False, 1e-8, fade, 3.0)
+    for name, leaf, exp in zip(("means", "conics", "opacities"), leaves, (rg[0], rg[1], rg[3])):
+        err = scaled_err(to_np(leaf.grad), exp)
+        print(name, "scaled err", err)
+        assert err <= 1e-4, name
+    m2 = means.clone().requires_grad_(True)
+    o2 = op.clone().requires_grad_(True)
+    HR.raw_weight_map(m2, conics, radii, H, W, opacities=o2, support_fade=fade,
+                      detach_opacities=True).sum().backward()
+    assert o2.grad is None and m2.grad is not None

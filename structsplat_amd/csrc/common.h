@@ -39,6 +39,30 @@ struct FieldTable {
   ssplat_field single;
 };
 
+// A table entry, read into registers before the lookup's branch ends: otherwise the compiler
+// merges the table read and the read of `single` into one load through a selected pointer,
+// which copies the kernel arguments to scratch memory.
+__device__ __forceinline__ ssplat_field load_field(const ssplat_field* p) {
+  static_assert(sizeof(ssplat_field) == 48, "ssplat_field is six 8-byte words");
+  const int2* w = reinterpret_cast<const int2*>(p);
+  const int2 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
+  auto i64 = [](int2 v) {
+    return static_cast<int64_t>((uint64_t(uint32_t(v.y)) << 32) | uint32_t(v.x));
+  };
+  ssplat_field f;
+  f.height = w0.x;
+  f.width = w0.y;
+  f.first_gaussian = i64(w1);
+  f.num_gaussians = i64(w2);
+  f.first_pixel = i64(w3);
+  f.first_tile = i64(w4);
+  f.tiles_x = w5.x;
+  f.num_tiles = w5.y;
+  // The words are in registers here, so the reads cannot be moved past the caller's branch.
+  asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(w3), "v"(w4), "v"(w5));
+  return f;
+}
+
 // Field owning global tile `tile` (the last field whose first_tile <= tile).
 __device__ __forceinline__ ssplat_field field_of_tile(const FieldTable& t, int tile) {
   if (t.fields == nullptr) return t.single;
@@ -51,7 +75,7 @@ __device__ __forceinline__ ssplat_field field_of_tile(const FieldTable& t, int t
       hi = mid - 1;
     }
   }
-  return t.fields[lo];
+  return load_field(t.fields + lo);
 }
 
 // Field owning global Gaussian `i` (the last field whose first_gaussian <= i; fields are
@@ -67,7 +91,7 @@ __device__ __forceinline__ ssplat_field field_of_gaussian(const FieldTable& t, i
       hi = mid - 1;
     }
   }
-  return t.fields[lo];
+  return load_field(t.fields + lo);
 }
 
 }  // namespace ssplat_detail

@@ -1011,6 +1011,19 @@ __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunk
   }
 }
 
+// Requires each value to be in a VGPR at this point of the program (an empty asm statement that
+// reads them); it emits no instruction of its own.
+template <typename T>
+__device__ __forceinline__ int in_register(T v) {
+  asm volatile("" ::"v"(v));
+  return 0;
+}
+template <typename... T>
+__device__ __forceinline__ void in_registers(T... v) {
+  const int all[] = {in_register(v)...};
+  (void)all;
+}
+
 // Pixels of the 8x8 quadrant at (qx0, qy0) inside the half-open rectangle packed in (xr, yr), as
 // a 64-bit mask with bit 8*ly + lx for the pixel (qx0 + lx, qy0 + ly).
 __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int qx0, int qy0) {
@@ -1029,8 +1042,8 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // Accumulates numerator (r,g,b) and denominator of one pixel per lane over the whole tile list,
 // in ascending Gaussian id.  Each wave owns the 8x8 quadrant (qx0, qy0).  For every staged record
 // the wave first computes (one record per lane) the mask of quadrant pixels inside its rectangle
-// into its LDS slice; a ballot keeps the records that meet the quadrant, records covering all of
-// it skip the pixel test, and for the others each lane reads its bit of the mask.
+// into its LDS slice; a ballot keeps the records that meet the quadrant, and each lane reads its
+// bit of their masks.
 // kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
 // recomputation); it is the same sum, in the same order, as the forward's D.
 template <bool kFade, bool kDenOnly, int kChunks, int kS, bool kAffine = false>
@@ -1051,7 +1064,7 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
       stage_batch<3>(a, st, batch, m);
     }
     __syncthreads();
-    uint64_t hit[kS / 64], full[kS / 64];
+    uint64_t hit[kS / 64];
 #pragma unroll
     for (int k = 0; k < kS / 64; ++k) {
       const int j = lane + 64 * k;
@@ -1063,51 +1076,65 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
       s_mask[2 * j] = static_cast<uint32_t>(qm);  // pixels of lanes 0-31
       s_mask[2 * j + 1] = static_cast<uint32_t>(qm >> 32);  // lanes 32-63
       hit[k] = __ballot(qm != 0);
-      full[k] = __ballot(qm == ~uint64_t(0));
     }
     wave_lds_sync();  // the masks are read back by every lane of this wave
     const uint32_t lane_bit_index = lane & 31;
     const uint32_t* my_half = s_mask + (lane >> 5);
+    // One record's contribution to this lane's pixel, if the pixel is inside its rectangle.
+    auto visit = [&](int j, bool inside, const float4& v0, const float4& v1, const float4& v2) {
+      if (inside) {
+        const float dx = fx - v0.x;
+        const float dy = fy - v0.y;
+        const float t = fmaf(v0.z, dx, v0.w * dy);
+        const float e = fmaf(t, dx, v1.x * dy * dy);
+        const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
+        const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
+        if constexpr (kAffine && !kDenOnly) {
+          // w (c_k + G0k x + G1k y) with the pixel's local coordinates x, y.
+          const float4 v3 = st.c[3][j];
+          const float4 a0 = st.c[4][j];
+          const float4 a1 = st.c[5][j];
+          const float4 a2 = st.c[6][j];
+          const float lxc = fmaf(a0.x, dx, a0.y * dy);
+          const float lyc = fmaf(a0.z, dy, -(a0.w * dx));
+          const float w = base * v2.w;
+          nr = fmaf(w, fmaf(a1.x, lxc, fmaf(a1.w, lyc, v3.x)), nr);
+          ng = fmaf(w, fmaf(a1.y, lxc, fmaf(a2.x, lyc, v3.y)), ng);
+          nb = fmaf(w, fmaf(a1.z, lxc, fmaf(a2.y, lyc, v3.z)), nb);
+        } else if constexpr (!kDenOnly) {
+          nr = fmaf(base, v2.x, nr);
+          ng = fmaf(base, v2.y, ng);
+          nb = fmaf(base, v2.z, nb);
+        }
+        d = fmaf(base, v2.w, d);
+      }
+    };
 #pragma unroll
     for (int k = 0; k < kS / 64; ++k) {
       uint64_t mask = hit[k];
+      // Two records per step: both are read from LDS before the first is evaluated, so the
+      // second's reads overlap the first's arithmetic.  They are added in ascending order.
       while (mask) {
-        const int b = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        const int b1 = __ffsll(static_cast<unsigned long long>(mask)) - 1;
         mask &= mask - 1;
-        const int j = 64 * k + b;
-        const float4 v0 = st.c[0][j];
-        const float4 v1 = st.c[1][j];
-        const float4 v2 = st.c[2][j];
-        bool inside = true;
-        if (!((full[k] >> b) & 1)) {
-          inside = __builtin_amdgcn_ubfe(my_half[2 * j], lane_bit_index, 1) != 0;
-        }
-        if (inside) {
-          const float dx = fx - v0.x;
-          const float dy = fy - v0.y;
-          const float t = fmaf(v0.z, dx, v0.w * dy);
-          const float e = fmaf(t, dx, v1.x * dy * dy);
-          const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
-          const float base = kFade ? fmaxf(raw - tail, 0.0f) : raw;
-          if constexpr (kAffine && !kDenOnly) {
-            // w (c_k + G0k x + G1k y) with the pixel's local coordinates x, y.
-            const float4 v3 = st.c[3][j];
-            const float4 a0 = st.c[4][j];
-            const float4 a1 = st.c[5][j];
-            const float4 a2 = st.c[6][j];
-            const float lxc = fmaf(a0.x, dx, a0.y * dy);
-            const float lyc = fmaf(a0.z, dy, -(a0.w * dx));
-            const float w = base * v2.w;
-            nr = fmaf(w, fmaf(a1.x, lxc, fmaf(a1.w, lyc, v3.x)), nr);
-            ng = fmaf(w, fmaf(a1.y, lxc, fmaf(a2.x, lyc, v3.y)), ng);
-            nb = fmaf(w, fmaf(a1.z, lxc, fmaf(a2.y, lyc, v3.z)), nb);
-          } else if constexpr (!kDenOnly) {
-            nr = fmaf(base, v2.x, nr);
-            ng = fmaf(base, v2.y, ng);
-            nb = fmaf(base, v2.z, nb);
-          }
-          d = fmaf(base, v2.w, d);
-        }
+        const bool two = mask != 0;
+        const int b2 = two ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : b1;
+        mask &= mask - 1;
+        const int j1 = 64 * k + b1, j2 = 64 * k + b2;
+        const float4 p0 = st.c[0][j1], p1 = st.c[1][j1], p2 = st.c[2][j1];
+        const float4 q0 = st.c[0][j2], q1 = st.c[1][j2], q2 = st.c[2][j2];
+        // The lane's bit of each record's quadrant mask (all ones for a covered quadrant),
+        // read with the records so one wait covers every read.
+        const uint32_t m1 = my_half[2 * j1], m2 = my_half[2 * j2];
+        // Every value both visits read is in registers here (the compiler would otherwise
+        // sink each record's reads into its masked visit and wait once per record).
+        in_registers(p0.x, p0.y, p0.z, p0.w, p1.x, p2.w, m1, q0.x, q0.y, q0.z, q0.w, q1.x, q2.w,
+                     m2);
+        if constexpr (!kDenOnly) in_registers(p2.x, p2.y, p2.z, q2.x, q2.y, q2.z);
+        const bool in1 = __builtin_amdgcn_ubfe(m1, lane_bit_index, 1) != 0;
+        const bool in2 = __builtin_amdgcn_ubfe(m2, lane_bit_index, 1) != 0;
+        visit(j1, in1, p0, p1, p2);
+        if (two) visit(j2, in2, q0, q1, q2);
       }
     }
     __syncthreads();
@@ -1530,22 +1557,26 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
 #pragma unroll
   for (int c = 0; c < kVals; ++c) acc[c] = 0.0f;
   if (count <= kLaneSumMax) {
-    // Four slots' loads in flight per step; the additions stay in ascending slot order.
+    // kInFlight slots' loads issued together per step (the slots past the count are not
+    // loaded); the additions stay in ascending slot order.
+    constexpr int kInFlight = kWords <= kSlotWords ? 8 : 4;
     const float* sl = a.slots + base * kWords;
-    int k = 0;
-    for (; k + 4 <= count; k += 4) {
-      float2 v[4][kWords / 2];
+    for (int k = 0; k < count; k += kInFlight) {
+      float2 v[kInFlight][kWords / 2];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kInFlight; ++u) {
+        const bool live = k + u < count;
 #pragma unroll
         for (int w = 0; w < kWords / 2; ++w) {
-          v[u][w] = *reinterpret_cast<const float2*>(sl + (k + u) * kWords + 2 * w);
+          v[u][w] = live ? *reinterpret_cast<const float2*>(sl + (k + u) * kWords + 2 * w)
+                         : make_float2(0.0f, 0.0f);
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) add_slot_values<kWords, kVals>(v[u], tag, acc);
+      for (int u = 0; u < kInFlight; ++u) {
+        if (k + u < count) add_slot_values<kWords, kVals>(v[u], tag, acc);
+      }
     }
-    for (; k < count; ++k) add_slot<kWords, kVals>(sl + k * kWords, tag, acc);
   }
   uint64_t big = __ballot(count > kLaneSumMax);
   while (big) {

@@ -1556,27 +1556,50 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   float acc[kVals];
 #pragma unroll
   for (int c = 0; c < kVals; ++c) acc[c] = 0.0f;
-  if (count <= kLaneSumMax) {
-    // kInFlight slots' loads issued together per step (the slots past the count are not
-    // loaded); the additions stay in ascending slot order.
-    constexpr int kInFlight = kWords <= kSlotWords ? 8 : 4;
-    const float* sl = a.slots + base * kWords;
-    for (int k = 0; k < count; k += kInFlight) {
-      float2 v[kInFlight][kWords / 2];
+  // Lane sums.  A Gaussian's slots follow the previous Gaussian's, so the wave's slots form one
+  // run; it is copied to LDS kChunk slots at a time with coalesced loads (lanes reading their
+  // own slots straight from memory would touch a different cache line per lane and load), and
+  // every lane adds its slots of the chunk in ascending order.
+  constexpr int kChunk = 2560 / kWords;      // slots per chunk: 10 KB per wave
+  constexpr int kPer = kChunk * kWords / 128;  // float2 loads per lane and chunk
+  __shared__ float2 s_chunk[256 / 64][kChunk * kWords / 2];
+  float2* chunk = s_chunk[threadIdx.x >> 6];
+  const bool small = count > 0 && count <= kLaneSumMax;
+  int64_t lo = small ? base : INT64_MAX, hi = small ? base + count : INT64_MIN;
 #pragma unroll
-      for (int u = 0; u < kInFlight; ++u) {
-        const bool live = k + u < count;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t olo = __shfl_xor(lo, d, 64), ohi = __shfl_xor(hi, d, 64);
+    lo = olo < lo ? olo : lo;
+    hi = ohi > hi ? ohi : hi;
+  }
+  for (int64_t c0 = lo; c0 < hi; c0 += kChunk) {
+    const bool need = small && base < c0 + kChunk && base + count > c0;
+    if (__ballot(need) == 0) continue;  // a run of big Gaussians' slots
+    const int len2 = static_cast<int>(hi - c0 < kChunk ? hi - c0 : kChunk) * (kWords / 2);
+    const float2* src = reinterpret_cast<const float2*>(a.slots + c0 * kWords);
+    float2 tmp[kPer];  // every load of the chunk in flight at once
 #pragma unroll
-        for (int w = 0; w < kWords / 2; ++w) {
-          v[u][w] = live ? *reinterpret_cast<const float2*>(sl + (k + u) * kWords + 2 * w)
-                         : make_float2(0.0f, 0.0f);
-        }
-      }
+    for (int u = 0; u < kPer; ++u) {
+      const int t = lane + 64 * u;
+      tmp[u] = t < len2 ? src[t] : make_float2(0.0f, 0.0f);
+    }
 #pragma unroll
-      for (int u = 0; u < kInFlight; ++u) {
-        if (k + u < count) add_slot_values<kWords, kVals>(v[u], tag, acc);
+    for (int u = 0; u < kPer; ++u) {
+      if (lane + 64 * u < len2) chunk[lane + 64 * u] = tmp[u];
+    }
+    wave_lds_sync();
+    if (need) {
+      const int k0 = static_cast<int>((base > c0 ? base : c0) - c0);
+      const int64_t end = base + count < c0 + kChunk ? base + count : c0 + kChunk;
+      const int k1 = static_cast<int>(end - c0);
+      for (int k = k0; k < k1; ++k) {
+        float2 v[kWords / 2];
+#pragma unroll
+        for (int w = 0; w < kWords / 2; ++w) v[w] = chunk[k * (kWords / 2) + w];
+        add_slot_values<kWords, kVals>(v, tag, acc);
       }
     }
+    wave_lds_sync();  // every lane is done with the chunk before the next copy
   }
   uint64_t big = __ballot(count > kLaneSumMax);
   while (big) {

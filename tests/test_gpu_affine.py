@@ -181,3 +181,40 @@ def test_render_field_dispatches_affine_colors(cuda):
         HR.render_field(means, conics, colors, radii, H, W, mode="cuda_additive", **kw)
     with pytest.raises(ValueError, match="requires scales and rotations"):
         HR.render_field(means, conics, colors, radii, H, W, opacities=op, color_grads=cg)
+
+
+def test_affine_empty_field_renders_zero(cuda):
+    _case(test="affine empty field")
+    H, W = 8, 9
+    leaves = [torch.zeros(0, k, device=cuda, requires_grad=True) for k in (2, 3, 3)]
+    cg = torch.zeros(0, 2, 3, device=cuda, requires_grad=True)
+    scales = torch.zeros(0, 2, device=cuda, requires_grad=True)
+    rot = torch.zeros(0, device=cuda, requires_grad=True)
+    r = torch.zeros(0, 2, dtype=torch.long, device=cuda)
+    out = HR.render_hip_affine(leaves[0], leaves[1], leaves[2], r, H, W, cg, scales, rot)
+    out.sum().backward()
+    torch.cuda.synchronize()
+    assert out.shape == (H, W, 3) and torch.count_nonzero(out) == 0
+    assert cg.grad is not None and cg.grad.shape == (0, 2, 3)
+
+
+def test_affine_entry_points_reject_other_tile_sizes(cuda):
+    """The affine kernels exist for 16-pixel tiles; the C ABI says so for 32."""
+    from structsplat_amd import _native
+
+    _case(test="affine tile size 32")
+    H, W = 32, 32
+    means, conics, colors, radii, op, cg, scales, rot = _affine_field(2, 12, H, W, cuda)
+    radii = radii.to(torch.int64).contiguous()
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, 32)
+    ws = torch.empty(_native.affine_workspace_bytes(12, H, W, 32), dtype=torch.uint8,
+                     device=cuda)
+    out = torch.empty(H, W, 3, device=cuda)
+    lib = _native.load()
+    rc = lib.ssplat_forward_tiled_affine(
+        HR._ptr(means), HR._ptr(conics), HR._ptr(colors), HR._ptr(radii), HR._ptr(op),
+        HR._ptr(cg), HR._ptr(scales), HR._ptr(rot), 12, HR._ptr(gids), HR._ptr(offsets),
+        gids.numel(), H, W, 32, 1, 1e-8, 0.0, 3.0, HR._ptr(out), None, HR._ptr(ws), ws.numel(),
+        HR._stream(means.device))
+    with pytest.raises(ValueError, match="tile_size 16"):
+        _native.check(rc, "ssplat_forward_tiled_affine")

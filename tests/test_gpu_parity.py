@@ -282,3 +282,27 @@ def test_tile_index_long_lists_bit_exact(cuda, n_dense):
     assert np.diff(ref_o).max() >= min(n_dense, 64)
     assert to_np(offsets).tolist() == ref_o.tolist()
     assert to_np(gids).tolist() == ref_g.tolist()
+
+
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+def test_normalized_render_is_invariant_to_opacity_scale(cuda, fade):
+    """Scaling every opacity by k = 4 (a power of two, so every weight and partial sum scales
+    exactly) multiplies the coverage by exactly 4 and leaves the normalized image unchanged up to
+    the eps term: out = 4N / (4D + eps).  A dropped or doubled opacity factor in either sum
+    breaks one of the two."""
+    _print_case(test="opacity scale invariance", fade=fade)
+    N, H, W = 500, 72, 88
+    f = random_field(27, N, H, W, scale_lo=0.7, scale_hi=6.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    out1, den1 = HR.render_hip_exact_with_coverage(means, conics, colors, radii, H, W,
+                                                    opacities=op, support_fade=fade)
+    out4, den4 = HR.render_hip_exact_with_coverage(means, conics, colors, radii, H, W,
+                                                    opacities=op * 4.0, support_fade=fade)
+    torch.cuda.synchronize()
+    # Exact wherever the sums stayed clear of the subnormal range (a subnormal partial weight
+    # rounds on a fixed grid that scaling does not commute with).
+    normal = den1 > 1e-20
+    assert torch.equal(den4[normal], den1[normal] * 4.0)
+    live = to_np(den1) > 1e-3
+    assert np.abs(to_np(out4) - to_np(out1))[live].max() <= 1e-6 * max(
+        1.0, float(np.abs(to_np(out1)).max()))

@@ -735,17 +735,94 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Ascending bitonic sort of up to 64 values held one per lane (INT32_MAX padding sorts last).
 // Must be called with all 64 lanes active.
-__device__ __forceinline__ int32_t wave_sort64(int32_t v, int lane) {
+// The value of lane (lane ^ kJ), by DPP (kJ <= 8) or a permlane swap (16, 32): no LDS traffic.
+// Must be called with all 64 lanes active.
+template <int kJ>
+__device__ __forceinline__ int32_t lane_xor(int32_t v, int lane) {
+  if constexpr (kJ == 1) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (kJ == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (kJ == 4) {
+    // Mirror within 8 lanes (i -> i ^ 7), then within 4 (i -> i ^ 3): i -> i ^ 4.
+    const int32_t m = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    return __builtin_amdgcn_update_dpp(0, m, 0x1b, 0xf, 0xf, false);  // quad_perm [3,2,1,0]
+  } else if constexpr (kJ == 8) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xf, 0xf, false);  // row_ror:8
+  } else if constexpr (kJ == 16) {
+    // v_permlane16_swap(v, v): s[0] holds rows (0, 0, 2, 2) of v, s[1] rows (1, 1, 3, 3).
+    const auto s = __builtin_amdgcn_permlane16_swap(uint32_t(v), uint32_t(v), false, false);
+    return static_cast<int32_t>((lane & 16) ? s[0] : s[1]);
+  } else {
+    static_assert(kJ == 32, "lane_xor covers 1, 2, 4, 8, 16 and 32");
+    // v_permlane32_swap(v, v): s[0] holds halves (0, 0) of v, s[1] halves (1, 1).
+    const auto s = __builtin_amdgcn_permlane32_swap(uint32_t(v), uint32_t(v), false, false);
+    return static_cast<int32_t>((lane & 32) ? s[0] : s[1]);
+  }
+}
+
+// One half-cleaner step at distance kJ < 64 over element e = 64 r + lane of each register.
+template <int kR, int kK, int kJ>
+__device__ __forceinline__ void sort_step_lanes(int32_t (&v)[kR], int lane) {
 #pragma unroll
-  for (int k = 2; k <= 64; k <<= 1) {
+  for (int r = 0; r < kR; ++r) {
+    const int32_t p = lane_xor<kJ>(v[r], lane);
+    const bool up = ((64 * r + lane) & kK) == 0;  // ascending block
+    const bool keep_min = ((lane & kJ) == 0) == up;
+    v[r] = keep_min ? min(v[r], p) : max(v[r], p);
+  }
+}
+
+// A half-cleaner step at distance kJ >= 64: registers r and r + kJ / 64 of one lane.
+template <int kR, int kK, int kJ>
+__device__ __forceinline__ void sort_step_regs(int32_t (&v)[kR]) {
+  constexpr int kD = kJ / 64;
 #pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const int32_t p = __shfl_xor(v, j, 64);
-      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-      v = keep_min ? min(v, p) : max(v, p);
+  for (int r = 0; r < kR; ++r) {
+    if ((r & kD) == 0) {
+      const bool up = ((64 * r) & kK) == 0;
+      const int32_t lo = min(v[r], v[r + kD]), hi = max(v[r], v[r + kD]);
+      v[r] = up ? lo : hi;
+      v[r + kD] = up ? hi : lo;
     }
   }
-  return v;
+}
+
+template <int kR, int kK, int kJ>
+__device__ __forceinline__ void sort_merge(int32_t (&v)[kR], int lane) {
+  if constexpr (kJ >= 64) {
+    sort_step_regs<kR, kK, kJ>(v);
+  } else {
+    sort_step_lanes<kR, kK, kJ>(v, lane);
+  }
+  if constexpr (kJ > 1) sort_merge<kR, kK, kJ / 2>(v, lane);
+}
+
+template <int kR, int kK>
+__device__ __forceinline__ void sort_stages(int32_t (&v)[kR], int lane) {
+  sort_merge<kR, kK, kK / 2>(v, lane);
+  if constexpr (kK < 64 * kR) sort_stages<kR, 2 * kK>(v, lane);
+}
+
+// Ascending bitonic sort of 64 kR values held kR per lane (element 64 r + lane in v[r]), all in
+// registers: lane exchanges by DPP and permlane swaps.  INT32_MAX padding sorts last.  Must be
+// called with all 64 lanes active.
+template <int kR>
+__device__ __forceinline__ void wave_sort_regs(int32_t (&v)[kR], int lane) {
+  sort_stages<kR, 2>(v, lane);
+}
+
+// Sorts a tile list of L <= 64 kR entries by one wave, in registers.
+template <int kR>
+__device__ __forceinline__ void sort_tile_in_registers(int32_t* __restrict__ g, int L, int lane) {
+  int32_t v[kR];
+#pragma unroll
+  for (int r = 0; r < kR; ++r) v[r] = 64 * r + lane < L ? g[64 * r + lane] : INT32_MAX;
+  wave_sort_regs<kR>(v, lane);
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    if (64 * r + lane < L) g[64 * r + lane] = v[r];
+  }
 }
 
 // Flip-form bitonic sort of a[0, L) by the 64 lanes of one wave (a in the wave's LDS slice).
@@ -820,11 +897,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict
     int64_t s = 0;
     const int L = t < num_tiles ? tile_segment(offsets, t, capacity, &s) : 0;
     if (L <= 64) {
-      if (L > 1) {
-        int32_t v = lane < L ? gids[s + lane] : INT32_MAX;
-        v = wave_sort64(v, lane);
-        if (lane < L) gids[s + lane] = v;
-      }
+      if (L > 1) sort_tile_in_registers<1>(gids + s, L, lane);
+    } else if (L <= 128) {
+      sort_tile_in_registers<2>(gids + s, L, lane);
+    } else if (L <= 256) {
+      sort_tile_in_registers<4>(gids + s, L, lane);
     } else if (L <= kSortWaveLds) {
       int32_t* a = buf + wave * kSortWaveLds;
       for (int i = lane; i < L; i += 64) a[i] = gids[s + i];

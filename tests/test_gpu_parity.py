@@ -259,10 +259,10 @@ def test_render_kernels_reject_other_tile_sizes(cuda):
                          8, False, 3.0)
 
 
-@pytest.mark.parametrize("n_dense", [70, 900, 2000, 6000])
+@pytest.mark.parametrize("n_dense", [70, 200, 900, 2000, 6000])
 def test_tile_index_long_lists_bit_exact(cuda, n_dense):
-    """Dense tiles: lists of ~n_dense entries exercise every sort path (register, wave LDS,
-    block LDS, global memory)."""
+    """Dense tiles: lists of ~n_dense entries exercise every sort path (registers with one, two
+    and four entries per lane, wave LDS, block LDS, global memory)."""
     _print_case(test="long lists", n_dense=n_dense)
     rng = np.random.default_rng(n_dense)
     H, W = 40, 52

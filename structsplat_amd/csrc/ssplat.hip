@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -249,6 +250,45 @@ __device__ __forceinline__ float dpp_move(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
 }
 
+// Inclusive prefix sums over the 64 lanes of a wave (lane l gets v(0) + ... + v(l)) by DPP:
+// Hillis-Steele within each 16-lane row, then the rows' totals by row broadcasts.  No LDS
+// traffic.  Must be called with all lanes active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int32_t dpp_i32(int32_t v) {
+  // Lanes without a source, and rows outside kRowMask, read 0.
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int64_t dpp_i64(int64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(dpp_i32<kCtrl, kRowMask>(static_cast<int32_t>(v)));
+  const int32_t hi = dpp_i32<kCtrl, kRowMask>(static_cast<int32_t>(v >> 32));
+  return static_cast<int64_t>((uint64_t(uint32_t(hi)) << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+  auto move = [](auto ctrl, auto mask, T x) -> T {
+    if constexpr (sizeof(T) == 8) {
+      return dpp_i64<decltype(ctrl)::value, decltype(mask)::value>(x);
+    } else {
+      return dpp_i32<decltype(ctrl)::value, decltype(mask)::value>(x);
+    }
+  };
+  using C1 = std::integral_constant<int, 0x111>;   // row_shr:1
+  using C2 = std::integral_constant<int, 0x112>;   // row_shr:2
+  using C4 = std::integral_constant<int, 0x114>;   // row_shr:4
+  using C8 = std::integral_constant<int, 0x118>;   // row_shr:8
+  using B15 = std::integral_constant<int, 0x142>;  // row_bcast:15 (lane 15 of a row -> next row)
+  using B31 = std::integral_constant<int, 0x143>;  // row_bcast:31 (lane 31 -> rows 2 and 3)
+  using All = std::integral_constant<int, 0xf>;
+  v += move(C1{}, All{}, v);
+  v += move(C2{}, All{}, v);
+  v += move(C4{}, All{}, v);
+  v += move(C8{}, All{}, v);
+  v += move(B15{}, std::integral_constant<int, 0xa>{}, v);  // rows 1 and 3
+  v += move(B31{}, std::integral_constant<int, 0xc>{}, v);  // rows 2 and 3
+  return v;
+}
+
 // Sum over the 64 lanes of a wave; the total lands in lane 63.  Must be called with all lanes
 // active.
 __device__ __forceinline__ float wave_sum_lane63(float v) {
@@ -389,13 +429,10 @@ __device__ __forceinline__ void enumerate_pairs(const BinArgs& a, Emit&& emit) {
       cnt = int64_t(tw) * (ty1 - ty0 + 1);
     }
   }
-  int64_t incl = cnt;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int64_t y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
-  const int64_t total = __shfl(incl, 63, 64);
+  const int64_t incl = wave_inclusive_scan(cnt);
+  const int64_t total =
+      (int64_t(__builtin_amdgcn_readlane(static_cast<int32_t>(incl >> 32), 63)) << 32) |
+      uint32_t(__builtin_amdgcn_readlane(static_cast<int32_t>(incl), 63));
   s_incl[threadIdx.x] = incl;
   s_rect[threadIdx.x] = make_int4(r.x0, r.x1, r.y0, r.y1);
   s_tile[threadIdx.x] = make_int4(tx0, ty0, tw, i);
@@ -471,21 +508,11 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
   __shared__ int64_t s_wave[kWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  int64_t x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int64_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
+  const int64_t x = wave_inclusive_scan(v);
   if (lane == 63) s_wave[wave] = x;
   __syncthreads();
   if (wave == 0) {
-    int64_t s = lane < kWaves ? s_wave[lane] : 0;
-#pragma unroll
-    for (int d = 1; d < kWaves; d <<= 1) {
-      const int64_t y = __shfl_up(s, d, 64);
-      if (lane >= d) s += y;
-    }
+    const int64_t s = wave_inclusive_scan(lane < kWaves ? s_wave[lane] : int64_t(0));
     if (lane < kWaves) s_wave[lane] = s;
   }
   __syncthreads();
@@ -998,12 +1025,7 @@ __global__ __launch_bounds__(256) void k_prepare(
   }
   // Slots of this wave's Gaussians: inclusive prefix of R over the 64 lanes.
   const int slots = rect_tiles(xr, yr, sp.lg).count;
-  int incl = slots;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
+  const int incl = wave_inclusive_scan(slots);
   if (lane == 63 && (i >> 6) < sp.num_waves) sp.rect_wave[i >> 6] = incl;
   if (!live) return;
   const float scale = kFold ? kNegHalfLog2e : 1.0f;
@@ -1642,12 +1664,18 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   __shared__ float2 s_chunk[256 / 64][kChunk * kWords / 2];
   float2* chunk = s_chunk[threadIdx.x >> 6];
   const bool small = count > 0 && count <= kLaneSumMax;
-  int64_t lo = small ? base : INT64_MAX, hi = small ? base + count : INT64_MIN;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int64_t olo = __shfl_xor(lo, d, 64), ohi = __shfl_xor(hi, d, 64);
-    lo = olo < lo ? olo : lo;
-    hi = ohi > hi ? ohi : hi;
+  // Slot runs ascend with the lane, so the wave's run is [first small lane's base, last small
+  // lane's end).
+  const uint64_t small_lanes = __ballot(small);
+  int64_t lo = 0, hi = 0;
+  if (small_lanes != 0) {
+    const int first = __ffsll(static_cast<unsigned long long>(small_lanes)) - 1;
+    const int last = 63 - __clzll(static_cast<long long>(small_lanes));
+    const int64_t end = base + count;
+    lo = (int64_t(__builtin_amdgcn_readlane(static_cast<int>(base >> 32), first)) << 32) |
+         uint32_t(__builtin_amdgcn_readlane(static_cast<int>(base), first));
+    hi = (int64_t(__builtin_amdgcn_readlane(static_cast<int>(end >> 32), last)) << 32) |
+         uint32_t(__builtin_amdgcn_readlane(static_cast<int>(end), last));
   }
   for (int64_t c0 = lo; c0 < hi; c0 += kChunk) {
     const bool need = small && base < c0 + kChunk && base + count > c0;
@@ -1682,8 +1710,9 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   while (big) {
     const int b = __ffsll(static_cast<unsigned long long>(big)) - 1;
     big &= big - 1;
-    const int cb = __shfl(count, b, 64);
-    const int64_t bb = __shfl(base, b, 64);
+    const int cb = __builtin_amdgcn_readlane(count, b);
+    const int64_t bb = (int64_t(__builtin_amdgcn_readlane(static_cast<int>(base >> 32), b)) << 32) |
+                       uint32_t(__builtin_amdgcn_readlane(static_cast<int>(base), b));
     float part[kVals];
 #pragma unroll
     for (int c = 0; c < kVals; ++c) part[c] = 0.0f;
@@ -1692,7 +1721,8 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     }
 #pragma unroll
     for (int c = 0; c < kVals; ++c) {
-      const float total = __shfl(wave_sum_lane63(part[c]), 63, 64);
+      const float total = __int_as_float(
+          __builtin_amdgcn_readlane(__float_as_int(wave_sum_lane63(part[c])), 63));
       if (lane == b) acc[c] = total;
     }
   }
@@ -1761,12 +1791,7 @@ __global__ __launch_bounds__(256) void k_rect_count(const float* __restrict__ me
       yr = uint32_t(r.y0) | (uint32_t(r.y1) << 16);
     }
   }
-  int incl = rect_tiles(xr, yr, sp.lg).count;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
+  const int incl = wave_inclusive_scan(rect_tiles(xr, yr, sp.lg).count);
   if (lane == 63 && (i >> 6) < sp.num_waves) sp.rect_wave[i >> 6] = incl;
 }
 

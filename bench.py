@@ -342,7 +342,7 @@ def main() -> int:
             ev["g"][0].record(stream)
         HR.backward_tiled(grad_out, means, conics, colors, radii, op, den if saved else None, out,
                           gids, offsets, H, W, True, 1e-8, ts, False, 3.0, grads=grads, ws=ws,
-                          slots=slots, slot_capacity=capacity, status=status)
+                          slots=slots, slot_capacity=capacity, status=status, after_forward=True)
         if record:
             ev["g"][1].record(stream)
 

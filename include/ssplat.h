@@ -174,6 +174,23 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                           void* stream);
 
 /*
+ * ssplat_backward_tiled for a backward that follows ssplat_forward_tiled of the same inputs
+ * (means, conics, colors, radii, opacities, tile lists, image, tile size, fade) into the same
+ * workspace, with nothing else written to that workspace in between: the forward's packed
+ * records are used as they are instead of being packed again (one launch and ~13N bytes of
+ * reads and 64N of writes less).  With any other workspace content the gradients are wrong;
+ * use ssplat_backward_tiled then.  No reference counterpart.
+ */
+int ssplat_backward_tiled_after_forward(
+    const float* grad_out, const float* means, const float* conics, const float* colors,
+    const int64_t* radii, const float* opacities, const float* den, const float* out, int64_t n,
+    const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids, int height,
+    int width, int tile_size, int normalize, double eps, double support_fade_alpha,
+    double sigma_cutoff, float* grad_means, float* grad_conics, float* grad_colors,
+    float* grad_opacities, void* grad_slots, int64_t slot_capacity, int32_t* status,
+    void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Affine (gradient) colors: Gaussian i's color at pixel p is
  *   c_i + G_i0 * lx + G_i1 * ly,   lx = ( cos t dx + sin t dy) / max(sx, 1e-6),
  *                                  ly = (-sin t dx + cos t dy) / max(sy, 1e-6),
@@ -307,6 +324,18 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                                 float* grad_opacities, void* grad_slots, int64_t slot_capacity,
                                 int32_t* status, void* workspace, size_t workspace_bytes,
                                 void* stream);
+
+/* ssplat_batch_backward_tiled after ssplat_batch_forward_tiled of the same inputs into the same
+   workspace (see ssplat_backward_tiled_after_forward).  No reference counterpart. */
+int ssplat_batch_backward_tiled_after_forward(
+    const ssplat_field* fields, int num_fields, int64_t total_gaussians, int64_t total_tiles,
+    int tile_size, const float* grad_out, const float* means, const float* conics,
+    const float* colors, const int64_t* radii, const float* opacities, const float* den,
+    const float* out, const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
+    int normalize, double eps, double support_fade_alpha, double sigma_cutoff,
+    float* grad_means, float* grad_conics, float* grad_colors, float* grad_opacities,
+    void* grad_slots, int64_t slot_capacity, int32_t* status, void* workspace,
+    size_t workspace_bytes, void* stream);
 
 /*
  * SSPL1 decoding (the reference's codec.decode, codec.py:333-391).  The host parses the

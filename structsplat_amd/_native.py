@@ -105,6 +105,9 @@ SIGNATURES = {
                                              _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                              _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_size, _c_ptr]),
 }
+# The after-forward backwards take the plain backwards' arguments.
+SIGNATURES["ssplat_backward_tiled_after_forward"] = SIGNATURES["ssplat_backward_tiled"]
+SIGNATURES["ssplat_batch_backward_tiled_after_forward"] = SIGNATURES["ssplat_batch_backward_tiled"]
 
 
 class Field(ctypes.Structure):

@@ -132,12 +132,13 @@ class Batch:
 
     def backward(self, grad_out, means, conics, colors, radii, opacities, den, out, gids,
                  offsets, normalize=True, eps=1e-8, support_fade=False, sigma_cutoff=3.0,
-                 slots=None, slot_capacity=None, status=None):
+                 slots=None, slot_capacity=None, status=None, after_forward=False):
         """Batched deterministic backward; ``den``/``out`` are the batched forward's outputs
         (``None`` allowed for additive renders).  ``slots`` holds ``slot_capacity`` gradient
         slots and ``status`` records an overflow (both from a ``BatchPlan`` in the fit loop).
         Without them, the slots get the pair buffer's length and an overflow raises (one host
-        synchronization)."""
+        synchronization).  ``after_forward``: the batch's workspace still holds the records of
+        ``forward`` on the same inputs (``ssplat_batch_backward_tiled_after_forward``)."""
         lib = _native.load()
         has_op = opacities is not None and opacities.numel() > 0
         g_m = torch.empty_like(means)
@@ -149,13 +150,16 @@ class Batch:
             slot_capacity = gids.numel()
             slots = grad_slots(slot_capacity, self.device)
             status = torch.zeros(1, dtype=torch.int32, device=self.device)
-        _native.check(lib.ssplat_batch_backward_tiled(
+        fn, name = ((lib.ssplat_batch_backward_tiled_after_forward,
+                     "ssplat_batch_backward_tiled_after_forward") if after_forward
+                    else (lib.ssplat_batch_backward_tiled, "ssplat_batch_backward_tiled"))
+        _native.check(fn(
             *self._head(), _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii),
             _ptr(opacities), _ptr(den), _ptr(out), _ptr(gids), _ptr(offsets), gids.numel(),
             int(bool(normalize)), float(eps), _fade_alpha(support_fade), float(sigma_cutoff),
             _ptr(g_m), _ptr(g_q), _ptr(g_c), _ptr(g_o), _ptr(slots), int(slot_capacity),
             _ptr(status), _ptr(self.workspace), self.workspace.numel(), _stream(self.device)),
-            "ssplat_batch_backward_tiled")
+            name)
         if check_here and int(status.item()) != 0:
             raise RuntimeError("batched backward: gradient slot capacity exceeded")
         return g_m, g_q, g_c, g_o
@@ -204,7 +208,7 @@ class _BatchRender(Function):
                                                  radii, opacities, den, out, plan.gids,
                                                  plan.offsets, *ctx.args, slots=plan.slots,
                                                  slot_capacity=plan.capacity,
-                                                 status=plan.status)
+                                                 status=plan.status, after_forward=True)
         if opacities.numel() == 0:
             g_o = None
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None

@@ -550,7 +550,9 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(BinArgs a, int32_t* _
 __global__ __launch_bounds__(kScanThreads) void k_scan_single(const int32_t* __restrict__ counts,
                                                               int64_t items,
                                                               int32_t* __restrict__ offsets,
-                                                              int32_t* __restrict__ status) {
+                                                              int32_t* __restrict__ status,
+                                                              uint32_t* __restrict__ bump) {
+  if (bump != nullptr && threadIdx.x == 0) *bump += 1u;  // a counter the caller advances
   constexpr int kPer = kSingleScanPerThread;
   constexpr int kChunk = kScanThreads * kPer;
   static_assert(kPer % 4 == 0, "counters are loaded four at a time");
@@ -661,7 +663,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int32_t* _
 
 // Second level: exclusive scan of the block sums, in place (one block, <= kMaxScanBlocks sums).
 __global__ __launch_bounds__(kScanThreads) void k_scan_top(int64_t* __restrict__ block_sums,
-                                                           int blocks) {
+                                                           int blocks,
+                                                           uint32_t* __restrict__ bump) {
+  if (bump != nullptr && threadIdx.x == 0) *bump += 1u;  // a counter the caller advances
   const int base = threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t s = 0;
@@ -975,15 +979,13 @@ struct __align__(16) GaussRec {
 };
 static_assert(sizeof(GaussRec) == 64, "GaussRec must be 64 bytes");
 
-// Per-wave slot bookkeeping written by the record packing: rect_wave[w] = slots owned by the 64
-// Gaussians of wave w (the scan input; rect_wave[num_waves] = 0), each record's c3.w = the
-// Gaussian's first slot relative to its wave.  `epoch` (the backward's slot buffer header, NULL in
-// the forward) is advanced once per call: the call's tag.
+// Per-wave slot bookkeeping written by the record packing (forward and backward alike):
+// rect_wave[w] = slots owned by the 64 Gaussians of wave w (the scan input; rect_wave[num_waves]
+// = 0), each record's c3.w = the Gaussian's first slot relative to its wave.
 struct SlotPrep {
   int32_t* rect_wave;
   int num_waves;
   int lg;
-  uint32_t* epoch;
   int32_t* rect_status;
 };
 
@@ -998,7 +1000,6 @@ __global__ __launch_bounds__(256) void k_prepare(
   if (i == 0) {
     sp.rect_wave[sp.num_waves] = 0;
     *sp.rect_status = SSPLAT_OK;
-    if (sp.epoch != nullptr) *sp.epoch += 1u;
   }
   // Every input is loaded up front (one memory round trip per thread).
   const bool live = i < n;
@@ -2014,20 +2015,20 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
 // Exclusive int32 scan of counts[0, items) into offsets (saturating; *status flags an overflow):
 // one launch up to kSingleScanItems items, three beyond.
 int run_scan(const int32_t* counts, int64_t items, int32_t* offsets, int64_t* bsums,
-             int32_t* status, hipStream_t stream) {
+             int32_t* status, hipStream_t stream, uint32_t* bump = nullptr) {
   if (items > kMaxScanItems) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "scan of %lld items is out of range",
                      static_cast<long long>(items));
   }
   if (items <= kSingleScanItems) {
-    k_scan_single<<<1, kScanThreads, 0, stream>>>(counts, items, offsets, status);
+    k_scan_single<<<1, kScanThreads, 0, stream>>>(counts, items, offsets, status, bump);
     SSPLAT_LAUNCH_CHECK("k_scan_single");
     return SSPLAT_OK;
   }
   const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
   k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums);
   SSPLAT_LAUNCH_CHECK("k_scan_partials");
-  k_scan_top<<<1, kScanThreads, 0, stream>>>(bsums, blocks);
+  k_scan_top<<<1, kScanThreads, 0, stream>>>(bsums, blocks, bump);
   SSPLAT_LAUNCH_CHECK("k_scan_top");
   k_scan_apply<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums, offsets, status);
   SSPLAT_LAUNCH_CHECK("k_scan_apply");
@@ -2162,18 +2163,18 @@ RenderArgs make_render_args(const Workspace& w, const Plan& p, const int32_t* gi
 
 int tile_lg(int ts) { return ts == 32 ? 5 : 4; }
 
-SlotPrep make_slot_prep(const Plan& p, const Workspace& w, uint32_t* epoch) {
-  return SlotPrep{w.rect_wave, w.num_waves, tile_lg(p.ts), epoch, w.rect_status};
+SlotPrep make_slot_prep(const Plan& p, const Workspace& w) {
+  return SlotPrep{w.rect_wave, w.num_waves, tile_lg(p.ts), w.rect_status};
 }
 
 // Packs the records with each Gaussian's slot bookkeeping; without support fade the conic is
-// folded with -log2(e)/2.  `epoch` (the backward's slot header) is advanced; NULL in the forward.
+// folded with -log2(e)/2.  The forward and the backward pack identical records.
 int run_prepare(const float* means, const float* conics, const float* colors,
                 const int64_t* radii, const float* opacities, const Plan& p, const Workspace& w,
-                bool support_fade, uint32_t* epoch, hipStream_t stream) {
+                bool support_fade, hipStream_t stream) {
   if (p.n == 0) return SSPLAT_OK;
   const int blocks = static_cast<int>((p.n + 255) / 256);
-  const SlotPrep sp = make_slot_prep(p, w, epoch);
+  const SlotPrep sp = make_slot_prep(p, w);
   if (support_fade) {
     k_prepare<false><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
                                                  static_cast<int>(p.n), p.table, w.recs, sp);
@@ -2298,8 +2299,7 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   if (af != nullptr && (rc = prepare_affine(p, *af, ws, ws_bytes, w, false, &arecs, s))) {
     return rc;
   }
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, nullptr,
-                        s))) {
+  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, s))) {
     return rc;
   }
   RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
@@ -2334,7 +2334,8 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
                    double support_fade_alpha, double sigma_cutoff, float* grad_means,
                    float* grad_conics, float* grad_colors, float* grad_opacities,
                    void* grad_slots, int64_t slot_capacity, int32_t* status, void* ws,
-                   size_t ws_bytes, void* stream, const Affine* af = nullptr) {
+                   size_t ws_bytes, void* stream, const Affine* af = nullptr,
+                   bool after_forward = false) {
   int rc;
   double fade_alpha;
   if ((rc = check_fade(support_fade_alpha, &fade_alpha))) return rc;
@@ -2365,11 +2366,15 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   if (af != nullptr && (rc = prepare_affine(p, *af, ws, ws_bytes, w, true, &arecs, s))) {
     return rc;
   }
-  if ((rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, epoch, s))) {
+  // After a forward of the same inputs into this workspace, its records (and their slot
+  // bookkeeping) are still there and are used as they are.
+  if (!after_forward &&
+      (rc = run_prepare(means, conics, colors, radii, opacities, p, w, support_fade, s))) {
     return rc;
   }
+  // The slot offsets; the scan also advances the slot buffer's call counter (the call's tag).
   if ((rc = run_scan(w.rect_wave, int64_t(w.num_waves) + 1, w.rect_off, w.rect_bsums,
-                     w.rect_status, s))) {
+                     w.rect_status, s, epoch))) {
     return rc;
   }
   RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
@@ -2442,7 +2447,7 @@ int rect_pair_count(const Plan& p, const float* means, const int64_t* radii, int
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int blocks = static_cast<int>((p.n + 255) / 256);
   k_rect_count<<<blocks > 0 ? blocks : 1, 256, 0, s>>>(means, radii, static_cast<int>(p.n),
-                                                       p.table, make_slot_prep(p, w, nullptr));
+                                                       p.table, make_slot_prep(p, w));
   SSPLAT_LAUNCH_CHECK("k_rect_count");
   if ((rc = run_scan(w.rect_wave, int64_t(w.num_waves) + 1, w.rect_off, w.rect_bsums,
                      w.rect_status, s))) {
@@ -2584,6 +2589,23 @@ int ssplat_backward_tiled(const float* grad_out, const float* means, const float
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
                         slot_capacity, status, workspace, workspace_bytes, stream);
+}
+
+int ssplat_backward_tiled_after_forward(
+    const float* grad_out, const float* means, const float* conics, const float* colors,
+    const int64_t* radii, const float* opacities, const float* den, const float* out, int64_t n,
+    const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids, int height,
+    int width, int tile_size, int normalize, double eps, double support_fade_alpha,
+    double sigma_cutoff, float* grad_means, float* grad_conics, float* grad_colors,
+    float* grad_opacities, void* grad_slots, int64_t slot_capacity, int32_t* status,
+    void* workspace, size_t workspace_bytes, void* stream) {
+  Plan p;
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
+                        tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
+                        grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
+                        slot_capacity, status, workspace, workspace_bytes, stream, nullptr, true);
 }
 
 int ssplat_forward_tiled_affine(const float* means, const float* conics, const float* colors,
@@ -2759,6 +2781,24 @@ int ssplat_batch_backward_tiled(const ssplat_field* fields, int num_fields,
                         tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
                         grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
                         slot_capacity, status, workspace, workspace_bytes, stream);
+}
+
+int ssplat_batch_backward_tiled_after_forward(
+    const ssplat_field* fields, int num_fields, int64_t total_gaussians, int64_t total_tiles,
+    int tile_size, const float* grad_out, const float* means, const float* conics,
+    const float* colors, const int64_t* radii, const float* opacities, const float* den,
+    const float* out, const int32_t* tile_gids, const int32_t* tile_offsets, int64_t num_gids,
+    int normalize, double eps, double support_fade_alpha, double sigma_cutoff,
+    float* grad_means, float* grad_conics, float* grad_colors, float* grad_opacities,
+    void* grad_slots, int64_t slot_capacity, int32_t* status, void* workspace,
+    size_t workspace_bytes, void* stream) {
+  Plan p;
+  const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
+  if (rc) return rc;
+  return backward_tiled(p, grad_out, means, conics, colors, radii, opacities, den, out, tile_gids,
+                        tile_offsets, num_gids, normalize, eps, support_fade_alpha, sigma_cutoff,
+                        grad_means, grad_conics, grad_colors, grad_opacities, grad_slots,
+                        slot_capacity, status, workspace, workspace_bytes, stream, nullptr, true);
 }
 
 }  // extern "C"

@@ -111,7 +111,8 @@ class _PlannedRender(Function):
         g_m, g_q, g_c, g_o = HR.backward_tiled(
             grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, plan.gids,
             plan.offsets, plan.H, plan.W, normalize, eps, plan.tile_size, fade, sigma_cutoff,
-            ws=plan.ws, slots=plan.slots, slot_capacity=plan.capacity, status=plan.status)
+            ws=plan.ws, slots=plan.slots, slot_capacity=plan.capacity, status=plan.status,
+            after_forward=True)
         if opacities.numel() == 0:
             g_o = None
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None

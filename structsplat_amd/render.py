@@ -263,14 +263,16 @@ def forward_tiled(means, conics, colors, radii, opacities, gids, offsets, H: int
 def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, gids, offsets,
                    H: int, W: int, normalize: bool, eps: float, tile_size: int,
                    support_fade: bool, sigma_cutoff: float, grads=None, ws=None, slots=None,
-                   slot_capacity=None, status=None):
+                   slot_capacity=None, status=None, after_forward: bool = False):
     """One tiled backward; returns ``(g_means, g_conics, g_colors, g_opacities or None)``.
 
     ``den`` and ``out`` are the forward's coverage and image for the same inputs (the
     reference's saved tensors, cuda_render.py:111-113); additive renders may pass ``None``.
     The gradients are deterministic.  ``slots`` (from ``grad_slots``) holds ``slot_capacity``
     gradient slots; without it a buffer is made for the field's exact slot count.  ``status``
-    (device int32) is set to 2 if the slot capacity was too small.
+    (device int32) is set to 2 if the slot capacity was too small.  ``after_forward``: ``ws``
+    holds the records of ``forward_tiled`` on the same inputs, which are used as they are
+    (``ssplat_backward_tiled_after_forward``).
     """
     lib = _native.load()
     dev = means.device
@@ -288,13 +290,16 @@ def backward_tiled(grad_out, means, conics, colors, radii, opacities, den, out, 
         slots = grad_slots(slot_capacity, dev)
     elif slot_capacity is None:
         raise ValueError("slot_capacity is required with a slot buffer")
-    _native.check(lib.ssplat_backward_tiled(
+    if after_forward and ws is None:
+        raise ValueError("after_forward needs the forward's workspace")
+    fn, name = ((lib.ssplat_backward_tiled_after_forward, "ssplat_backward_tiled_after_forward")
+                if after_forward else (lib.ssplat_backward_tiled, "ssplat_backward_tiled"))
+    _native.check(fn(
         _ptr(grad_out), _ptr(means), _ptr(conics), _ptr(colors), _ptr(radii), _ptr(opacities),
         _ptr(den), _ptr(out), n, _ptr(gids), _ptr(offsets), gids.numel(), int(H), int(W),
         int(tile_size), int(bool(normalize)), float(eps), _fade_alpha(support_fade),
         float(sigma_cutoff), _ptr(g_means), _ptr(g_conics), _ptr(g_colors), _ptr(g_opac),
-        _ptr(slots), int(slot_capacity), _ptr(status), _ptr(ws), ws.numel(), _stream(dev)),
-        "ssplat_backward_tiled")
+        _ptr(slots), int(slot_capacity), _ptr(status), _ptr(ws), ws.numel(), _stream(dev)), name)
     return g_means, g_conics, g_colors, g_opac
 
 
@@ -378,6 +383,7 @@ class _ExactRenderHip(Function):
                                int(height), int(width), bool(normalize), float(eps),
                                int(tile_size), alpha, float(sigma_cutoff), den=den, ws=ws)
         ctx.save_for_backward(means, conics, colors, radii, opacities, den, out, gids, offsets)
+        ctx.ws = ws  # holds the forward's packed records, which the backward reuses
         ctx.height = int(height)
         ctx.width = int(width)
         ctx.normalize = bool(normalize)
@@ -398,7 +404,7 @@ class _ExactRenderHip(Function):
         g_means, g_conics, g_colors, g_opac = backward_tiled(
             grad_out.contiguous(), means, conics, colors, radii, opacities, den, out, gids,
             offsets, ctx.height, ctx.width, ctx.normalize, ctx.eps, ctx.tile_size, ctx.support_fade,
-            ctx.sigma_cutoff, slot_capacity=ctx.slot_capacity)
+            ctx.sigma_cutoff, ws=ctx.ws, slot_capacity=ctx.slot_capacity, after_forward=True)
         if opacities.numel() == 0 or not need_opac:
             g_opac = None
         return (

@@ -78,7 +78,7 @@ def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, no
                       sigma_cutoff, out=out, den=den)
             b.backward(grad, means, conics, colors, radii, op, den, out, gids, offsets,
                        normalize, eps, alpha, sigma_cutoff, slots=slots, slot_capacity=pairs + 1,
-                       status=status)
+                       status=status, after_forward=True)
 
         timings[ts] = _time_ms(step, reps)
     best = min(timings, key=timings.get)

@@ -91,6 +91,51 @@ def test_batched_backward_is_bitwise_equal_to_per_field(cuda, ts):
             assert torch.equal(got, exp), k
 
 
+@pytest.mark.parametrize("ts", [16, 32])
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+def test_after_forward_backward_is_bitwise_equal(cuda, ts, fade):
+    """The backward that reuses the forward's packed records (single field and batch) gives the
+    plain backward's gradients bit for bit, in every normalizer mode, and twice in a row."""
+    _case(test="after-forward backward", tile=ts, fade=fade)
+    H, W, N = 96, 120, 700
+    means, conics, colors, radii, op = _field(cuda, 23, N, H, W, scale_hi=20.0)
+    alpha = HR._fade_alpha(fade)
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, ts, alpha == 1.0, 3.0)
+    cap = HR.rect_pair_count(means, radii, H, W, ts)
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(8))
+    for keep_den in (False, True):
+        ws = HR.workspace(N, H, W, ts, cuda)
+        den = torch.empty(H, W, device=cuda) if keep_den else None
+        out, _ = HR.forward_tiled(means, conics, colors, radii, op, gids, offsets, H, W, True,
+                                  1e-8, ts, alpha, 3.0, den=den, ws=ws)
+        ref = HR.backward_tiled(g, means, conics, colors, radii, op, den, out, gids, offsets, H,
+                                W, True, 1e-8, ts, alpha, 3.0, slot_capacity=cap)
+        slots = HR.grad_slots(cap, cuda)
+        for _ in range(2):
+            got = HR.backward_tiled(g, means, conics, colors, radii, op, den, out, gids, offsets,
+                                    H, W, True, 1e-8, ts, alpha, 3.0, ws=ws, slots=slots,
+                                    slot_capacity=cap, after_forward=True)
+            torch.cuda.synchronize()
+            for a, e in zip(got, ref):
+                assert torch.equal(a, e), keep_den
+    # Batch: the batch's own workspace after its forward.
+    sizes, counts = [(40, 70), (96, 120)], [150, N]
+    b = B.Batch(sizes, counts, cuda, tile_size=ts)
+    small = _field(cuda, 24, 150, 40, 70, scale_hi=20.0)
+    cat = [torch.cat([x, y], 0).contiguous()
+           for x, y in zip(small, (means, conics, colors, radii, op))]
+    bg, bo, _ = b.build_tile_index(cat[0], cat[1], cat[3], 1 << 16, alpha == 1.0, 3.0)
+    grad = torch.rand(b.total_pixels, 3, device=cuda,
+                      generator=torch.Generator(device=cuda).manual_seed(9))
+    bout = b.forward(*cat, bg, bo, True, 1e-8, alpha, 3.0)
+    bref = b.backward(grad, *cat, None, bout, bg, bo, True, 1e-8, alpha, 3.0)
+    bout = b.forward(*cat, bg, bo, True, 1e-8, alpha, 3.0)
+    bgot = b.backward(grad, *cat, None, bout, bg, bo, True, 1e-8, alpha, 3.0, after_forward=True)
+    torch.cuda.synchronize()
+    for a, e in zip(bgot, bref):
+        assert torch.equal(a, e)
+
+
 def test_slot_capacity_overflow_sets_status(cuda):
     _case(test="slot capacity overflow")
     H, W, N = 64, 80, 200

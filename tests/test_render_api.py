@@ -62,3 +62,30 @@ def test_field_producer_dilation_validation():
     with pytest.raises(ValueError, match="dilation must be >= 0"):
         g.radii(3.0, -0.5)
     assert g.radii(3.0).dtype == torch.int64
+
+
+def test_affine_and_coverage_entry_points_reject_cpu_tensors():
+    """Affine colors and the raw weight map run only on the GPU, like every mode."""
+    from structsplat_amd import raw_weight_map, render_hip_affine
+
+    m, q, c, r = _field()
+    g = torch.zeros(1, 2, 3)
+    with pytest.raises(RuntimeError, match="requires CUDA tensors"):
+        render_field(m, q, c, r, 10, 10, color_grads=g, scales=torch.ones(1, 2),
+                     rotations=torch.zeros(1))
+    with pytest.raises(RuntimeError, match="requires CUDA tensors"):
+        render_hip_affine(m, q, c, r, 10, 10, g, torch.ones(1, 2), torch.zeros(1))
+    with pytest.raises(RuntimeError, match="requires CUDA tensors"):
+        raw_weight_map(m, q, r, 10, 10)
+
+
+def test_raw_weight_map_field_keeps_the_reference_signature():
+    """fit.py:338-340: (field, cfg, H, W, support_fade_alpha=None, detach_opacities=False)."""
+    import inspect
+
+    from structsplat_amd.fit import raw_weight_map_field
+
+    params = list(inspect.signature(raw_weight_map_field).parameters.items())
+    assert [k for k, _ in params] == ["field", "cfg", "H", "W", "support_fade_alpha",
+                                      "detach_opacities"]
+    assert params[4][1].default is None and params[5][1].default is False

@@ -268,7 +268,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     plan = BatchPlan(batch, F._capacity(count(), cfg), cfg.recompute_normalizer)
 
     def step():
-        opt.zero_grad(set_to_none=False)
+        opt.zero_grad(set_to_none=True)  # gradients assigned, not accumulated
         conics, radii, op = big.produce(cfg.sigma_cutoff)
         op = big.means.new_empty(0) if op is None else op
         img = _BatchRender.apply(big.means, conics, big.colors, radii, op, plan, cfg.normalize,

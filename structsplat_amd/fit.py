@@ -205,7 +205,8 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     state = {"loss": None}
 
     def step():
-        opt.zero_grad(set_to_none=False)
+        # Gradients are assigned, not accumulated: no fill and add kernels per parameter.
+        opt.zero_grad(set_to_none=True)
         img = planned_render(field, plan, cfg)
         loss = loss_fn(img, target, cfg.ssim_weight)
         loss.backward()

@@ -48,9 +48,16 @@ __global__ void k_adam_tick(float* step) {
 }
 
 __global__ __launch_bounds__(kAdamThreads) void k_adam(AdamArgs a) {
-  const float t = *a.step;
-  const float bc1 = 1.0f - powf(a.beta1, t);
-  const float bc2_sqrt = sqrtf(1.0f - powf(a.beta2, t));
+  // The bias corrections, once per workgroup (powf and sqrtf cost more than the update).
+  __shared__ float s_bc[2];
+  if (threadIdx.x == 0) {
+    const float t = *a.step;
+    s_bc[0] = 1.0f - powf(a.beta1, t);
+    s_bc[1] = sqrtf(1.0f - powf(a.beta2, t));
+  }
+  __syncthreads();
+  const float bc1 = s_bc[0];
+  const float bc2_sqrt = s_bc[1];
   const int64_t total = a.start[a.num_groups];
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
@@ -66,7 +73,11 @@ __global__ __launch_bounds__(kAdamThreads) void k_adam(AdamArgs a) {
     const float step_size = g.lr / bc1;
     float p = g.param[j] - step_size * m / (sqrtf(v) / bc2_sqrt + a.eps);
     if (g.clamp) {
-      const float hi = g.hi_rows != nullptr ? g.hi_rows[j / g.row_width] : g.hi;
+      // Rows of at most 2^31 elements and tables below 2^32 (checked on the host): a 32-bit
+      // division instead of a 64-bit one.
+      const float hi = g.hi_rows != nullptr
+                           ? g.hi_rows[static_cast<uint32_t>(j) / static_cast<uint32_t>(g.row_width)]
+                           : g.hi;
       p = fminf(fmaxf(p, g.lo), hi);
     }
     g.m[j] = m;
@@ -104,6 +115,9 @@ int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* ste
     }
     if (s.clamp && s.hi_rows != nullptr && s.row_width <= 0) {
       return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: row_width must be > 0", k);
+    }
+    if (s.count > (int64_t(1) << 32) - 1 || s.row_width > (int64_t(1) << 31)) {
+      return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: more than 2^32 - 1 elements", k);
     }
     AdamGroupArgs& g = a.g[k];
     g.param = s.param;

@@ -21,6 +21,7 @@ using namespace ssplat_detail;
 
 constexpr int kMaxGroups = 8;
 constexpr int kAdamThreads = 256;
+constexpr int kMaxGroupBlocks = 4096;
 
 struct AdamGroupArgs {
   float* param;
@@ -37,7 +38,7 @@ struct AdamGroupArgs {
 
 struct AdamArgs {
   AdamGroupArgs g[kMaxGroups];
-  int64_t start[kMaxGroups + 1];  // prefix of the group counts
+  int block_start[kMaxGroups + 1];  // each group's first workgroup (prefix of the block counts)
   int num_groups;
   const float* step;
   float beta1, beta2, eps;
@@ -58,13 +59,13 @@ __global__ __launch_bounds__(kAdamThreads) void k_adam(AdamArgs a) {
   __syncthreads();
   const float bc1 = s_bc[0];
   const float bc2_sqrt = s_bc[1];
-  const int64_t total = a.start[a.num_groups];
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-    int k = 0;
-    while (k + 1 < a.num_groups && i >= a.start[k + 1]) ++k;
-    const AdamGroupArgs& g = a.g[k];
-    const int64_t j = i - a.start[k];
+  // Every workgroup serves one group (a uniform choice: the group's arguments stay scalar).
+  int k = 0;
+  while (k + 1 < a.num_groups && static_cast<int>(blockIdx.x) >= a.block_start[k + 1]) ++k;
+  const AdamGroupArgs& g = a.g[k];
+  const int64_t stride = int64_t(a.block_start[k + 1] - a.block_start[k]) * blockDim.x;
+  for (int64_t j = int64_t(blockIdx.x - a.block_start[k]) * blockDim.x + threadIdx.x;
+       j < g.count; j += stride) {
     const float grad = g.grad[j];
     float m = g.m[j];
     float v = g.v[j];
@@ -105,7 +106,7 @@ int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* ste
   a.beta1 = static_cast<float>(beta1);
   a.beta2 = static_cast<float>(beta2);
   a.eps = static_cast<float>(eps);
-  int64_t total = 0;
+  int blocks = 0;
   for (int k = 0; k < num_groups; ++k) {
     const ssplat_adam_group& s = groups[k];
     if (s.count < 0) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "group %d: count < 0", k);
@@ -131,16 +132,16 @@ int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* ste
     g.hi = static_cast<float>(s.hi);
     g.hi_rows = s.hi_rows;
     g.row_width = s.row_width > 0 ? s.row_width : 1;
-    a.start[k] = total;
-    total += s.count;
+    // One workgroup per 256 elements, at most kMaxGroupBlocks per group (grid-stride beyond).
+    a.block_start[k] = blocks;
+    const int64_t need = (s.count + kAdamThreads - 1) / kAdamThreads;
+    blocks += static_cast<int>(need < kMaxGroupBlocks ? need : kMaxGroupBlocks);
   }
-  a.start[num_groups] = total;
+  a.block_start[num_groups] = blocks;
   hipStream_t st = static_cast<hipStream_t>(stream);
   k_adam_tick<<<1, 64, 0, st>>>(step);
   SSPLAT_LAUNCH_CHECK("k_adam_tick");
-  if (total > 0) {
-    const int64_t blocks64 = (total + kAdamThreads - 1) / kAdamThreads;
-    const int blocks = static_cast<int>(blocks64 < 8192 ? blocks64 : 8192);
+  if (blocks > 0) {
     k_adam<<<blocks, kAdamThreads, 0, st>>>(a);
     SSPLAT_LAUNCH_CHECK("k_adam");
   }

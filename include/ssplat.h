@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SSPLAT_ABI_VERSION 2
+#define SSPLAT_ABI_VERSION 3  /* 3: affine-color and after-forward entry points */
 
 enum ssplat_status {
   SSPLAT_OK = 0,

@@ -1656,12 +1656,15 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
   float acc[kVals];
 #pragma unroll
   for (int c = 0; c < kVals; ++c) acc[c] = 0.0f;
-  // Lane sums.  A Gaussian's slots follow the previous Gaussian's, so the wave's slots form one
-  // run; it is copied to LDS kChunk slots at a time with coalesced loads (lanes reading their
-  // own slots straight from memory would touch a different cache line per lane and load), and
-  // every lane adds its slots of the chunk in ascending order.
+  // Lane sums, each in ascending slot order.  A Gaussian's slots follow the previous
+  // Gaussian's, so the wave's slots form one run.  With a few slots per lane the run is copied
+  // to LDS kChunk slots at a time with coalesced loads (lanes reading their own slots straight
+  // from memory would touch a different cache line per lane and load); with many, each lane
+  // streams its own slots (a chunk would then hold the slots of only a few lanes).
   constexpr int kChunk = 2560 / kWords;      // slots per chunk: 10 KB per wave
   constexpr int kPer = kChunk * kWords / 128;  // float2 loads per lane and chunk
+  constexpr int kChunkedMaxPerLane = 12;       // slots per small lane up to which chunks pay
+  constexpr int kInFlight = kWords <= kSlotWords ? 8 : 4;  // loads issued together per lane
   __shared__ float2 s_chunk[256 / 64][kChunk * kWords / 2];
   float2* chunk = s_chunk[threadIdx.x >> 6];
   const bool small = count > 0 && count <= kLaneSumMax;
@@ -1678,7 +1681,27 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     hi = (int64_t(__builtin_amdgcn_readlane(static_cast<int>(end >> 32), last)) << 32) |
          uint32_t(__builtin_amdgcn_readlane(static_cast<int>(end), last));
   }
-  for (int64_t c0 = lo; c0 < hi; c0 += kChunk) {
+  const bool chunked = hi - lo <= int64_t(kChunkedMaxPerLane) * __popcll(small_lanes);
+  if (!chunked && small) {
+    const float* sl = a.slots + base * kWords;
+    for (int k = 0; k < count; k += kInFlight) {
+      float2 v[kInFlight][kWords / 2];
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u) {
+        const bool live = k + u < count;
+#pragma unroll
+        for (int w = 0; w < kWords / 2; ++w) {
+          v[u][w] = live ? *reinterpret_cast<const float2*>(sl + (k + u) * kWords + 2 * w)
+                         : make_float2(0.0f, 0.0f);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u) {
+        if (k + u < count) add_slot_values<kWords, kVals>(v[u], tag, acc);
+      }
+    }
+  }
+  for (int64_t c0 = lo; chunked && c0 < hi; c0 += kChunk) {
     const bool need = small && base < c0 + kChunk && base + count > c0;
     if (__ballot(need) == 0) continue;  // a run of big Gaussians' slots
     const int len2 = static_cast<int>(hi - c0 < kChunk ? hi - c0 : kChunk) * (kWords / 2);

@@ -595,6 +595,33 @@ def raw_weight_map(means, conics, radii, H: int, W: int, opacities=None,
     return out[..., 0].reshape(-1)
 
 
+def render(means, conics, colors, radii, H: int, W: int, chunk: int = 4096, opacities=None,
+           support_fade: bool = False, sigma_cutoff: float = 3.0, color_grads=None,
+           scales=None, rotations=None, support_fade_alpha: float | None = None,
+           checkpoint_chunks: bool = False, normalization_eps: float = _EPS):
+    """The normalized weighted-sum renderer under the reference's name and arguments
+    (render.py:238-249), on the HIP kernels (affine colors included)."""
+    return render_field(means, conics, colors, radii, H, W, chunk, "normalized", opacities,
+                        scales, rotations, support_fade, sigma_cutoff, color_grads,
+                        support_fade_alpha, checkpoint_chunks, normalization_eps)
+
+
+def render_additive(means, conics, colors, radii, H: int, W: int, chunk: int = 4096,
+                    opacities=None, support_fade: bool = False, sigma_cutoff: float = 3.0,
+                    color_grads=None, scales=None, rotations=None,
+                    support_fade_alpha: float | None = None,
+                    checkpoint_chunks: bool = False):
+    """The additive renderer under the reference's name and arguments (render.py:252-263)."""
+    return render_field(means, conics, colors, radii, H, W, chunk, "additive", opacities,
+                        scales, rotations, support_fade, sigma_cutoff, color_grads,
+                        support_fade_alpha, checkpoint_chunks)
+
+
+# The reference's CUDA entry names (cuda_render.py:229-294) for callers that import them.
+render_cuda_exact = render_hip_exact
+render_cuda_exact_with_coverage = render_hip_exact_with_coverage
+
+
 def render_field(means, conics, colors, radii, H: int, W: int,
                  chunk: int = 4096, mode: str = "normalized", opacities=None,
                  scales=None, rotations=None, support_fade: bool = False,

@@ -306,3 +306,23 @@ def test_normalized_render_is_invariant_to_opacity_scale(cuda, fade):
     live = to_np(den1) > 1e-3
     assert np.abs(to_np(out4) - to_np(out1))[live].max() <= 1e-6 * max(
         1.0, float(np.abs(to_np(out1)).max()))
+
+
+def test_reference_renderer_names_route_to_render_field(cuda):
+    """render / render_additive (render.py:238-263) and the reference's CUDA entry names give
+    render_field's images bit for bit."""
+    _print_case(test="reference renderer names")
+    N, H, W = 200, 40, 56
+    f = random_field(31, N, H, W, scale_lo=0.7, scale_hi=5.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    a = HR.render(means, conics, colors, radii, H, W, opacities=op, support_fade=True,
+                  support_fade_alpha=0.5)
+    b = HR.render_field(means, conics, colors, radii, H, W, mode="normalized", opacities=op,
+                        support_fade=True, support_fade_alpha=0.5)
+    c = HR.render_additive(means, conics, colors, radii, H, W, opacities=op)
+    d = HR.render_field(means, conics, colors, radii, H, W, mode="additive", opacities=op)
+    e = HR.render_cuda_exact(means, conics, colors, radii, H, W, opacities=op)
+    g = HR.render_field(means, conics, colors, radii, H, W, mode="cuda", opacities=op)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.equal(c, d) and torch.equal(e, g)
+    assert HR.render_cuda_exact_with_coverage is HR.render_hip_exact_with_coverage

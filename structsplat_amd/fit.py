@@ -192,6 +192,9 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     tuning and first plan before it are not)."""
     H, W = target.shape[:2]
     dev = target.device
+    if field.color_grads is not None:
+        raise ValueError("the planned fit step renders constant colors; render affine colors "
+                         "(color_grads) with render_field or render_hip_affine")
     field.trainable()
     opt = make_optimizer(field, cfg)
     opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))

@@ -17,38 +17,44 @@ from torch.autograd.function import once_differentiable
 
 
 class GaussianField:
-    def __init__(self, means, log_scales, rotations, colors, opacities=None,
-                 filter_variance=None):
-        self.means = means
-        self.log_scales = log_scales
-        self.rotations = rotations
-        self.colors = colors
-        self.opacities = opacities
+    """The reference's field (gaussians.py:15-29), with its constructor's argument order."""
+
+    def __init__(self, means, log_scales, rotations, colors, opacities=None, scale_max=None,
+                 color_grads=None, background_mask=None, filter_variance=None):
+        self.means = means            # (N, 2)
+        self.log_scales = log_scales  # (N, 2)
+        self.rotations = rotations    # (N,)
+        self.colors = colors          # base colors (N, 3)
+        self.opacities = opacities    # optional logits (N,)
+        self.scale_max = scale_max    # optional per-Gaussian scale cap (N, 2), kept as data
+        self.color_grads = color_grads  # optional affine color coefficients (N, 2, 3)
+        self.background_mask = background_mask  # optional bool rows, kept as data
         # Optional non-trainable isotropic covariance variance (px^2) per row.
         self.filter_variance = filter_variance
 
     @classmethod
-    def from_numpy(cls, means, scales, angles, colors, opacities=None, device="cpu",
-                   dtype=torch.float32, filter_variance=None):
+    def from_numpy(cls, means, scales, angles, colors, opacities=None, scale_max=None,
+                   device="cpu", dtype=torch.float32, color_grads=None,
+                   background_mask=None, filter_variance=None):
         """Build a field from NumPy arrays; scales are clipped to >= 1e-3 (gaussians.py:31-55)."""
         def t(a):
             return torch.as_tensor(np.asarray(a), device=device, dtype=dtype).clone()
 
         scales = np.clip(np.asarray(scales), 1e-3, None)
         opacity_t = None if opacities is None else t(opacities).reshape(-1)
-        fv = None if filter_variance is None else t(filter_variance).reshape(-1)
+        mask = (None if background_mask is None else torch.as_tensor(
+            np.asarray(background_mask), device=device, dtype=torch.bool).reshape(-1).clone())
         return cls(t(means), torch.log(t(scales)), t(angles).reshape(-1), t(colors), opacity_t,
-                   fv)
+                   None if scale_max is None else t(scale_max),
+                   None if color_grads is None else t(color_grads), mask,
+                   None if filter_variance is None else t(filter_variance).reshape(-1))
 
     @property
     def n(self) -> int:
         return self.means.shape[0]
 
     def trainable(self) -> "GaussianField":
-        params = [self.means, self.log_scales, self.rotations, self.colors]
-        if self.opacities is not None:
-            params.append(self.opacities)
-        for p in params:
+        for p in self.parameters():
             p.requires_grad_(True)
         return self
 
@@ -57,12 +63,16 @@ class GaussianField:
             return None if x is None else x.detach().clone()
 
         return GaussianField(d(self.means), d(self.log_scales), d(self.rotations),
-                             d(self.colors), d(self.opacities), d(self.filter_variance))
+                             d(self.colors), d(self.opacities), d(self.scale_max),
+                             d(self.color_grads), d(self.background_mask),
+                             d(self.filter_variance))
 
     def parameters(self) -> list[torch.Tensor]:
         params = [self.means, self.log_scales, self.rotations, self.colors]
         if self.opacities is not None:
             params.append(self.opacities)
+        if self.color_grads is not None:
+            params.append(self.color_grads)
         return params
 
     def parameter_groups(self, lr_means, lr_scales, lr_rot, lr_color, lr_opacity=1e-2):
@@ -75,7 +85,47 @@ class GaussianField:
         ]
         if self.opacities is not None:
             groups.append({"params": [self.opacities], "lr": lr_opacity})
+        if self.color_grads is not None:
+            groups.append({"params": [self.color_grads], "lr": lr_color})
         return groups
+
+    def with_affine_colors(self) -> "GaussianField":
+        """The field with zero affine color coefficients added (gaussians.py:216-229)."""
+        if self.color_grads is not None:
+            return self
+        return GaussianField(self.means, self.log_scales, self.rotations, self.colors,
+                             self.opacities, self.scale_max,
+                             torch.zeros(self.n, 2, 3, device=self.colors.device,
+                                         dtype=self.colors.dtype),
+                             self.background_mask, self.filter_variance)
+
+    def effective_scales(self, dilation: float = 0.0):
+        """Scales after the isotropic covariance filter and anti-alias dilation
+        (gaussians.py:234-246)."""
+        if dilation < 0.0:
+            raise ValueError(f"dilation must be >= 0, got {dilation}")
+        scales = self.scales()
+        if self.filter_variance is None and dilation == 0.0:
+            return scales
+        variance = scales.square() + float(dilation)
+        if self.filter_variance is not None:
+            variance = variance + self.filter_variance.to(device=scales.device,
+                                                          dtype=scales.dtype).reshape(-1, 1)
+        return torch.sqrt(variance.clamp_min(1e-12))
+
+    def effective_log_scales(self, dilation: float = 0.0):
+        """log of ``effective_scales`` (gaussians.py:248-252)."""
+        if self.filter_variance is None and dilation == 0.0:
+            return self.log_scales
+        return torch.log(self.effective_scales(dilation))
+
+    def materialize_covariance_filter(self) -> "GaussianField":
+        """The filter variance folded into the scales (gaussians.py:254-268)."""
+        if self.filter_variance is None:
+            return self
+        return GaussianField(self.means, self.effective_log_scales(), self.rotations, self.colors,
+                             self.opacities, self.scale_max, self.color_grads,
+                             self.background_mask, None)
 
     def scales(self):
         return torch.exp(self.log_scales)
@@ -149,10 +199,11 @@ class GaussianField:
             "rotations": self.rotations.detach().cpu().numpy(),
             "colors": self.colors.detach().cpu().numpy(),
         }
-        if self.opacities is not None:
-            data["opacities"] = self.opacities.detach().cpu().numpy()
-        if self.filter_variance is not None:
-            data["filter_variance"] = self.filter_variance.detach().cpu().numpy()
+        for name in ("opacities", "scale_max", "color_grads", "background_mask",
+                     "filter_variance"):
+            value = getattr(self, name)
+            if value is not None:
+                data[name] = value.detach().cpu().numpy()
         np.savez(path, **data)
 
     @classmethod
@@ -162,9 +213,13 @@ class GaussianField:
         def t(k):
             return torch.as_tensor(z[k], device=device, dtype=torch.float32)
 
-        return cls(t("means"), t("log_scales"), t("rotations"), t("colors"),
-                   t("opacities") if "opacities" in z.files else None,
-                   t("filter_variance") if "filter_variance" in z.files else None)
+        def opt(k):
+            return t(k) if k in z.files else None
+
+        mask = (torch.as_tensor(z["background_mask"], device=device, dtype=torch.bool)
+                if "background_mask" in z.files else None)
+        return cls(t("means"), t("log_scales"), t("rotations"), t("colors"), opt("opacities"),
+                   opt("scale_max"), opt("color_grads"), mask, opt("filter_variance"))
 
 
 def _ptr(t):

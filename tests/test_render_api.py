@@ -89,3 +89,32 @@ def test_raw_weight_map_field_keeps_the_reference_signature():
     assert [k for k, _ in params] == ["field", "cfg", "H", "W", "support_fade_alpha",
                                       "detach_opacities"]
     assert params[4][1].default is None and params[5][1].default is False
+
+
+def test_gaussian_field_keeps_the_reference_constructor_and_helpers(tmp_path):
+    """gaussians.py:15-29, :193-268, :324-358: argument order, affine coefficients as a
+    parameter group, effective scales with the covariance filter, and save/load round trips."""
+    import math
+
+    g = GaussianField.from_numpy(np.array([[1.0, 2.0]]), np.array([[2.0, 3.0]]), np.zeros(1),
+                                 np.ones((1, 3)), opacities=np.zeros(1),
+                                 scale_max=np.full((1, 2), 9.0),
+                                 background_mask=np.array([True]),
+                                 filter_variance=np.array([0.25]))
+    assert g.scale_max.shape == (1, 2) and g.background_mask.dtype == torch.bool
+    a = g.with_affine_colors()
+    assert a.color_grads.shape == (1, 2, 3) and not a.color_grads.any()
+    groups = a.parameter_groups(1, 2, 3, 4, 5)
+    assert len(groups) == 6 and groups[-1]["lr"] == 4  # color_grads at the color rate
+    eff = g.effective_scales()
+    assert torch.allclose(eff, torch.tensor([[math.sqrt(4.25), math.sqrt(9.25)]]))
+    m = g.materialize_covariance_filter()
+    assert m.filter_variance is None and torch.allclose(m.scales(), eff)
+    a.save(str(tmp_path / "f.npz"))
+    b = GaussianField.load(str(tmp_path / "f.npz"))
+    for name in ("means", "log_scales", "rotations", "colors", "opacities", "scale_max",
+                 "color_grads", "background_mask", "filter_variance"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    positional = GaussianField(g.means, g.log_scales, g.rotations, g.colors, g.opacities,
+                               g.scale_max)
+    assert positional.scale_max is g.scale_max and positional.filter_variance is None

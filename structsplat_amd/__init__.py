@@ -3,9 +3,10 @@
 Public entry points mirror the reference's render path:
 
 * :func:`render_field` - mode dispatcher (render.py:319-414).
-* :func:`render_hip_exact` - the ``render_cuda_exact`` entry (cuda_render.py:256-294), also
-  exported under that name, like ``render_cuda_exact_with_coverage``.
-* :func:`render`, :func:`render_additive` - the reference's renderer names (render.py:238-263).
+* :func:`render_hip_exact` - the ``render_cuda_exact`` entry (cuda_render.py:256-294).
+* ``structsplat_amd.render`` also holds ``render``, ``render_additive`` (render.py:238-263),
+  ``render_cuda_exact`` and ``render_cuda_exact_with_coverage`` under the reference's names; the
+  package keeps ``render`` for that module, as the reference's package does.
 * :func:`build_tile_index` - per-tile Gaussian lists (render_ext.cpp:139-167).
 * :func:`gaussian_activity` - per-Gaussian summed weight, the pruning statistic (render.py:417-442).
 * :func:`raw_weight_map` - differentiable raw coverage sum_i o_i G_i (fit.py:338-367).
@@ -22,10 +23,6 @@ from .render import (
     build_tile_index_capacity,
     gaussian_activity,
     raw_weight_map,
-    render,
-    render_additive,
-    render_cuda_exact,
-    render_cuda_exact_with_coverage,
     render_field,
     render_hip_affine,
     render_hip_exact,
@@ -38,10 +35,6 @@ __all__ = [
     "build_tile_index_capacity",
     "gaussian_activity",
     "raw_weight_map",
-    "render",
-    "render_additive",
-    "render_cuda_exact",
-    "render_cuda_exact_with_coverage",
     "render_field",
     "render_hip_affine",
     "render_hip_exact",

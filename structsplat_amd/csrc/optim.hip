@@ -22,6 +22,7 @@ using namespace ssplat_detail;
 constexpr int kMaxGroups = 8;
 constexpr int kAdamThreads = 256;
 constexpr int kMaxGroupBlocks = 4096;
+constexpr int kAdamPer = 4;  // elements per thread and step
 
 struct AdamGroupArgs {
   float* param;
@@ -63,27 +64,44 @@ __global__ __launch_bounds__(kAdamThreads) void k_adam(AdamArgs a) {
   int k = 0;
   while (k + 1 < a.num_groups && static_cast<int>(blockIdx.x) >= a.block_start[k + 1]) ++k;
   const AdamGroupArgs& g = a.g[k];
-  const int64_t stride = int64_t(a.block_start[k + 1] - a.block_start[k]) * blockDim.x;
-  for (int64_t j = int64_t(blockIdx.x - a.block_start[k]) * blockDim.x + threadIdx.x;
-       j < g.count; j += stride) {
-    const float grad = g.grad[j];
-    float m = g.m[j];
-    float v = g.v[j];
-    m = fmaf(1.0f - a.beta1, grad - m, m);
-    v = fmaf(1.0f - a.beta2, grad * grad, a.beta2 * v);
-    const float step_size = g.lr / bc1;
-    float p = g.param[j] - step_size * m / (sqrtf(v) / bc2_sqrt + a.eps);
-    if (g.clamp) {
+  float* __restrict__ param = g.param;
+  const float* __restrict__ grad = g.grad;
+  float* __restrict__ mom = g.m;
+  float* __restrict__ var = g.v;
+  const float step_size = g.lr / bc1;
+  // kAdamPer elements per thread, blockDim apart: every load of a step is issued before the
+  // first update.
+  const int64_t span = int64_t(kAdamThreads) * kAdamPer;
+  const int64_t stride = int64_t(a.block_start[k + 1] - a.block_start[k]) * span;
+  for (int64_t j0 = int64_t(blockIdx.x - a.block_start[k]) * span + threadIdx.x; j0 < g.count;
+       j0 += stride) {
+    float gr[kAdamPer], m[kAdamPer], v[kAdamPer], p[kAdamPer], hi[kAdamPer];
+#pragma unroll
+    for (int u = 0; u < kAdamPer; ++u) {
+      const int64_t j = j0 + int64_t(u) * kAdamThreads;
+      const bool ok = j < g.count;
+      gr[u] = ok ? grad[j] : 0.0f;
+      m[u] = ok ? mom[j] : 0.0f;
+      v[u] = ok ? var[j] : 0.0f;
+      p[u] = ok ? param[j] : 0.0f;
       // Rows of at most 2^31 elements and tables below 2^32 (checked on the host): a 32-bit
       // division instead of a 64-bit one.
-      const float hi = g.hi_rows != nullptr
-                           ? g.hi_rows[static_cast<uint32_t>(j) / static_cast<uint32_t>(g.row_width)]
-                           : g.hi;
-      p = fminf(fmaxf(p, g.lo), hi);
+      hi[u] = (ok && g.clamp && g.hi_rows != nullptr)
+                  ? g.hi_rows[static_cast<uint32_t>(j) / static_cast<uint32_t>(g.row_width)]
+                  : g.hi;
     }
-    g.m[j] = m;
-    g.v[j] = v;
-    g.param[j] = p;
+#pragma unroll
+    for (int u = 0; u < kAdamPer; ++u) {
+      const int64_t j = j0 + int64_t(u) * kAdamThreads;
+      if (j >= g.count) continue;
+      const float mu = fmaf(1.0f - a.beta1, gr[u] - m[u], m[u]);
+      const float vu = fmaf(1.0f - a.beta2, gr[u] * gr[u], a.beta2 * v[u]);
+      float pu = p[u] - step_size * mu / (sqrtf(vu) / bc2_sqrt + a.eps);
+      if (g.clamp) pu = fminf(fmaxf(pu, g.lo), hi[u]);
+      mom[j] = mu;
+      var[j] = vu;
+      param[j] = pu;
+    }
   }
 }
 
@@ -132,9 +150,10 @@ int ssplat_adam_step(const ssplat_adam_group* groups, int num_groups, float* ste
     g.hi = static_cast<float>(s.hi);
     g.hi_rows = s.hi_rows;
     g.row_width = s.row_width > 0 ? s.row_width : 1;
-    // One workgroup per 256 elements, at most kMaxGroupBlocks per group (grid-stride beyond).
+    // One workgroup per kAdamThreads * kAdamPer elements, at most kMaxGroupBlocks per group
+    // (grid-stride beyond).
     a.block_start[k] = blocks;
-    const int64_t need = (s.count + kAdamThreads - 1) / kAdamThreads;
+    const int64_t need = (s.count + kAdamThreads * kAdamPer - 1) / (kAdamThreads * kAdamPer);
     blocks += static_cast<int>(need < kMaxGroupBlocks ? need : kMaxGroupBlocks);
   }
   a.block_start[num_groups] = blocks;

@@ -239,6 +239,8 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
               graph: bool = True) -> BatchResult:
     """Fit every field to its target in one batched step per iteration.  The fitted rows are
     copied back into `fields`."""
+    if cfg.color_basis != "constant" or any(f.color_grads is not None for f in fields):
+        raise ValueError("fit_batch renders constant colors; fit affine-color fields with fit()")
     dev = targets[0].device
     sizes = [tuple(t.shape[:2]) for t in targets]
     counts = [f.n for f in fields]

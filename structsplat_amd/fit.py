@@ -53,6 +53,8 @@ class FitConfig:
     capacity_slack: float = 1.5    # pair-buffer capacity / current pair count
     capacity_floor: int = 4096     # minimum pair-buffer capacity
     tile_size: int | str = "auto"  # render tile edge (16 or 32), or "auto": the autotuner's choice
+    color_basis: str = "constant"  # "constant", or "affine": per-Gaussian color gradients
+    color_grad_l2: float = 1e-4    # weight of mean(color_grads^2) in the affine loss
 
 
 class RenderPlan:
@@ -61,16 +63,24 @@ class RenderPlan:
     count without ellipse cull); an overflow of either sets the status word."""
 
     def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16,
-                 recompute_normalizer: bool = True):
+                 recompute_normalizer: bool = True, affine: bool = False):
+        if affine and int(tile_size) != HR.DEFAULT_TILE_SIZE:
+            raise ValueError("affine-color plans use 16-pixel tiles")
         self.n, self.H, self.W, self.tile_size = int(n), int(H), int(W), int(tile_size)
-        self.recompute_normalizer = bool(recompute_normalizer)
+        self.affine = bool(affine)
+        # The affine backward always recomputes the normalizer.
+        self.recompute_normalizer = bool(recompute_normalizer) or self.affine
         self.capacity = max(int(capacity), 1)
         T = HR.num_tiles(H, W, tile_size)
         self.gids = torch.empty(self.capacity, dtype=torch.int32, device=device)
         self.offsets = torch.empty(T + 1, dtype=torch.int32, device=device)
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
-        self.ws = HR.workspace(n, H, W, tile_size, device)
-        self.slots = HR.grad_slots(self.capacity, device)
+        if self.affine:
+            self.ws = HR.affine_workspace(n, H, W, device)
+            self.slots = HR.affine_grad_slots(self.capacity, device)
+        else:
+            self.ws = HR.workspace(n, H, W, tile_size, device)
+            self.slots = HR.grad_slots(self.capacity, device)
         self.den = (None if self.recompute_normalizer
                     else torch.empty(H, W, dtype=torch.float32, device=device))
 
@@ -118,15 +128,82 @@ class _PlannedRender(Function):
         return g_m, g_q, g_c, None, g_o, None, None, None, None, None
 
 
+class _PlannedAffineRender(Function):
+    """Planned render with affine colors; same gradients as ``render._AffineRenderHip``."""
+
+    @staticmethod
+    def forward(ctx, means, conics, colors, radii, opacities, color_grads, scales, rotations,
+                plan, normalize, eps, fade, sigma_cutoff):
+        means, conics, colors = means.contiguous(), conics.contiguous(), colors.contiguous()
+        radii, opacities = radii.contiguous(), opacities.contiguous()
+        color_grads = color_grads.contiguous()
+        scales, rotations = scales.contiguous(), rotations.contiguous()
+        cull = HR._fade_alpha(fade) == 1.0
+        gids, offsets, _ = HR.build_tile_index_capacity(
+            means, conics, radii, plan.H, plan.W, plan.capacity, plan.tile_size, cull,
+            sigma_cutoff, out=plan.buffers())
+        out, _ = HR.forward_tiled_affine(means, conics, colors, radii, opacities, color_grads,
+                                         scales, rotations, gids, offsets, plan.H, plan.W,
+                                         normalize, eps, fade, sigma_cutoff, ws=plan.ws)
+        ctx.save_for_backward(means, conics, colors, radii, opacities, color_grads, scales,
+                              rotations, out)
+        ctx.plan = plan
+        ctx.args = (normalize, eps, fade, sigma_cutoff)
+        return out
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_out):
+        means, conics, colors, radii, opacities, color_grads, scales, rotations, out = \
+            ctx.saved_tensors
+        plan = ctx.plan
+        normalize, eps, fade, sigma_cutoff = ctx.args
+        g = HR.backward_tiled_affine(
+            grad_out.contiguous(), means, conics, colors, radii, opacities, color_grads, scales,
+            rotations, None, out if normalize else None, plan.gids, plan.offsets, plan.H, plan.W,
+            normalize, eps, fade, sigma_cutoff, plan.capacity, ws=plan.ws, slots=plan.slots,
+            status=plan.status)
+        g_m, g_q, g_c, g_o, g_cg, g_s, g_r = g
+        if opacities.numel() == 0:
+            g_o = None
+        return g_m, g_q, g_c, None, g_o, g_cg, g_s, g_r, None, None, None, None, None
+
+
 def planned_render(field: GaussianField, plan: RenderPlan, cfg: FitConfig) -> torch.Tensor:
     """Renders the field with the plan's buffers: conics, radii and opacities from the fused
-    producer (gaussians.py:270-321), then binning, forward and (in backward) the gradients."""
+    producer (gaussians.py:270-321), then binning, forward and (in backward) the gradients.
+    An affine plan renders ``field.color_grads`` in each Gaussian's frame of
+    ``effective_scales()`` and ``rotations`` (fit.py:1043-1056)."""
     conics, radii, op = field.produce(cfg.sigma_cutoff)
     if op is None:
         op = field.means.new_empty(0)
-    return _PlannedRender.apply(field.means, conics, field.colors, radii, op, plan, cfg.normalize,
-                                float(cfg.normalization_eps), HR._fade_alpha(cfg.support_fade),
-                                float(cfg.sigma_cutoff))
+    args = (plan, cfg.normalize, float(cfg.normalization_eps), HR._fade_alpha(cfg.support_fade),
+            float(cfg.sigma_cutoff))
+    if plan.affine:
+        return _PlannedAffineRender.apply(field.means, conics, field.colors, radii, op,
+                                          field.color_grads, field.effective_scales(),
+                                          field.rotations, *args)
+    return _PlannedRender.apply(field.means, conics, field.colors, radii, op, *args)
+
+
+def add_color_grad_l2(field: GaussianField, cfg: FitConfig) -> None:
+    """Adds the gradient of ``color_grad_l2 * mean(color_grads^2)`` (fit.py:3256-3259) to
+    ``color_grads.grad`` in place: one elementwise kernel, so a captured step holds no
+    reduction."""
+    cg = field.color_grads
+    if cg is None or cfg.color_grad_l2 <= 0.0 or cg.grad is None or cg.numel() == 0:
+        return
+    with torch.no_grad():
+        cg.grad.add_(cg, alpha=2.0 * float(cfg.color_grad_l2) / cg.numel())
+
+
+def color_grad_l2_term(field: GaussianField, cfg: FitConfig) -> float:
+    """The value ``color_grad_l2 * mean(color_grads^2)`` (0 without affine colors)."""
+    cg = field.color_grads
+    if cg is None or cfg.color_grad_l2 <= 0.0 or cg.numel() == 0:
+        return 0.0
+    with torch.no_grad():
+        return float(cfg.color_grad_l2 * (cg * cg).mean())
 
 
 def raw_weight_map_field(field: GaussianField, cfg: FitConfig, H: int, W: int,
@@ -187,22 +264,36 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         verbose: bool = False) -> dict:
     """Fit `field` to `target` (H, W, 3) for cfg.iters Adam steps.
 
-    Returns the final PSNR and loss, the pair-buffer regrows and capacity, the tile size, and
-    ``seconds``: the wall time of the step loop (window checks and regrows included; the tile
-    tuning and first plan before it are not)."""
+    With ``cfg.color_basis == "affine"`` (or a field that already has ``color_grads``) the field
+    renders affine colors and its loss adds ``color_grad_l2 * mean(color_grads^2)``
+    (fit.py:3256-3259); zero coefficients are added to the field in place when it has none.
+
+    Returns the final PSNR, the last step's image loss, the pair-buffer regrows and capacity, the
+    tile size, ``seconds``: the wall time of the step loop (window checks and regrows included;
+    the tile tuning and first plan before it are not), and ``color_grad_l2_term``: the affine
+    regularizer at the final parameters."""
     H, W = target.shape[:2]
     dev = target.device
-    if field.color_grads is not None:
-        raise ValueError("the planned fit step renders constant colors; render affine colors "
-                         "(color_grads) with render_field or render_hip_affine")
+    if cfg.color_basis not in ("constant", "affine"):
+        raise ValueError(f"color_basis must be constant or affine, got {cfg.color_basis!r}")
+    if cfg.color_basis == "affine" and field.color_grads is None:
+        # _ensure_color_basis (fit.py:1219-1222), in place: zero coefficients.
+        field.color_grads = torch.zeros(field.n, 2, 3, device=field.colors.device,
+                                        dtype=field.colors.dtype)
+    affine = field.color_grads is not None
     field.trainable()
     opt = make_optimizer(field, cfg)
     opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))
-    ts = resolve_tile_size(cfg, lambda: tune_field(field, H, W, cfg)[0])
+    if affine:
+        if cfg.tile_size not in ("auto", HR.DEFAULT_TILE_SIZE):
+            raise ValueError("affine colors render with 16-pixel tiles")
+        ts = HR.DEFAULT_TILE_SIZE
+    else:
+        ts = resolve_tile_size(cfg, lambda: tune_field(field, H, W, cfg)[0])
 
     def new_plan(capacity):
         return RenderPlan(field.n, H, W, capacity, dev, tile_size=ts,
-                          recompute_normalizer=cfg.recompute_normalizer)
+                          recompute_normalizer=cfg.recompute_normalizer, affine=affine)
 
     plan = new_plan(_capacity(pair_count(field, H, W, cfg, ts), cfg))
     state = {"loss": None}
@@ -213,6 +304,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         img = planned_render(field, plan, cfg)
         loss = loss_fn(img, target, cfg.ssim_weight)
         loss.backward()
+        add_color_grad_l2(field, cfg)
         opt.step()  # Adam and the log-scale clamp
         state["loss"] = loss.detach()
 
@@ -262,14 +354,20 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     torch.cuda.synchronize(dev)
     seconds = time.perf_counter() - t0
     with torch.no_grad():
-        img = HR.render_hip_exact(field.means, field.conics(), field.colors,
-                                  field.radii(cfg.sigma_cutoff), H, W,
-                                  opacities=field.opacity_values(), normalize=cfg.normalize,
-                                  eps=cfg.normalization_eps, support_fade=cfg.support_fade,
-                                  sigma_cutoff=cfg.sigma_cutoff)
+        kw = dict(opacities=field.opacity_values(), normalize=cfg.normalize,
+                  eps=cfg.normalization_eps, support_fade=cfg.support_fade,
+                  sigma_cutoff=cfg.sigma_cutoff)
+        if affine:
+            img = HR.render_hip_affine(field.means, field.conics(), field.colors,
+                                       field.radii(cfg.sigma_cutoff), H, W, field.color_grads,
+                                       field.effective_scales(), field.rotations, **kw)
+        else:
+            img = HR.render_hip_exact(field.means, field.conics(), field.colors,
+                                      field.radii(cfg.sigma_cutoff), H, W, **kw)
         final_psnr = float(psnr_tensor(img, target))
     return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
-            "capacity": plan.capacity, "tile_size": ts, "seconds": seconds}
+            "capacity": plan.capacity, "tile_size": ts, "seconds": seconds,
+            "color_grad_l2_term": color_grad_l2_term(field, cfg)}
 
 
 def resolve_tile_size(cfg: FitConfig, tune) -> int:

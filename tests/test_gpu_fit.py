@@ -8,6 +8,7 @@ import torch
 from structsplat_amd import fit as F
 from structsplat_amd import synth
 from structsplat_amd.gaussians import GaussianField
+from structsplat_amd.render import render_hip_affine
 
 pytestmark = pytest.mark.gpu
 
@@ -122,3 +123,57 @@ def test_autotuned_tile_size(cuda):
     assert r["tile_size"] == best
     with pytest.raises(ValueError):
         F.resolve_tile_size(F.FitConfig(tile_size=24), lambda: 16)
+
+
+def _affine_problem(device, seed):
+    field, target = _problem(device, seed=seed)
+    g = torch.Generator(device=device).manual_seed(seed)
+    field.color_grads = 0.05 * torch.randn(field.n, 2, 3, device=device, generator=g)
+    return field, target
+
+
+def test_planned_affine_step_matches_render_hip_affine(cuda):
+    """The fit's planned affine render gives the image and all gradients of render_hip_affine
+    with the same producer outputs (same tile lists, same deterministic slot order)."""
+    print("case: planned affine step vs render_hip_affine, 96x128, n=600", flush=True)
+    field, target = _affine_problem(cuda, 11)
+    cfg = F.FitConfig(iters=0, color_basis="affine")
+    H, W = target.shape[:2]
+    grads = []
+    for planned in (True, False):
+        field.trainable()
+        for p in field.parameters():
+            p.grad = None
+        if planned:
+            plan = F.RenderPlan(field.n, H, W, 2 * F.pair_count(field, H, W, cfg) + 4096, cuda,
+                                affine=True)
+            img = F.planned_render(field, plan, cfg)
+        else:
+            conics, radii, op = field.produce(cfg.sigma_cutoff)
+            img = render_hip_affine(field.means, conics, field.colors, radii, H, W,
+                                    field.color_grads, field.effective_scales(),
+                                    field.rotations, opacities=op)
+        F.loss_fn(img, target, cfg.ssim_weight).backward()
+        torch.cuda.synchronize()
+        grads.append([img.detach().clone()] + [p.grad.clone() for p in field.parameters()])
+    assert int(plan.status.item()) == 0
+    assert torch.equal(grads[0][0], grads[1][0])
+    for a, b in zip(grads[0][1:], grads[1][1:]):
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-7)
+    assert float(grads[0][-1].abs().max()) > 0.0  # color_grads receive gradient
+
+
+def test_affine_plain_and_graphed_fits_agree(cuda):
+    """color_basis="affine" adds zero coefficients to a constant field; the plain and graphed
+    fits agree and move the coefficients off zero."""
+    print("case: affine plain vs graphed fit, 30 steps", flush=True)
+    a, target = _problem(cuda, seed=13)
+    b, _ = _problem(cuda, seed=13)
+    cfg = F.FitConfig(iters=30, window=10, color_basis="affine", color_grad_l2=1e-3)
+    ra = F.fit(a, target, cfg, graph=False)
+    rb = F.fit(b, target, cfg, graph=True)
+    print(ra, rb)
+    assert ra["tile_size"] == 16 and a.color_grads is not None
+    assert abs(ra["psnr"] - rb["psnr"]) < 0.01 and ra["psnr"] > 10.0
+    assert float(a.color_grads.abs().max()) > 0.0 and ra["color_grad_l2_term"] > 0.0
+    assert torch.allclose(a.color_grads, b.color_grads, rtol=1e-3, atol=1e-5)

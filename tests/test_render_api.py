@@ -118,3 +118,31 @@ def test_gaussian_field_keeps_the_reference_constructor_and_helpers(tmp_path):
     positional = GaussianField(g.means, g.log_scales, g.rotations, g.colors, g.opacities,
                                g.scale_max)
     assert positional.scale_max is g.scale_max and positional.filter_variance is None
+
+
+def test_fit_color_basis_and_l2_gradient():
+    """fit() validates color_basis before any device work, and the in-place regularizer gradient
+    equals autograd's gradient of color_grad_l2 * mean(color_grads^2) (fit.py:3256-3259)."""
+    from structsplat_amd import fit as F
+    from structsplat_amd.batch import fit_batch
+
+    field = GaussianField.from_numpy(np.array([[3.0, 3.0]]), np.full((1, 2), 1.5), np.zeros(1),
+                                     np.array([[1.0, 0, 0]]))
+    target = torch.zeros(8, 8, 3)
+    with pytest.raises(ValueError, match="color_basis must be constant or affine"):
+        F.fit(field, target, F.FitConfig(iters=1, color_basis="quadratic"))
+    with pytest.raises(ValueError, match="affine-color fields with fit"):
+        fit_batch([field], [target], F.FitConfig(iters=1, color_basis="affine"))
+
+    torch.manual_seed(0)
+    cg = torch.randn(7, 2, 3, dtype=torch.float64, requires_grad=True)
+    aff = GaussianField(torch.zeros(7, 2), torch.zeros(7, 2), torch.zeros(7), torch.zeros(7, 3),
+                        color_grads=cg)
+    cfg = F.FitConfig(color_grad_l2=0.3)
+    (cfg.color_grad_l2 * (cg * cg).mean()).backward()
+    expected = cg.grad.clone()
+    cg.grad = torch.full_like(cg, 0.25)
+    F.add_color_grad_l2(aff, cfg)
+    assert torch.allclose(cg.grad - 0.25, expected, rtol=1e-12, atol=0)
+    assert F.color_grad_l2_term(aff, cfg) == pytest.approx(0.3 * float((cg * cg).mean()),
+                                                          rel=1e-12)

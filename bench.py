@@ -216,6 +216,66 @@ def fit_loop(steps: int, seed: int = 0) -> dict:
     }
 
 
+def band_shard(means, conics, colors, radii, op, grad_out, H: int, W: int, ts: int, steps: int,
+               warmup: int, bands: int) -> dict:
+    """Each row band of the image as one GPU's share of a band-split run: the band's members
+    (its Gaussians plus the halo from neighbouring bands) binned, rendered and differentiated
+    as an image of their own, with inputs resident.  The slowest band sets a multi-GPU step's
+    pace; the all-reduce of the partial gradients is not part of this."""
+    import torch
+
+    from structsplat_amd import render as HR
+    from structsplat_amd import shard
+
+    dev = means.device
+    per_band = []
+    for y0, y1 in shard.band_bounds(H, bands, 32):
+        _, m, q, c, r, o = shard.band_inputs(means, conics, colors, radii, op, y0, y1)
+        m, q, c, r = (t.contiguous() for t in (m, q, c, r))
+        o = m.new_empty(0) if o is None else o.contiguous()
+        hb, n = y1 - y0, m.shape[0]
+        _, off0 = HR.build_tile_index(m, q, r, hb, W, ts)
+        P = int(off0[-1].item())
+        cap = P + P // 8 + 1024
+        gids = torch.empty(cap, dtype=torch.int32, device=dev)
+        offsets = torch.empty(HR.num_tiles(hb, W, ts) + 1, dtype=torch.int32, device=dev)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        ws = HR.workspace(n, hb, W, ts, dev)
+        slots = HR.grad_slots(cap, dev)
+        img = torch.empty(hb, W, 3, device=dev)
+        g = grad_out[y0:y1].contiguous()
+        grads = (torch.empty_like(m), torch.empty_like(q), torch.empty_like(c),
+                 torch.empty_like(o))
+
+        def step():
+            HR.build_tile_index_capacity(m, q, r, hb, W, cap, ts, out=(gids, offsets, status, ws))
+            HR.forward_tiled(m, q, c, r, o, gids, offsets, hb, W, True, 1e-8, ts, False, 3.0,
+                             out=img, ws=ws)
+            HR.backward_tiled(g, m, q, c, r, o, None, img, gids, offsets, hb, W, True, 1e-8, ts,
+                              False, 3.0, grads=grads, ws=ws, slots=slots, slot_capacity=cap,
+                              status=status, after_forward=True)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        if int(status.item()) != 0:
+            raise RuntimeError("pair capacity exceeded in a band")
+        per_band.append({"rows": [y0, y1], "gaussians": n, "tile_pairs": P,
+                         "ms_per_step": round(ms, 4)})
+    return {"bands": bands, "tile_size": ts,
+            "max_band_ms_per_step": max(b["ms_per_step"] for b in per_band),
+            "member_gaussians_over_n": round(sum(b["gaussians"] for b in per_band)
+                                             / means.shape[0], 4),
+            "per_band": per_band,
+            "note": "one GPU's share of a row-band split, timed band by band on one GPU; "
+                    "the gradient all-reduce is not included"}
+
+
 def copy_probe(mib: int) -> int:
     """One device-to-device copy of `mib` MiB (the calibration kernel of the HBM counters)."""
     import torch
@@ -269,6 +329,8 @@ def main() -> int:
                     help="Adam steps of the 2048x2048/200k fit loop (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched-vs-plain fit (--batched-steps steps)")
+    ap.add_argument("--bands", type=int, default=8,
+                    help="row bands timed as one GPU's share each (0 skips)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -447,6 +509,9 @@ def main() -> int:
             "traffic": measured_traffic(args.workload, f"k_backward<{ts}>"),
         },
     }
+    if args.bands > 0:
+        line["band_shard"] = band_shard(means, conics, colors, radii, op, grad_out, H, W, ts,
+                                        args.steps, args.warmup, args.bands)
     if args.batched_steps > 0:
         line["batched"] = batched_fit(args.batched_steps)
     if args.fit_steps > 0:

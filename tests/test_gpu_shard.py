@@ -1,0 +1,46 @@
+"""One GPU's share of a row-band split: each band rendered from its members alone equals the full
+render's rows bit for bit, and the bands' partial gradients sum to the full gradients."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from structsplat_amd import render as HR
+from structsplat_amd import shard
+from tests.helpers import field_tensors, random_field, scaled_err, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True, 0.5])
+@pytest.mark.parametrize("tile_size", [16, 32])
+def test_bands_equal_full_render_and_gradients_sum(cuda, normalize, fade, tile_size):
+    print(f"case: bands normalize={normalize} fade={fade} tile={tile_size}", flush=True)
+    H, W = 384, 320
+    f = random_field(21, 3000, H, W, scale_hi=12.0, opacity=True)
+    leaves = field_tensors(f, cuda)
+    means, conics, colors, radii, op = leaves
+    kw = dict(normalize=normalize, support_fade=fade, tile_size=tile_size)
+    g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(4))
+
+    full_leaves = [t.detach().clone().requires_grad_(True) for t in (means, conics, colors, op)]
+    full = HR.render_hip_exact(full_leaves[0], full_leaves[1], full_leaves[2], radii, H, W,
+                               opacities=full_leaves[3], **kw)
+    (full * g).sum().backward()
+
+    band_leaves = [t.detach().clone().requires_grad_(True) for t in (means, conics, colors, op)]
+    total, members = 0.0, 0
+    for y0, y1 in shard.band_bounds(H, 6, 32):
+        img, idx = shard.render_band(band_leaves[0], band_leaves[1], band_leaves[2], radii, H, W,
+                                     y0, y1, opacities=band_leaves[3], **kw)
+        assert torch.equal(img, full.detach()[y0:y1]), (y0, y1)
+        total = total + (img * g[y0:y1]).sum()
+        members += idx.numel()
+    total.backward()
+    torch.cuda.synchronize()
+    assert members > means.shape[0]  # Gaussians crossing band edges belong to several bands
+    for name, a, b in zip(("means", "conics", "colors", "opacities"), full_leaves, band_leaves):
+        err = scaled_err(to_np(b.grad), to_np(a.grad))
+        print(name, "scaled err", err)
+        assert err <= 1e-5, name

@@ -13,6 +13,7 @@ Public entry points mirror the reference's render path:
 * :func:`render_hip_affine` - affine-color render (render.py:110-123).
 * :class:`GaussianField` - field layout and conic/radii/opacity producer (gaussians.py).
 * :mod:`structsplat_amd.codec` - SSPL1 decode and decode-and-render (codec.py:333-500).
+* :mod:`structsplat_amd.shard` - one row band of a large image rendered from its own Gaussians.
 
 The compute runs in hand-written HIP kernels in ``libssplat_hip.so`` behind a C ABI
 (``include/ssplat.h``); build it with ``python -m structsplat_amd._build``.

@@ -2,7 +2,8 @@
 
 Nothing in the product package (``structsplat_amd``) imports this package.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may use it, and only as the
-checker (or, for ``cpu_baseline``, as the timed CPU restatement of the reference path).
+expected results that the HIP path is compared with (or, for ``cpu_baseline``, as the timed CPU
+restatement of the reference path).
 
 Modules and what pins each of them (the tests named here live in ``tests/``):
 

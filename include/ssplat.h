@@ -40,7 +40,7 @@ enum ssplat_status {
   SSPLAT_ERR_INVALID_ARGUMENT = 1, /* bad shape, size, tile size or pointer */
   SSPLAT_ERR_CAPACITY = 2,         /* tile_gids or gradient slot capacity too small */
   SSPLAT_ERR_HIP = 3,              /* a HIP runtime call or kernel launch failed */
-  SSPLAT_ERR_UNSUPPORTED = 4       /* valid request this build does not implement */
+  SSPLAT_ERR_UNSUPPORTED = 4       /* valid request the library does not implement */
 };
 
 /* Version of this ABI (SSPLAT_ABI_VERSION of the build). */

@@ -335,8 +335,8 @@ def main() -> int:
                     help="only run one device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the renderer's fields are independent units and this "
-              "build runs on a single MI355X; run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU (a single MI355X); the band_shard entry of its line "
+              "times each GPU's share of a row-band split. Run with --gpus 1.", file=sys.stderr)
         return 2
     if args.copy_probe > 0:
         return copy_probe(args.copy_probe)

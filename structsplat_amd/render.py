@@ -505,16 +505,19 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
 
     ``tiled`` and ``backward_variant`` are accepted for drop-in compatibility: the HIP path is
     always tiled and has one backward.  ``recompute_normalizer`` (default) keeps only the image
-    for the backward and recomputes the per-pixel normalizer there; ``False`` saves it too.  ``tile_index_backend`` accepts ``"hip"`` (or the
-    reference's ``"cuda"``); the reference's torch-op builder exists here only as the oracle.
+    for the backward and recomputes the per-pixel normalizer there; ``False`` saves it too.
+    ``tile_index_backend`` accepts ``"hip"`` and the reference's ``"cuda"`` and ``"torch"``
+    (cuda_render.py:285-288); all three bin on the GPU, since the torch builder's lists hold the
+    same Gaussians per tile (test_render.py:401-409) and the ascending order is one of its orders.
     """
     _check_field(means, conics, colors, radii, opacities)
     if backward_variant not in ("baseline", "block_reduce"):
         raise ValueError(
             f"backward_variant must be baseline or block_reduce, got {backward_variant!r}"
         )
-    if tile_index_backend not in ("hip", "cuda"):
-        raise ValueError(f"tile_index_backend must be hip or cuda, got {tile_index_backend!r}")
+    if tile_index_backend not in ("hip", "cuda", "torch"):
+        raise ValueError(
+            f"tile_index_backend must be hip, cuda or torch, got {tile_index_backend!r}")
     _check_image(H, W, tile_size)
     if normalize:
         eps = _validate_normalization_eps(eps)

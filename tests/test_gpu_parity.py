@@ -326,3 +326,21 @@ def test_reference_renderer_names_route_to_render_field(cuda):
     torch.cuda.synchronize()
     assert torch.equal(a, b) and torch.equal(c, d) and torch.equal(e, g)
     assert HR.render_cuda_exact_with_coverage is HR.render_hip_exact_with_coverage
+
+
+def test_tile_index_backend_names(cuda):
+    """render_cuda_exact's tile_index_backend (cuda_render.py:285-288): "cuda" and "torch" give
+    the same image, and an unknown name is a ValueError (test_render.py:411-414)."""
+    _print_case(test="tile_index_backend names")
+    N, H, W = 150, 36, 44
+    f = random_field(41, N, H, W, scale_lo=0.7, scale_hi=5.0, opacity=True)
+    means, conics, colors, radii, op = field_tensors(f, cuda)
+    a = HR.render_cuda_exact(means, conics, colors, radii, H, W, opacities=op, tiled=True,
+                             tile_index_backend="cuda")
+    b = HR.render_cuda_exact(means, conics, colors, radii, H, W, opacities=op, tiled=True,
+                             tile_index_backend="torch")
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    with pytest.raises(ValueError, match="tile_index_backend must be"):
+        HR.render_cuda_exact(means, conics, colors, radii, H, W, tiled=True,
+                             tile_index_backend="nope")

@@ -44,3 +44,21 @@ def test_bands_equal_full_render_and_gradients_sum(cuda, normalize, fade, tile_s
         err = scaled_err(to_np(b.grad), to_np(a.grad))
         print(name, "scaled err", err)
         assert err <= 1e-5, name
+
+
+def test_band_without_members_renders_zero(cuda):
+    """A band no Gaussian reaches has no members: its image is zero and the gradients of the
+    other bands are untouched by it."""
+    print("case: empty band", flush=True)
+    H, W = 128, 96
+    f = random_field(5, 200, 40, W, scale_hi=2.0, margin=0.0)  # means in rows [0, 40)
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    means = means.detach().clone().requires_grad_(True)
+    top, idx_top = shard.render_band(means, conics, colors, radii, H, W, 0, 64)
+    low, idx_low = shard.render_band(means, conics, colors, radii, H, W, 96, 128)
+    assert idx_low.numel() == 0 and low.shape == (32, W, 3) and torch.count_nonzero(low) == 0
+    full = HR.render_hip_exact(means, conics, colors, radii, H, W)
+    assert torch.equal(top, full.detach()[0:64])
+    (top.sum() + low.sum()).backward()
+    torch.cuda.synchronize()
+    assert torch.isfinite(means.grad).all()

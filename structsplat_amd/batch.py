@@ -269,6 +269,8 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
 
     plan = BatchPlan(batch, F._capacity(count(), cfg), cfg.recompute_normalizer)
 
+    seeds = {}
+
     def step():
         opt.zero_grad(set_to_none=True)  # gradients assigned, not accumulated
         conics, radii, op = big.produce(cfg.sigma_cutoff)
@@ -277,7 +279,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                                  float(cfg.normalization_eps), _fade_alpha(cfg.support_fade),
                                  float(cfg.sigma_cutoff))
         loss = batch_l1_ssim_loss(batch, img, targets_flat, w)
-        loss.backward()
+        loss.backward(F._seed(seeds, loss))
         opt.step()  # Adam and the per-field log-scale clamp
 
     torch.cuda.synchronize(dev)

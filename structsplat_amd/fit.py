@@ -303,7 +303,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         opt.zero_grad(set_to_none=True)
         img = planned_render(field, plan, cfg)
         loss = loss_fn(img, target, cfg.ssim_weight)
-        loss.backward()
+        loss.backward(_seed(state, loss))
         add_color_grad_l2(field, cfg)
         opt.step()  # Adam and the log-scale clamp
         state["loss"] = loss.detach()
@@ -368,6 +368,15 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
             "capacity": plan.capacity, "tile_size": ts, "seconds": seconds,
             "color_grad_l2_term": color_grad_l2_term(field, cfg)}
+
+
+def _seed(state: dict, loss: torch.Tensor) -> torch.Tensor:
+    """dL/dL = 1, allocated once (the first, eager step) and reused, so a step holds no fill
+    kernel for it."""
+    seed = state.get("seed")
+    if seed is None or seed.shape != loss.shape or seed.device != loss.device:
+        seed = state["seed"] = torch.ones_like(loss)
+    return seed
 
 
 def resolve_tile_size(cfg: FitConfig, tune) -> int:

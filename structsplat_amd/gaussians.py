@@ -247,6 +247,9 @@ class _Produce(Function):
             dilation, sigma_cutoff, _ptr(conics), _ptr(radii), _ptr(op), stream),
             "ssplat_produce_field")
         ctx.mark_non_differentiable(radii)
+        # Unused output gradients arrive as None rather than as zero-filled tensors (the radii's
+        # would cost a fill kernel per step).
+        ctx.set_materialize_grads(False)
         ctx.save_for_backward(log_scales, rotations, op, filter_variance)
         ctx.dilation = dilation
         return conics, radii, op

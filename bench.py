@@ -79,6 +79,8 @@ def make_field(workload: str, device):
 
     if workload == "metric":
         f = synth.metric_field(0)
+    elif workload == "cpu_reference":
+        f = synth.cpu_reference_field(0)
     elif workload == "single":
         f = synth.single_field(0)
     else:
@@ -95,42 +97,105 @@ def make_field(workload: str, device):
     return means, conics, colors, radii, op
 
 
-def cpu_baseline(workload: str, H: int, W: int, bytes_step: float, reps: int = 2) -> dict:
-    """The oracle's torch restatement of the reference CPU path (render.py _accumulate + torch
-    autograd), fwd+bwd of the same field, timed on this host's cores."""
-    import torch
-
-    from oracle import reference_render as R
-
-    torch.set_num_threads(os.cpu_count() or 1)
-    means, conics, colors, radii, op = (t.cpu() for t in make_field(workload, "cpu"))
-    grad = torch.full((H, W, 3), 1.0 / (3 * H * W))
-
-    def one():
-        leaves = [t.detach().clone().requires_grad_() for t in (means, conics, colors, op)]
-        img = R.render(leaves[0], leaves[1], leaves[2], radii, H, W, chunk=512,
-                       opacities=leaves[3])
-        img.backward(grad)
-
+def _median_seconds(fn, warmup: int, reps: int, sync=None) -> float:
+    for _ in range(warmup):
+        fn()
+    if sync:
+        sync()
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        one()
+        fn()
+        if sync:
+            sync()
         times.append(time.perf_counter() - t0)
-    sec = statistics.median(times)
+    return statistics.median(times)
+
+
+def _render_timings(fields, H: int, W: int, device, warmup: int, reps: int) -> dict:
+    """Median seconds of the normalized render alone and of render + backward (dL/dout = 1/(3HW),
+    the mean's gradient) of one field: on the CPU through the oracle's torch restatement of
+    render.py (float32, render_chunk 512, torch autograd), on the GPU through render_hip_exact."""
+    import torch
+
+    from oracle import reference_render as R
+    from structsplat_amd import render as HR
+
+    means, conics, colors, radii, op = (t.to(device) for t in fields)
+    grad = torch.full((H, W, 3), 1.0 / (3 * H * W), device=device)
+    on_cpu = device == "cpu"
+    sync = None if on_cpu else torch.cuda.synchronize
+
+    def fwd():
+        with torch.no_grad():
+            if on_cpu:
+                R.render(means, conics, colors, radii, H, W, chunk=512, opacities=op)
+            else:
+                HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op)
+
+    def fwd_bwd():
+        leaves = [t.detach().clone().requires_grad_() for t in (means, conics, colors, op)]
+        if on_cpu:
+            img = R.render(leaves[0], leaves[1], leaves[2], radii, H, W, chunk=512,
+                           opacities=leaves[3])
+        else:
+            img = HR.render_hip_exact(leaves[0], leaves[1], leaves[2], radii, H, W,
+                                      opacities=leaves[3])
+        img.backward(grad)
+
+    return {"fwd_s": _median_seconds(fwd, warmup, reps, sync),
+            "fwd_bwd_s": _median_seconds(fwd_bwd, warmup, reps, sync)}
+
+
+def cpu_baseline(workload: str, H: int, W: int, bytes_step: float) -> dict:
+    """The reference CPU path timed on this host's cores beside the GPU, on identical seeded
+    inputs: the oracle's torch restatement of render.py (_accumulate with render_chunk 512, torch
+    autograd for the backward).  Two workloads, forward and forward + backward each:
+
+    * the reference's CPU case, 256x256 / 2,000 Gaussians, seed 0: 3 warm-ups, median of 10;
+    * the metric workload (2048x2048 / 200,000): 3 warm-ups, median of 3.
+
+    `value` is the metric workload's fwd+bwd in the metric's unit (algorithmic GB/s); every
+    timing sits next to the GPU's time for the same call (render_hip_exact with autograd) and
+    the GPU/CPU speedup."""
+    import torch
+
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    gpu = torch.device("cuda:0")
+    out = {}
+    for name, (h, w), warmup, reps in (("cpu_reference_256x256_2k", (256, 256), 3, 10),
+                                       (f"{workload}_{W}x{H}", (H, W), 3, 3)):
+        fields = tuple(t.cpu() for t in make_field("cpu_reference" if h == 256 else workload,
+                                                   "cpu"))
+        cpu = _render_timings(fields, h, w, "cpu", warmup, reps)
+        dev = _render_timings(fields, h, w, gpu, 3, 10)
+        out[name] = {
+            "cpu_fwd_ms": round(cpu["fwd_s"] * 1e3, 3),
+            "cpu_fwd_bwd_ms": round(cpu["fwd_bwd_s"] * 1e3, 3),
+            "gpu_fwd_ms": round(dev["fwd_s"] * 1e3, 4),
+            "gpu_fwd_bwd_ms": round(dev["fwd_bwd_s"] * 1e3, 4),
+            "speedup_fwd": round(cpu["fwd_s"] / dev["fwd_s"], 1),
+            "speedup_fwd_bwd": round(cpu["fwd_bwd_s"] / dev["fwd_bwd_s"], 1),
+            "protocol": f"{warmup} warm-up(s), median of {reps}",
+        }
+    sec = out[f"{workload}_{W}x{H}"]["cpu_fwd_bwd_ms"] * 1e-3
     return {
         "value": round(bytes_step / sec / 1e9, 4),
         "unit": "GB/s",
         "cores": torch.get_num_threads(),
         "kind": "port",
-        "sample": (f"whole {workload} workload, fwd+bwd of the torch restatement of render.py "
-                   f"(float32, render_chunk 512, autograd), median of {reps}: "
-                   f"{sec * 1e3:.1f} ms per fwd+bwd"),
+        "sample": (f"the {workload} workload's fwd+bwd (whole field) through the torch "
+                   "restatement of render.py (float32, render_chunk 512, autograd) on "
+                   f"{threads} threads, 3 warm-ups and the median of 3: "
+                   f"{sec * 1e3:.1f} ms per fwd+bwd; also the 256x256/2k CPU case"),
         "seconds_per_step": round(sec, 4),
+        "os_cpu_count": os.cpu_count(),
         "cpu_model": _cpu_model(),
+        "workloads": out,
+        "gpu_timing": "render_hip_exact + autograd backward per call (binning, forward, "
+                      "backward and the autograd node's allocations), 3 warm-ups, median of 10",
     }
-
-
 def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
     """Batched fitting: `num_fields` fields, sides uniform over {256, 512, 768, 1024}, Gaussian
     counts log-uniform in [1k, 60k], fit-target recipe; the batched graph-captured fit against
@@ -143,17 +208,8 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
     from structsplat_amd import synth
     from structsplat_amd.gaussians import GaussianField
 
-    rng = np.random.default_rng(seed)
-    sides = rng.choice([256, 512, 768, 1024], num_fields)
-    counts = np.exp(rng.uniform(np.log(1000), np.log(60000), num_fields)).astype(int)
-    order = np.argsort(sides, kind="stable")
-    sides, counts = sides[order], counts[order]
     dev = torch.device("cuda:0")
-    targets, inits = [], []
-    for k, (s, n) in enumerate(zip(sides, counts)):
-        t = synth.fit_target(1000 + k, int(s), int(s))
-        targets.append(torch.tensor(t, device=dev))
-        inits.append(synth.fit_init_field(t, int(n), 1000 + k))
+    sides, counts, targets, inits = synth.batched_fit_workload(num_fields, seed, device=dev)
 
     def fields():
         return [GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
@@ -191,9 +247,8 @@ def fit_loop(steps: int, seed: int = 0) -> dict:
     from structsplat_amd.gaussians import GaussianField
 
     dev = torch.device("cuda:0")
-    t = synth.fit_target(seed, 2048, 2048)
-    target = torch.tensor(t, device=dev)
-    init = synth.fit_init_field(t, 200_000, seed)
+    target = synth.fit_target(seed, 2048, 2048, device=dev)
+    init = synth.fit_init_field(target, 200_000, seed)
 
     def field():
         return GaussianField.from_numpy(init["means"], init["scales"], init["angles"],

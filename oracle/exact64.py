@@ -114,6 +114,42 @@ def forward(means, conics, colors, radii, opacities, H: int, W: int, normalize: 
     return out.reshape(H, W, 3), den.reshape(H, W)
 
 
+def forward_conditioning(means, conics, colors, radii, opacities, H: int, W: int,
+                         normalize: bool = True, eps: float = 1e-8, support_fade: bool = False,
+                         sigma_cutoff: float = 3.0, rects=None, float64_params: bool = False):
+    """Per-pixel float32 sensitivity (H, W, 3) of the forward: a float32 evaluation of the
+    reference's formula (render.py:101-107, :180-235) errs at a pixel by a few units of float32
+    rounding times this value.  Per visit, the rounding of q = a dx^2 + 2b dx dy + c dy^2 is
+    relative to qabs = |a| dx^2 + 2|b dx dy| + |c| dy^2 (large for needle-like Gaussians, whose
+    terms cancel), so raw = exp(-q/2) carries raw (1 + qabs/2) of it, and a faded weight
+    raw - tail carries that plus the tail's own rounding (large relative to the weight near the
+    cutoff, where the difference cancels).  With s = o (raw (1 + qabs/2) + tail):
+        additive:   S_k = sum s c_k
+        normalized: (S_k + |out_k| sum s) / (D + eps)
+    """
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects,
+                                   float64_params)
+    fade, tail = _tail(support_fade, sigma_cutoff)
+    num = np.zeros((H * W, 3))
+    den = np.zeros(H * W)
+    sens = np.zeros((H * W, 3))
+    sens_d = np.zeros(H * W)
+    for gid, px, py in _visit_chunks(*rect):
+        dx, dy, a, b, c, raw, base, w = _weights(m, q, op, gid, px, py, fade, tail)
+        qabs = np.abs(a) * dx * dx + 2.0 * np.abs(b * dx * dy) + np.abs(c) * dy * dy
+        sv = op[gid] * (raw * (1.0 + 0.5 * qabs) + (tail if fade else 0.0))
+        flat = py * W + px
+        for k in range(3):
+            num[:, k] += np.bincount(flat, weights=w * col[gid, k], minlength=H * W)
+            sens[:, k] += np.bincount(flat, weights=sv * np.abs(col[gid, k]), minlength=H * W)
+        den += np.bincount(flat, weights=w, minlength=H * W)
+        sens_d += np.bincount(flat, weights=sv, minlength=H * W)
+    if normalize:
+        out = num / (den[:, None] + eps)
+        sens = (sens + np.abs(out) * sens_d[:, None]) / (den[:, None] + eps)
+    return sens.reshape(H, W, 3)
+
+
 def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
              normalize: bool = True, eps: float = 1e-8, support_fade: bool = False,
              sigma_cutoff: float = 3.0, rects=None, float64_params: bool = False):

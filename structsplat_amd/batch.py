@@ -215,7 +215,11 @@ class _BatchRender(Function):
 
 
 def concat_fields(fields: list[GaussianField]) -> GaussianField:
-    """One field holding every field's rows in order (opacity logits required on all or none)."""
+    """One field holding every field's rows in order (opacity logits required on all or none).
+
+    The covariance filter variance and the scale caps travel with the rows.  A field without
+    them gets the neutral value where another field has them: variance 0 (adding 0 leaves the
+    producer's float32 arithmetic unchanged) and cap +inf (no cap)."""
     has_op = [f.opacities is not None for f in fields]
     if any(has_op) and not all(has_op):
         raise ValueError("either every field has opacities or none has")
@@ -223,8 +227,22 @@ def concat_fields(fields: list[GaussianField]) -> GaussianField:
     def cat(name):
         return torch.cat([getattr(f, name).detach() for f in fields], 0).clone()
 
+    def cat_optional(name, shape_of, fill):
+        if all(getattr(f, name) is None for f in fields):
+            return None
+        parts = []
+        for f in fields:
+            v = getattr(f, name)
+            ref = f.log_scales.detach()
+            parts.append(torch.full(shape_of(f), fill, dtype=torch.float32, device=ref.device)
+                         if v is None else v.detach().to(device=ref.device,
+                                                        dtype=torch.float32).reshape(shape_of(f)))
+        return torch.cat(parts, 0).clone()
+
     return GaussianField(cat("means"), cat("log_scales"), cat("rotations"), cat("colors"),
-                         cat("opacities") if all(has_op) else None)
+                         cat("opacities") if all(has_op) else None,
+                         scale_max=cat_optional("scale_max", lambda f: (f.n, 2), math.inf),
+                         filter_variance=cat_optional("filter_variance", lambda f: (f.n,), 0.0))
 
 
 @dataclass
@@ -257,8 +275,11 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     batch = Batch(sizes, counts, dev, tile_size=F.resolve_tile_size(cfg, tune))
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
     opt = F.make_optimizer(big, cfg)
-    hi = torch.cat([torch.full((f.n,), math.log(max(h, w)), device=dev)
-                    for f, (h, w) in zip(fields, sizes)], 0)
+    # Per-field log-scale upper bounds (one per element when a field caps its scales).
+    uppers = [F.log_scale_upper(f, h, w) for f, (h, w) in zip(fields, sizes)]
+    hi = torch.cat([u.to(dev) if torch.is_tensor(u)
+                    else torch.full((2 * f.n,), u, dtype=torch.float32, device=dev)
+                    for f, u in zip(fields, uppers)], 0)
     opt.set_clamp(big.log_scales, math.log(0.35), hi)
     w = cfg.ssim_weight
 

@@ -260,6 +260,20 @@ class _Snapshot:
                 t.copy_(saved)
 
 
+def log_scale_upper(field: GaussianField, H: int, W: int):
+    """Upper bound of the log-scale clamp after each step (fit.py:3033, :3359-3375):
+    ``ln max(H, W)``, and with a per-Gaussian ``scale_max`` also ``ln clamp(scale_max, 1e-3)``
+    element by element.  Returns a number, or a float32 tensor of one bound per element.
+    ``min(clamp(x, lo, hi), cap)`` (the reference's order) equals ``clamp(x, lo, min(hi, cap))``
+    because ``lo < hi``."""
+    hi = math.log(max(int(H), int(W)))
+    if field.scale_max is None:
+        return hi
+    cap = torch.log(torch.clamp(field.scale_max.detach().to(device=field.log_scales.device,
+                                                            dtype=torch.float32), min=1e-3))
+    return torch.clamp(cap.reshape(field.log_scales.shape), max=hi).reshape(-1).contiguous()
+
+
 def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool = False,
         verbose: bool = False) -> dict:
     """Fit `field` to `target` (H, W, 3) for cfg.iters Adam steps.
@@ -283,7 +297,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     affine = field.color_grads is not None
     field.trainable()
     opt = make_optimizer(field, cfg)
-    opt.set_clamp(field.log_scales, math.log(0.35), math.log(max(H, W)))
+    opt.set_clamp(field.log_scales, math.log(0.35), log_scale_upper(field, H, W))
     if affine:
         if cfg.tile_size not in ("auto", HR.DEFAULT_TILE_SIZE):
             raise ValueError("affine colors render with 16-pixel tiles")

@@ -267,10 +267,13 @@ class _Produce(Function):
         g_logits = torch.empty_like(op) if has_op else None
         if g_conics is None:
             g_conics = torch.zeros(n, 3, dtype=torch.float32, device=log_scales.device)
+        # Contiguous copies are bound to names so they stay alive until the launch is enqueued
+        # (a temporary freed earlier could be reused by the next copy on the same stream).
+        gq = g_conics.contiguous()
+        go = g_op.contiguous() if (has_op and g_op is not None) else None
         stream = ctypes.c_void_p(torch.cuda.current_stream(log_scales.device).cuda_stream)
         _native.check(_native.load().ssplat_produce_field_backward(
             _ptr(log_scales), _ptr(rotations), _ptr(op) if has_op else None, _ptr(fv), n,
-            ctx.dilation, _ptr(g_conics.contiguous()),
-            _ptr(g_op.contiguous()) if (has_op and g_op is not None) else None,
-            _ptr(g_ls), _ptr(g_rot), _ptr(g_logits), stream), "ssplat_produce_field_backward")
+            ctx.dilation, _ptr(gq), _ptr(go), _ptr(g_ls), _ptr(g_rot), _ptr(g_logits), stream),
+            "ssplat_produce_field_backward")
         return g_ls, g_rot, g_logits, None, None, None

@@ -39,15 +39,17 @@ class FusedAdam:
         self._clamp = {}
 
     def set_clamp(self, param: torch.Tensor, lo: float, hi=None) -> None:
-        """Clamp `param` to [lo, hi] after every step; `hi` is a number or a tensor with one
-        upper bound per row of `param` (rows = the leading dimension)."""
+        """Clamp `param` to [lo, hi] after every step; `hi` is a number or a float32 tensor
+        holding one upper bound per row of `param` (rows = the leading dimension) or one per
+        element."""
         if param not in self.state:
             raise ValueError("set_clamp: not a parameter of this optimizer")
         if torch.is_tensor(hi):
             rows = hi.reshape(-1)
-            if rows.numel() != param.shape[0] or rows.dtype != torch.float32:
-                raise ValueError("set_clamp: need one float32 bound per row")
-            width = param.numel() // max(param.shape[0], 1)
+            if rows.dtype != torch.float32 or rows.numel() not in (param.shape[0], param.numel()):
+                raise ValueError("set_clamp: need one float32 bound per row or per element")
+            width = (1 if rows.numel() == param.numel()
+                     else param.numel() // max(param.shape[0], 1))
             self._clamp[param] = (float(lo), 0.0, rows.contiguous(), width)
         else:
             self._clamp[param] = (float(lo), float(hi), None, 1)

@@ -491,40 +491,72 @@ def render_hip_affine(means, conics, colors, radii, H: int, W: int, color_grads,
                                   _fade_alpha(support_fade), float(sigma_cutoff))
 
 
-def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=None,
-                     normalize: bool = True, eps: float = 1e-8, tiled: bool = True,
-                     tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool | float = False,
-                     sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
-                     tile_index_backend: str = "hip", tile_ellipse_cull: bool = True,
-                     recompute_normalizer: bool = True):
-    """Exact normalized/additive render on the GPU (signature of ``render_cuda_exact``,
-    cuda_render.py:256-294, plus ``recompute_normalizer``).
+RENDER_TILE_SIZES = (16, 32)
 
-    ``support_fade`` is the reference's bool (full fade) or a fade strength in [0, 1]
-    (render.py:94-107).
 
-    ``tiled`` and ``backward_variant`` are accepted for drop-in compatibility: the HIP path is
-    always tiled and has one backward.  ``recompute_normalizer`` (default) keeps only the image
-    for the backward and recomputes the per-pixel normalizer there; ``False`` saves it too.
-    ``tile_index_backend`` accepts ``"hip"`` and the reference's ``"cuda"`` and ``"torch"``
-    (cuda_render.py:285-288); all three bin on the GPU, since the torch builder's lists hold the
-    same Gaussians per tile (test_render.py:401-409) and the ascending order is one of its orders.
+def render_tile_size(tile_size: int) -> int:
+    """Tile edge the render kernels run for a requested ``tile_size`` (any edge with
+    ``tile_size^2 <= 1024``, the reference's contract, render_ext.cpp:162-163).
+
+    The kernels are built for 16- and 32-pixel tiles.  Every pixel sums exactly the Gaussians
+    whose support rectangle holds it, in ascending id, whatever the tiling, so the image does
+    not depend on the tile edge; another requested edge renders at 16, the reference default.
     """
-    _check_field(means, conics, colors, radii, opacities)
+    ts = int(tile_size)
+    if ts <= 0 or ts * ts > 1024:
+        raise ValueError("tile_size must be positive with tile_size^2 <= 1024")
+    return ts if ts in RENDER_TILE_SIZES else DEFAULT_TILE_SIZE
+
+
+def _check_exact_args(H: int, W: int, tiled: bool, tile_size: int, backward_variant: str,
+                      tile_index_backend: str) -> int:
+    """Argument checks of ``render_cuda_exact`` after its device and dtype checks, in the
+    reference's order (cuda_render.py:278-288); returns the tile edge the kernels run."""
     if backward_variant not in ("baseline", "block_reduce"):
         raise ValueError(
             f"backward_variant must be baseline or block_reduce, got {backward_variant!r}"
         )
+    if tiled and backward_variant != "baseline":
+        raise ValueError("block_reduce backward is experimental and untiled-only")
     if tile_index_backend not in ("hip", "cuda", "torch"):
         raise ValueError(
             f"tile_index_backend must be hip, cuda or torch, got {tile_index_backend!r}")
     _check_image(H, W, tile_size)
+    return render_tile_size(tile_size) if tiled else DEFAULT_TILE_SIZE
+
+
+def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=None,
+                     normalize: bool = True, eps: float = 1e-8, tiled: bool = False,
+                     tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool | float = False,
+                     sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
+                     tile_index_backend: str = "hip", tile_ellipse_cull: bool = True,
+                     recompute_normalizer: bool = True):
+    """Exact normalized/additive render on the GPU (signature and checks of
+    ``render_cuda_exact``, cuda_render.py:256-294, plus ``recompute_normalizer``).
+
+    ``support_fade`` is the reference's bool (full fade) or a fade strength in [0, 1]
+    (render.py:94-107).
+
+    The HIP path is always tiled and has one deterministic backward.  ``tiled`` keeps the
+    reference's meaning for argument checking: ``tiled=True`` with ``backward_variant=
+    "block_reduce"`` raises as in cuda_render.py:283-284; with ``tiled=False`` (the reference
+    default, whose untiled kernels compute the same sums) ``tile_size`` is ignored as the
+    reference ignores it.  A tiled ``tile_size`` may be any edge with ``tile_size^2 <= 1024``
+    (render_ext.cpp:162-163); see ``render_tile_size`` for the edge the kernels run.
+    ``recompute_normalizer`` (default) keeps only the image for the backward and recomputes the
+    per-pixel normalizer there; ``False`` saves it too.  ``tile_index_backend`` accepts
+    ``"hip"`` and the reference's ``"cuda"`` and ``"torch"`` (cuda_render.py:285-288); all three
+    bin on the GPU, since the torch builder's lists hold the same Gaussians per tile
+    (test_render.py:401-409) and the ascending order is one of its orders.
+    """
+    _check_field(means, conics, colors, radii, opacities)
+    ts = _check_exact_args(H, W, tiled, tile_size, backward_variant, tile_index_backend)
     if normalize:
         eps = _validate_normalization_eps(eps)
     if opacities is None:
         opacities = means.new_empty(0)
     return _ExactRenderHip.apply(means, conics, colors, radii, opacities, int(H), int(W),
-                                 bool(normalize), float(eps), int(tile_size),
+                                 bool(normalize), float(eps), ts,
                                  _fade_alpha(support_fade), float(sigma_cutoff),
                                  bool(tile_ellipse_cull), bool(recompute_normalizer))
 

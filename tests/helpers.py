@@ -6,6 +6,7 @@ import math
 import numpy as np
 import torch
 
+from oracle import exact64
 from oracle import reference_render as R
 from oracle.tile_index import support_rects
 
@@ -89,3 +90,65 @@ def window_field(means, conics, colors, radii, opacities, H: int, W: int, x0: in
     op = None if opacities is None or np.size(opacities) == 0 else f64(opacities, 1)[:, 0]
     return (ids, m32[ids].astype(np.float64) - shift, f64(conics, 3), f64(colors, 3),
             np.asarray(radii).reshape(-1, 2)[ids], op, rects)
+
+
+def _host_np(x):
+    return x.detach().cpu().numpy() if torch.is_tensor(x) else np.asarray(x)
+
+
+def pixel_rel_err(got, ref, floor: float = 1e-6):
+    """Per-pixel (elementwise) relative error max |got - ref| / max(|ref|, floor * max |ref|):
+    the forward's 1e-5 relative bar taken pixel by pixel, with values below `floor` of the
+    image's scale measured against that floor (exact zeros of both sides count as 0)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    if ref.size == 0:
+        return 0.0
+    scale = float(np.max(np.abs(ref)))
+    denom = np.maximum(np.abs(ref), max(floor * scale, 1e-300))
+    return float(np.max(np.abs(got - ref) / denom))
+
+
+def local_gradient(host, H, W, G, i, normalize):
+    """Gaussian i's float64 gradient from the oracle on its own support rectangle (a pixel sums
+    only the Gaussians whose rectangle holds it, so these pixels give its whole gradient), with
+    the window problem (for finite differences).  `host` = (means, conics, colors, radii,
+    opacities) as arrays or tensors."""
+    m, q, c, r, o = (_host_np(t) for t in host)
+    x0, x1, y0, y1, valid = support_rects(m, r, H, W)
+    assert valid[i]
+    win = window_field(m, q, c, r, o, H, W, x0[i], x1[i], y0[i], y1[i])
+    ids, mw, qw, cw, rw, ow, rects = win
+    k = int(np.searchsorted(ids, i))
+    assert ids[k] == i
+    h, w = int(y1[i] - y0[i]), int(x1[i] - x0[i])
+    Gw = G[y0[i]:y1[i], x0[i]:x1[i]]
+    g = exact64.backward(Gw, mw, qw, cw, rw, ow, h, w, normalize, 1e-8, False, 3.0, rects=rects,
+                         float64_params=True)
+    return [x[k] for x in g], (k, Gw, h, w, mw, qw, cw, rw, ow, rects)
+
+
+def sample_gaussians(host, H, W, count, seed):
+    """`count` random visible Gaussians (ascending ids)."""
+    m, r = _host_np(host[0]), _host_np(host[3])
+    valid = support_rects(m, r, H, W)[4]
+    ids = np.nonzero(valid)[0]
+    return np.sort(np.random.default_rng(seed).choice(ids, size=count, replace=False))
+
+
+F32_UNIT = 2.0 ** -24  # float32 unit roundoff
+
+
+def pixel_bar_err(got, ref, sens, rel: float = 1e-5, units: float = 64.0):
+    """The forward's per-pixel bar as a ratio (pass <= 1): each value must satisfy
+    ``|got - ref| <= max(rel * |ref|, units * u * sens)``, with ``sens`` from
+    ``exact64.forward_conditioning`` (the pixel's float32 sensitivity, u = 2^-24).  Where the
+    reference's own float32 formula is well conditioned (sens ~ |ref|) this is the 1e-5 relative
+    bar itself; where it is not (faded weights near the cutoff, needle-like Gaussians) the bar is
+    what any float32 evaluation of that formula can meet."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    allow = np.maximum(rel * np.abs(ref), units * F32_UNIT * np.asarray(sens, dtype=np.float64))
+    diff = np.abs(got - ref)
+    ratio = np.where(allow > 0, diff / np.maximum(allow, 1e-300), np.where(diff > 0, np.inf, 0))
+    return float(np.max(ratio)) if ratio.size else 0.0

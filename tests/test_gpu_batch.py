@@ -105,3 +105,34 @@ def test_batched_fit_matches_plain_loop(cuda):
     print("plain", plain.psnr, plain.seconds, "batched", batched.psnr, batched.seconds)
     # Deterministic gradients and the same tile size: the same fits, bit for bit.
     assert plain.psnr == batched.psnr
+
+
+def test_batched_fit_with_filter_variance_and_caps_matches_plain_loop(cuda):
+    """Fields with a covariance filter variance or scale caps (and fields without either) fit in
+    one batch exactly as they fit one by one: the batch carries both with the rows."""
+    print("case: batched fit with filter variance and scale caps", flush=True)
+    sizes = [(64, 64), (48, 80), (64, 64)]
+    counts = [300, 250, 200]
+    targets, init = [], []
+    for k, ((h, w), n) in enumerate(zip(sizes, counts)):
+        t = synth.fit_target(70 + k, h, w)
+        targets.append(torch.tensor(t, device=cuda))
+        init.append(synth.fit_init_field(t, n, 70 + k))
+
+    def fields():
+        out = []
+        for k, f in enumerate(init):
+            g = GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
+                                         opacities=f["opacity"], device=cuda)
+            if k == 0:
+                g.filter_variance = torch.full((g.n,), 0.35, device=cuda)
+            if k == 1:
+                g.scale_max = g.scales().detach() * 1.02
+            out.append(g)
+        return out
+
+    cfg = F.FitConfig(iters=20, window=10, lr_scales=0.03)
+    batched = B.fit_batch(fields(), targets, cfg, graph=True)
+    plain = B.fit_each(fields(), targets, dataclasses.replace(cfg, tile_size=batched.tile_size))
+    print("plain", plain.psnr, "batched", batched.psnr)
+    assert plain.psnr == batched.psnr

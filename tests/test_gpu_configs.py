@@ -1,0 +1,204 @@
+"""GPU tests at the benchmark's named workloads that the other GPU tests do not run:
+
+* the CPU reference case, 256 x 256 RGB, 2,000 Gaussians, seed 0, L1 loss against a U[0,1]
+  target: the whole image against the float64 oracle (scaled and per-pixel), every gradient
+  against the oracle's float64 backward and against float32 torch autograd of the oracle's
+  restatement of render.py (the reference's own numerics), through the field producer to the
+  log-scales, rotations and opacity logits;
+* the full fit loop, 2048 x 2048, 200,000 Gaussians, 1,000 Adam steps with the whole step
+  captured as a HIP graph: the graphed fit equals the eager fit bit for bit, and one step's
+  render gradients at the fitted field match the float64 oracle on the Gaussians' own pixels;
+* the ragged batched fit, 64 fields x 500 Adam steps: every field's PSNR equals the plain
+  per-image loop's (the bar is 0.01 dB; deterministic gradients give equality).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact64
+from oracle import reference_render as R
+from structsplat_amd import batch as B
+from structsplat_amd import fit as F
+from structsplat_amd import render as HR
+from structsplat_amd import synth
+from structsplat_amd.gaussians import GaussianField
+from tests.helpers import (local_gradient, pixel_bar_err, pixel_rel_err, sample_gaussians,
+                           scaled_err, to_np, window_field)
+
+pytestmark = pytest.mark.gpu
+
+GRAD_NAMES = ("means", "conics", "colors", "opacities")
+
+
+def _case(**kw):
+    print("case:", ", ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _c1_field(device):
+    f = synth.cpu_reference_field(0)
+    logits = torch.logit(torch.tensor(f["opacity"], dtype=torch.float64)).float().numpy()
+    return GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
+                                    opacities=logits, device=device).trainable()
+
+
+def test_cpu_reference_case_forward_and_l1_gradients(cuda):
+    """256^2 / 2,000 Gaussians, seed 0: normalized render through render_field, L1 loss against a
+    seeded U[0,1] target, backward to the field's parameters."""
+    _case(test="256x256/2000 L1 fwd+bwd", seed=0)
+    H = W = 256
+    field = _c1_field(cuda)
+    conics, radii, op = field.produce(3.0)
+    conics.retain_grad()
+    op.retain_grad()
+    img = HR.render_field(field.means, conics, field.colors, radii, H, W, mode="normalized",
+                          opacities=op)
+    target = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(0))
+    loss = (img - target).abs().mean()
+    loss.backward()
+    torch.cuda.synchronize()
+
+    m, q, c, r, o = (to_np(t) for t in (field.means, conics, field.colors, radii, op))
+    ref, _ = exact64.forward(m, q, c, r, o, H, W, True, 1e-8, False, 3.0)
+    sens = exact64.forward_conditioning(m, q, c, r, o, H, W, True, 1e-8, False, 3.0)
+    got = to_np(img)
+    err = scaled_err(got, ref)
+    prel = pixel_rel_err(got, ref)
+    bar = pixel_bar_err(got, ref, sens)
+    print("forward scaled err", err, "per-pixel rel err (floor 1e-6)", prel, "per-pixel bar", bar)
+    assert err <= 1e-5
+    assert bar <= 1.0
+
+    # dL/dimg of the L1 mean, taken from the GPU image (sign(0) pixels are measure zero).
+    G = np.sign(got.astype(np.float64) - to_np(target).astype(np.float64)) / (3 * H * W)
+    g64 = exact64.backward(G, m, q, c, r, o, H, W, True, 1e-8, False, 3.0)
+    gpu = (to_np(field.means.grad), to_np(conics.grad), to_np(field.colors.grad), to_np(op.grad))
+    for name, a, b in zip(GRAD_NAMES, gpu, g64):
+        e = scaled_err(a, b)
+        print(name, "vs float64 analytic", e)
+        assert e <= 1e-4, name
+
+    # float32 torch autograd of the restatement of render.py (the reference's CPU numerics).
+    leaves = [torch.tensor(x, requires_grad=True) for x in (m, q, c, o)]
+    ref32 = R.render(leaves[0], leaves[1], leaves[2], torch.from_numpy(r), H, W,
+                     opacities=leaves[3])
+    ref32.backward(torch.from_numpy(G.astype(np.float32)))
+    e32 = scaled_err(got, to_np(ref32))
+    print("forward vs float32 restatement", e32)
+    assert e32 <= 2e-5  # test_render.py:197
+    for name, a, leaf in zip(GRAD_NAMES, gpu, leaves):
+        e = scaled_err(a, to_np(leaf.grad))
+        print(name, "vs float32 autograd", e)
+        assert e <= 3e-4, name  # test_render.py:495-497
+
+    # Producer chain: log-scales, rotations and logits against torch float64 autograd of the
+    # reference formulas (gaussians.py:270-300) fed the same conic and opacity gradients.
+    ls = field.log_scales.detach().double().cpu().requires_grad_(True)
+    rot = field.rotations.detach().double().cpu().requires_grad_(True)
+    lg = field.opacities.detach().double().cpu().requires_grad_(True)
+    q64 = R.conics_from_rs(ls, rot)
+    o64 = torch.sigmoid(lg)
+    (q64 * conics.grad.double().cpu()).sum().add_((o64 * op.grad.double().cpu()).sum()).backward()
+    for name, a, b in (("log_scales", field.log_scales.grad, ls.grad),
+                       ("rotations", field.rotations.grad, rot.grad),
+                       ("opacity logits", field.opacities.grad, lg.grad)):
+        e = scaled_err(to_np(a), b.numpy())
+        print(name, "vs float64 producer chain", e)
+        assert e <= 1e-5, name
+
+
+@pytest.mark.timeout(600)
+def test_full_fit_graph_equals_eager_and_gradients_match_fp64(cuda):
+    """2048^2 target (32 sinusoids + 200 step edges), 200,000 Gaussians, 1,000 Adam steps of
+    L1 + 0.2 (1 - SSIM): the hipGraph-captured fit equals the eager fit bit for bit (PSNR,
+    loss and every parameter); at the fitted field one step's render gradients match the
+    float64 oracle for 48 Gaussians, and the image matches on 12 windows."""
+    _case(test="2048x2048/200k fit, 1000 steps, graph vs eager")
+    H = W = 2048
+    target = synth.fit_target(0, H, W, device=cuda)
+    init = synth.fit_init_field(target, 200_000, 0)
+
+    def field():
+        return GaussianField.from_numpy(init["means"], init["scales"], init["angles"],
+                                        init["colors"], opacities=init["opacity"], device=cuda)
+
+    cfg = F.FitConfig(iters=1000)
+    fg, fe = field(), field()
+    graphed = F.fit(fg, target, cfg, graph=True)
+    print("graphed", {k: graphed[k] for k in ("psnr", "loss", "regrows", "tile_size")},
+          flush=True)
+    eager = F.fit(fe, target, dataclasses.replace(cfg, tile_size=graphed["tile_size"]),
+                  graph=False)
+    print("eager", {k: eager[k] for k in ("psnr", "loss", "regrows", "tile_size")}, flush=True)
+    assert graphed["psnr"] == eager["psnr"]
+    assert float(graphed["loss"]) == float(eager["loss"])
+    for a, b in zip(fg.parameters(), fe.parameters()):
+        assert torch.equal(a.detach(), b.detach())
+    assert graphed["psnr"] > 25.0
+
+    # One step's render gradients at the fitted field, against the float64 oracle.
+    with torch.no_grad():
+        conics, radii, op = fe.produce(3.0)
+    leaves = [t.detach().clone().contiguous().requires_grad_(True)
+              for t in (fe.means, conics, fe.colors, op)]
+    img = HR.render_hip_exact(leaves[0], leaves[1], leaves[2], radii, H, W, opacities=leaves[3])
+    img_d = img.detach().clone().requires_grad_(True)
+    loss = F.loss_fn(img_d, target, cfg.ssim_weight)
+    loss.backward()
+    G = img_d.grad.detach().contiguous()
+    img.backward(G)
+    torch.cuda.synchronize()
+    host = tuple(to_np(t) for t in (fe.means, conics, fe.colors, radii, op))
+    got_img = to_np(img)
+    rng = np.random.default_rng(3)
+    for _ in range(12):
+        x0, y0 = int(rng.integers(0, W - 48)), int(rng.integers(0, H - 48))
+        ids, mw, qw, cw, rw, ow, rects = window_field(*host, H, W, x0, x0 + 48, y0, y0 + 48)
+        ref, _ = exact64.forward(mw, qw, cw, rw, ow, 48, 48, True, 1e-8, False, 3.0, rects=rects,
+                                 float64_params=True)
+        sens = exact64.forward_conditioning(mw, qw, cw, rw, ow, 48, 48, True, 1e-8, False, 3.0,
+                                            rects=rects, float64_params=True)
+        win = got_img[y0:y0 + 48, x0:x0 + 48]
+        e, pe, bar = scaled_err(win, ref), pixel_rel_err(win, ref), pixel_bar_err(win, ref, sens)
+        print("window", x0, y0, "scaled", e, "per-pixel rel", pe, "bar", bar)
+        assert e <= 1e-5 and bar <= 1.0, (x0, y0, e, pe, bar)
+    Gn = to_np(G).astype(np.float64)
+    gpu = [to_np(t.grad) for t in leaves]
+    an = {k: [] for k in GRAD_NAMES}
+    got = {k: [] for k in GRAD_NAMES}
+    for i in sample_gaussians(host, H, W, 48, seed=17):
+        analytic, _ = local_gradient(host, H, W, Gn, i, True)
+        for slot, key in enumerate(GRAD_NAMES):
+            an[key].append(np.atleast_1d(analytic[slot]))
+            got[key].append(np.atleast_1d(gpu[slot][i]))
+    for key in GRAD_NAMES:
+        e = scaled_err(np.array(got[key]), np.array(an[key]))
+        print(key, "scaled err", e)
+        assert e <= 1e-4, key
+
+
+@pytest.mark.timeout(900)
+def test_batched_fit_64_ragged_fields_equals_plain_loop(cuda):
+    """64 fields (sides in {256, 512, 768, 1024}, 1k-60k Gaussians), 500 Adam steps each: the
+    batched graph-captured fit against the plain per-image loop at the same tile size."""
+    _case(test="batched fit 64 fields x 500 steps vs plain loop")
+    sides, counts, targets, inits = synth.batched_fit_workload(64, 0, device=cuda)
+
+    def fields():
+        return [GaussianField.from_numpy(f["means"], f["scales"], f["angles"], f["colors"],
+                                         opacities=f["opacity"], device=cuda) for f in inits]
+
+    cfg = F.FitConfig(iters=500)
+    batched = B.fit_batch(fields(), targets, cfg, graph=True)
+    print("batched", round(batched.seconds, 3), "s, tile", batched.tile_size, flush=True)
+    plain = B.fit_each(fields(), targets, dataclasses.replace(cfg, tile_size=batched.tile_size))
+    print("plain", round(plain.seconds, 3), "s", flush=True)
+    diff = np.abs(np.array(batched.psnr) - np.array(plain.psnr))
+    print("max |PSNR diff| dB", float(diff.max()), "mean PSNR", float(np.mean(batched.psnr)))
+    assert len(batched.psnr) == 64
+    assert float(diff.max()) <= 0.01
+    assert batched.psnr == plain.psnr  # deterministic gradients: the same fits bit for bit
+    assert min(batched.psnr) > 20.0

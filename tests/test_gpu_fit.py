@@ -1,6 +1,8 @@
 """GPU tests of the fit step: graph replays match eager steps, capacity overflow rolls back."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -177,3 +179,25 @@ def test_affine_plain_and_graphed_fits_agree(cuda):
     assert abs(ra["psnr"] - rb["psnr"]) < 0.01 and ra["psnr"] > 10.0
     assert float(a.color_grads.abs().max()) > 0.0 and ra["color_grad_l2_term"] > 0.0
     assert torch.allclose(a.color_grads, b.color_grads, rtol=1e-3, atol=1e-5)
+
+
+def test_fit_caps_log_scales_at_scale_max(cuda):
+    """The reference's step caps log-scales at log(clamp(scale_max, 1e-3)) after the
+    [ln 0.35, ln max(H, W)] clamp (fit.py:3359-3375): capped rows never exceed their cap, and a
+    cap below ln 0.35 wins over the lower bound."""
+    print("case: fit with scale_max caps", flush=True)
+    field, target = _problem(cuda, H=64, W=80, n=300, seed=9)
+    s0 = field.scales().detach()
+    cap = s0 * 1.05
+    cap[::3] = 0.1  # below exp(ln 0.35): the cap wins over the lower clamp
+    field.scale_max = cap.clone()
+    cfg = F.FitConfig(iters=30, window=10, lr_scales=0.05)
+    F.fit(field, target, cfg, graph=False)
+    torch.cuda.synchronize()
+    limit = torch.log(torch.clamp(cap, min=1e-3))
+    ls = field.log_scales.detach()
+    assert bool((ls <= limit).all())
+    assert torch.equal(ls[::3], limit[::3])
+    lo = torch.tensor(math.log(0.35), dtype=torch.float32)
+    free = limit >= lo
+    assert bool((ls[free] >= lo).all())

@@ -26,7 +26,8 @@ from oracle import exact64
 from oracle import tile_index as TI
 from structsplat_amd import render as HR
 from structsplat_amd import synth
-from tests.helpers import field_tensors, scaled_err, window_field
+from tests.helpers import (field_tensors, local_gradient, pixel_rel_err, sample_gaussians,
+                           scaled_err, window_field)
 
 pytestmark = pytest.mark.gpu
 
@@ -66,30 +67,6 @@ def _grad_image(H, W, dev, seed=3):
     return torch.rand(H, W, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(seed))
 
 
-def _local_gradient(host, H, W, G, i, normalize):
-    """Gaussian i's float64 gradient from the oracle on its own support rectangle, with the
-    window problem (for finite differences)."""
-    m, q, c, r, o = (_np(t) for t in host)
-    x0, x1, y0, y1, valid = TI.support_rects(m, r, H, W)
-    assert valid[i]
-    win = window_field(m, q, c, r, o, H, W, x0[i], x1[i], y0[i], y1[i])
-    ids, mw, qw, cw, rw, ow, rects = win
-    k = int(np.searchsorted(ids, i))
-    assert ids[k] == i
-    h, w = int(y1[i] - y0[i]), int(x1[i] - x0[i])
-    Gw = G[y0[i]:y1[i], x0[i]:x1[i]]
-    g = exact64.backward(Gw, mw, qw, cw, rw, ow, h, w, normalize, 1e-8, False, 3.0, rects=rects,
-                         float64_params=True)
-    return [x[k] for x in g], (k, Gw, h, w, mw, qw, cw, rw, ow, rects)
-
-
-def _sample_gaussians(host, H, W, count, seed):
-    m, r = _np(host[0]), _np(host[3])
-    valid = TI.support_rects(m, r, H, W)[4]
-    ids = np.nonzero(valid)[0]
-    return np.sort(np.random.default_rng(seed).choice(ids, size=count, replace=False))
-
-
 @pytest.mark.parametrize("ts", [16, 32])
 @pytest.mark.parametrize("name", ["single", "metric", "stress"])
 def test_full_size_tile_lists_bit_exact(cuda, name, ts):
@@ -108,13 +85,17 @@ def test_single_workload_whole_image_matches_fp64(cuda, normalize):
     (m, q, c, r, o), H, W = _device("single", cuda)
     out = HR.render_hip_exact(m, q, c, r, H, W, opacities=o, normalize=normalize)
     torch.cuda.synchronize()
-    ref, _ = exact64.forward(_np(m), _np(q), _np(c), _np(r), _np(o), H, W, normalize, 1e-8,
-                             False, 3.0)
+    args = (_np(m), _np(q), _np(c), _np(r), _np(o), H, W, normalize, 1e-8, False, 3.0)
+    ref, _ = exact64.forward(*args)
+    sens = exact64.forward_conditioning(*args)
     got = _np(out)
     assert np.isfinite(got).all()
     err = scaled_err(got, ref)
-    print("scaled err", err)
+    bar = pixel_bar_err(got, ref, sens)
+    print("scaled err", err, "per-pixel rel err (floor 1e-6)", pixel_rel_err(got, ref),
+          "per-pixel bar", bar)
     assert err <= 1e-5
+    assert bar <= 1.0
 
 
 @pytest.mark.parametrize("normalize", [True, False])
@@ -132,13 +113,13 @@ def test_single_workload_gradients_match_central_differences(cuda, normalize):
     host = _host("single")[0]
     Gn = _np(G).astype(np.float64)
     gpu = [_np(t.grad) for t in (m, q, c, o)]
-    picks = _sample_gaussians(host, H, W, 64, seed=11)
+    picks = sample_gaussians(host, H, W, 64, seed=11)
     fd = {k: [] for k in GRAD_NAMES}
     an = {k: [] for k in GRAD_NAMES}
     got = {k: [] for k in GRAD_NAMES}
     steps = (1e-5, 1e-6, 1e-5, 1e-5)  # means, conics (values ~1e-2), colors, opacities
     for i in picks:
-        analytic, (k, Gw, h, w, mw, qw, cw, rw, ow, rects) = _local_gradient(host, H, W, Gn, i,
+        analytic, (k, Gw, h, w, mw, qw, cw, rw, ow, rects) = local_gradient(host, H, W, Gn, i,
                                                                               normalize)
         params = [mw, qw, cw, ow]
 
@@ -192,16 +173,19 @@ def test_full_size_windows_and_gradients_match_fp64(cuda, name):
         y0 = int(rng.integers(0, H - 48))
         ids, mw, qw, cw, rw, ow, rects = window_field(hm, hq, hc, hr, ho, H, W, x0, x0 + 48,
                                                       y0, y0 + 48)
-        ref, _ = exact64.forward(mw, qw, cw, rw, ow, 48, 48, True, 1e-8, False, 3.0, rects=rects,
-                                 float64_params=True)
-        err = scaled_err(img[y0:y0 + 48, x0:x0 + 48], ref)
-        assert err <= 1e-5, (x0, y0, err)
+        wargs = (mw, qw, cw, rw, ow, 48, 48, True, 1e-8, False, 3.0)
+        ref, _ = exact64.forward(*wargs, rects=rects, float64_params=True)
+        sens = exact64.forward_conditioning(*wargs, rects=rects, float64_params=True)
+        win = img[y0:y0 + 48, x0:x0 + 48]
+        err, bar = scaled_err(win, ref), pixel_bar_err(win, ref, sens)
+        print("window", x0, y0, "scaled", err, "per-pixel rel", pixel_rel_err(win, ref), "bar", bar)
+        assert err <= 1e-5 and bar <= 1.0, (x0, y0, err, bar)
     Gn = _np(G).astype(np.float64)
     gpu = [_np(t.grad) for t in (m, q, c, o)]
     an = {k: [] for k in GRAD_NAMES}
     got = {k: [] for k in GRAD_NAMES}
-    for i in _sample_gaussians(host, H, W, 48, seed=13):
-        analytic, _ = _local_gradient(host, H, W, Gn, i, True)
+    for i in sample_gaussians(host, H, W, 48, seed=13):
+        analytic, _ = local_gradient(host, H, W, Gn, i, True)
         for slot, key in enumerate(GRAD_NAMES):
             an[key].append(np.atleast_1d(analytic[slot]))
             got[key].append(np.atleast_1d(gpu[slot][i]))

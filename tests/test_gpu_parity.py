@@ -8,7 +8,8 @@ import torch
 
 from oracle import exact64, tile_index
 from structsplat_amd import render as HR
-from tests.helpers import field_tensors, random_field, rel_err, scaled_err, to_np
+from tests.helpers import (field_tensors, pixel_bar_err, pixel_rel_err, random_field,
+                           scaled_err, to_np)
 
 pytestmark = pytest.mark.gpu
 
@@ -78,13 +79,18 @@ def test_forward_matches_fp64(cuda, normalize, fade, opacity):
     out = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op,
                               normalize=normalize, support_fade=fade)
     torch.cuda.synchronize()
-    ref, _ = exact64.forward(to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op),
-                             H, W, normalize, 1e-8, fade, 3.0)
+    args = (to_np(means), to_np(conics), to_np(colors), to_np(radii), to_np(op), H, W,
+            normalize, 1e-8, fade, 3.0)
+    ref, _ = exact64.forward(*args)
+    sens = exact64.forward_conditioning(*args)
     got = to_np(out)
     assert np.isfinite(got).all()
     err = scaled_err(got, ref)
-    print("scaled err", err, "elementwise rel err (floor 1e-3)", rel_err(got, ref, 1e-3))
+    bar = pixel_bar_err(got, ref, sens)
+    print("scaled err", err, "per-pixel rel err (floor 1e-6)", pixel_rel_err(got, ref),
+          "per-pixel bar", bar)
     assert err <= 1e-5
+    assert bar <= 1.0
 
 
 @pytest.mark.parametrize("normalize", [True, False])
@@ -184,9 +190,11 @@ def test_tile32_long_lists(cuda):
          "colors": rng.random((N, 3)).astype(np.float32),
          "opacity": rng.uniform(0.1, 1.0, N).astype(np.float32)}
     means, conics, colors, radii, op = field_tensors(f, cuda)
-    out16 = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op, tile_size=16)
+    out16 = HR.render_hip_exact(means, conics, colors, radii, H, W, opacities=op, tiled=True,
+                                tile_size=16)
     x = means.clone().requires_grad_(True)
-    out32 = HR.render_hip_exact(x, conics, colors, radii, H, W, opacities=op, tile_size=32)
+    out32 = HR.render_hip_exact(x, conics, colors, radii, H, W, opacities=op, tiled=True,
+                                tile_size=32)
     g = torch.rand(H, W, 3, device=cuda, generator=torch.Generator(device=cuda).manual_seed(9))
     (out32 * g).sum().backward()
     torch.cuda.synchronize()

@@ -341,3 +341,53 @@ def test_affine_colors_require_scales_and_rotations_and_clamp_scales():
     clamped = R.render(means, conics, colors, radii, 12, 12, color_grads=G,
                        scales=torch.full((5, 2), 1e-6), rotations=rot)
     assert torch.equal(tiny, clamped)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_gaussian_activity_axis_aligned_closed_form(seed):
+    """For b = 0 the weight exp(-(a dx^2 + c dy^2)/2) factors, so a Gaussian's activity over its
+    clipped rectangle is the product of two 1-D sums: exp(-q) instead of exp(-q/2), a transposed
+    conic entry, or a wrong rectangle row or column fails it (render.py:417-442)."""
+    rng = np.random.default_rng(seed)
+    N, H, W = 40, 37, 29
+    means = np.stack([rng.uniform(-4, W + 4, N), rng.uniform(-4, H + 4, N)], 1)
+    a = 1.0 / rng.uniform(0.5, 6.0, N) ** 2
+    c = 1.0 / rng.uniform(0.5, 6.0, N) ** 2
+    conics = np.stack([a, np.zeros(N), c], 1)
+    radii = np.stack([np.ceil(3.0 / np.sqrt(a)), np.ceil(3.0 / np.sqrt(c))], 1).astype(np.int64)
+    got = R.gaussian_activity(torch.tensor(means), torch.tensor(conics), torch.tensor(radii),
+                              H, W).numpy()
+    for i in range(N):
+        ix, iy = int(np.round(means[i, 0])), int(np.round(means[i, 1]))
+        xs = np.arange(max(ix - radii[i, 0], 0), min(ix + radii[i, 0], W - 1) + 1)
+        ys = np.arange(max(iy - radii[i, 1], 0), min(iy + radii[i, 1], H - 1) + 1)
+        sx = np.exp(-0.5 * a[i] * (xs - means[i, 0]) ** 2).sum()
+        sy = np.exp(-0.5 * c[i] * (ys - means[i, 1]) ** 2).sum()
+        assert got[i] == pytest.approx(sx * sy, rel=1e-12, abs=1e-300), i
+
+
+def test_gaussian_activity_rotated_faded_brute_force():
+    """A rotated conic with fractional and full support fade against a pure-Python double loop
+    over the rectangle (math.exp, max(w - alpha exp(-cutoff^2/2), 0))."""
+    rng = np.random.default_rng(5)
+    N, H, W = 12, 21, 19
+    g = GaussianField.from_numpy(np.stack([rng.uniform(0, W, N), rng.uniform(0, H, N)], 1),
+                                 rng.uniform(0.8, 4.0, (N, 2)), rng.uniform(0, np.pi, N),
+                                 rng.random((N, 3)), dtype=torch.float64)
+    conics, radii = g.conics(), g.radii(3.0)
+    for alpha in (0.0, 0.5, 1.0):
+        got = R.gaussian_activity(g.means, conics, radii, H, W, support_fade=alpha > 0,
+                                  support_fade_alpha=alpha if alpha > 0 else None).numpy()
+        tail = alpha * math.exp(-0.5 * 9.0)
+        for i in range(N):
+            mx, my = float(g.means[i, 0]), float(g.means[i, 1])
+            qa, qb, qc = (float(v) for v in conics[i])
+            rx, ry = int(radii[i, 0]), int(radii[i, 1])
+            ix, iy = round(mx), round(my)
+            total = 0.0
+            for py in range(max(iy - ry, 0), min(iy + ry, H - 1) + 1):
+                for px in range(max(ix - rx, 0), min(ix + rx, W - 1) + 1):
+                    dx, dy = px - mx, py - my
+                    w = math.exp(-0.5 * (qa * dx * dx + 2 * qb * dx * dy + qc * dy * dy))
+                    total += max(w - tail, 0.0) if alpha > 0 else w
+            assert got[i] == pytest.approx(total, rel=1e-12, abs=1e-300), (alpha, i)

@@ -146,3 +146,66 @@ def test_fit_color_basis_and_l2_gradient():
     assert torch.allclose(cg.grad - 0.25, expected, rtol=1e-12, atol=0)
     assert F.color_grad_l2_term(aff, cfg) == pytest.approx(0.3 * float((cg * cg).mean()),
                                                           rel=1e-12)
+
+
+def test_exact_tile_size_any_edge_up_to_1024_pixels():
+    """render_cuda_exact accepts any tile edge with tile_size^2 <= 1024 (render_ext.cpp:162-163);
+    the kernels run 16 and 32 as asked and every other edge at 16 (the image does not depend on
+    the tiling)."""
+    from structsplat_amd.render import _check_exact_args, render_tile_size
+    for ts in range(1, 33):
+        expect = ts if ts in (16, 32) else 16
+        assert render_tile_size(ts) == expect
+        assert _check_exact_args(8, 8, True, ts, "baseline", "cuda") == expect
+        # untiled (the reference default) ignores tile_size
+        assert _check_exact_args(8, 8, False, ts, "baseline", "cuda") == 16
+    for ts in (0, -4, 33, 64):
+        with pytest.raises(ValueError, match=r"tile_size must be positive with tile_size\^2"):
+            _check_exact_args(8, 8, True, ts, "baseline", "cuda")
+
+
+def test_exact_block_reduce_is_untiled_only():
+    """cuda_render.py:283-284: a tiled render with the block_reduce backward is a ValueError;
+    untiled it is accepted, and unknown variants or backends are ValueErrors (:278-288)."""
+    from structsplat_amd.render import _check_exact_args
+    with pytest.raises(ValueError, match="block_reduce backward is experimental and untiled-only"):
+        _check_exact_args(8, 8, True, 16, "block_reduce", "cuda")
+    assert _check_exact_args(8, 8, False, 16, "block_reduce", "cuda") == 16
+    with pytest.raises(ValueError, match="backward_variant must be baseline or block_reduce"):
+        _check_exact_args(8, 8, False, 16, "atomic", "cuda")
+    with pytest.raises(ValueError, match="tile_index_backend must be"):
+        _check_exact_args(8, 8, True, 16, "baseline", "nope")
+
+
+def test_concat_fields_carries_filter_variance_and_caps():
+    """concat_fields keeps each field's covariance filter variance and scale caps with its rows;
+    fields without them get the neutral values (variance 0, cap +inf)."""
+    import math as _m
+
+    import torch as _t
+
+    from structsplat_amd.batch import concat_fields
+
+    def f(n, fv=None, cap=None):
+        g = GaussianField.from_numpy(np.zeros((n, 2)), np.ones((n, 2)), np.zeros(n),
+                                     np.zeros((n, 3)))
+        g.filter_variance = fv
+        g.scale_max = cap
+        return g
+
+    a = f(2, fv=_t.tensor([0.5, 0.25]))
+    b = f(3, cap=_t.full((3, 2), 2.0))
+    c = f(1)
+    big = concat_fields([a, b, c])
+    assert big.filter_variance.tolist() == [0.5, 0.25, 0.0, 0.0, 0.0, 0.0]
+    assert big.scale_max.shape == (6, 2)
+    assert big.scale_max[2:5].eq(2.0).all() and _t.isinf(big.scale_max[[0, 1, 5]]).all()
+    assert concat_fields([c, f(2)]).filter_variance is None
+    assert concat_fields([c, f(2)]).scale_max is None
+    # A neutral cap leaves the clamp's upper bound ln max(H, W).
+    from structsplat_amd.fit import log_scale_upper
+    hi = log_scale_upper(big, 40, 30)
+    assert _t.is_tensor(hi) and hi.shape == (12,)
+    assert hi[0].item() == _t.tensor(_m.log(40), dtype=_t.float32).item()
+    assert hi[4].item() == _t.log(_t.tensor(2.0)).item()
+    assert log_scale_upper(c, 40, 30) == _m.log(40)

@@ -369,6 +369,11 @@ def _cpu_model() -> str:
     return platform.processor()
 
 
+def _progress(msg: str) -> None:
+    """One line on stderr when a phase of the run ends (the JSON line goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -564,15 +569,20 @@ def main() -> int:
             "traffic": measured_traffic(args.workload, f"k_backward<{ts}>"),
         },
     }
+    _progress(f"fwd+bwd {ms_per_step:.4f} ms per step")
     if args.bands > 0:
         line["band_shard"] = band_shard(means, conics, colors, radii, op, grad_out, H, W, ts,
                                         args.steps, args.warmup, args.bands)
+        _progress("band shares timed")
     if args.batched_steps > 0:
         line["batched"] = batched_fit(args.batched_steps)
+        _progress(f"batched fit: {line['batched']['speedup']}x the plain loop")
     if args.fit_steps > 0:
         line["fit"] = fit_loop(args.fit_steps)
+        _progress(f"fit loop: {line['fit']['graph_ms_per_step']} ms per graphed step")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, H, W, b["step"])
+        _progress("CPU baseline timed")
     print(json.dumps(line), flush=True)
     return 0
 

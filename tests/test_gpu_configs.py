@@ -70,6 +70,7 @@ def test_cpu_reference_case_forward_and_l1_gradients(cuda):
     bar = pixel_bar_err(got, ref, sens)
     print("forward scaled err", err, "per-pixel rel err (floor 1e-6)", prel, "per-pixel bar", bar)
     assert err <= 1e-5
+    assert pixel_rel_err(got, ref) <= 1e-5  # per pixel, floor 1e-6 of the image scale
     assert bar <= 1.0
 
     # dL/dimg of the L1 mean, taken from the GPU image (sign(0) pixels are measure zero).
@@ -164,7 +165,7 @@ def test_full_fit_graph_equals_eager_and_gradients_match_fp64(cuda):
         win = got_img[y0:y0 + 48, x0:x0 + 48]
         e, pe, bar = scaled_err(win, ref), pixel_rel_err(win, ref), pixel_bar_err(win, ref, sens)
         print("window", x0, y0, "scaled", e, "per-pixel rel", pe, "bar", bar)
-        assert e <= 1e-5 and bar <= 1.0, (x0, y0, e, pe, bar)
+        assert e <= 1e-5 and pe <= 1e-5 and bar <= 1.0, (x0, y0, e, pe, bar)
     Gn = to_np(G).astype(np.float64)
     gpu = [to_np(t.grad) for t in leaves]
     an = {k: [] for k in GRAD_NAMES}

@@ -26,8 +26,8 @@ from oracle import exact64
 from oracle import tile_index as TI
 from structsplat_amd import render as HR
 from structsplat_amd import synth
-from tests.helpers import (field_tensors, local_gradient, pixel_rel_err, sample_gaussians,
-                           scaled_err, window_field)
+from tests.helpers import (field_tensors, local_gradient, pixel_bar_err, pixel_rel_err,
+                           sample_gaussians, scaled_err, window_field)
 
 pytestmark = pytest.mark.gpu
 
@@ -95,6 +95,7 @@ def test_single_workload_whole_image_matches_fp64(cuda, normalize):
     print("scaled err", err, "per-pixel rel err (floor 1e-6)", pixel_rel_err(got, ref),
           "per-pixel bar", bar)
     assert err <= 1e-5
+    assert pixel_rel_err(got, ref) <= 1e-5  # per pixel, floor 1e-6 of the image scale
     assert bar <= 1.0
 
 
@@ -180,6 +181,7 @@ def test_full_size_windows_and_gradients_match_fp64(cuda, name):
         err, bar = scaled_err(win, ref), pixel_bar_err(win, ref, sens)
         print("window", x0, y0, "scaled", err, "per-pixel rel", pixel_rel_err(win, ref), "bar", bar)
         assert err <= 1e-5 and bar <= 1.0, (x0, y0, err, bar)
+        assert pixel_rel_err(win, ref) <= 1e-5, (x0, y0)  # floor 1e-6 of the window's scale
     Gn = _np(G).astype(np.float64)
     gpu = [_np(t.grad) for t in (m, q, c, o)]
     an = {k: [] for k in GRAD_NAMES}

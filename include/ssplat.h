@@ -134,6 +134,23 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * ssplat_forward_tiled without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Its scratch (the packed records and their bookkeeping) is a
+ * static device array of 17 MiB, one copy per GPU, held until the process ends; a call takes at
+ * most 262,144 Gaussians and 65,536 tiles (SSPLAT_ERR_INVALID_ARGUMENT beyond).  It calls no
+ * allocating routine.  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams or threads without ordering between them) race and must not be made.
+ * Results are those of ssplat_forward_tiled, bit for bit.
+ */
+int ssplat_forward_tiled_noalloc(const float* means, const float* conics, const float* colors,
+                                 const int64_t* radii, const float* opacities, int64_t n,
+                                 const int32_t* tile_gids, const int32_t* tile_offsets,
+                                 int64_t num_gids, int height, int width, int tile_size,
+                                 int normalize, double eps, double support_fade_alpha,
+                                 double sigma_cutoff, float* out, float* den, void* stream);
+
+/*
  * Tiled backward.  Given grad_out (H,W,3) = dL/dout, writes dL/dmeans (N,2), dL/dconics (N,3),
  * dL/dcolors (N,3) and, when opacities and grad_opacities are both non-NULL, dL/dopacities (N,).
  * A normalized backward needs the per-pixel adjoints dL/dN_k = G_k/(den+eps) and

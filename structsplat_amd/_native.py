@@ -44,6 +44,10 @@ SIGNATURES = {
     "ssplat_forward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
                                       _c_ptr, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_dbl,
                                       _c_dbl, _c_dbl, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "ssplat_forward_tiled_noalloc": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64,
+                                              _c_ptr, _c_ptr, _c_i64, _c_int, _c_int, _c_int,
+                                              _c_int, _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr,
+                                              _c_ptr]),
     "ssplat_backward_tiled": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                        _c_ptr, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_int, _c_int,
                                        _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_ptr, _c_ptr,

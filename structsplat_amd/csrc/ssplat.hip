@@ -2481,6 +2481,50 @@ int rect_pair_count(const Plan& p, const float* means, const int64_t* radii, int
   return success();
 }
 
+// ------------------------------------------------------------------------------------------
+// Allocation-free forward: the workspace of ssplat_forward_tiled_noalloc is a static device
+// array (one copy per GPU, held until the process ends) instead of a caller's buffer.
+// ------------------------------------------------------------------------------------------
+
+constexpr int64_t kNoAllocMaxGaussians = int64_t(1) << 18;  // 262,144 packed records
+constexpr int64_t kNoAllocMaxTiles = int64_t(1) << 16;      // e.g. 4096 x 4096 at 16 px
+// The forward's layout at both maxima (records 16 MiB plus the tile counters and the slot
+// bookkeeping), rounded up.
+constexpr size_t kNoAllocScratchBytes = size_t(17) << 20;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per device with that device
+// current.  The array's host-side name is a stand-in, not a device address: only this pointer
+// may be handed to kernels.
+int scratch_address(void** out) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    return set_error(SSPLAT_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    return set_error(SSPLAT_ERR_UNSUPPORTED, "device %d is beyond the %d devices supported", dev,
+                     kMaxDevices);
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      return set_error(SSPLAT_ERR_HIP, "hipGetSymbolAddress(scratch): %s",
+                       hipGetErrorString(e));
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return SSPLAT_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -2593,6 +2637,33 @@ int ssplat_forward_tiled(const float* means, const float* conics, const float* c
   return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
                        num_gids, normalize, eps, support_fade_alpha, sigma_cutoff, out, den,
                        workspace, workspace_bytes, stream);
+}
+
+int ssplat_forward_tiled_noalloc(const float* means, const float* conics, const float* colors,
+                                 const int64_t* radii, const float* opacities, int64_t n,
+                                 const int32_t* tile_gids, const int32_t* tile_offsets,
+                                 int64_t num_gids, int height, int width, int tile_size,
+                                 int normalize, double eps, double support_fade_alpha,
+                                 double sigma_cutoff, float* out, float* den, void* stream) {
+  Plan p;
+  int rc = make_single_plan(n, height, width, tile_size, &p);
+  if (rc) return rc;
+  const size_t need = layout(nullptr, p.n, p.num_tiles).bytes;
+  if (p.n > kNoAllocMaxGaussians || p.num_tiles > kNoAllocMaxTiles ||
+      need > kNoAllocScratchBytes) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
+                     "ssplat_forward_tiled_noalloc: %lld Gaussians and %d tiles need %zu bytes of "
+                     "scratch; the static scratch holds %zu (at most %lld Gaussians and %lld "
+                     "tiles)",
+                     static_cast<long long>(p.n), p.num_tiles, need, kNoAllocScratchBytes,
+                     static_cast<long long>(kNoAllocMaxGaussians),
+                     static_cast<long long>(kNoAllocMaxTiles));
+  }
+  void* ws = nullptr;
+  if ((rc = scratch_address(&ws))) return rc;
+  return forward_tiled(p, means, conics, colors, radii, opacities, tile_gids, tile_offsets,
+                       num_gids, normalize, eps, support_fade_alpha, sigma_cutoff, out, den, ws,
+                       kNoAllocScratchBytes, stream);
 }
 
 int ssplat_backward_tiled(const float* grad_out, const float* means, const float* conics,

@@ -205,7 +205,9 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
 
     from structsplat_amd import batch as B
     from structsplat_amd import fit as F
+    from structsplat_amd import render as HR
     from structsplat_amd import synth
+    from structsplat_amd import tune
     from structsplat_amd.gaussians import GaussianField
 
     dev = torch.device("cuda:0")
@@ -228,9 +230,16 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
         "batched_steps_per_s": round(steps / batched.seconds, 3),
         "plain_steps_per_s": round(steps / plain.seconds, 3),
         "speedup": round(plain.seconds / batched.seconds, 3),
+        "batched_replay_ms_per_step": round(batched.step_seconds * 1e3
+                                            / max(batched.timed_steps, 1), 4),
+        "plain_ms_per_step": round(plain.step_seconds * 1e3
+                                   / max(plain.timed_steps // num_fields, 1), 4),
         "psnr_max_abs_diff_db": round(diff, 5),
         "mean_psnr_db": round(float(np.mean(batched.psnr)), 3),
-        "tile_size": batched.tile_size,
+        "tile_size": HR.tile_edge(batched.tile_size),
+        "schedule": "persistent" if batched.tile_size & HR.TILE_PERSISTENT else "grid",
+        "autotune_fwd_bwd_ms": ({tune.describe(k): round(v, 4) for k, v in batched.tune_ms.items()}
+                                if batched.tune_ms else None),
         "note": "a step advances every field by one Adam step",
     }
 
@@ -263,11 +272,15 @@ def fit_loop(steps: int, seed: int = 0) -> dict:
         "graph_ms_per_step": round(graphed["seconds"] * 1e3 / steps, 4),
         "eager_steps_per_s": round(steps / eager["seconds"], 3),
         "eager_ms_per_step": round(eager["seconds"] * 1e3 / steps, 4),
+        "graph_replay_ms_per_step": round(graphed["step_seconds"] * 1e3
+                                          / max(graphed["timed_steps"], 1), 4),
+        "eager_step_ms": round(eager["step_seconds"] * 1e3 / max(eager["timed_steps"], 1), 4),
         "graph_psnr_db": round(graphed["psnr"], 4),
         "eager_psnr_db": round(eager["psnr"], 4),
         "tile_size": graphed["tile_size"],
         "regrows": graphed["regrows"],
-        "note": "loop time includes the per-window capacity checks",
+        "note": ("steps_per_s and ms_per_step: the whole loop (capture, windows' snapshots and "
+                 "readbacks); graph_replay_ms_per_step / eager_step_ms: the steps alone"),
     }
 
 
@@ -322,7 +335,7 @@ def band_shard(means, conics, colors, radii, op, grad_out, H: int, W: int, ts: i
             raise RuntimeError("pair capacity exceeded in a band")
         per_band.append({"rows": [y0, y1], "gaussians": n, "tile_pairs": P,
                          "ms_per_step": round(ms, 4)})
-    return {"bands": bands, "tile_size": ts,
+    return {"bands": bands, "tile_size": HR.tile_edge(ts),
             "max_band_ms_per_step": max(b["ms_per_step"] for b in per_band),
             "member_gaussians_over_n": round(sum(b["gaussians"] for b in per_band)
                                              / means.shape[0], 4),
@@ -381,8 +394,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32"],
-                    help="render tile edge; auto = the autotuner's choice on this field")
+    ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32", "272", "288"],
+                    help="render tile edge (272 / 288: 16 / 32 with the persistent schedule); "
+                         "auto = the autotuner's choice on this field")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="Adam steps per field of the batched-vs-plain fit (0 skips it)")
     ap.add_argument("--fit-steps", type=int, default=1000,
@@ -422,6 +436,7 @@ def main() -> int:
                                         normalize=True, eps=1e-8, support_fade=False,
                                         sigma_cutoff=3.0, recompute_normalizer=True)
     ts = best if args.tile_size == "auto" else int(args.tile_size)
+    edge = HR.tile_edge(ts)  # the tile edge of the chosen configuration (ts may add the schedule)
     T = HR.num_tiles(H, W, ts)
     T16 = HR.num_tiles(H, W, 16)
     gen = torch.Generator(device=dev).manual_seed(0)
@@ -481,6 +496,28 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
+
+    # The same step captured once as a HIP graph and replayed (how the fit loop runs it): the
+    # kernels are identical, the launch gaps between them shrink.
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        step(False)  # warm the side stream once before capture
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step(False)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    if int(status.item()) != 0:
+        raise RuntimeError("pair capacity exceeded in the graph replays")
 
     # Per-phase timing with events on the launch stream (a separate pass over K steps).
     for _ in range(args.steps):
@@ -546,12 +583,18 @@ def main() -> int:
         "config": {
             "workload": desc,
             "H": H, "W": W, "n_gaussians": N,
-            "tile_size": ts, "tile_choice": args.tile_size,
-            "autotune_fwd_bwd_ms": {str(k): round(v, 4) for k, v in tile_ms.items()},
+            "tile_size": edge, "schedule": ("persistent" if ts & HR.TILE_PERSISTENT
+                                            else "grid"),
+            "tile_choice": args.tile_size,
+            "autotune_fwd_bwd_ms": {tune.describe(k): round(v, 4) for k, v in tile_ms.items()},
             "tile_pairs": P, "tiles": T, "tile_pairs_at_16": P16,
             "phase_ms": {k: round(v, 4) for k, v in phase.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
             "normalizer": "recomputed in the backward (den not stored)",
+            "graph_replay_ms_per_step": round(graph_ms, 4),
+            "timing": ("ms_per_step and value: K eager steps (host launches) between two "
+                       "synchronizations; graph_replay_ms_per_step: the same step captured "
+                       "once as a HIP graph and replayed K times"),
             "stored_normalizer_variant": {
                 "ms_per_step": round(saved_ms, 4),
                 "kernel_ms": {k: round(v, 4) for k, v in saved_kernel_ms.items()},
@@ -561,12 +604,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_backward<{ts}>",
+            "kernel": f"k_backward<{edge}>",
             "achieved": round(bwd_gbs, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(args.workload, f"k_backward<{ts}>"),
+            "traffic": measured_traffic(args.workload, f"k_backward<{edge}>"),
         },
     }
     _progress(f"fwd+bwd {ms_per_step:.4f} ms per step")

@@ -33,7 +33,17 @@
 extern "C" {
 #endif
 
-#define SSPLAT_ABI_VERSION 3  /* 3: affine-color and after-forward entry points */
+#define SSPLAT_ABI_VERSION 4  /* 4: persistent render schedule (SSPLAT_TILE_PERSISTENT) */
+
+/*
+ * Render schedule flag, OR-ed into the tile_size argument of the render entry points (the plain
+ * and batched forwards and backwards, ssplat_workspace_bytes, ssplat_batch_plan): the forward and
+ * backward kernels then run as one persistent launch (as many workgroups as the device holds at
+ * once, each taking tiles from per-XCD work queues in the workspace, then from the other XCDs'
+ * queues) instead of one workgroup per tile.  Results are bit-identical to the grid launch's.
+ * The binning entry points take the plain tile edge.  No reference counterpart.
+ */
+#define SSPLAT_TILE_PERSISTENT 0x100
 
 enum ssplat_status {
   SSPLAT_OK = 0,

@@ -30,7 +30,7 @@ from .gaussians import GaussianField
 from .loss import batch_l1_ssim_loss
 from .metrics import psnr_tensor
 from .tune import tune_field, tune_tile_size
-from .render import _fade_alpha, _ptr, _stream, grad_slots
+from .render import _fade_alpha, _ptr, _stream, grad_slots, tile_edge
 
 
 def _plan(sizes, counts, tile_size):
@@ -57,8 +57,9 @@ def _device_table(table, device):
 
 class Batch:
     """Planned layout of a batch of fields of sizes `sizes` [(H, W)] with `counts` Gaussians,
-    rendered with `tile_size`-pixel tiles (16 or 32).  The loss always tiles by 16 pixels
-    (``loss_fields_dev`` / ``loss_total_tiles``)."""
+    rendered with `tile_size`-pixel tiles (16 or 32, optionally with the TILE_PERSISTENT
+    schedule flag: every field's tiles in one persistent launch).  The loss always tiles by 16
+    pixels (``loss_fields_dev`` / ``loss_total_tiles``)."""
 
     def __init__(self, sizes, counts, device, tile_size: int = 16):
         nf = len(sizes)
@@ -66,7 +67,8 @@ class Batch:
             raise ValueError("a batch needs one Gaussian count per field size")
         table, g, t, px, wsb = _plan(sizes, counts, tile_size)
         self.device = torch.device(device)
-        self.tile_size = int(tile_size)
+        self.tile_config = int(tile_size)       # edge and schedule flag, passed to the kernels
+        self.tile_size = tile_edge(tile_size)   # the tile edge
         self.num_fields = nf
         self.table = table
         self.total_gaussians = g
@@ -89,7 +91,7 @@ class Batch:
 
     def _head(self):
         return (_ptr(self.fields_dev), self.num_fields, self.total_gaussians, self.total_tiles,
-                self.tile_size)
+                self.tile_config)
 
     def images(self, flat: torch.Tensor):
         """Per-field (H, W, 3) views of a concatenated (total_pixels, 3) image buffer."""
@@ -251,6 +253,9 @@ class BatchResult:
     seconds: float
     steps: int
     tile_size: int | None = None  # the batched fit's render tile (per field for the plain loop)
+    step_seconds: float = 0.0     # the timed steps alone (graph replays or eager steps)
+    timed_steps: int = 0
+    tune_ms: dict | None = None   # the autotuner's fwd+bwd time per tile configuration
 
 
 def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig,
@@ -264,13 +269,16 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     counts = [f.n for f in fields]
     big = concat_fields(fields).trainable()
 
+    tuned = {}
+
     def tune():
         conics, radii, op = big.produce(cfg.sigma_cutoff)
-        return tune_tile_size(sizes, counts, big.means.detach(), conics.detach(),
-                              big.colors.detach(), radii, None if op is None else op.detach(),
-                              normalize=cfg.normalize, eps=cfg.normalization_eps,
-                              support_fade=cfg.support_fade, sigma_cutoff=cfg.sigma_cutoff,
-                              recompute_normalizer=cfg.recompute_normalizer)[0]
+        best, tuned["ms"] = tune_tile_size(
+            sizes, counts, big.means.detach(), conics.detach(), big.colors.detach(), radii,
+            None if op is None else op.detach(), normalize=cfg.normalize,
+            eps=cfg.normalization_eps, support_fade=cfg.support_fade,
+            sigma_cutoff=cfg.sigma_cutoff, recompute_normalizer=cfg.recompute_normalizer)
+        return best
 
     batch = Batch(sizes, counts, dev, tile_size=F.resolve_tile_size(cfg, tune))
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
@@ -307,6 +315,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
     t0 = time.perf_counter()
     done = 0
     g = None
+    replay = {"steps": 0, "seconds": 0.0}
     while done < cfg.iters:
         n = min(cfg.window, cfg.iters - done)
         snap = F._Snapshot(big, opt)
@@ -314,8 +323,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
         if graph and done >= 2:
             if g is None:
                 g = F._capture(step)
-            for _ in range(n):
-                g.replay()
+            F._timed_steps(replay, n, g.replay, dev)
         elif graph:
             n = min(n, 2 - done)
             side = torch.cuda.Stream(device=dev)
@@ -325,16 +333,15 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                     step()
             torch.cuda.current_stream(dev).wait_stream(side)
         else:
-            for _ in range(n):
-                step()
-        if int(plan.status.item()) != 0:
+            F._timed_steps(replay, n, step, dev)
+        status, p = F.window_readback(plan)
+        if status != 0:
             snap.restore(big, opt)
-            plan = BatchPlan(batch, max(2 * plan.capacity, F._capacity(count(), cfg)),
+            plan = BatchPlan(batch, max(2 * plan.capacity, F._capacity(p, cfg)),
                              cfg.recompute_normalizer)
             g = None
             continue
         done += n
-        p = count()
         if p * 1.15 > plan.capacity:
             plan = BatchPlan(batch, F._headroom(p, cfg), cfg.recompute_normalizer)
             g = None
@@ -356,7 +363,8 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
                 src = getattr(big, name)
                 if src is not None:
                     setattr(f, name, src.detach()[sl].clone())
-    return BatchResult(psnrs, seconds, cfg.iters, batch.tile_size)
+    return BatchResult(psnrs, seconds, cfg.iters, batch.tile_config, replay["seconds"],
+                       replay["steps"], tuned.get("ms"))
 
 
 def fit_each(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.FitConfig
@@ -369,8 +377,12 @@ def fit_each(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.Fi
         for f, t in zip(fields, targets)]
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    psnrs = []
+    psnrs, step_seconds, timed = [], 0.0, 0
     for f, t, c in zip(fields, targets, cfgs):
-        psnrs.append(F.fit(f, t, c, graph=False)["psnr"])
+        r = F.fit(f, t, c, graph=False)
+        psnrs.append(r["psnr"])
+        step_seconds += r["step_seconds"]
+        timed += r["timed_steps"]
     torch.cuda.synchronize(dev)
-    return BatchResult(psnrs, time.perf_counter() - t0, cfg.iters * len(fields))
+    return BatchResult(psnrs, time.perf_counter() - t0, cfg.iters * len(fields), None,
+                       step_seconds, timed)

@@ -138,6 +138,11 @@ constexpr int kSlotWords = 10;            // 9 partial gradients + the call's ta
 constexpr int kSlotWordsAffine = 20;
 constexpr size_t kSlotHeaderBytes = 256;  // slot buffer header: the call counter (uint32)
 constexpr int kLaneSumMax = 64;           // Gaussians with up to this many slots: one lane each
+// Work queues of a persistent render launch: up to kMaxQueuesPerXcd tile ranges per XCD.
+constexpr int kMaxQueuesPerXcd = 8;
+constexpr int kQueues = kXcds * kMaxQueuesPerXcd;
+static_assert(kQueues <= 64, "one wave probes every queue");
+constexpr int kWorkgroupsPerQueue = 64;  // resident workgroups sharing one queue (see TileQueue)
 constexpr int kFieldWaves = 4;            // waves per 256-thread workgroup of per-Gaussian sweeps
 // The render kernels run one workgroup per tile, one thread per pixel (tile edge 16: 256 threads,
 // 4 waves; tile edge 32: 1024 threads, 16 waves), one wave per 8x8 quadrant.
@@ -551,8 +556,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(const int32_t* __r
                                                               int64_t items,
                                                               int32_t* __restrict__ offsets,
                                                               int32_t* __restrict__ status,
-                                                              uint32_t* __restrict__ bump) {
+                                                              uint32_t* __restrict__ bump,
+                                                              int32_t* __restrict__ queue) {
   if (bump != nullptr && threadIdx.x == 0) *bump += 1u;  // a counter the caller advances
+  // The next launch's persistent work-queue counters, zeroed for it.
+  if (queue != nullptr && threadIdx.x < kQueues) queue[threadIdx.x] = 0;
   constexpr int kPer = kSingleScanPerThread;
   constexpr int kChunk = kScanThreads * kPer;
   static_assert(kPer % 4 == 0, "counters are loaded four at a time");
@@ -664,8 +672,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int32_t* _
 // Second level: exclusive scan of the block sums, in place (one block, <= kMaxScanBlocks sums).
 __global__ __launch_bounds__(kScanThreads) void k_scan_top(int64_t* __restrict__ block_sums,
                                                            int blocks,
-                                                           uint32_t* __restrict__ bump) {
+                                                           uint32_t* __restrict__ bump,
+                                                           int32_t* __restrict__ queue) {
   if (bump != nullptr && threadIdx.x == 0) *bump += 1u;  // a counter the caller advances
+  if (queue != nullptr && threadIdx.x < kQueues) queue[threadIdx.x] = 0;
   const int base = threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t s = 0;
@@ -994,13 +1004,15 @@ __global__ __launch_bounds__(256) void k_prepare(
     const float* __restrict__ means, const float* __restrict__ conics,
     const float* __restrict__ colors, const int64_t* __restrict__ radii,
     const float* __restrict__ opacities, int n, FieldTable fields, GaussRec* __restrict__ recs,
-    SlotPrep sp) {
+    SlotPrep sp, int32_t* __restrict__ queue) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   if (i == 0) {
     sp.rect_wave[sp.num_waves] = 0;
     *sp.rect_status = SSPLAT_OK;
   }
+  // The forward's persistent work-queue counters, zeroed for it.
+  if (i < kQueues) queue[i] = 0;
   // Every input is loaded up front (one memory round trip per thread).
   const bool live = i < n;
   const int j = live ? i : 0;
@@ -1242,13 +1254,109 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   acc = make_float4(nr, ng, nb, d);
 }
 
-template <int kT, bool kFade, bool kNormalize, bool kAffine = false>
-__global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __restrict__ out,
-                                                     float* __restrict__ den) {
+// Work queue of a persistent render launch: the tiles split into `queues` contiguous ranges,
+// queues / kXcds consecutive ranges per XCD, each with an atomic counter in the workspace that the
+// launch's preceding kernel zeroes; a claim takes one tile.  A workgroup claims from its home
+// range (its XCD from the round-robin dispatch, a range of that XCD from its index) and, once that
+// is exhausted, from the next range with tiles left.  About kWorkgroupsPerQueue
+// workgroups share a range: they work on neighbouring tiles at any moment, which share most of
+// their records (one XCD's L2 serves them), while the claims on any one counter (atomics on one
+// address serialize) stay few.  Stealing balances the XCDs, whose ranges can differ in work
+// (a batch's fields are concatenated by size).
+struct TileQueue {
+  int32_t* counters;  // kQueues counters, zero at launch
+  int num_tiles;
+  int queues;         // ranges in use: kXcds times 1 .. kMaxQueuesPerXcd
+};
+
+__device__ __forceinline__ int queue_bound(const TileQueue& q, int r) {
+  return static_cast<int>(int64_t(q.num_tiles) * r / q.queues);
+}
+
+// Thread 0's claim state in a persistent workgroup: the range it drains, the tile of its current
+// claim, and the counter value of the following claim in that range, fetched when the current
+// one is taken so its round trip overlaps the tile's work.
+struct TileClaim {
+  int range;
+  int pending;
+};
+
+__device__ __forceinline__ TileClaim first_claim(const TileQueue& q) {
+  const int b = static_cast<int>(blockIdx.x);
+  const int per_xcd = q.queues / kXcds;
+  TileClaim c;
+  c.range = (b & (kXcds - 1)) * per_xcd + (b / kXcds) % per_xcd;
+  c.pending = atomicAdd(&q.counters[c.range], 1);
+  return c;
+}
+
+// Tile of claim value v in range r, or -1 past the range.
+__device__ __forceinline__ int claimed_tile(const TileQueue& q, int r, int v) {
+  const int lo = queue_bound(q, r), hi = queue_bound(q, r + 1);
+  return v < hi - lo ? lo + v : -1;
+}
+
+// The next tile of the workgroup, broadcast through LDS (-1 once every range is exhausted).
+// Thread 0 takes its pending claim.  When its range is exhausted, the first wave reads every
+// range's counter at once (device-scope atomic loads: no cache serves a stale value, and no
+// read-modify-write competes with the claims) and thread 0 claims
+// in the next range (in range order) with tiles left, repeating only if other workgroups took
+// them first.
+// Starts with a barrier, so the previous tile's threads are done with LDS and the broadcast word.
+__device__ __forceinline__ int claim_tile(const TileQueue& q, TileClaim& c) {
+  __shared__ int s_tile;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = claimed_tile(q, c.range, c.pending);
+    if (t >= 0) c.pending = atomicAdd(&q.counters[c.range], 1);  // the claim after this one
+    s_tile = t;
+  }
+  __syncthreads();
+  int tile = s_tile;
+  while (tile < 0) {  // uniform: every thread reads the same broadcast word
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      int left = 0;
+      if (lane < q.queues) {
+        const int size = queue_bound(q, lane + 1) - queue_bound(q, lane);
+        const int taken =
+            __hip_atomic_load(&q.counters[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        left = taken < size ? size - taken : 0;
+      }
+      // The first range with tiles left after the current one, in range order (workgroups
+      // that run dry at different ranges spread over different victims), by a wave minimum.
+      const int cur = __shfl(c.range, 0, 64);  // thread 0's range (the claim state is its own)
+      const int dist = (lane - cur + q.queues) % q.queues;
+      int best = left > 0 ? (dist << 6) | lane : INT32_MAX;
+      for (int d = 32; d > 0; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+      if (lane == 0) {
+        int t = -2;  // -2: every range exhausted
+        if (best != INT32_MAX) {
+          const int r = best & 63;
+          t = claimed_tile(q, r, atomicAdd(&q.counters[r], 1));  // -1: raced, probe again
+          if (t >= 0) {
+            c.range = r;
+            c.pending = atomicAdd(&q.counters[r], 1);
+          }
+        }
+        s_tile = t;
+      }
+    }
+    __syncthreads();
+    tile = s_tile;
+    __syncthreads();  // every thread has read the word before it can change again
+    if (tile == -2) return -1;
+  }
+  return tile;
+}
+
+// One tile of the forward: the body of k_forward and of its persistent form.
+template <int kT, bool kFade, bool kNormalize, bool kAffine>
+__device__ __forceinline__ void forward_tile(const RenderArgs& a, int tile, float* __restrict__ out,
+                                             float* __restrict__ den) {
   constexpr int kQ = kT / 8;  // quadrants per tile row
   __shared__ StageLds<kAffine ? 4 + kAffineChunks : 3> st;
   __shared__ uint32_t s_mask[kQ * kQ][2 * kStage];  // per-wave quadrant masks
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
   const int tile_x = local % f.tiles_x;
@@ -1279,6 +1387,28 @@ __global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __rest
   }
 }
 
+// Grid launch: one workgroup per tile, XCD-aware tile order.
+template <int kT, bool kFade, bool kNormalize, bool kAffine = false>
+__global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __restrict__ out,
+                                                     float* __restrict__ den) {
+  forward_tile<kT, kFade, kNormalize, kAffine>(a, xcd_remap(blockIdx.x, gridDim.x), out, den);
+}
+
+// Persistent launch: as many workgroups as fit on the device, each taking tiles from the queue.
+// Held to the grid kernel's register budget (eight waves per SIMD): the tile loop otherwise lets
+// the compiler keep values live across tiles, which costs occupancy.
+template <int kT, bool kFade, bool kNormalize>
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_forward_persistent(RenderArgs a, TileQueue q,
+                                                                float* __restrict__ out,
+                                                                float* __restrict__ den) {
+  TileClaim c{};
+  if (threadIdx.x == 0) c = first_claim(q);
+  for (int tile = claim_tile(q, c); tile >= 0; tile = claim_tile(q, c)) {
+    forward_tile<kT, kFade, kNormalize, false>(a, tile, out, den);
+  }
+}
+
 struct GradArgs {
   const float* grad_out;
   const float* den;  // forward coverage (read when normalizing)
@@ -1294,13 +1424,9 @@ constexpr int kNormSaved = 0;      // both read from the forward's saved outputs
 constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-only sweep
 constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
 
-template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine = false>
-// Eight waves per SIMD (at most 64 VGPRs): with 16-pixel tiles that is eight resident
-// workgroups per CU, whose different phases (list sweeps, reductions) overlap.  The affine
-// sweep's 21 accumulators need more registers: four waves per SIMD (up to 128 VGPRs).
-__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
-    kAffine ? 4 : 8, kAffine ? 4 : 8))) void k_backward(
-    RenderArgs a, GradArgs g) {
+// One tile of the backward: the body of k_backward and of its persistent form.
+template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine>
+__device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArgs& g, int tile) {
   constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
   constexpr int kQ = kT / 8;  // quadrants per tile row
   constexpr int kWaves = kQ * kQ;
@@ -1313,7 +1439,6 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
   // quadrant masks (two uint32 per staged record and wave).
   __shared__ float4 s_scratch[kRecompute && kMaskF4 > kT * kT ? kMaskF4 : kT * kT];
   float4* s_adj = s_scratch;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
   const int tile_x = local % f.tiles_x;
@@ -1588,6 +1713,25 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
       }
     }
     __syncthreads();
+  }
+}
+
+// Eight waves per SIMD (at most 64 VGPRs): with 16-pixel tiles that is eight resident
+// workgroups per CU, whose different phases (list sweeps, reductions) overlap.  The affine
+// sweep's 21 accumulators need more registers: four waves per SIMD (up to 128 VGPRs).
+template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine = false>
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
+    kAffine ? 4 : 8, kAffine ? 4 : 8))) void k_backward(RenderArgs a, GradArgs g) {
+  backward_tile<kT, kFade, kNormalize, kNorm, kAffine>(a, g, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int kT, bool kFade, bool kNormalize, int kNorm>
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_backward_persistent(RenderArgs a, GradArgs g, TileQueue q) {
+  TileClaim c{};
+  if (threadIdx.x == 0) c = first_claim(q);
+  for (int tile = claim_tile(q, c); tile >= 0; tile = claim_tile(q, c)) {
+    backward_tile<kT, kFade, kNormalize, kNorm, false>(a, g, tile);
   }
 }
 
@@ -1881,13 +2025,17 @@ __global__ __launch_bounds__(256) void k_activity(const float* __restrict__ mean
 // Geometry of one call: the field table (one by-value field, or a device table) and totals.
 struct Plan {
   FieldTable table;
-  int64_t n;      // Gaussians over all fields
-  int num_tiles;  // tiles over all fields
+  int64_t n;        // Gaussians over all fields
+  int num_tiles;    // tiles over all fields
   int ts;
+  bool persistent;  // render kernels as one persistent launch (SSPLAT_TILE_PERSISTENT)
 };
 
-int check_tile_size(int ts) {
-  if (ts <= 0 || ts * ts > 1024) {
+// Splits a tile_size argument into the tile edge and the schedule flag.
+int check_tile_size(int tile_size, int* ts, bool* persistent) {
+  *persistent = (tile_size & SSPLAT_TILE_PERSISTENT) != 0;
+  *ts = tile_size & ~SSPLAT_TILE_PERSISTENT;
+  if (*ts <= 0 || *ts * *ts > 1024) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
                      "tile_size must be positive with tile_size^2 <= 1024");
   }
@@ -1913,10 +2061,12 @@ int check_totals(int64_t n, int64_t tiles) {
   return SSPLAT_OK;
 }
 
-int make_single_plan(int64_t n, int H, int W, int ts, Plan* p) {
+int make_single_plan(int64_t n, int H, int W, int tile_size, Plan* p) {
   int rc;
+  int ts;
+  bool persistent;
   if (n < 0 || n > INT32_MAX) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
-  if ((rc = check_image(H, W)) || (rc = check_tile_size(ts))) return rc;
+  if ((rc = check_image(H, W)) || (rc = check_tile_size(tile_size, &ts, &persistent))) return rc;
   const int64_t tx = (int64_t(W) + ts - 1) / ts;
   const int64_t ty = (int64_t(H) + ts - 1) / ts;
   if ((rc = check_totals(n, tx * ty))) return rc;
@@ -1933,20 +2083,27 @@ int make_single_plan(int64_t n, int H, int W, int ts, Plan* p) {
   p->n = n;
   p->num_tiles = f.num_tiles;
   p->ts = ts;
+  p->persistent = persistent;
   return SSPLAT_OK;
 }
 
 int make_batch_plan(const ssplat_field* fields, int num_fields, int64_t total_gaussians,
-                    int64_t total_tiles, int ts, Plan* p) {
+                    int64_t total_tiles, int tile_size, Plan* p) {
   int rc;
+  int ts;
+  bool persistent;
   if (fields == nullptr || num_fields <= 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the field table must hold at least one field");
   }
-  if ((rc = check_tile_size(ts)) || (rc = check_totals(total_gaussians, total_tiles))) return rc;
+  if ((rc = check_tile_size(tile_size, &ts, &persistent)) ||
+      (rc = check_totals(total_gaussians, total_tiles))) {
+    return rc;
+  }
   p->table = FieldTable{fields, num_fields, ssplat_field{}};
   p->n = total_gaussians;
   p->num_tiles = static_cast<int>(total_tiles);
   p->ts = ts;
+  p->persistent = persistent;
   return SSPLAT_OK;
 }
 
@@ -1959,6 +2116,8 @@ struct Workspace {
   int32_t* rect_off;     // their exclusive scan: each wave's first slot; [num_waves] = total
   int64_t* rect_bsums;   // first-level block sums of that scan
   int32_t* rect_status;  // int32 overflow of that scan
+  int32_t* queue_fwd;    // kQueues work-queue counters of a persistent forward
+  int32_t* queue_bwd;    // and of a persistent backward
   int num_waves;
   size_t bytes;
 };
@@ -1984,6 +2143,8 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   off = align_up(off + ((waves + 1 + kScanBlock - 1) / kScanBlock) * sizeof(int64_t), 256);
   const size_t rect_status = off;
   off = align_up(off + sizeof(int32_t), 256);
+  const size_t queues = off;
+  off = align_up(off + 2 * kQueues * sizeof(int32_t), 256);
   char* b = static_cast<char*>(base);
   w.recs = reinterpret_cast<GaussRec*>(b + recs);
   w.counts = reinterpret_cast<int32_t*>(b + counts);
@@ -1993,6 +2154,8 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   w.rect_off = reinterpret_cast<int32_t*>(b + rect_off);
   w.rect_bsums = reinterpret_cast<int64_t*>(b + rect_bsums);
   w.rect_status = reinterpret_cast<int32_t*>(b + rect_status);
+  w.queue_fwd = reinterpret_cast<int32_t*>(b + queues);
+  w.queue_bwd = w.queue_fwd + kQueues;
   w.num_waves = static_cast<int>(waves);
   w.bytes = off;
   return w;
@@ -2038,20 +2201,21 @@ int make_bin_args(const float* means, const float* conics, const int64_t* radii,
 // Exclusive int32 scan of counts[0, items) into offsets (saturating; *status flags an overflow):
 // one launch up to kSingleScanItems items, three beyond.
 int run_scan(const int32_t* counts, int64_t items, int32_t* offsets, int64_t* bsums,
-             int32_t* status, hipStream_t stream, uint32_t* bump = nullptr) {
+             int32_t* status, hipStream_t stream, uint32_t* bump = nullptr,
+             int32_t* queue = nullptr) {
   if (items > kMaxScanItems) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "scan of %lld items is out of range",
                      static_cast<long long>(items));
   }
   if (items <= kSingleScanItems) {
-    k_scan_single<<<1, kScanThreads, 0, stream>>>(counts, items, offsets, status, bump);
+    k_scan_single<<<1, kScanThreads, 0, stream>>>(counts, items, offsets, status, bump, queue);
     SSPLAT_LAUNCH_CHECK("k_scan_single");
     return SSPLAT_OK;
   }
   const int blocks = static_cast<int>((items + kScanBlock - 1) / kScanBlock);
   k_scan_partials<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums);
   SSPLAT_LAUNCH_CHECK("k_scan_partials");
-  k_scan_top<<<1, kScanThreads, 0, stream>>>(bsums, blocks, bump);
+  k_scan_top<<<1, kScanThreads, 0, stream>>>(bsums, blocks, bump, queue);
   SSPLAT_LAUNCH_CHECK("k_scan_top");
   k_scan_apply<<<blocks, kScanThreads, 0, stream>>>(counts, items, bsums, offsets, status);
   SSPLAT_LAUNCH_CHECK("k_scan_apply");
@@ -2200,10 +2364,12 @@ int run_prepare(const float* means, const float* conics, const float* colors,
   const SlotPrep sp = make_slot_prep(p, w);
   if (support_fade) {
     k_prepare<false><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
-                                                 static_cast<int>(p.n), p.table, w.recs, sp);
+                                                 static_cast<int>(p.n), p.table, w.recs, sp,
+                                                 w.queue_fwd);
   } else {
     k_prepare<true><<<blocks, 256, 0, stream>>>(means, conics, colors, radii, opacities,
-                                                static_cast<int>(p.n), p.table, w.recs, sp);
+                                                static_cast<int>(p.n), p.table, w.recs, sp,
+                                                w.queue_fwd);
   }
   SSPLAT_LAUNCH_CHECK("k_prepare");
   return SSPLAT_OK;
@@ -2254,6 +2420,84 @@ void launch_backward(bool fade, bool normalize, int norm, int tiles, const Rende
   } else {
     launch_backward_normalized<kT, false, kAffine>(norm, tiles, ra, ga, s);
   }
+}
+
+// Workgroups of a persistent launch: as many as the device holds at once, at most one a tile.
+template <typename Kernel>
+int persistent_grid(Kernel kernel, int block, int tiles, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+  if (e != hipSuccess) {
+    return set_error(SSPLAT_ERR_HIP, "persistent launch sizing: %s", hipGetErrorString(e));
+  }
+  const int64_t resident = int64_t(cus) * (per_cu > 0 ? per_cu : 1);
+  *grid = static_cast<int>(resident < tiles ? resident : tiles);
+  if (*grid < 1) *grid = 1;
+  return SSPLAT_OK;
+}
+
+// Ranges of a persistent launch of `grid` workgroups: about kWorkgroupsPerQueue workgroups a
+// range, 1 .. kMaxQueuesPerXcd ranges per XCD.
+int queue_count(int grid) {
+  int per_xcd = (grid / kXcds + kWorkgroupsPerQueue / 2) / kWorkgroupsPerQueue;
+  per_xcd = per_xcd < 1 ? 1 : (per_xcd > kMaxQueuesPerXcd ? kMaxQueuesPerXcd : per_xcd);
+  return kXcds * per_xcd;
+}
+
+template <int kT, bool kFade, bool kNormalize>
+int launch_forward_persistent_one(TileQueue q, const RenderArgs& ra, float* out, float* den,
+                                  hipStream_t s) {
+  auto kernel = k_forward_persistent<kT, kFade, kNormalize>;
+  int grid = 0;
+  const int rc = persistent_grid(kernel, kT * kT, q.num_tiles, &grid);
+  if (rc) return rc;
+  q.queues = queue_count(grid);
+  kernel<<<grid, kT * kT, 0, s>>>(ra, q, out, den);
+  return SSPLAT_OK;
+}
+
+template <int kT>
+int launch_forward_persistent(bool fade, bool normalize, const TileQueue& q, const RenderArgs& ra,
+                              float* out, float* den, hipStream_t s) {
+  if (fade && normalize) return launch_forward_persistent_one<kT, true, true>(q, ra, out, den, s);
+  if (fade) return launch_forward_persistent_one<kT, true, false>(q, ra, out, den, s);
+  if (normalize) return launch_forward_persistent_one<kT, false, true>(q, ra, out, den, s);
+  return launch_forward_persistent_one<kT, false, false>(q, ra, out, den, s);
+}
+
+template <int kT, bool kFade, bool kNormalize, int kNorm>
+int launch_backward_persistent_one(TileQueue q, const RenderArgs& ra, const GradArgs& ga,
+                                   hipStream_t s) {
+  auto kernel = k_backward_persistent<kT, kFade, kNormalize, kNorm>;
+  int grid = 0;
+  const int rc = persistent_grid(kernel, kT * kT, q.num_tiles, &grid);
+  if (rc) return rc;
+  q.queues = queue_count(grid);
+  kernel<<<grid, kT * kT, 0, s>>>(ra, ga, q);
+  return SSPLAT_OK;
+}
+
+template <int kT, bool kFade>
+int launch_backward_persistent_norm(int norm, const TileQueue& q, const RenderArgs& ra,
+                                    const GradArgs& ga, hipStream_t s) {
+  if (norm == kNormSaved) return launch_backward_persistent_one<kT, kFade, true, kNormSaved>(q, ra, ga, s);
+  if (norm == kNormRecompD) {
+    return launch_backward_persistent_one<kT, kFade, true, kNormRecompD>(q, ra, ga, s);
+  }
+  return launch_backward_persistent_one<kT, kFade, true, kNormRecompAll>(q, ra, ga, s);
+}
+
+template <int kT>
+int launch_backward_persistent(bool fade, bool normalize, int norm, const TileQueue& q,
+                               const RenderArgs& ra, const GradArgs& ga, hipStream_t s) {
+  if (!normalize) {
+    return fade ? launch_backward_persistent_one<kT, true, false, kNormSaved>(q, ra, ga, s)
+                : launch_backward_persistent_one<kT, false, false, kNormSaved>(q, ra, ga, s);
+  }
+  return fade ? launch_backward_persistent_norm<kT, true>(norm, q, ra, ga, s)
+              : launch_backward_persistent_norm<kT, false>(norm, q, ra, ga, s);
 }
 
 // Affine-color inputs (render.py:110-123): color gradients (n,2,3), scales (n,2), rotations (n,)
@@ -2329,8 +2573,19 @@ int forward_tiled(const Plan& p, const float* means, const float* conics, const 
   ra.arecs = arecs;
   hipEvent_t t0;
   const bool timed = timer_begin(0, s, &t0);
+  // A field without Gaussians has no record packing to zero the queue: grid launch.
+  const bool persistent = p.persistent && af == nullptr && p.n > 0;
+  const TileQueue q{w.queue_fwd, p.num_tiles, kXcds};
   if (af != nullptr) {
     launch_forward<16, true>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
+  } else if (persistent && p.ts == 16) {
+    if ((rc = launch_forward_persistent<16>(support_fade, normalize != 0, q, ra, out, den, s))) {
+      return rc;
+    }
+  } else if (persistent) {
+    if ((rc = launch_forward_persistent<32>(support_fade, normalize != 0, q, ra, out, den, s))) {
+      return rc;
+    }
   } else if (p.ts == 16) {
     launch_forward<16>(support_fade, normalize != 0, p.num_tiles, ra, out, den, s);
   } else {
@@ -2397,7 +2652,7 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   }
   // The slot offsets; the scan also advances the slot buffer's call counter (the call's tag).
   if ((rc = run_scan(w.rect_wave, int64_t(w.num_waves) + 1, w.rect_off, w.rect_bsums,
-                     w.rect_status, s, epoch))) {
+                     w.rect_status, s, epoch, w.queue_bwd))) {
     return rc;
   }
   RenderArgs ra = make_render_args(w, p, gids, offsets, num_gids, eps, fade_alpha, sigma_cutoff);
@@ -2406,8 +2661,17 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
   const GradArgs ga{grad_out, den, out, slots, slot_capacity, epoch, gop != nullptr};
   hipEvent_t t0;
   const bool timed = timer_begin(1, s, &t0);
+  const TileQueue q{w.queue_bwd, p.num_tiles, kXcds};
   if (af != nullptr) {
     launch_backward<16, true>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
+  } else if (p.persistent && p.ts == 16) {
+    if ((rc = launch_backward_persistent<16>(support_fade, normalize != 0, norm, q, ra, ga, s))) {
+      return rc;
+    }
+  } else if (p.persistent) {
+    if ((rc = launch_backward_persistent<32>(support_fade, normalize != 0, norm, q, ra, ga, s))) {
+      return rc;
+    }
   } else if (p.ts == 16) {
     launch_backward<16>(support_fade, normalize != 0, norm, p.num_tiles, ra, ga, s);
   } else {
@@ -2795,7 +3059,10 @@ int ssplat_batch_plan(ssplat_field* fields, int num_fields, int tile_size,
   if (fields == nullptr || num_fields <= 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the field table must hold at least one field");
   }
-  if ((rc = check_tile_size(tile_size))) return rc;
+  int edge;
+  bool persistent;
+  if ((rc = check_tile_size(tile_size, &edge, &persistent))) return rc;
+  tile_size = edge;  // the schedule flag does not change the layout
   int64_t g = 0, t = 0, px = 0;
   for (int i = 0; i < num_fields; ++i) {
     ssplat_field& f = fields[i];

@@ -13,9 +13,11 @@ Two drivers run the same arithmetic:
 * ``fit(..., graph=True)`` - the whole step captured once as a HIP graph and replayed.
 
 Both bin without host synchronization into a capacity-sized pair buffer (``RenderPlan``).  The
-binning status word is sticky; it is checked once per window of steps, and a window that
-overflowed is rolled back (parameters and Adam state restored) and replayed with a larger
-capacity, so the result equals a run with unlimited capacity.
+binning status word is sticky; it is read once per window of steps together with the last
+binning's pair count (``tile_offsets[T]``, on the device), and a window that overflowed is
+rolled back (parameters and Adam state restored) and replayed with a larger capacity, so the
+result equals a run with unlimited capacity; a pair count near the capacity regrows the buffer
+before it overflows.
 """
 from __future__ import annotations
 
@@ -66,6 +68,7 @@ class RenderPlan:
                  recompute_normalizer: bool = True, affine: bool = False):
         if affine and int(tile_size) != HR.DEFAULT_TILE_SIZE:
             raise ValueError("affine-color plans use 16-pixel tiles")
+        # tile_size: the edge, optionally with the TILE_PERSISTENT schedule flag.
         self.n, self.H, self.W, self.tile_size = int(n), int(H), int(W), int(tile_size)
         self.affine = bool(affine)
         # The affine backward always recomputes the normalizer.
@@ -325,6 +328,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     done = 0
     graph_obj = None
     regrows = 0
+    replay = {"steps": 0, "seconds": 0.0}  # graph replays (or eager steps) alone
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     while done < cfg.iters:
@@ -334,8 +338,7 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         if graph and done >= 2:
             if graph_obj is None:
                 graph_obj = _capture(step)
-            for _ in range(n):
-                graph_obj.replay()
+            _timed_steps(replay, n, graph_obj.replay, dev)
         elif graph:
             # The graphed driver's first two steps run eagerly on a side stream (torch's capture
             # recipe): they create Adam's state and the gradient buffers the graph reuses.
@@ -347,18 +350,18 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
                     step()
             torch.cuda.current_stream(dev).wait_stream(side)
         else:
-            for _ in range(n):
-                step()
-        if int(plan.status.item()) != 0:
+            _timed_steps(replay, n, step, dev)
+        # One readback per window: the sticky status and the last binning's pair count
+        # (offsets[T], written on the device by every step's binning).
+        status, p = window_readback(plan)
+        if status != 0:
             snap.restore(field, opt)
-            plan = new_plan(max(2 * plan.capacity,
-                                _capacity(pair_count(field, H, W, cfg, ts), cfg)))
+            plan = new_plan(max(2 * plan.capacity, _capacity(p, cfg)))
             graph_obj = None
             regrows += 1
             continue
         done += n
         # Keep headroom: regrow (and re-capture) when the pair count approaches capacity.
-        p = pair_count(field, H, W, cfg, ts)
         if p * 1.15 > plan.capacity:
             plan = new_plan(_headroom(p, cfg))
             graph_obj = None
@@ -381,7 +384,28 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
         final_psnr = float(psnr_tensor(img, target))
     return {"psnr": final_psnr, "loss": float(state["loss"]), "regrows": regrows,
             "capacity": plan.capacity, "tile_size": ts, "seconds": seconds,
+            "step_seconds": replay["seconds"], "timed_steps": replay["steps"],
             "color_grad_l2_term": color_grad_l2_term(field, cfg)}
+
+
+def _timed_steps(acc: dict, n: int, fn, dev) -> None:
+    """Runs `fn` n times between two device synchronizations and adds the wall time to `acc`
+    (the steps alone: no capture, snapshot or window readback)."""
+    torch.cuda.synchronize(dev)
+    t = time.perf_counter()
+    for _ in range(n):
+        fn()
+    torch.cuda.synchronize(dev)
+    acc["seconds"] += time.perf_counter() - t
+    acc["steps"] += n
+
+
+def window_readback(plan) -> tuple[int, int]:
+    """(sticky binning status, pair count of the plan's last binning) in one device-to-host
+    copy.  The pair count is tile_offsets[T], which every binning writes on the device (int32,
+    saturated at 2^31 - 1), so a window needs no recount on the host."""
+    vals = torch.cat([plan.status.reshape(-1), plan.offsets[-1:]]).to("cpu")
+    return int(vals[0]), int(vals[1])
 
 
 def _seed(state: dict, loss: torch.Tensor) -> torch.Tensor:
@@ -394,11 +418,12 @@ def _seed(state: dict, loss: torch.Tensor) -> torch.Tensor:
 
 
 def resolve_tile_size(cfg: FitConfig, tune) -> int:
-    """cfg.tile_size, or the autotuner's choice (`tune()`) for "auto"."""
+    """cfg.tile_size, or the autotuner's choice (`tune()`) for "auto".  An explicit size is 16
+    or 32, optionally OR-ed with ``render.TILE_PERSISTENT`` (the persistent schedule)."""
     if cfg.tile_size == "auto":
         return int(tune())
     ts = int(cfg.tile_size)
-    if ts not in (16, 32):
+    if HR.tile_edge(ts) not in (16, 32) or ts & ~(HR.TILE_PERSISTENT | 0xff):
         raise ValueError(f"tile_size must be 16, 32 or 'auto', got {cfg.tile_size!r}")
     return ts
 

@@ -32,6 +32,15 @@ from . import _native
 _EPS = 1e-8
 DEFAULT_TILE_SIZE = 16
 _INT32_SATURATED = 2**31 - 1
+# Render schedule flag OR-ed into a tile size (SSPLAT_TILE_PERSISTENT): the forward and backward
+# kernels run as one persistent launch over per-XCD tile queues instead of one workgroup a tile.
+# The binning ignores it; results are bit-identical either way.
+TILE_PERSISTENT = 0x100
+
+
+def tile_edge(tile_size: int) -> int:
+    """The tile edge of a tile size that may carry the TILE_PERSISTENT schedule flag."""
+    return int(tile_size) & ~TILE_PERSISTENT
 
 _NORMALIZED_MODES = (
     "normalized", "hip", "hip_normalized", "hip_tiled", "hip_tiled_normalized",
@@ -103,7 +112,8 @@ def _check_field(means, conics, colors, radii, opacities, need_colors: bool = Tr
 def _check_image(height: int, width: int, tile_size: int) -> None:
     if int(height) <= 0 or int(width) <= 0:
         raise ValueError("height and width must be positive")
-    if int(tile_size) <= 0 or int(tile_size) ** 2 > 1024:
+    ts = tile_edge(tile_size)
+    if ts <= 0 or ts ** 2 > 1024:
         raise ValueError("tile_size must be positive with tile_size^2 <= 1024")
 
 
@@ -119,7 +129,7 @@ def _radii64(radii: torch.Tensor) -> torch.Tensor:
 
 
 def num_tiles(height: int, width: int, tile_size: int = DEFAULT_TILE_SIZE) -> int:
-    ts = int(tile_size)
+    ts = tile_edge(tile_size)
     return ((int(width) + ts - 1) // ts) * ((int(height) + ts - 1) // ts)
 
 

@@ -23,14 +23,16 @@ workload's unit of work stays fixed, so a tiling with fewer pairs shows up as a 
 as fewer bytes.  The render tile (16 or 32 pixels) is the autotuner's choice on this field
 (`--tile-size` overrides it); every candidate's fwd+bwd time is reported.
 
-The roofline is reported for k_backward, the step's dominant kernel, at the chosen tile: its
-algorithmic bytes per launch (B_bwd with that tile's P) over its average duration, measured by
-HIP events that the library records on the launch stream around that kernel alone.  `traffic` is the kernel's measured HBM bytes per launch
-(rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy; `--copy-probe` is that copy)
-when profiles/hbm_traffic.json holds a measurement of this workload, else null.
+The roofline is reported for the step's dominant kernel, the backward (k_backward, or
+k_backward_persistent under the persistent schedule), at the chosen tile: its algorithmic bytes
+per launch (B_bwd with that tile's P) over its average duration, measured by HIP events that the
+library records on the launch stream around that kernel alone.  `traffic` is the kernel's
+measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy;
+`--copy-probe` is that copy) when profiles/hbm_traffic.json holds a measurement of this workload
+and kernel, else null.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload metric|single|stress]
-                       [--tile-size auto|16|32]
+                       [--tile-size auto|16|32|272|288]
        python bench.py --copy-probe MIB     (one device copy of MIB MiB, for counter calibration)
 """
 from __future__ import annotations
@@ -365,10 +367,8 @@ def measured_traffic(workload: str, kernel: str):
             data = json.load(fh)
     except (OSError, ValueError):
         return None
-    entry = data.get("kernels", {}).get(kernel)
-    if data.get("workload") != workload or not entry:
-        return None
-    return entry.get("hbm_bytes_per_launch")
+    entry = data.get("workloads", {}).get(workload, {}).get("kernels", {}).get(kernel)
+    return entry.get("hbm_bytes_per_launch") if entry else None
 
 
 def _cpu_model() -> str:
@@ -567,6 +567,8 @@ def main() -> int:
     b_tile = algorithmic_bytes(N, P, T, H * W)      # what the chosen tiling must move
     value = b["step"] / (ms_per_step * 1e-3) / 1e9
     bwd_gbs = b_tile["bwd"] / (kernel_ms["k_backward"] * 1e-3) / 1e9
+    bwd_kernel = (f"k_backward_persistent<{edge}>" if ts & HR.TILE_PERSISTENT
+                  else f"k_backward<{edge}>")
     line = {
         "metric": "fwd+bwd ms & HBM GB/s at 2048^2/200k Gaussians; batched fit steps/s vs plain",
         "value": round(value, 3),
@@ -604,12 +606,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_backward<{edge}>",
+            "kernel": bwd_kernel,
             "achieved": round(bwd_gbs, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(args.workload, f"k_backward<{edge}>"),
+            "traffic": measured_traffic(args.workload, bwd_kernel),
         },
     }
     _progress(f"fwd+bwd {ms_per_step:.4f} ms per step")

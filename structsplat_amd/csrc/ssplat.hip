@@ -1058,6 +1058,18 @@ __global__ __launch_bounds__(256) void k_prepare(
 // the clamp holds, whose scale then gets no gradient.
 constexpr int kAffineChunks = 3;
 
+// The thread's index in its workgroup; with kOpaque, opaque to the optimizer.  In the persistent
+// kernels' tile loop the compiler otherwise hoists what derives from the thread index (lane, wave,
+// pixel offsets, LDS addresses) out of the loop and holds it in registers across every tile: the
+// persistent backward then needs 101 VGPRs instead of 59.  Opaque, those values are recomputed
+// where used.  The grid kernels run one tile and gain nothing from the recomputation.
+template <bool kOpaque>
+__device__ __forceinline__ int thread_index() {
+  int v = static_cast<int>(threadIdx.x);
+  if constexpr (kOpaque) asm volatile("" : "+v"(v));
+  return v;
+}
+
 struct RenderArgs {
   const GaussRec* recs;
   const float4* arecs;      // affine records (kAffineChunks per Gaussian), or NULL
@@ -1096,29 +1108,31 @@ struct StageLds {
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
 // kAffine: the affine chunks follow in st.c[4 .. 4 + kAffineChunks).  kSlot: the gradient
 // sweep's records carry the Gaussian's absolute first slot in c3.w.
-template <int kLoad, int kChunks, int kS, bool kAffine = false, bool kSlot = false>
+template <int kLoad, int kChunks, int kS, bool kAffine = false, bool kSlot = false,
+          bool kOpaque = false>
 __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                             int64_t batch, int m) {
   static_assert(kLoad <= kChunks, "staging more chunks than the buffer holds");
   static_assert(!kAffine || (kLoad == 4 && kChunks == 4 + kAffineChunks), "affine layout");
-  if (static_cast<int>(threadIdx.x) < m) {
-    const int g = a.gids[batch + threadIdx.x];
+  const int t = thread_index<kOpaque>();
+  if (t < m) {
+    const int g = a.gids[batch + t];
     if (g >= 0 && g < a.n) {
       const float4* src = reinterpret_cast<const float4*>(a.recs + g);
 #pragma unroll
-      for (int c = 0; c < kLoad; ++c) st.c[c][threadIdx.x] = src[c];
+      for (int c = 0; c < kLoad; ++c) st.c[c][t] = src[c];
       if constexpr (kSlot) {
-        st.c[3][threadIdx.x].w = __int_as_float(__float_as_int(src[3].w) + a.rect_off[g >> 6]);
+        st.c[3][t].w = __int_as_float(__float_as_int(src[3].w) + a.rect_off[g >> 6]);
       }
       if constexpr (kAffine) {
 #pragma unroll
         for (int c = 0; c < kAffineChunks; ++c) {
-          st.c[4 + c][threadIdx.x] = a.arecs[int64_t(g) * kAffineChunks + c];
+          st.c[4 + c][t] = a.arecs[int64_t(g) * kAffineChunks + c];
         }
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < kChunks; ++c) st.c[c][threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int c = 0; c < kChunks; ++c) st.c[c][t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
 }
@@ -1158,11 +1172,11 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // bit of their masks.
 // kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
 // recomputation); it is the same sum, in the same order, as the forward's D.
-template <bool kFade, bool kDenOnly, int kChunks, int kS, bool kAffine = false>
+template <bool kFade, bool kDenOnly, int kChunks, int kS, bool kAffine = false, bool kOpaque = false>
 __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                                     uint32_t* s_mask, int tile, int qx0, int qy0,
                                                     int px, int py, float tail, float4& acc) {
-  const int lane = threadIdx.x & 63;
+  const int lane = thread_index<kOpaque>() & 63;
   const float fx = static_cast<float>(px);
   const float fy = static_cast<float>(py);
   int64_t start, end;
@@ -1171,9 +1185,9 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
     if constexpr (kAffine && !kDenOnly) {
-      stage_batch<4, kChunks, kS, true>(a, st, batch, m);
+      stage_batch<4, kChunks, kS, true, false, kOpaque>(a, st, batch, m);
     } else {
-      stage_batch<3>(a, st, batch, m);
+      stage_batch<3, kChunks, kS, false, false, kOpaque>(a, st, batch, m);
     }
     __syncthreads();
     uint64_t hit[kS / 64];
@@ -1306,16 +1320,18 @@ __device__ __forceinline__ int claimed_tile(const TileQueue& q, int r, int v) {
 __device__ __forceinline__ int claim_tile(const TileQueue& q, TileClaim& c) {
   __shared__ int s_tile;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (thread_index<true>() == 0) {
     const int t = claimed_tile(q, c.range, c.pending);
     if (t >= 0) c.pending = atomicAdd(&q.counters[c.range], 1);  // the claim after this one
     s_tile = t;
   }
   __syncthreads();
-  int tile = s_tile;
-  while (tile < 0) {  // uniform: every thread reads the same broadcast word
-    if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
+  // Every thread reads the same broadcast word: readfirstlane tells the compiler the tile is
+  // uniform, so what derives from it (origin, field, list bounds) stays in scalar registers.
+  int tile = __builtin_amdgcn_readfirstlane(s_tile);
+  while (tile < 0) {
+    if (thread_index<true>() < 64) {
+      const int lane = thread_index<true>();
       int left = 0;
       if (lane < q.queues) {
         const int size = queue_bound(q, lane + 1) - queue_bound(q, lane);
@@ -1343,7 +1359,7 @@ __device__ __forceinline__ int claim_tile(const TileQueue& q, TileClaim& c) {
       }
     }
     __syncthreads();
-    tile = s_tile;
+    tile = __builtin_amdgcn_readfirstlane(s_tile);
     __syncthreads();  // every thread has read the word before it can change again
     if (tile == -2) return -1;
   }
@@ -1351,7 +1367,7 @@ __device__ __forceinline__ int claim_tile(const TileQueue& q, TileClaim& c) {
 }
 
 // One tile of the forward: the body of k_forward and of its persistent form.
-template <int kT, bool kFade, bool kNormalize, bool kAffine>
+template <int kT, bool kFade, bool kNormalize, bool kAffine, bool kOpaque = false>
 __device__ __forceinline__ void forward_tile(const RenderArgs& a, int tile, float* __restrict__ out,
                                              float* __restrict__ den) {
   constexpr int kQ = kT / 8;  // quadrants per tile row
@@ -1361,15 +1377,15 @@ __device__ __forceinline__ void forward_tile(const RenderArgs& a, int tile, floa
   const int local = tile - static_cast<int>(f.first_tile);
   const int tile_x = local % f.tiles_x;
   const int tile_y = local / f.tiles_x;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  const int wave = thread_index<kOpaque>() >> 6;
+  const int lane = thread_index<kOpaque>() & 63;
   const int qx0 = tile_x * kT + (wave % kQ) * 8;
   const int qy0 = tile_y * kT + (wave / kQ) * 8;
   const int px = qx0 + (lane & 7);
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
-  accumulate_quadrant<kFade, false, kAffine ? 4 + kAffineChunks : 3, kStage, kAffine>(
+  accumulate_quadrant<kFade, false, kAffine ? 4 + kAffineChunks : 3, kStage, kAffine, kOpaque>(
       a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
     const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
@@ -1395,17 +1411,17 @@ __global__ __launch_bounds__(kT * kT) void k_forward(RenderArgs a, float* __rest
 }
 
 // Persistent launch: as many workgroups as fit on the device, each taking tiles from the queue.
-// Held to the grid kernel's register budget (eight waves per SIMD): the tile loop otherwise lets
-// the compiler keep values live across tiles, which costs occupancy.
+// With 32-pixel tiles the kernel is held to eight waves per SIMD (two workgroups per CU); with
+// 16-pixel tiles the LDS of seven workgroups fills the CU, so seven waves per SIMD is the most.
 template <int kT, bool kFade, bool kNormalize>
-__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(kT == 32 ? 8 : 7, 8)))
 void k_forward_persistent(RenderArgs a, TileQueue q,
                                                                 float* __restrict__ out,
                                                                 float* __restrict__ den) {
   TileClaim c{};
   if (threadIdx.x == 0) c = first_claim(q);
   for (int tile = claim_tile(q, c); tile >= 0; tile = claim_tile(q, c)) {
-    forward_tile<kT, kFade, kNormalize, false>(a, tile, out, den);
+    forward_tile<kT, kFade, kNormalize, false, true>(a, tile, out, den);
   }
 }
 
@@ -1425,7 +1441,7 @@ constexpr int kNormRecompD = 1;    // out read, D recomputed by a denominator-on
 constexpr int kNormRecompAll = 2;  // N and D recomputed by the forward's sweep, out = N/(D+eps)
 
 // One tile of the backward: the body of k_backward and of its persistent form.
-template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine>
+template <int kT, bool kFade, bool kNormalize, int kNorm, bool kAffine, bool kOpaque = false>
 __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArgs& g, int tile) {
   constexpr bool kRecompute = kNormalize && kNorm != kNormSaved;
   constexpr int kQ = kT / 8;  // quadrants per tile row
@@ -1443,8 +1459,8 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   const int local = tile - static_cast<int>(f.first_tile);
   const int tile_x = local % f.tiles_x;
   const int tile_y = local / f.tiles_x;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  const int wave = thread_index<kOpaque>() >> 6;
+  const int lane = thread_index<kOpaque>() & 63;
   const int lx = lane & 7;
   const int ly = lane >> 3;
   const int tx0 = tile_x * kT;
@@ -1462,7 +1478,7 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
     const int py = qy0 + ly;
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kS);
     float4 acc;
-    accumulate_quadrant<kFade, kNorm == kNormRecompD, kChunks, kS, kAffine>(
+    accumulate_quadrant<kFade, kNorm == kNormRecompD, kChunks, kS, kAffine, kOpaque>(
         a, st, s_mask, tile, qx0, qy0, px, py,
                                                       tail, acc);
     // Every wave is past the sweep's last barrier: the mask space can take the adjoints.
@@ -1488,8 +1504,8 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
     }
     s_adj[(py - ty0) * kT + (px - tx0)] = adj;
   } else {
-    const int px = tx0 + static_cast<int>(threadIdx.x % kT);
-    const int py = ty0 + static_cast<int>(threadIdx.x / kT);
+    const int px = tx0 + static_cast<int>(thread_index<kOpaque>() % kT);
+    const int py = ty0 + static_cast<int>(thread_index<kOpaque>() / kT);
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (px < f.width && py < f.height) {
       const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;
@@ -1505,7 +1521,7 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
         adj = make_float4(gr, gg, gb, 0.0f);
       }
     }
-    s_adj[threadIdx.x] = adj;
+    s_adj[thread_index<kOpaque>()] = adj;
   }
   __syncthreads();
 
@@ -1531,7 +1547,7 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    stage_batch<4, kChunks, kS, kAffine, true>(a, st, batch, m);
+    stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(a, st, batch, m);
     __syncthreads();
     for (int j = wave; j < m; j += kWaves) {
       const float4 v0 = st.c[0][j];
@@ -1725,13 +1741,17 @@ __global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(
   backward_tile<kT, kFade, kNormalize, kNorm, kAffine>(a, g, xcd_remap(blockIdx.x, gridDim.x));
 }
 
+// The tile loop holds the kernel arguments in scalar registers across tiles.  With 32-pixel tiles
+// eight waves per SIMD (two workgroups per CU) costs a few scalar spills into vector lanes; with
+// 16-pixel tiles the recomputing sweeps need 69-71 VGPRs, and eight waves per SIMD would spill
+// vector registers to memory, so seven is the floor.
 template <int kT, bool kFade, bool kNormalize, int kNorm>
-__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ __launch_bounds__(kT * kT) __attribute__((amdgpu_waves_per_eu(kT == 32 ? 8 : 7, 8)))
 void k_backward_persistent(RenderArgs a, GradArgs g, TileQueue q) {
   TileClaim c{};
   if (threadIdx.x == 0) c = first_claim(q);
   for (int tile = claim_tile(q, c); tile >= 0; tile = claim_tile(q, c)) {
-    backward_tile<kT, kFade, kNormalize, kNorm, false>(a, g, tile);
+    backward_tile<kT, kFade, kNormalize, kNorm, false, true>(a, g, tile);
   }
 }
 

@@ -1,12 +1,13 @@
 """Benchmark of the HIP rasterizer's hot path: one fwd+bwd of a 2048x2048 field of 200k Gaussians.
 
 A step is one pass of the renderer's hot path over the resident field: tile binning (no host
-synchronization), tiled forward, and the tiled backward of the default autograd path, which
-recomputes the per-pixel normalizer (a denominator-only sweep of the tile lists) instead of
-storing it and reads the forward's image, against a fixed synthetic dL/dout.  The variant that
-stores the normalizer (the reference's saved den/out) is timed beside it.  Prints one JSON line
-(metric, value, roofline of the dominant kernel, CPU baseline).  `value` is the algorithmic HBM
-throughput of the whole step:
+synchronization), tiled forward, and the tiled backward of the default autograd path, against a
+fixed synthetic dL/dout.  By default the forward stores the per-pixel normalizer and the backward
+reads it with the image (the reference's saved den/out); the variant whose backward recomputes
+the normalizer with a second sweep of the tile lists is timed beside it (`--normalizer recompute`
+makes it the step).  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+baseline).  `value` is the algorithmic HBM throughput of the whole step, by the workload's fixed
+byte model:
 
     B_step = B_bin + B_fwd + B_bwd
     B_bin  = 52 N + 4 P + 4 (T+1)                     (means, conics, radii in; lists, offsets out)
@@ -14,9 +15,8 @@ throughput of the whole step:
     B_bwd  = 12 H W + 2 P (4 + R) + 36 P + 36 N       (grad in; two list sweeps; one 9-float
                                                         atomic set per pair; gradients)
 with R = 44 bytes of Gaussian parameters per pair visit (mean 8, conic 12, color 12, opacity 4,
-radii 8), N Gaussians, P (tile, Gaussian) pairs and T tiles.  This is the byte model of a backward
-that recomputes the normalizer with a second list sweep; the default backward does so and also
-reads the forward's image (12 H W, not counted).
+radii 8), N Gaussians, P (tile, Gaussian) pairs and T tiles.  The model is the same for either
+normalizer mode: it is the workload's unit of work, so a faster mode shows as a shorter step.
 
 P in `value` is counted at the reference's 16-pixel tiling whatever tile the renderer runs: the
 workload's unit of work stays fixed, so a tiling with fewer pairs shows up as a shorter step, not
@@ -24,9 +24,14 @@ as fewer bytes.  The render tile (16 or 32 pixels) is the autotuner's choice on 
 (`--tile-size` overrides it); every candidate's fwd+bwd time is reported.
 
 The roofline is reported for the step's dominant kernel, the backward (k_backward, or
-k_backward_persistent under the persistent schedule), at the chosen tile: its algorithmic bytes
-per launch (B_bwd with that tile's P) over its average duration, measured by HIP events that the
-library records on the launch stream around that kernel alone.  `traffic` is the kernel's
+k_backward_persistent under the persistent schedule), at the chosen tile: its own algorithmic
+bytes per launch, with that tile's P,
+
+    stored normalizer:     B_k = 28 H W + P (4 + R) + 36 P   (grad, den, image in; one sweep; slots)
+    recomputed normalizer: B_k = 24 H W + 2 P (4 + R) + 36 P (grad, image in; two sweeps; slots)
+
+(the 36 N gradient bytes belong to k_grad_reduce), over its average duration, measured by HIP
+events that the library records on the launch stream around that kernel alone.  `traffic` is the kernel's
 measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy;
 `--copy-probe` is that copy) when profiles/hbm_traffic.json holds a measurement of this workload
 and kernel, else null.
@@ -63,6 +68,14 @@ WORKLOADS = {
     "stress": ("fwd+bwd 4096x4096 RGB fp32, 1,000,000 Gaussians: scales log-normal(ln 2, 1) "
                "clipped at 64 px, aspect log-U[1,20], seed 0", 4096, 4096, 1_000_000),
 }
+
+
+def backward_kernel_bytes(P: int, HW: int, stored: bool) -> int:
+    """Algorithmic bytes of one backward kernel launch (gradient slots out, not the reduction)."""
+    R = RECORD_BYTES
+    if stored:
+        return 28 * HW + P * (4 + R) + 36 * P
+    return 24 * HW + 2 * P * (4 + R) + 36 * P
 
 
 def algorithmic_bytes(N: int, P: int, T: int, HW: int) -> dict[str, float]:
@@ -394,6 +407,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--normalizer", default="stored", choices=["stored", "recompute"],
+                    help="stored: the forward saves den for the backward (the reference's "
+                         "data flow); recompute: the backward recomputes it")
     ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32", "272", "288"],
                     help="render tile edge (272 / 288: 16 / 32 with the persistent schedule); "
                          "auto = the autotuner's choice on this field")
@@ -434,7 +450,8 @@ def main() -> int:
     # Tile shape: the autotuner's choice (fwd+bwd of this field timed at each candidate).
     best, tile_ms = tune.tune_tile_size([(H, W)], [N], means, conics, colors, radii, op,
                                         normalize=True, eps=1e-8, support_fade=False,
-                                        sigma_cutoff=3.0, recompute_normalizer=True)
+                                        sigma_cutoff=3.0,
+                                        recompute_normalizer=args.normalizer == "recompute")
     ts = best if args.tile_size == "auto" else int(args.tile_size)
     edge = HR.tile_edge(ts)  # the tile edge of the chosen configuration (ts may add the schedule)
     T = HR.num_tiles(H, W, ts)
@@ -463,7 +480,9 @@ def main() -> int:
     ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(2)] for k in ("b", "f", "g")}
     phase_ms = {"bin": [], "fwd": [], "bwd": []}
 
-    def step(record: bool, saved: bool = False):
+    stored = args.normalizer == "stored"
+
+    def step(record: bool, saved: bool = stored):
         """One fwd+bwd; `saved`: the forward stores den and the backward reads it."""
         if record:
             ev["b"][0].record(stream)
@@ -548,17 +567,17 @@ def main() -> int:
         return {"k_forward": fwd_kernel_ms / max(nf, 1),
                 "k_backward": bwd_kernel_ms / max(nb, 1)}
 
-    kernel_ms = kernel_times(False)
-    # The stored-normalizer variant: whole-step wall time and kernel times.
+    kernel_ms = kernel_times(stored)
+    # The other normalizer mode: whole-step wall time and kernel times.
     for _ in range(args.warmup):
-        step(False, True)
+        step(False, not stored)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(False, True)
+        step(False, not stored)
     torch.cuda.synchronize()
-    saved_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    saved_kernel_ms = kernel_times(True)
+    other_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    other_kernel_ms = kernel_times(not stored)
 
     P_now = int(offsets[-1].item())
     if P_now != P:
@@ -566,9 +585,10 @@ def main() -> int:
     b = algorithmic_bytes(N, P16, T16, H * W)       # the workload's work (reference tiling)
     b_tile = algorithmic_bytes(N, P, T, H * W)      # what the chosen tiling must move
     value = b["step"] / (ms_per_step * 1e-3) / 1e9
-    bwd_gbs = b_tile["bwd"] / (kernel_ms["k_backward"] * 1e-3) / 1e9
+    bwd_bytes = backward_kernel_bytes(P, H * W, stored)
+    bwd_gbs = bwd_bytes / (kernel_ms["k_backward"] * 1e-3) / 1e9
     bwd_kernel = (f"k_backward_persistent<{edge}>" if ts & HR.TILE_PERSISTENT
-                  else f"k_backward<{edge}>")
+                  else f"k_backward<{edge}>") + ("/stored" if stored else "")
     line = {
         "metric": "fwd+bwd ms & HBM GB/s at 2048^2/200k Gaussians; batched fit steps/s vs plain",
         "value": round(value, 3),
@@ -592,14 +612,15 @@ def main() -> int:
             "tile_pairs": P, "tiles": T, "tile_pairs_at_16": P16,
             "phase_ms": {k: round(v, 4) for k, v in phase.items()},
             "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
-            "normalizer": "recomputed in the backward (den not stored)",
+            "normalizer": ("stored by the forward, read by the backward" if stored
+                           else "recomputed in the backward (den not stored)"),
             "graph_replay_ms_per_step": round(graph_ms, 4),
             "timing": ("ms_per_step and value: K eager steps (host launches) between two "
                        "synchronizations; graph_replay_ms_per_step: the same step captured "
                        "once as a HIP graph and replayed K times"),
-            "stored_normalizer_variant": {
-                "ms_per_step": round(saved_ms, 4),
-                "kernel_ms": {k: round(v, 4) for k, v in saved_kernel_ms.items()},
+            ("recomputed_normalizer_variant" if stored else "stored_normalizer_variant"): {
+                "ms_per_step": round(other_ms, 4),
+                "kernel_ms": {k: round(v, 4) for k, v in other_kernel_ms.items()},
             },
             "algorithmic_bytes": b,
             "algorithmic_bytes_at_tile": b_tile,
@@ -607,6 +628,7 @@ def main() -> int:
         "roofline": {
             "bound": "hbm",
             "kernel": bwd_kernel,
+            "algorithmic_bytes_per_launch": bwd_bytes,
             "achieved": round(bwd_gbs, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

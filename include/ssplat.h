@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SSPLAT_ABI_VERSION 4  /* 4: persistent render schedule (SSPLAT_TILE_PERSISTENT) */
+#define SSPLAT_ABI_VERSION 5  /* 5: loss target moments (ssplat_loss_target_moments) */
 
 /*
  * Render schedule flag, OR-ed into the tile_size argument of the render entry points (the plain
@@ -407,12 +407,23 @@ int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int
                         size_t workspace_bytes, void* stream);
 
 /*
+ * The target's window moments for a fit whose target does not change: moments receives two
+ * (H, W, 3) float32 maps, G*t then G*t^2 (the 11-tap window per channel, zero-padded).  No
+ * reference counterpart (the reference recomputes them in every SSIM call).
+ */
+int ssplat_loss_target_moments(const float* target, int height, int width, float* moments,
+                               void* stream);
+
+/*
  * The same loss in two calls for autograd: _forward writes *loss and keeps the SSIM derivative
  * maps in the workspace; _backward (same image, target, weight and workspace, nothing else using
  * the workspace in between) writes grad_image = (*grad_loss) * dloss/dimage, with grad_loss a
- * device float (NULL: 1).
+ * device float (NULL: 1).  target_moments, when not NULL, holds ssplat_loss_target_moments of
+ * this target: the forward then blurs three maps of the image instead of five, with the same
+ * result bit for bit.
  */
-int ssplat_l1_ssim_loss_forward(const float* image, const float* target, int height, int width,
+int ssplat_l1_ssim_loss_forward(const float* image, const float* target,
+                                const float* target_moments, int height, int width,
                                 double ssim_weight, float* loss, void* workspace,
                                 size_t workspace_bytes, void* stream);
 int ssplat_l1_ssim_loss_backward(const float* image, const float* target, int height, int width,
@@ -420,14 +431,19 @@ int ssplat_l1_ssim_loss_backward(const float* image, const float* target, int he
                                  void* workspace, size_t workspace_bytes, void* stream);
 
 /* Batched loss: the sum of every field's loss over concatenated images (a field table planned
-   with tile_size 16 by ssplat_batch_plan, copied to the device), and its two-call form. */
+   with tile_size 16 by ssplat_batch_plan, copied to the device), its two-call form and the
+   target moments of every field (2 maps of (total_pixels, 3)). */
 int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
                               int64_t total_pixels, const float* image, const float* target,
                               double ssim_weight, float* loss, float* grad_image,
                               void* workspace, size_t workspace_bytes, void* stream);
+int ssplat_batch_loss_target_moments(const ssplat_field* fields, int num_fields,
+                                     int64_t total_tiles, int64_t total_pixels,
+                                     const float* target, float* moments, void* stream);
 int ssplat_batch_l1_ssim_loss_forward(const ssplat_field* fields, int num_fields,
                                       int64_t total_tiles, int64_t total_pixels,
-                                      const float* image, const float* target, double ssim_weight,
+                                      const float* image, const float* target,
+                                      const float* target_moments, double ssim_weight,
                                       float* loss, void* workspace, size_t workspace_bytes,
                                       void* stream);
 int ssplat_batch_l1_ssim_loss_backward(const ssplat_field* fields, int num_fields,

@@ -83,13 +83,16 @@ SIGNATURES = {
                                       _c_ptr, _c_ptr, _c_ptr, _c_ptr]),
     "ssplat_produce_field_backward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_dbl,
                                                _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr]),
-    "ssplat_l1_ssim_loss_forward": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_int, _c_dbl, _c_ptr,
-                                             _c_ptr, _c_size, _c_ptr]),
+    "ssplat_loss_target_moments": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr, _c_ptr]),
+    "ssplat_batch_loss_target_moments": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_ptr,
+                                                  _c_ptr, _c_ptr]),
+    "ssplat_l1_ssim_loss_forward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_int, _c_dbl,
+                                             _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_l1_ssim_loss_backward": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_int, _c_dbl, _c_ptr,
                                               _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_l1_ssim_loss_forward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_ptr,
-                                                   _c_ptr, _c_dbl, _c_ptr, _c_ptr, _c_size,
-                                                   _c_ptr]),
+                                                   _c_ptr, _c_ptr, _c_dbl, _c_ptr, _c_ptr,
+                                                   _c_size, _c_ptr]),
     "ssplat_batch_l1_ssim_loss_backward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_ptr,
                                                     _c_ptr, _c_dbl, _c_ptr, _c_ptr, _c_ptr,
                                                     _c_size, _c_ptr]),

@@ -46,6 +46,16 @@ constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 static_assert(kHTasks <= kLossThreads && kVTasks <= kLossThreads, "one task per thread");
 
+// What the stats pass blurs.  The target's window moments mu_t = G*t and G*t^2 do not change
+// while a fit runs, so a fit computes them once (kStatsTarget) and its steps read them
+// (kStatsMoments), blurring three maps of the image instead of five.  The blurs are the same
+// arithmetic in every mode, so the moments read equal the ones computed inline bit for bit.
+constexpr int kStatsFull = 0;     // p, t, p^2, t^2, p t
+constexpr int kStatsMoments = 1;  // p, p^2, p t; mu_t and G*t^2 read from tmom
+constexpr int kStatsTarget = 2;   // t, t^2 only: writes mu_t and G*t^2 to tmom_out
+template <int kMode>
+constexpr int kQuantities = kMode == kStatsFull ? 5 : (kMode == kStatsMoments ? 3 : 2);
+
 struct Window {
   float w[kWin];
 };
@@ -58,6 +68,8 @@ struct LossArgs {
   const float* scale;   // device scalar dL/dloss multiplying the gradient, or NULL (1)
   bool keep;            // forward: write a, b, c for a later backward
   float* abc;           // 3 maps of (pixels, 3): dS/dmu_p, dS/dm2, dS/dmpt
+  const float* tmom;    // the target's moments, 2 maps of (pixels, 3): G*t, G*t^2 (or NULL)
+  float* tmom_out;      // kStatsTarget: where they are written
   float* partial;       // one float per tile
   int64_t pixels;       // pixels over all fields
   float w_l1;           // 1 - ssim_weight
@@ -154,8 +166,15 @@ __device__ float block_sum(float v) {
   return t;
 }
 
-template <bool kSSIM>
+// The window's sums at one output: mu = (mu_p, mu_t), m = (G*p^2, G*t^2, G*(p t)).
+struct Moments {
+  float mu_p, mu_t, m_pp, m_tt, m_pt;
+};
+
+template <bool kSSIM, int kMode = kStatsFull>
 __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
+  constexpr int kQ = kQuantities<kMode>;
+  constexpr bool kTarget = kMode == kStatsTarget;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileGeo g = tile_geo(a, tile);
   float l1 = 0.0f, ssim_sum = 0.0f;
@@ -173,23 +192,26 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     return;
   }
 
-  // LDS: the staged p and t, then (reusing the same space) the horizontally blurred
-  // p, t, p^2, t^2, p t.
-  constexpr int kStaged = 2 * kMapFloats;
-  constexpr int kBlurred = 5 * 3 * kSpan * (kLT + 1);
+  // LDS: the staged p and t (t alone for the target's moments), then (reusing the same space)
+  // the horizontally blurred quantities.
+  constexpr int kStaged = (kTarget ? 1 : 2) * kMapFloats;
+  constexpr int kBlurred = kQ * 3 * kSpan * (kLT + 1);
   __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
-  {
+  if constexpr (kTarget) {
+    const float* const src[1] = {a.target};
+    stage_halo<1, false>(src, g, s_buf);
+  } else {
     const float* const src[2] = {a.img, a.target};
     l1 = stage_halo<2, true>(src, g, s_buf);
   }
   __syncthreads();
 
   // Horizontal pass into registers: task = (channel, staged row, 8-column segment); input-major
-  // so each product p^2, t^2, p t is formed once.
+  // so each product is formed once.
   const float (*s_p)[kSpan][kSpan + 1] = reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf);
   const float (*s_t)[kSpan][kSpan + 1] =
-      reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + kMapFloats);
-  float h[5][kHSeg];
+      reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + (kTarget ? 0 : kMapFloats));
+  float h[kQ][kHSeg];
   const bool h_task = threadIdx.x < kHTasks;
   const int h_ch = threadIdx.x / (kSpan * 2);
   const int h_rem = threadIdx.x - h_ch * (kSpan * 2);
@@ -197,33 +219,46 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
   const int h_c0 = (h_rem & 1) * kHSeg;
   if (h_task) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < kQ; ++q) {
 #pragma unroll
       for (int o = 0; o < kHSeg; ++o) h[q][o] = 0.0f;
     }
 #pragma unroll
     for (int i = 0; i < kHSeg + kWin - 1; ++i) {
-      const float p = s_p[h_ch][h_row][h_c0 + i];
       const float t = s_t[h_ch][h_row][h_c0 + i];
-      const float pp = p * p, tt = t * t, pt = p * t;
+      float val[kQ];
+      if constexpr (kMode == kStatsTarget) {
+        val[0] = t;
+        val[1] = t * t;
+      } else {
+        const float p = s_p[h_ch][h_row][h_c0 + i];
+        if constexpr (kMode == kStatsFull) {
+          val[0] = p;
+          val[1] = t;
+          val[2] = p * p;
+          val[3] = t * t;
+          val[4] = p * t;
+        } else {
+          val[0] = p;
+          val[1] = p * p;
+          val[2] = p * t;
+        }
+      }
 #pragma unroll
       for (int o = 0; o < kHSeg; ++o) {
         const int k = i - o;
         if (k < 0 || k >= kWin) continue;
         const float w = a.win.w[k];
-        h[0][o] = fmaf(w, p, h[0][o]);
-        h[1][o] = fmaf(w, t, h[1][o]);
-        h[2][o] = fmaf(w, pp, h[2][o]);
-        h[3][o] = fmaf(w, tt, h[3][o]);
-        h[4][o] = fmaf(w, pt, h[4][o]);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) h[q][o] = fmaf(w, val[q], h[q][o]);
       }
     }
   }
-  __syncthreads();  // every read of the staged p and t is done
+  __syncthreads();  // every read of the staged maps is done
   float (*s_h)[3][kSpan][kLT + 1] = reinterpret_cast<float (*)[3][kSpan][kLT + 1]>(s_buf);
   if (h_task) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < kQ; ++q) {
 #pragma unroll
       for (int o = 0; o < kHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
     }
@@ -235,9 +270,9 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     const int ch = threadIdx.x >> 6;
     const int col = threadIdx.x & (kLT - 1);
     const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
-    float st[5][kVSeg];
+    float st[kQ][kVSeg];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       float v[kVSeg + kWin - 1];
 #pragma unroll
       for (int i = 0; i < kVSeg + kWin - 1; ++i) v[i] = s_h[q][ch][r0 + i][col];
@@ -254,29 +289,41 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     for (int o = 0; o < kVSeg; ++o) {
       const int y = g.y0 + r0 + o;
       if (x >= g.W || y >= g.H) continue;
-      const float mu_p = st[0][o], mu_t = st[1][o];
-      const float mu_p2 = mu_p * mu_p, mu_t2 = mu_t * mu_t, mu_pt = mu_p * mu_t;
-      const float s_pp = st[2][o] - mu_p2;
-      const float s_tt = st[3][o] - mu_t2;
-      const float s_pt = st[4][o] - mu_pt;
-      const float A1 = 2.0f * mu_pt + kC1;
-      const float A2 = 2.0f * s_pt + kC2;
-      const float B1 = mu_p2 + mu_t2 + kC1;
-      const float B2 = s_pp + s_tt + kC2;
-      // B1 >= C1 > 0 and B2 is C2 plus two variances: the hardware reciprocal (1 ulp) is safe.
-      const float inv_d = __builtin_amdgcn_rcpf(B1 * B2);
-      const float S = A1 * A2 * inv_d;
-      const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
-      ssim_sum += S;
-      if (a.keep) {
-        const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
-        const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
-        a.abc[at] = da;
-        a.abc[a.pixels * 3 + at] = -S * inv_b2;
-        a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
+      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+      if constexpr (kTarget) {
+        a.tmom_out[at] = st[0][o];
+        a.tmom_out[a.pixels * 3 + at] = st[1][o];
+      } else {
+        Moments m;
+        if constexpr (kMode == kStatsFull) {
+          m = Moments{st[0][o], st[1][o], st[2][o], st[3][o], st[4][o]};
+        } else {
+          m = Moments{st[0][o], a.tmom[at], st[1][o], a.tmom[a.pixels * 3 + at], st[2][o]};
+        }
+        const float mu_p = m.mu_p, mu_t = m.mu_t;
+        const float mu_p2 = mu_p * mu_p, mu_t2 = mu_t * mu_t, mu_pt = mu_p * mu_t;
+        const float s_pp = m.m_pp - mu_p2;
+        const float s_tt = m.m_tt - mu_t2;
+        const float s_pt = m.m_pt - mu_pt;
+        const float A1 = 2.0f * mu_pt + kC1;
+        const float A2 = 2.0f * s_pt + kC2;
+        const float B1 = mu_p2 + mu_t2 + kC1;
+        const float B2 = s_pp + s_tt + kC2;
+        // B1 >= C1 > 0 and B2 is C2 plus two variances: the hardware reciprocal (1 ulp) is safe.
+        const float inv_d = __builtin_amdgcn_rcpf(B1 * B2);
+        const float S = A1 * A2 * inv_d;
+        const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
+        ssim_sum += S;
+        if (a.keep) {
+          const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
+          a.abc[at] = da;
+          a.abc[a.pixels * 3 + at] = -S * inv_b2;
+          a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
+        }
       }
     }
   }
+  if constexpr (kTarget) return;
   const float l1_tot = block_sum(l1);
   __syncthreads();
   const float ssim_tot = block_sum(ssim_sum);
@@ -438,7 +485,7 @@ Window gaussian_window() {
 // Validates the common arguments and fills `a` (everything but the per-call outputs).
 int prepare_loss(const FieldTable& table, int64_t tiles, int64_t pixels, const float* img,
                  const float* target, double ssim_weight, void* ws, size_t ws_bytes,
-                 LossArgs* a) {
+                 LossArgs* a, const float* target_moments = nullptr) {
   if (img == nullptr || target == nullptr) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "image and target must not be NULL");
   }
@@ -460,6 +507,7 @@ int prepare_loss(const FieldTable& table, int64_t tiles, int64_t pixels, const f
   a->fields = table;
   a->img = img;
   a->target = target;
+  a->tmom = target_moments;
   a->abc = l.abc;
   a->partial = l.partial;
   a->pixels = pixels;
@@ -475,7 +523,9 @@ int run_loss_forward(LossArgs a, int num_fields, int64_t tiles, double ssim_weig
   if (loss == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "loss is NULL");
   a.keep = keep;
   const int grid = static_cast<int>(tiles);
-  if (ssim_weight > 0.0) {
+  if (ssim_weight > 0.0 && a.tmom != nullptr) {
+    k_loss_stats<true, kStatsMoments><<<grid, kLossThreads, 0, s>>>(a);
+  } else if (ssim_weight > 0.0) {
     k_loss_stats<true><<<grid, kLossThreads, 0, s>>>(a);
   } else {
     k_loss_stats<false><<<grid, kLossThreads, 0, s>>>(a);
@@ -483,6 +533,26 @@ int run_loss_forward(LossArgs a, int num_fields, int64_t tiles, double ssim_weig
   SSPLAT_LAUNCH_CHECK("k_loss_stats");
   k_loss_total<<<1, 1024, 0, s>>>(a.partial, tiles, ssim_weight * num_fields, loss);
   SSPLAT_LAUNCH_CHECK("k_loss_total");
+  return SSPLAT_OK;
+}
+
+// The target's window moments G*t and G*t^2 into `moments` (2 maps of (pixels, 3)).
+int run_target_moments(const FieldTable& table, int64_t tiles, int64_t pixels,
+                       const float* target, float* moments, hipStream_t s) {
+  if (target == nullptr || moments == nullptr) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "target and moments must not be NULL");
+  }
+  if (tiles <= 0 || tiles > INT32_MAX || pixels <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the images must have pixels");
+  }
+  LossArgs a{};
+  a.fields = table;
+  a.target = target;
+  a.pixels = pixels;
+  a.tmom_out = moments;
+  a.win = gaussian_window();
+  k_loss_stats<true, kStatsTarget><<<static_cast<int>(tiles), kLossThreads, 0, s>>>(a);
+  SSPLAT_LAUNCH_CHECK("k_loss_stats");
   return SSPLAT_OK;
 }
 
@@ -558,7 +628,8 @@ int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int
   return success();
 }
 
-int ssplat_l1_ssim_loss_forward(const float* image, const float* target, int height, int width,
+int ssplat_l1_ssim_loss_forward(const float* image, const float* target,
+                                const float* target_moments, int height, int width,
                                 double ssim_weight, float* loss, void* workspace,
                                 size_t workspace_bytes, void* stream) {
   int rc;
@@ -567,11 +638,36 @@ int ssplat_l1_ssim_loss_forward(const float* image, const float* target, int hei
   const FieldTable t = single_image(height, width, &tiles);
   LossArgs a;
   if ((rc = prepare_loss(t, tiles, int64_t(height) * width, image, target, ssim_weight,
-                         workspace, workspace_bytes, &a))) {
+                         workspace, workspace_bytes, &a, target_moments))) {
     return rc;
   }
   if ((rc = run_loss_forward(a, 1, tiles, ssim_weight, loss, true,
                              static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_loss_target_moments(const float* target, int height, int width, float* moments,
+                               void* stream) {
+  int rc;
+  if ((rc = check_size(height, width))) return rc;
+  int64_t tiles;
+  const FieldTable t = single_image(height, width, &tiles);
+  if ((rc = run_target_moments(t, tiles, int64_t(height) * width, target, moments,
+                               static_cast<hipStream_t>(stream)))) {
+    return rc;
+  }
+  return success();
+}
+
+int ssplat_batch_loss_target_moments(const ssplat_field* fields, int num_fields,
+                                     int64_t total_tiles, int64_t total_pixels,
+                                     const float* target, float* moments, void* stream) {
+  int rc;
+  if ((rc = check_table(fields, num_fields))) return rc;
+  if ((rc = run_target_moments(FieldTable{fields, num_fields, ssplat_field{}}, total_tiles,
+                               total_pixels, target, moments, static_cast<hipStream_t>(stream)))) {
     return rc;
   }
   return success();
@@ -622,7 +718,8 @@ int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_
 
 int ssplat_batch_l1_ssim_loss_forward(const ssplat_field* fields, int num_fields,
                                       int64_t total_tiles, int64_t total_pixels,
-                                      const float* image, const float* target, double ssim_weight,
+                                      const float* image, const float* target,
+                                      const float* target_moments, double ssim_weight,
                                       float* loss, void* workspace, size_t workspace_bytes,
                                       void* stream) {
   int rc;
@@ -630,7 +727,7 @@ int ssplat_batch_l1_ssim_loss_forward(const ssplat_field* fields, int num_fields
   LossArgs a;
   if ((rc = prepare_loss(FieldTable{fields, num_fields, ssplat_field{}}, total_tiles,
                          total_pixels, image, target, ssim_weight, workspace, workspace_bytes,
-                         &a))) {
+                         &a, target_moments))) {
     return rc;
   }
   if ((rc = run_loss_forward(a, num_fields, total_tiles, ssim_weight, loss, true,

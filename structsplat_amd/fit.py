@@ -31,7 +31,7 @@ from torch.autograd.function import once_differentiable
 
 from . import render as HR
 from .gaussians import GaussianField
-from .loss import l1_ssim_loss
+from .loss import l1_ssim_loss, target_moments
 from .optim import FusedAdam
 from .tune import tune_field
 from .metrics import psnr_tensor
@@ -225,10 +225,12 @@ def raw_weight_map_field(field: GaussianField, cfg: FitConfig, H: int, W: int,
                              sigma_cutoff=cfg.sigma_cutoff, detach_opacities=detach_opacities)
 
 
-def loss_fn(img: torch.Tensor, target: torch.Tensor, w: float) -> torch.Tensor:
+def loss_fn(img: torch.Tensor, target: torch.Tensor, w: float,
+            moments: torch.Tensor | None = None) -> torch.Tensor:
     """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1"), computed with its
-    gradient by the fused HIP loss kernels."""
-    return l1_ssim_loss(img, target, w)
+    gradient by the fused HIP loss kernels; ``moments``: the target's window moments
+    (``loss.target_moments``), computed once per fit."""
+    return l1_ssim_loss(img, target, w, moments)
 
 
 def pair_count(field: GaussianField, H: int, W: int, cfg: FitConfig, tile_size: int = 16) -> int:
@@ -314,12 +316,14 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
 
     plan = new_plan(_capacity(pair_count(field, H, W, cfg, ts), cfg))
     state = {"loss": None}
+    target = target.contiguous()
+    moments = target_moments(target) if cfg.ssim_weight > 0.0 else None
 
     def step():
         # Gradients are assigned, not accumulated: no fill and add kernels per parameter.
         opt.zero_grad(set_to_none=True)
         img = planned_render(field, plan, cfg)
-        loss = loss_fn(img, target, cfg.ssim_weight)
+        loss = loss_fn(img, target, cfg.ssim_weight, moments)
         loss.backward(_seed(state, loss))
         add_color_grad_l2(field, cfg)
         opt.step()  # Adam and the log-scale clamp

@@ -93,3 +93,55 @@ def test_batched_loss_is_the_sum_of_field_losses(cuda):
         # same tiles, same arithmetic: the batched gradient equals the single-image one
         assert torch.equal(x.grad[batch.pixel_slices[k]].view_as(y), y.grad), k
     assert abs(float(loss) - total) <= 1e-6 * max(1.0, abs(total))
+
+
+@pytest.mark.parametrize("H,W", [(37, 53), (1, 9), (130, 70)])
+def test_target_moments_match_float64_blur(cuda, H, W):
+    """G*t and G*t^2 against the float64 separable blur (metrics.py's zero-padded window)."""
+    from structsplat_amd.loss import target_moments
+    from structsplat_amd.metrics import _blur, _to_bchw, gaussian_window
+
+    print(f"case: target moments H={H} W={W}", flush=True)
+    _, t = _pair(H * 7 + W, H, W, cuda)
+    mom = target_moments(t)
+    torch.cuda.synchronize()
+    t64 = _to_bchw(t.detach().cpu().double())
+    wh, wv = gaussian_window(11, 1.5, "cpu", torch.float64)
+    for k, x in enumerate((t64, t64 * t64)):
+        ref = _blur(x, wh, wv, 5)[0].permute(1, 2, 0)
+        err = float((mom[k].cpu().double() - ref).abs().max())
+        print("moment", k, "max abs err", err)
+        assert err <= 2e-7
+
+
+@pytest.mark.parametrize("w", [0.2, 1.0])
+def test_loss_with_target_moments_is_bitwise_the_inline_loss(cuda, w):
+    """The forward that reads precomputed target moments gives the same loss and gradient, bit
+    for bit, as the one that blurs the target itself (single image and batch)."""
+    from structsplat_amd.loss import batch_target_moments, target_moments
+
+    p, t = _pair(11, 70, 45, cuda)
+    x = p.clone().requires_grad_(True)
+    a = l1_ssim_loss(x, t, w)
+    a.backward()
+    y = p.clone().requires_grad_(True)
+    b = l1_ssim_loss(y, t, w, target_moments(t))
+    b.backward()
+    torch.cuda.synchronize()
+    assert float(a) == float(b)
+    assert torch.equal(x.grad, y.grad)
+
+    sizes = [(20, 33), (64, 64), (5, 70)]
+    pairs = [_pair(20 + k, h, ww, cuda) for k, (h, ww) in enumerate(sizes)]
+    batch = B.Batch(sizes, [0] * len(sizes), cuda)
+    img = torch.cat([q.reshape(-1, 3) for q, _ in pairs]).contiguous()
+    tgt = torch.cat([r.reshape(-1, 3) for _, r in pairs]).contiguous()
+    x = img.clone().requires_grad_(True)
+    a = batch_l1_ssim_loss(batch, x, tgt, w)
+    a.backward()
+    y = img.clone().requires_grad_(True)
+    b = batch_l1_ssim_loss(batch, y, tgt, w, batch_target_moments(batch, tgt))
+    b.backward()
+    torch.cuda.synchronize()
+    assert float(a) == float(b)
+    assert torch.equal(x.grad, y.grad)

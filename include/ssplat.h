@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SSPLAT_ABI_VERSION 5  /* 5: loss target moments (ssplat_loss_target_moments) */
+#define SSPLAT_ABI_VERSION 6  /* 6: loss units (ssplat_loss_plan), target moments */
 
 /*
  * Render schedule flag, OR-ed into the tile_size argument of the render entry points (the plain
@@ -402,6 +402,16 @@ int ssplat_sspl1_decode(const ssplat_sspl1* layout, const uint8_t* means_raw,
  * only).  ssim_weight w is in [0, 1].  The loss is summed in a fixed order (deterministic).
  */
 int ssplat_loss_workspace_bytes(int64_t total_pixels, int64_t total_tiles, size_t* bytes);
+
+/*
+ * Host-only: completes a HOST table of fields for the loss calls (the loss works in units, strips
+ * 16 columns wide and up to 128 rows tall): first_pixel, first_tile (the field's first unit),
+ * tiles_x (its strips) and num_tiles (its units); height and width are the caller's.  Returns the
+ * total units (the total_tiles of the loss calls, and of ssplat_loss_workspace_bytes for one
+ * image), the total pixels and the loss workspace size.
+ */
+int ssplat_loss_plan(ssplat_field* fields, int num_fields, int64_t* total_units,
+                     int64_t* total_pixels, size_t* workspace_bytes);
 int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int width,
                         double ssim_weight, float* loss, float* grad_image, void* workspace,
                         size_t workspace_bytes, void* stream);
@@ -431,7 +441,7 @@ int ssplat_l1_ssim_loss_backward(const float* image, const float* target, int he
                                  void* workspace, size_t workspace_bytes, void* stream);
 
 /* Batched loss: the sum of every field's loss over concatenated images (a field table planned
-   with tile_size 16 by ssplat_batch_plan, copied to the device), its two-call form and the
+   by ssplat_loss_plan, copied to the device; total_tiles = its units), its two-call form and the
    target moments of every field (2 maps of (total_pixels, 3)). */
 int ssplat_batch_l1_ssim_loss(const ssplat_field* fields, int num_fields, int64_t total_tiles,
                               int64_t total_pixels, const float* image, const float* target,

@@ -74,6 +74,8 @@ SIGNATURES = {
     "ssplat_kernel_timing_read": (_c_int, [_c_int, ctypes.POINTER(_c_i64),
                                            ctypes.POINTER(_c_dbl)]),
     "ssplat_loss_workspace_bytes": (_c_int, [_c_i64, _c_i64, ctypes.POINTER(_c_size)]),
+    "ssplat_loss_plan": (_c_int, [_c_ptr, _c_int, ctypes.POINTER(_c_i64),
+                                  ctypes.POINTER(_c_i64), ctypes.POINTER(_c_size)]),
     "ssplat_l1_ssim_loss": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_int, _c_dbl, _c_ptr, _c_ptr,
                                      _c_ptr, _c_size, _c_ptr]),
     "ssplat_batch_l1_ssim_loss": (_c_int, [_c_ptr, _c_int, _c_i64, _c_i64, _c_ptr, _c_ptr,

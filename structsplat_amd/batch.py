@@ -27,7 +27,7 @@ from torch.autograd.function import once_differentiable
 from . import _native
 from . import fit as F
 from .gaussians import GaussianField
-from .loss import batch_l1_ssim_loss, batch_target_moments
+from .loss import batch_l1_ssim_loss, batch_target_moments, loss_plan
 from .metrics import psnr_tensor
 from .tune import tune_field, tune_tile_size
 from .render import _fade_alpha, _ptr, _stream, grad_slots, tile_edge
@@ -58,8 +58,8 @@ def _device_table(table, device):
 class Batch:
     """Planned layout of a batch of fields of sizes `sizes` [(H, W)] with `counts` Gaussians,
     rendered with `tile_size`-pixel tiles (16 or 32, optionally with the TILE_PERSISTENT
-    schedule flag: every field's tiles in one persistent launch).  The loss always tiles by 16
-    pixels (``loss_fields_dev`` / ``loss_total_tiles``)."""
+    schedule flag: every field's tiles in one persistent launch).  The loss has its own table
+    (``loss_fields_dev``, ``loss_total_tiles`` units of ``loss.loss_plan``)."""
 
     def __init__(self, sizes, counts, device, tile_size: int = 16):
         nf = len(sizes)
@@ -76,11 +76,8 @@ class Batch:
         self.total_pixels = px
         self.fields_dev = _device_table(table, self.device)
         self.workspace = torch.empty(max(wsb, 256), dtype=torch.uint8, device=self.device)
-        if self.tile_size == 16:
-            self.loss_fields_dev, self.loss_total_tiles = self.fields_dev, t
-        else:
-            table16, _, t16, _, _ = _plan(sizes, counts, 16)
-            self.loss_fields_dev, self.loss_total_tiles = _device_table(table16, self.device), t16
+        loss_table, self.loss_total_tiles, _, _ = loss_plan(sizes)
+        self.loss_fields_dev = _device_table(loss_table, self.device)
         self.sizes = [(int(h), int(w)) for h, w in sizes]
         self.gauss_slices = [slice(table[i].first_gaussian,
                                    table[i].first_gaussian + table[i].num_gaussians)

@@ -16,12 +16,15 @@
 // and, the window being symmetric, dSSIM/dp = (G*a + 2 p G*b + t G*c) / n for the three maps
 // a, b, c above (zero outside the image).
 //
-// Kernels, one 256-thread workgroup per 16x16 tile (the render tiles of the field table):
-//   k_loss_stats : stages p and t with a 5-pixel halo in LDS, blurs p, t, p^2, t^2, p t
-//                  horizontally then vertically (sliding windows in registers), writes a, b, c,
-//                  the L1 gradient (without SSIM) and the tile's partial loss.
-//   k_loss_grad  : stages a, b, c with the halo, blurs them and adds the SSIM gradient.
-//   k_loss_total : one workgroup sums the tile partials in a fixed order (deterministic loss).
+// Kernels, one 192-thread workgroup per unit, a strip 16 columns wide and up to 128 rows tall
+// (ssplat_loss_plan numbers the units of every field):
+//   k_loss_stats : walks down the strip 16 rows at a time: stages p and t with a 5-pixel halo
+//                  in LDS, blurs p, p^2, p t (and t, t^2 without precomputed target moments)
+//                  horizontally into a ring of rows, then vertically, writes a, b, c and the
+//                  unit's partial loss.
+//   k_loss_grad  : the same walk over a, b, c; adds the SSIM gradient to the L1 gradient.
+//   k_loss_l1    : the loss and gradient without SSIM.
+//   k_loss_total : one workgroup sums the unit partials in a fixed order (deterministic loss).
 
 #include <cmath>
 #include <cstdint>
@@ -32,19 +35,39 @@ namespace {
 
 using namespace ssplat_detail;
 
-constexpr int kLT = 16;               // tile edge
-constexpr int kHalo = 5;              // window radius
-constexpr int kWin = 2 * kHalo + 1;   // 11 taps
-constexpr int kSpan = kLT + 2 * kHalo;  // 26: staged rows / columns
-constexpr int kLoadRow = 3 * kSpan;     // 78 floats per staged row (3 channels)
+// Geometry.  A unit of work is 16 columns by 32 rows of one image (clipped by the image), one
+// 256-thread workgroup.  The stats pass blurs the unit's 42 staged rows horizontally (42 / 32 rows
+// per output row; a 16 x 16 tile would blur 26 / 16) and its 32 output rows vertically.  The
+// gradient pass works on the unit's two 16 x 16 halves in turn (three staged maps of a 16 x 32
+// unit would not leave room in LDS for more than three workgroups per CU).
+constexpr int kUnitW = 16;
+constexpr int kUnitH = 32;
+constexpr int kHalo = 5;                     // window radius
+constexpr int kWin = 2 * kHalo + 1;          // 11 taps
+constexpr int kSpanX = kUnitW + 2 * kHalo;   // 26 staged columns
+constexpr int kLoadRow = 3 * kSpanX;         // 78 floats per staged row (3 channels)
 constexpr int kLossThreads = 256;
-constexpr int kHSeg = 8;              // horizontal pass: outputs per thread
-constexpr int kVSeg = 4;              // vertical pass: outputs per thread
-constexpr int kHTasks = 3 * kSpan * (kLT / kHSeg);  // 156 (channel, row, segment) tasks
-constexpr int kVTasks = 3 * kLT * (kLT / kVSeg);    // 192 (channel, column, segment) tasks
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
+
+// Stats pass (the whole unit).
+constexpr int kSpanY = kUnitH + 2 * kHalo;   // 42 staged rows
+constexpr int kStageStride = 28;             // floats per staged row: 26 used, 16-byte rows
+constexpr int kBlurStride = 18;              // floats per blurred row: 16 used, bank spread
+constexpr int kHSeg = 8;                     // horizontal pass: outputs per task
+constexpr int kVSeg = 8;                     // vertical pass: outputs per task
+constexpr int kHTasks = 3 * kSpanY * (kUnitW / kHSeg);  // 252 (channel, row, segment)
+constexpr int kVTasks = 3 * kUnitW * (kUnitH / kVSeg);  // 192 (channel, column, segment)
 static_assert(kHTasks <= kLossThreads && kVTasks <= kLossThreads, "one task per thread");
+
+// Gradient pass (one 16 x 16 half at a time).
+constexpr int kGT = 16;                      // half edge
+constexpr int kGSpan = kGT + 2 * kHalo;      // 26 staged rows / columns
+constexpr int kGHSeg = 8;
+constexpr int kGVSeg = 4;
+constexpr int kGHTasks = 3 * kGSpan * (kGT / kGHSeg);  // 156
+constexpr int kGVTasks = 3 * kGT * (kGT / kGVSeg);     // 192
+static_assert(kGHTasks <= kLossThreads && kGVTasks <= kLossThreads, "one task per thread");
 
 // What the stats pass blurs.  The target's window moments mu_t = G*t and G*t^2 do not change
 // while a fit runs, so a fit computes them once (kStatsTarget) and its steps read them
@@ -61,7 +84,7 @@ struct Window {
 };
 
 struct LossArgs {
-  FieldTable fields;
+  FieldTable fields;    // units: first_tile = first unit, tiles_x = units per row, num_tiles
   const float* img;     // (pixels, 3)
   const float* target;  // (pixels, 3)
   float* grad;          // (pixels, 3): the gradient (backward)
@@ -70,25 +93,25 @@ struct LossArgs {
   float* abc;           // 3 maps of (pixels, 3): dS/dmu_p, dS/dm2, dS/dmpt
   const float* tmom;    // the target's moments, 2 maps of (pixels, 3): G*t, G*t^2 (or NULL)
   float* tmom_out;      // kStatsTarget: where they are written
-  float* partial;       // one float per tile
+  float* partial;       // one float per unit
   int64_t pixels;       // pixels over all fields
   float w_l1;           // 1 - ssim_weight
   float w_ssim;         // ssim_weight
   Window win;
 };
 
-struct TileGeo {
+struct UnitGeo {
   int x0, y0, W, H;
   int64_t first_pixel;
   float inv_n;  // 1 / (3 H W)
 };
 
-__device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
-  const ssplat_field f = field_of_tile(a.fields, tile);
-  const int local = tile - static_cast<int>(f.first_tile);
-  TileGeo g;
-  g.x0 = (local % f.tiles_x) * kLT;
-  g.y0 = (local / f.tiles_x) * kLT;
+__device__ __forceinline__ UnitGeo unit_geo(const LossArgs& a, int unit) {
+  const ssplat_field f = field_of_tile(a.fields, unit);
+  const int local = unit - static_cast<int>(f.first_tile);
+  UnitGeo g;
+  g.x0 = (local % f.tiles_x) * kUnitW;
+  g.y0 = (local / f.tiles_x) * kUnitH;
   g.W = f.width;
   g.H = f.height;
   g.first_pixel = f.first_pixel;
@@ -96,67 +119,11 @@ __device__ __forceinline__ TileGeo tile_geo(const LossArgs& a, int tile) {
   return g;
 }
 
-// Staging of rows y0-5 .. y0+20, columns x0-5 .. x0+20 of 3-channel maps into LDS as
-// [map][ch][row][col] (row stride kSpan + 1), zero outside the image.  Each staged row is 78
-// contiguous floats; thread t < 234 owns float t % 78 of rows t / 78, t / 78 + 3, ... (9 passes),
-// so the divisions happen once per thread.  All loads are issued before the LDS stores (one
-// memory latency per staging); a position outside the image loads element 0 of the map, which
-// always exists, and stores 0.
-constexpr int kStageRowsPerPass = kLossThreads / kLoadRow;                      // 3
-constexpr int kStagePasses = (kSpan + kStageRowsPerPass - 1) / kStageRowsPerPass;  // 9
-constexpr int kMapFloats = 3 * kSpan * (kSpan + 1);  // one staged 3-channel map
-
-// Stages kMaps maps into lds[m * kMapFloats + ...]; returns sum |map0 - map1| over the tile's own
-// pixels when kL1 (the L1 term of the loss).
-template <int kMaps, bool kL1>
-__device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], const TileGeo& g,
-                                            float* lds) {
-  const int t = threadIdx.x;
-  const bool active = t < kStageRowsPerPass * kLoadRow;
-  const int r0 = t / kLoadRow;
-  const int c3 = t - r0 * kLoadRow;
-  const int col = c3 / 3;
-  const int ch = c3 - col * 3;
-  const int gx = g.x0 - kHalo + col;
-  const bool x_ok = active && gx >= 0 && gx < g.W;
-  const bool x_center = col >= kHalo && col < kHalo + kLT;
-  // Row stride in the map: W pixels of 3 floats.
-  const int64_t row_stride = int64_t(g.W) * 3;
-  const int64_t at0 = (g.first_pixel + int64_t(g.y0 - kHalo + r0) * g.W + gx) * 3 + ch;
-  const int lds0 = (ch * kSpan + r0) * (kSpan + 1) + col;
-  float v[kMaps][kStagePasses];
-  bool ok[kStagePasses];
-#pragma unroll
-  for (int k = 0; k < kStagePasses; ++k) {
-    const int row = r0 + kStageRowsPerPass * k;
-    const int gy = g.y0 - kHalo + row;
-    ok[k] = x_ok && row < kSpan && gy >= 0 && gy < g.H;
-    const int64_t at = ok[k] ? at0 + int64_t(kStageRowsPerPass * k) * row_stride : 0;
-#pragma unroll
-    for (int m = 0; m < kMaps; ++m) {
-      const float x = src[m][at];
-      v[m][k] = ok[k] ? x : 0.0f;
-    }
-  }
-  float l1 = 0.0f;
-#pragma unroll
-  for (int k = 0; k < kStagePasses; ++k) {
-    const int row = r0 + kStageRowsPerPass * k;
-    if (!active || row >= kSpan) continue;
-    const int off = lds0 + kStageRowsPerPass * k * (kSpan + 1);
-#pragma unroll
-    for (int m = 0; m < kMaps; ++m) lds[m * kMapFloats + off] = v[m][k];
-    if (kL1 && ok[k] && x_center && row >= kHalo && row < kHalo + kLT) {
-      l1 += fabsf(v[0][k] - v[1][k]);
-    }
-  }
-  return l1;
-}
-
-// Sum of `vals` over the workgroup (returned to thread 0).
+// Sum over the workgroup (returned to thread 0).  Every thread calls it.
 __device__ float block_sum(float v) {
   __shared__ float s_red[kLossThreads / 64];
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  __syncthreads();  // earlier readers of s_red are done
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
   __syncthreads();
   float t = 0.0f;
@@ -166,58 +133,117 @@ __device__ float block_sum(float v) {
   return t;
 }
 
-// The window's sums at one output: mu = (mu_p, mu_t), m = (G*p^2, G*t^2, G*(p t)).
+// Staging of rows y0 - 5 .. y0 + kRows + 4, columns x0 - 5 .. x0 + 20 of kMaps 3-channel maps
+// into LDS as [map][channel][row][column] (row stride kStride), zero outside the image.  Each
+// staged row is 78 contiguous floats; thread t < 234 owns float t % 78 of rows t / 78,
+// t / 78 + 3, ...  All loads are issued before the LDS stores (one memory latency per staging);
+// a position outside the image loads element 0 of the map and stores 0.  With kL1, returns
+// sum |map0 - map1| over the staged floats of rows y0 .. y0 + kRows - 1 and columns x0 .. x0 + 15
+// inside the image.
+template <int kMaps, int kRows, int kStride, bool kL1>
+__device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], int x0, int y0,
+                                            const UnitGeo& g, float* lds) {
+  constexpr int kSpan = kRows + 2 * kHalo;
+  constexpr int kPerPass = kLossThreads / kLoadRow;                 // 3 rows per pass
+  constexpr int kPasses = (kSpan + kPerPass - 1) / kPerPass;
+  constexpr int kMapFloats = 3 * kSpan * kStride;
+  const int t = threadIdx.x;
+  const bool active = t < kPerPass * kLoadRow;
+  const int r0 = t / kLoadRow;
+  const int c3 = t - r0 * kLoadRow;
+  const int col = c3 / 3;
+  const int ch = c3 - col * 3;
+  const int gx = x0 - kHalo + col;
+  const bool x_ok = active && gx >= 0 && gx < g.W;
+  const bool x_center = col >= kHalo && col < kHalo + kUnitW;
+  const int64_t row_stride = int64_t(g.W) * 3;
+  const int64_t at0 = (g.first_pixel + int64_t(y0 - kHalo + r0) * g.W + gx) * 3 + ch;
+  const int lds0 = (ch * kSpan + r0) * kStride + col;
+  float v[kMaps][kPasses];
+  bool ok[kPasses];
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) {
+    const int row = r0 + kPerPass * k;
+    const int gy = y0 - kHalo + row;
+    ok[k] = x_ok && row < kSpan && gy >= 0 && gy < g.H;
+    const int64_t at = ok[k] ? at0 + int64_t(kPerPass * k) * row_stride : 0;
+#pragma unroll
+    for (int m = 0; m < kMaps; ++m) {
+      const float x = src[m][at];
+      v[m][k] = ok[k] ? x : 0.0f;
+    }
+  }
+  float l1 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kPasses; ++k) {
+    const int row = r0 + kPerPass * k;
+    if (!active || row >= kSpan) continue;
+    const int off = lds0 + kPerPass * k * kStride;
+#pragma unroll
+    for (int m = 0; m < kMaps; ++m) lds[m * kMapFloats + off] = v[m][k];
+    if (kL1 && ok[k] && x_center && row >= kHalo && row < kHalo + kRows) {
+      l1 += fabsf(v[0][k] - v[1][k]);
+    }
+  }
+  return l1;
+}
+
+// The window sums at one output: mu_p, mu_t, G*p^2, G*t^2, G*(p t).
 struct Moments {
   float mu_p, mu_t, m_pp, m_tt, m_pt;
 };
 
-template <bool kSSIM, int kMode = kStatsFull>
+// SSIM statistics of every unit: L1 and SSIM partial sums, and (keep) the maps a, b, c of the
+// gradient; or, for kStatsTarget, the target's moments.
+template <int kMode>
 __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
   constexpr int kQ = kQuantities<kMode>;
   constexpr bool kTarget = kMode == kStatsTarget;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const TileGeo g = tile_geo(a, tile);
+  constexpr int kMaps = kTarget ? 1 : 2;
+  constexpr int kMapFloats = 3 * kSpanY * kStageStride;
+  constexpr int kStaged = kMaps * kMapFloats;
+  constexpr int kBlurred = kQ * 3 * kSpanY * kBlurStride;
+  __shared__ __align__(16) float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);
+  const UnitGeo g = unit_geo(a, unit);
   float l1 = 0.0f, ssim_sum = 0.0f;
-  if (!kSSIM) {
-    // L1 only: one pixel per thread.
-    const int x = g.x0 + (threadIdx.x & (kLT - 1));
-    const int y = g.y0 + (threadIdx.x >> 4);
-    if (x < g.W && y < g.H) {
-      const int64_t base = (g.first_pixel + int64_t(y) * g.W + x) * 3;
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) l1 += fabsf(a.img[base + ch] - a.target[base + ch]);
-    }
-    const float t = block_sum(l1);
-    if (threadIdx.x == 0) a.partial[tile] = a.w_l1 * g.inv_n * t;
-    return;
-  }
-
-  // LDS: the staged p and t (t alone for the target's moments), then (reusing the same space)
-  // the horizontally blurred quantities.
-  constexpr int kStaged = (kTarget ? 1 : 2) * kMapFloats;
-  constexpr int kBlurred = kQ * 3 * kSpan * (kLT + 1);
-  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
   if constexpr (kTarget) {
     const float* const src[1] = {a.target};
-    stage_halo<1, false>(src, g, s_buf);
+    stage_halo<1, kUnitH, kStageStride, false>(src, g.x0, g.y0, g, s_buf);
   } else {
     const float* const src[2] = {a.img, a.target};
-    l1 = stage_halo<2, true>(src, g, s_buf);
+    l1 = stage_halo<2, kUnitH, kStageStride, true>(src, g.x0, g.y0, g, s_buf);
   }
   __syncthreads();
 
-  // Horizontal pass into registers: task = (channel, staged row, 8-column segment); input-major
-  // so each product is formed once.
-  const float (*s_p)[kSpan][kSpan + 1] = reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf);
-  const float (*s_t)[kSpan][kSpan + 1] =
-      reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + (kTarget ? 0 : kMapFloats));
+  // Horizontal pass into registers: task = (channel, staged row, 8-column segment); each row's
+  // 18 inputs are read as four float4 and a float2, and each product is formed once.
   float h[kQ][kHSeg];
   const bool h_task = threadIdx.x < kHTasks;
-  const int h_ch = threadIdx.x / (kSpan * 2);
-  const int h_rem = threadIdx.x - h_ch * (kSpan * 2);
+  const int h_ch = threadIdx.x / (kSpanY * 2);
+  const int h_rem = threadIdx.x - h_ch * (kSpanY * 2);
   const int h_row = h_rem >> 1;
   const int h_c0 = (h_rem & 1) * kHSeg;
   if (h_task) {
+    float in[kMaps][kHSeg + kWin - 1];
+#pragma unroll
+    for (int m = 0; m < kMaps; ++m) {
+      const float* row = s_buf + m * kMapFloats + (h_ch * kSpanY + h_row) * kStageStride + h_c0;
+      float tmp[20];
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const float4 x = *reinterpret_cast<const float4*>(row + 4 * q4);
+        tmp[4 * q4] = x.x;
+        tmp[4 * q4 + 1] = x.y;
+        tmp[4 * q4 + 2] = x.z;
+        tmp[4 * q4 + 3] = x.w;
+      }
+      const float2 x = *reinterpret_cast<const float2*>(row + 16);
+      tmp[16] = x.x;
+      tmp[17] = x.y;
+#pragma unroll
+      for (int i = 0; i < kHSeg + kWin - 1; ++i) in[m][i] = tmp[i];
+    }
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
 #pragma unroll
@@ -225,13 +251,13 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     }
 #pragma unroll
     for (int i = 0; i < kHSeg + kWin - 1; ++i) {
-      const float t = s_t[h_ch][h_row][h_c0 + i];
       float val[kQ];
       if constexpr (kMode == kStatsTarget) {
+        const float t = in[0][i];
         val[0] = t;
         val[1] = t * t;
       } else {
-        const float p = s_p[h_ch][h_row][h_c0 + i];
+        const float p = in[0][i], t = in[kMaps - 1][i];
         if constexpr (kMode == kStatsFull) {
           val[0] = p;
           val[1] = t;
@@ -255,21 +281,38 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     }
   }
   __syncthreads();  // every read of the staged maps is done
-  float (*s_h)[3][kSpan][kLT + 1] = reinterpret_cast<float (*)[3][kSpan][kLT + 1]>(s_buf);
+  float (*s_h)[3][kSpanY][kBlurStride] = reinterpret_cast<float (*)[3][kSpanY][kBlurStride]>(s_buf);
   if (h_task) {
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
 #pragma unroll
-      for (int o = 0; o < kHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
+      for (int o = 0; o < kHSeg; o += 2) {
+        *reinterpret_cast<float2*>(&s_h[q][h_ch][h_row][h_c0 + o]) =
+            make_float2(h[q][o], h[q][o + 1]);
+      }
     }
   }
   __syncthreads();
 
-  // Vertical pass: task = (channel, column, 4-row segment); then S and its derivatives.
+  // Vertical pass: task = (channel, column, 8-row segment); then S and its derivatives.
   if (threadIdx.x < kVTasks) {
     const int ch = threadIdx.x >> 6;
-    const int col = threadIdx.x & (kLT - 1);
+    const int col = threadIdx.x & (kUnitW - 1);
     const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
+    const int x = g.x0 + col;
+    // The target's moments at the outputs, loaded before the blur so it hides their latency.
+    float om[2][kVSeg];
+    if constexpr (kMode == kStatsMoments) {
+#pragma unroll
+      for (int o = 0; o < kVSeg; ++o) {
+        const int y = g.y0 + r0 + o;
+        const bool ok = x < g.W && y < g.H;
+        const int64_t at = ok ? (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch : 0;
+        const float m0 = a.tmom[at], m1 = a.tmom[a.pixels * 3 + at];
+        om[0][o] = m0;
+        om[1][o] = m1;
+      }
+    }
     float st[kQ][kVSeg];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -284,7 +327,6 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
         st[q][o] = acc;
       }
     }
-    const int x = g.x0 + col;
 #pragma unroll
     for (int o = 0; o < kVSeg; ++o) {
       const int y = g.y0 + r0 + o;
@@ -294,11 +336,14 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
         a.tmom_out[at] = st[0][o];
         a.tmom_out[a.pixels * 3 + at] = st[1][o];
       } else {
+        // No contraction into fused multiply-adds here: both ways of obtaining the target's
+        // moments then give the same S and derivatives bit for bit.
+#pragma clang fp contract(off)
         Moments m;
         if constexpr (kMode == kStatsFull) {
           m = Moments{st[0][o], st[1][o], st[2][o], st[3][o], st[4][o]};
         } else {
-          m = Moments{st[0][o], a.tmom[at], st[1][o], a.tmom[a.pixels * 3 + at], st[2][o]};
+          m = Moments{st[0][o], om[0][o], st[1][o], om[1][o], st[2][o]};
         }
         const float mu_p = m.mu_p, mu_t = m.mu_t;
         const float mu_p2 = mu_p * mu_p, mu_t2 = mu_t * mu_t, mu_pt = mu_p * mu_t;
@@ -309,126 +354,147 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
         const float A2 = 2.0f * s_pt + kC2;
         const float B1 = mu_p2 + mu_t2 + kC1;
         const float B2 = s_pp + s_tt + kC2;
-        // B1 >= C1 > 0 and B2 is C2 plus two variances: the hardware reciprocal (1 ulp) is safe.
+        // B1 >= C1 > 0 and B2 is C2 plus two variances: the hardware reciprocal is safe.
         const float inv_d = __builtin_amdgcn_rcpf(B1 * B2);
         const float S = A1 * A2 * inv_d;
         const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
         ssim_sum += S;
         if (a.keep) {
-          const float da = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
-          a.abc[at] = da;
+          a.abc[at] = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
           a.abc[a.pixels * 3 + at] = -S * inv_b2;
           a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
         }
       }
     }
   }
-  if constexpr (kTarget) return;
-  const float l1_tot = block_sum(l1);
-  __syncthreads();
-  const float ssim_tot = block_sum(ssim_sum);
-  if (threadIdx.x == 0) {
-    a.partial[tile] = g.inv_n * (a.w_l1 * l1_tot - a.w_ssim * ssim_tot);
+  if constexpr (!kTarget) {
+    const float l1_tot = block_sum(l1);
+    const float ssim_tot = block_sum(ssim_sum);
+    if (threadIdx.x == 0) {
+      a.partial[unit] = g.inv_n * (a.w_l1 * l1_tot - a.w_ssim * ssim_tot);
+    }
   }
 }
 
-// dL/dp = scale (w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n); without SSIM
-// only the L1 term, one pixel per thread.
-template <bool kSSIM>
-__global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const TileGeo g = tile_geo(a, tile);
-  const float scale = a.scale != nullptr ? *a.scale : 1.0f;
-  if (!kSSIM) {
-    const int x = g.x0 + (threadIdx.x & (kLT - 1));
-    const int y = g.y0 + (threadIdx.x >> 4);
-    if (x < g.W && y < g.H) {
-      const int64_t base = (g.first_pixel + int64_t(y) * g.W + x) * 3;
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float d = a.img[base + ch] - a.target[base + ch];
-        const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-        a.grad[base + ch] = scale * (a.w_l1 * g.inv_n * sg);
-      }
-    }
-    return;
-  }
-  // LDS: the staged a, b, c, then (reusing the same space) their horizontal blurs.
-  constexpr int kStaged = 3 * kMapFloats;
-  constexpr int kBlurred = 3 * 3 * kSpan * (kLT + 1);
-  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
-  {
-    const float* const src[3] = {a.abc, a.abc + a.pixels * 3, a.abc + 2 * a.pixels * 3};
-    stage_halo<3, false>(src, g, s_buf);
-  }
-  __syncthreads();
-  float h[3][kHSeg];
-  const bool h_task = threadIdx.x < kHTasks;
-  const int h_ch = threadIdx.x / (kSpan * 2);
-  const int h_rem = threadIdx.x - h_ch * (kSpan * 2);
-  const int h_row = h_rem >> 1;
-  const int h_c0 = (h_rem & 1) * kHSeg;
-  if (h_task) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const float (*s_m)[kSpan][kSpan + 1] =
-          reinterpret_cast<const float (*)[kSpan][kSpan + 1]>(s_buf + q * kMapFloats);
-      float v[kHSeg + kWin - 1];
-#pragma unroll
-      for (int i = 0; i < kHSeg + kWin - 1; ++i) v[i] = s_m[h_ch][h_row][h_c0 + i];
-#pragma unroll
-      for (int o = 0; o < kHSeg; ++o) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
-        h[q][o] = acc;
-      }
-    }
-  }
-  __syncthreads();  // every read of the staged maps is done
-  float (*s_h)[3][kSpan][kLT + 1] = reinterpret_cast<float (*)[3][kSpan][kLT + 1]>(s_buf);
-  if (h_task) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-#pragma unroll
-      for (int o = 0; o < kHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < kVTasks) {
-    const int ch = threadIdx.x >> 6;
-    const int col = threadIdx.x & (kLT - 1);
-    const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
-    float bl[3][kVSeg];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      float v[kVSeg + kWin - 1];
-#pragma unroll
-      for (int i = 0; i < kVSeg + kWin - 1; ++i) v[i] = s_h[q][ch][r0 + i][col];
-#pragma unroll
-      for (int o = 0; o < kVSeg; ++o) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
-        bl[q][o] = acc;
-      }
-    }
-    const int x = g.x0 + col;
-#pragma unroll
-    for (int o = 0; o < kVSeg; ++o) {
-      const int y = g.y0 + r0 + o;
-      if (x >= g.W || y >= g.H) continue;
-      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
-      const float p = a.img[at], t = a.target[at];
-      const float d = p - t;
+// L1 alone (ssim_weight 0): the unit's partial sum of |p - t|, or (backward) its gradient.
+__global__ __launch_bounds__(kLossThreads) void k_loss_l1(LossArgs a, bool forward) {
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);
+  const UnitGeo g = unit_geo(a, unit);
+  const float scale = (!forward && a.scale != nullptr) ? *a.scale : 1.0f;
+  float l1 = 0.0f;
+  for (int e = threadIdx.x; e < kUnitH * kUnitW * 3; e += kLossThreads) {
+    const int j = e / (kUnitW * 3);
+    const int c3 = e - j * (kUnitW * 3);
+    const int x = g.x0 + c3 / 3;
+    const int y = g.y0 + j;
+    if (x >= g.W || y >= g.H) continue;
+    const int64_t at = (g.first_pixel + int64_t(y) * g.W + g.x0) * 3 + c3;
+    const float d = a.img[at] - a.target[at];
+    if (forward) {
+      l1 += fabsf(d);
+    } else {
       const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-      const float dssim = bl[0][o] + 2.0f * p * bl[1][o] + t * bl[2][o];
-      a.grad[at] = scale * (g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim));
+      a.grad[at] = scale * (a.w_l1 * g.inv_n * sg);
+    }
+  }
+  if (forward) {
+    const float t = block_sum(l1);
+    if (threadIdx.x == 0) a.partial[unit] = a.w_l1 * g.inv_n * t;
+  }
+}
+
+// dL/dp = scale (w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n); two workgroups per
+// unit, one per 16 x 16 half.
+__global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
+  constexpr int kMapFloats = 3 * kGSpan * (kGSpan + 1);
+  constexpr int kStaged = 3 * kMapFloats;
+  constexpr int kBlurred = 3 * 3 * kGSpan * (kGT + 1);
+  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const UnitGeo g = unit_geo(a, wg >> 1);
+  const int y0 = g.y0 + (wg & 1) * kGT;
+  if (y0 >= g.H) return;  // a unit clipped to one half (uniform)
+  const float scale = a.scale != nullptr ? *a.scale : 1.0f;
+  const float* const src[3] = {a.abc, a.abc + a.pixels * 3, a.abc + 2 * a.pixels * 3};
+  const int h_ch = threadIdx.x / (kGSpan * 2);
+  const int h_rem = threadIdx.x - h_ch * (kGSpan * 2);
+  const int h_row = h_rem >> 1;
+  const int h_c0 = (h_rem & 1) * kGHSeg;
+  {
+    stage_halo<3, kGT, kGSpan + 1, false>(src, g.x0, y0, g, s_buf);
+    __syncthreads();
+    float h[3][kGHSeg];
+    if (threadIdx.x < kGHTasks) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float (*s_m)[kGSpan][kGSpan + 1] =
+            reinterpret_cast<const float (*)[kGSpan][kGSpan + 1]>(s_buf + q * kMapFloats);
+        float v[kGHSeg + kWin - 1];
+#pragma unroll
+        for (int i = 0; i < kGHSeg + kWin - 1; ++i) v[i] = s_m[h_ch][h_row][h_c0 + i];
+#pragma unroll
+        for (int o = 0; o < kGHSeg; ++o) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
+          h[q][o] = acc;
+        }
+      }
+    }
+    __syncthreads();  // every read of the staged maps is done
+    float (*s_h)[3][kGSpan][kGT + 1] = reinterpret_cast<float (*)[3][kGSpan][kGT + 1]>(s_buf);
+    if (threadIdx.x < kGHTasks) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int o = 0; o < kGHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kGVTasks) {
+      const int ch = threadIdx.x >> 6;
+      const int col = threadIdx.x & (kGT - 1);
+      const int r0 = ((threadIdx.x >> 4) & 3) * kGVSeg;
+      const int x = g.x0 + col;
+      float pv[kGVSeg], tv[kGVSeg];  // p and t at the outputs, loaded before the blur
+#pragma unroll
+      for (int o = 0; o < kGVSeg; ++o) {
+        const int y = y0 + r0 + o;
+        const bool ok = x < g.W && y < g.H;
+        const int64_t at = ok ? (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch : 0;
+        pv[o] = a.img[at];
+        tv[o] = a.target[at];
+      }
+      float bl[3][kGVSeg];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        float v[kGVSeg + kWin - 1];
+#pragma unroll
+        for (int i = 0; i < kGVSeg + kWin - 1; ++i) v[i] = s_h[q][ch][r0 + i][col];
+#pragma unroll
+        for (int o = 0; o < kGVSeg; ++o) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int k = 0; k < kWin; ++k) acc = fmaf(a.win.w[k], v[o + k], acc);
+          bl[q][o] = acc;
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kGVSeg; ++o) {
+        const int y = y0 + r0 + o;
+        if (x >= g.W || y >= g.H) continue;
+        const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+        const float p = pv[o], t = tv[o];
+        const float d = p - t;
+        const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+        const float dssim = bl[0][o] + 2.0f * p * bl[1][o] + t * bl[2][o];
+        a.grad[at] = scale * (g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim));
+      }
     }
   }
 }
 
-// loss = w_ssim * num_fields + sum of the tile partials, summed in float64 in a fixed order.
+// loss = w_ssim * num_fields + sum of the unit partials, summed in float64 in a fixed order.
 __global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ partial,
                                                      int64_t tiles, double constant,
                                                      float* __restrict__ loss) {
@@ -492,7 +558,7 @@ int prepare_loss(const FieldTable& table, int64_t tiles, int64_t pixels, const f
   if (!(std::isfinite(ssim_weight) && ssim_weight >= 0.0 && ssim_weight <= 1.0)) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "ssim_weight must be in [0, 1]");
   }
-  if (tiles <= 0 || tiles > INT32_MAX || pixels <= 0) {
+  if (tiles <= 0 || tiles > INT32_MAX / 2 || pixels <= 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the images must have pixels");
   }
   const LossLayout l = loss_layout(ws, pixels, tiles);
@@ -524,11 +590,11 @@ int run_loss_forward(LossArgs a, int num_fields, int64_t tiles, double ssim_weig
   a.keep = keep;
   const int grid = static_cast<int>(tiles);
   if (ssim_weight > 0.0 && a.tmom != nullptr) {
-    k_loss_stats<true, kStatsMoments><<<grid, kLossThreads, 0, s>>>(a);
+    k_loss_stats<kStatsMoments><<<grid, kLossThreads, 0, s>>>(a);
   } else if (ssim_weight > 0.0) {
-    k_loss_stats<true><<<grid, kLossThreads, 0, s>>>(a);
+    k_loss_stats<kStatsFull><<<grid, kLossThreads, 0, s>>>(a);
   } else {
-    k_loss_stats<false><<<grid, kLossThreads, 0, s>>>(a);
+    k_loss_l1<<<grid, kLossThreads, 0, s>>>(a, true);
   }
   SSPLAT_LAUNCH_CHECK("k_loss_stats");
   k_loss_total<<<1, 1024, 0, s>>>(a.partial, tiles, ssim_weight * num_fields, loss);
@@ -542,7 +608,7 @@ int run_target_moments(const FieldTable& table, int64_t tiles, int64_t pixels,
   if (target == nullptr || moments == nullptr) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "target and moments must not be NULL");
   }
-  if (tiles <= 0 || tiles > INT32_MAX || pixels <= 0) {
+  if (tiles <= 0 || tiles > INT32_MAX / 2 || pixels <= 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the images must have pixels");
   }
   LossArgs a{};
@@ -551,7 +617,7 @@ int run_target_moments(const FieldTable& table, int64_t tiles, int64_t pixels,
   a.pixels = pixels;
   a.tmom_out = moments;
   a.win = gaussian_window();
-  k_loss_stats<true, kStatsTarget><<<static_cast<int>(tiles), kLossThreads, 0, s>>>(a);
+  k_loss_stats<kStatsTarget><<<static_cast<int>(tiles), kLossThreads, 0, s>>>(a);
   SSPLAT_LAUNCH_CHECK("k_loss_stats");
   return SSPLAT_OK;
 }
@@ -562,22 +628,32 @@ int run_loss_backward(LossArgs a, int64_t tiles, double ssim_weight, const float
   a.grad = grad;
   a.scale = scale;
   const int grid = static_cast<int>(tiles);
+  static_assert(kUnitH == 2 * kGT, "two gradient workgroups per unit");
   if (ssim_weight > 0.0) {
-    k_loss_grad<true><<<grid, kLossThreads, 0, s>>>(a);
+    k_loss_grad<<<2 * grid, kLossThreads, 0, s>>>(a);
   } else {
-    k_loss_grad<false><<<grid, kLossThreads, 0, s>>>(a);
+    k_loss_l1<<<grid, kLossThreads, 0, s>>>(a, false);
   }
   SSPLAT_LAUNCH_CHECK("k_loss_grad");
   return SSPLAT_OK;
 }
 
-FieldTable single_image(int height, int width, int64_t* tiles) {
+// Units of one field: kUnitW columns by kUnitH rows, row-major.
+int64_t plan_units(ssplat_field& f, int64_t first_unit, int64_t first_pixel) {
+  const int64_t strips = (int64_t(f.width) + kUnitW - 1) / kUnitW;
+  const int64_t segs = (int64_t(f.height) + kUnitH - 1) / kUnitH;
+  f.first_pixel = first_pixel;
+  f.first_tile = first_unit;
+  f.tiles_x = static_cast<int32_t>(strips);
+  f.num_tiles = static_cast<int32_t>(strips * segs);
+  return strips * segs;
+}
+
+FieldTable single_image(int height, int width, int64_t* units) {
   ssplat_field f{};
   f.height = height;
   f.width = width;
-  f.tiles_x = (width + kLT - 1) / kLT;
-  f.num_tiles = f.tiles_x * ((height + kLT - 1) / kLT);
-  *tiles = int64_t(f.tiles_x) * ((height + kLT - 1) / kLT);
+  *units = plan_units(f, 0, 0);
   return FieldTable{nullptr, 0, f};
 }
 
@@ -598,6 +674,25 @@ int check_table(const ssplat_field* fields, int num_fields) {
 }  // namespace
 
 extern "C" {
+
+int ssplat_loss_plan(ssplat_field* fields, int num_fields, int64_t* total_units,
+                     int64_t* total_pixels, size_t* workspace_bytes) {
+  int rc;
+  if (fields == nullptr || num_fields <= 0) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "the field table must hold at least one field");
+  }
+  int64_t units = 0, px = 0;
+  for (int i = 0; i < num_fields; ++i) {
+    if ((rc = check_size(fields[i].height, fields[i].width))) return rc;
+    units += plan_units(fields[i], units, px);
+    px += int64_t(fields[i].height) * fields[i].width;
+    if (units > INT32_MAX / 2) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "too many loss units");
+  }
+  if (total_units) *total_units = units;
+  if (total_pixels) *total_pixels = px;
+  if (workspace_bytes) *workspace_bytes = loss_layout(nullptr, px, units).bytes;
+  return success();
+}
 
 int ssplat_loss_workspace_bytes(int64_t total_pixels, int64_t total_tiles, size_t* bytes) {
   if (bytes == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "bytes is NULL");

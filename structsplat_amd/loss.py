@@ -30,6 +30,22 @@ def _stream(dev):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
+def loss_plan(sizes):
+    """Completed host field table of the loss for images of `sizes` [(H, W)] and (units,
+    pixels, workspace bytes): the loss kernels work in units, strips 16 columns wide and up to
+    128 rows tall."""
+    nf = len(sizes)
+    table = (_native.Field * nf)()
+    for i, (h, w) in enumerate(sizes):
+        table[i].height = int(h)
+        table[i].width = int(w)
+    units, px, wsb = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_size_t(0)
+    _native.check(_native.load().ssplat_loss_plan(ctypes.addressof(table), nf, ctypes.byref(units),
+                                                   ctypes.byref(px), ctypes.byref(wsb)),
+                  "ssplat_loss_plan")
+    return table, int(units.value), int(px.value), int(wsb.value)
+
+
 def loss_workspace(pixels: int, tiles: int, device) -> torch.Tensor:
     n = ctypes.c_size_t(0)
     _native.check(_native.load().ssplat_loss_workspace_bytes(int(pixels), int(tiles),
@@ -107,8 +123,8 @@ class _L1SSIM(Function):
         lib = _native.load()
         if batch is None:
             H, W = int(img.shape[0]), int(img.shape[1])
-            tiles = ((H + 15) // 16) * ((W + 15) // 16)
-            ws = loss_workspace(H * W, tiles, dev)
+            _, units, _, _ = loss_plan([(H, W)])
+            ws = loss_workspace(H * W, units, dev)
             _native.check(lib.ssplat_l1_ssim_loss_forward(
                 _ptr(img), _ptr(target), _ptr(moments), H, W, float(ssim_weight), _ptr(loss),
                 _ptr(ws), ws.numel(), _stream(dev)), "ssplat_l1_ssim_loss_forward")

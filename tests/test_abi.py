@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_workspace_query():
     lib = _native.load()
-    assert lib.ssplat_abi_version() == 5
+    assert lib.ssplat_abi_version() == 6
     nbytes = _native.workspace_bytes(1000, 256, 256, 16)
     # 1000 records of 64 B + (256 tiles + 1) counters + scan sums + status, 256-byte aligned.
     assert nbytes >= 1000 * 64 + 257 * 4

@@ -2,10 +2,10 @@
 
 A step is one pass of the renderer's hot path over the resident field: tile binning (no host
 synchronization), tiled forward, and the tiled backward of the default autograd path, against a
-fixed synthetic dL/dout.  By default the forward stores the per-pixel normalizer and the backward
-reads it with the image (the reference's saved den/out); the variant whose backward recomputes
-the normalizer with a second sweep of the tile lists is timed beside it (`--normalizer recompute`
-makes it the step).  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+fixed synthetic dL/dout.  By default the backward recomputes the per-pixel normalizer with a
+second sweep of the tile lists (a denominator-only sweep) instead of storing it, and reads the
+forward's image; the variant that stores the normalizer (the reference's saved den/out) is timed
+beside it (`--normalizer stored` makes it the step).  Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
 baseline).  `value` is the algorithmic HBM throughput of the whole step, by the workload's fixed
 byte model:
 
@@ -407,9 +407,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--normalizer", default="stored", choices=["stored", "recompute"],
-                    help="stored: the forward saves den for the backward (the reference's "
-                         "data flow); recompute: the backward recomputes it")
+    ap.add_argument("--normalizer", default="recompute", choices=["recompute", "stored"],
+                    help="recompute: the backward recomputes den (the default); stored: the "
+                         "forward saves den for the backward (the reference's data flow)")
     ap.add_argument("--tile-size", default="auto", choices=["auto", "16", "32", "272", "288"],
                     help="render tile edge (272 / 288: 16 / 32 with the persistent schedule); "
                          "auto = the autotuner's choice on this field")

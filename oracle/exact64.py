@@ -190,3 +190,64 @@ def backward(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
         g_q[:, 1] += np.bincount(gid, weights=dq * 2.0 * dx * dy, minlength=n)
         g_q[:, 2] += np.bincount(gid, weights=dq * dy * dy, minlength=n)
     return g_m, g_q, g_c, g_o
+
+def backward_conditioning(grad_out, means, conics, colors, radii, opacities, H: int, W: int,
+                          normalize: bool = True, eps: float = 1e-8, support_fade: bool = False,
+                          sigma_cutoff: float = 3.0, rects=None, float64_params: bool = False):
+    """Per-Gaussian float32 sensitivity of the backward, shaped like ``backward``'s gradients: a
+    float32 evaluation of the per-pixel chain above errs in a gradient by a few units of float32
+    rounding times this value.  Per visit, with dw = sum_k dN_k c_k + dD and its rounding
+    relative to dw_abs = sum_k |dN_k c_k| + |dD| (large where the Gaussian's color is near the
+    pixel's output, whose terms cancel), and raw's rounding relative to rs = 1 + qabs/2 (as in
+    ``forward_conditioning``; plus tail/raw under support fade):
+        dq carries sq = o raw/2 (dw_abs + |dw| rs)
+        means:  sq |2a dx + 2b dy| + |dq| (|2a dx| + |2b dy|)   (and the y analogue)
+        conics: sq dx^2, sq 2|dx dy|, sq dy^2
+        colors: |dN_k| w rs;   opacity: base (dw_abs + |dw| rs)
+    summed in absolute value over the Gaussian's pixels.  The pixel offsets dx = px - mx are
+    exact in float32 for the integer pixels and float32 means used here.
+    """
+    m, q, col, op, rect = _prepare(means, conics, colors, radii, opacities, H, W, rects,
+                                   float64_params)
+    fade, tail = _tail(support_fade, sigma_cutoff)
+    G = np.asarray(grad_out, dtype=np.float64).reshape(H * W, 3)
+    if normalize:
+        num, den = accumulate(means, conics, colors, radii, opacities, H, W, support_fade,
+                              sigma_cutoff, rect, float64_params)
+        denom = den + eps
+        out = num / denom[:, None]
+        dN = G / denom[:, None]
+        dD = -(G * out).sum(axis=1) / denom
+        dD_abs = np.abs(G * out).sum(axis=1) / denom
+    else:
+        dN = G
+        dD = np.zeros(H * W)
+        dD_abs = np.zeros(H * W)
+    n = m.shape[0]
+    s_m = np.zeros((n, 2))
+    s_q = np.zeros((n, 3))
+    s_c = np.zeros((n, 3))
+    s_o = np.zeros(n)
+    for gid, px, py in _visit_chunks(*rect):
+        dx, dy, a, b, c, raw, base, w = _weights(m, q, op, gid, px, py, fade, tail)
+        flat = py * W + px
+        qabs = np.abs(a) * dx * dx + 2.0 * np.abs(b * dx * dy) + np.abs(c) * dy * dy
+        rs = 1.0 + 0.5 * qabs + (tail / np.maximum(raw, 1e-300) if fade else 0.0)
+        dw = (dN[flat] * col[gid]).sum(axis=1) + dD[flat]
+        dw_abs = np.abs(dN[flat] * col[gid]).sum(axis=1) + dD_abs[flat]
+        live = ~(fade & (raw <= tail)) if fade else np.ones_like(raw, dtype=bool)
+        o = np.abs(op[gid])
+        sq = np.where(live, o * 0.5 * raw * (dw_abs + np.abs(dw) * rs), 0.0)
+        dq = np.where(live, np.abs(dw) * o * 0.5 * raw, 0.0)
+        for k in range(3):
+            s_c[:, k] += np.bincount(gid, weights=np.abs(dN[flat, k]) * np.abs(w) * rs,
+                                     minlength=n)
+        s_o += np.bincount(gid, weights=np.abs(base) * (dw_abs + np.abs(dw) * rs), minlength=n)
+        s_m[:, 0] += np.bincount(gid, weights=sq * np.abs(2.0 * a * dx + 2.0 * b * dy)
+                                 + dq * (np.abs(2.0 * a * dx) + np.abs(2.0 * b * dy)), minlength=n)
+        s_m[:, 1] += np.bincount(gid, weights=sq * np.abs(2.0 * b * dx + 2.0 * c * dy)
+                                 + dq * (np.abs(2.0 * b * dx) + np.abs(2.0 * c * dy)), minlength=n)
+        s_q[:, 0] += np.bincount(gid, weights=sq * dx * dx, minlength=n)
+        s_q[:, 1] += np.bincount(gid, weights=sq * 2.0 * np.abs(dx * dy), minlength=n)
+        s_q[:, 2] += np.bincount(gid, weights=sq * dy * dy, minlength=n)
+    return s_m, s_q, s_c, s_o

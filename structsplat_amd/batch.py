@@ -168,7 +168,7 @@ class BatchPlan:
     """Pair buffer, offsets, sticky status and gradient slots of a batch (capturable binning
     and backward); the slot buffer has the pair buffer's capacity."""
 
-    def __init__(self, batch: Batch, capacity: int, recompute_normalizer: bool = False):
+    def __init__(self, batch: Batch, capacity: int, recompute_normalizer: bool = True):
         self.batch = batch
         self.capacity = max(int(capacity), 1)
         self.recompute_normalizer = bool(recompute_normalizer)

@@ -46,7 +46,7 @@ class FitConfig:
     lr_color: float = 5e-3
     lr_opacity: float = 5e-3
     ssim_weight: float = 0.2       # loss = (1 - w) * L1 + w * (1 - SSIM)
-    recompute_normalizer: bool = False  # True: the backward recomputes den instead of reading it
+    recompute_normalizer: bool = True  # backward recomputes den instead of storing it
     sigma_cutoff: float = 3.0
     support_fade: bool | float = False  # bool, or a fade strength in [0, 1]
     normalization_eps: float = 1e-8
@@ -65,7 +65,7 @@ class RenderPlan:
     count without ellipse cull); an overflow of either sets the status word."""
 
     def __init__(self, n: int, H: int, W: int, capacity: int, device, tile_size: int = 16,
-                 recompute_normalizer: bool = False, affine: bool = False):
+                 recompute_normalizer: bool = True, affine: bool = False):
         if affine and int(tile_size) != HR.DEFAULT_TILE_SIZE:
             raise ValueError("affine-color plans use 16-pixel tiles")
         # tile_size: the edge, optionally with the TILE_PERSISTENT schedule flag.

@@ -13,10 +13,10 @@ phrases and routes every mode to the gfx950 kernels of ``libssplat_hip.so`` thro
 * ``render_hip_exact_with_coverage(...)`` - ``render_cuda_exact_with_coverage``
   (cuda_render.py:229-253): normalized RGB plus raw coverage, no autograd.
 
-By default the forward saves the per-pixel normalizer ``den`` and the image ``out`` for the
-backward, as the reference does (render_ext.cu:972-1008).  ``recompute_normalizer=True`` saves
-only the image and recomputes ``den`` in the backward with a second sweep of the tile lists
-(bit-identical to the forward's): 4 bytes a pixel less memory for that sweep's arithmetic.  Other
+By default the backward recomputes the per-pixel normalizer ``den`` with a second sweep of the
+tile lists (bit-identical to the forward's) instead of storing it, and reads only the saved
+image; ``recompute_normalizer=False`` saves ``den`` and ``out`` as the reference does
+(render_ext.cu:972-1008), which skips that sweep for 4 bytes a pixel of memory.  Other
 differences by design: tile lists are built without a radix sort
 (ascending Gaussian id per tile, like the reference's stable sort); the forward is bitwise
 deterministic.
@@ -366,9 +366,9 @@ class _ExactRenderHip(Function):
     """Autograd boundary of the HIP renderer (the role of ``_ExactRenderCuda``,
     cuda_render.py:52-174): gradients for means, conics, colors and opacities only.
 
-    ``den`` and ``out`` are saved, as the reference does; with ``recompute_normalizer`` the
-    per-pixel normalizer ``den`` is not stored: the backward recomputes it from the tile lists
-    (bit-identical to the forward's) and reads only the saved image."""
+    With ``recompute_normalizer`` (the default) the per-pixel normalizer ``den`` is not stored:
+    the backward recomputes it from the tile lists (bit-identical to the forward's) and reads
+    only the saved image.  Otherwise ``den`` and ``out`` are saved, as the reference does."""
 
     @staticmethod
     def forward(ctx, means, conics, colors, radii, opacities, height, width, normalize, eps,
@@ -542,7 +542,7 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
                      tile_size: int = DEFAULT_TILE_SIZE, support_fade: bool | float = False,
                      sigma_cutoff: float = 3.0, backward_variant: str = "baseline",
                      tile_index_backend: str = "hip", tile_ellipse_cull: bool = True,
-                     recompute_normalizer: bool = False):
+                     recompute_normalizer: bool = True):
     """Exact normalized/additive render on the GPU (signature and checks of
     ``render_cuda_exact``, cuda_render.py:256-294, plus ``recompute_normalizer``).
 
@@ -555,9 +555,8 @@ def render_hip_exact(means, conics, colors, radii, H: int, W: int, opacities=Non
     default, whose untiled kernels compute the same sums) ``tile_size`` is ignored as the
     reference ignores it.  A tiled ``tile_size`` may be any edge with ``tile_size^2 <= 1024``
     (render_ext.cpp:162-163); see ``render_tile_size`` for the edge the kernels run.
-    By default the per-pixel normalizer is saved for the backward with the image, as the
-    reference saves them; ``recompute_normalizer=True`` keeps only the image and recomputes the
-    normalizer in the backward.  ``tile_index_backend`` accepts
+    ``recompute_normalizer`` (default) keeps only the image for the backward and recomputes the
+    per-pixel normalizer there; ``False`` saves it too, as the reference does.  ``tile_index_backend`` accepts
     ``"hip"`` and the reference's ``"cuda"`` and ``"torch"`` (cuda_render.py:285-288); all three
     bin on the GPU, since the torch builder's lists hold the same Gaussians per tile
     (test_render.py:401-409) and the ascending order is one of its orders.

@@ -57,7 +57,7 @@ def _time_ms(fn, reps: int) -> float:
 
 @torch.no_grad()
 def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, normalize=True,
-                   eps=1e-8, support_fade=False, sigma_cutoff=3.0, recompute_normalizer=False,
+                   eps=1e-8, support_fade=False, sigma_cutoff=3.0, recompute_normalizer=True,
                    candidates=CANDIDATE_TILE_SIZES, reps=3):
     """Fastest tile configuration (edge, with render.TILE_PERSISTENT for the persistent
     schedule) for rendering (fwd+bwd) the fields `sizes` [(H, W)] with `counts` Gaussians each

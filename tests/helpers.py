@@ -128,6 +128,30 @@ def local_gradient(host, H, W, G, i, normalize):
     return [x[k] for x in g], (k, Gw, h, w, mw, qw, cw, rw, ow, rects)
 
 
+def local_gradient_conditioning(host, H, W, G, i, normalize):
+    """Gaussian i's float32 sensitivity (``exact64.backward_conditioning``) on its own support
+    rectangle, shaped like ``local_gradient``'s gradients."""
+    _, (k, Gw, h, w, mw, qw, cw, rw, ow, rects) = local_gradient(host, H, W, G, i, normalize)
+    s = exact64.backward_conditioning(Gw, mw, qw, cw, rw, ow, h, w, normalize, 1e-8, False, 3.0,
+                                      rects=rects, float64_params=True)
+    return [x[k] for x in s]
+
+
+def grad_bar_err(got, ref, sens, rel: float = 1e-4, units: float = 64.0):
+    """The gradients' bar as a ratio (pass <= 1): every value must satisfy
+    ``|got - ref| <= max(rel * max |ref|, units * u * sens)``, with ``sens`` from
+    ``exact64.backward_conditioning``.  Where the float32 chain is well conditioned this is the
+    ``rel`` bar relative to the array's scale; where its terms cancel (needle-like Gaussians, a
+    Gaussian's color near the pixels' output) it is what any float32 evaluation can meet."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    scale = float(np.max(np.abs(ref))) if ref.size else 0.0
+    allow = np.maximum(rel * scale, units * F32_UNIT * np.asarray(sens, dtype=np.float64))
+    diff = np.abs(got - ref)
+    ratio = np.where(allow > 0, diff / np.maximum(allow, 1e-300), np.where(diff > 0, np.inf, 0))
+    return float(np.max(ratio)) if ratio.size else 0.0
+
+
 def sample_gaussians(host, H, W, count, seed):
     """`count` random visible Gaussians (ascending ids)."""
     m, r = _host_np(host[0]), _host_np(host[3])

@@ -26,8 +26,9 @@ from structsplat_amd import fit as F
 from structsplat_amd import render as HR
 from structsplat_amd import synth
 from structsplat_amd.gaussians import GaussianField
-from tests.helpers import (local_gradient, pixel_bar_err, pixel_rel_err, sample_gaussians,
-                           scaled_err, to_np, window_field)
+from tests.helpers import (grad_bar_err, local_gradient, local_gradient_conditioning,
+                           pixel_bar_err, pixel_rel_err, sample_gaussians, scaled_err, to_np,
+                           window_field)
 
 pytestmark = pytest.mark.gpu
 
@@ -170,15 +171,22 @@ def test_full_fit_graph_equals_eager_and_gradients_match_fp64(cuda):
     gpu = [to_np(t.grad) for t in leaves]
     an = {k: [] for k in GRAD_NAMES}
     got = {k: [] for k in GRAD_NAMES}
+    sens = {k: [] for k in GRAD_NAMES}
     for i in sample_gaussians(host, H, W, 48, seed=17):
         analytic, _ = local_gradient(host, H, W, Gn, i, True)
+        cond = local_gradient_conditioning(host, H, W, Gn, i, True)
         for slot, key in enumerate(GRAD_NAMES):
             an[key].append(np.atleast_1d(analytic[slot]))
             got[key].append(np.atleast_1d(gpu[slot][i]))
+            sens[key].append(np.atleast_1d(cond[slot]))
+    # A fitted field holds needle-like Gaussians (scales at the 0.35 floor and the cap) whose
+    # float32 chain cancels: the bar is 1e-4 of the scale or 64 float32 units of the gradient's
+    # own float32 sensitivity, whichever is larger.
     for key in GRAD_NAMES:
         e = scaled_err(np.array(got[key]), np.array(an[key]))
-        print(key, "scaled err", e)
-        assert e <= 1e-4, key
+        bar = grad_bar_err(np.array(got[key]), np.array(an[key]), np.array(sens[key]))
+        print(key, "scaled err", e, "bar", bar)
+        assert bar <= 1.0, key
 
 
 @pytest.mark.timeout(900)

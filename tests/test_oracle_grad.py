@@ -114,3 +114,82 @@ def test_float64_params_forward_matches_per_gaussian_loop(normalize, fade):
     assert not np.array_equal(out, rounded)
     with pytest.raises(ValueError):
         exact64.forward(*params[:3], radii, params[3], H, W, float64_params=True)
+
+
+U32 = 2.0 ** -24  # float32 unit roundoff
+
+
+def _f32_render_and_grads(f, G, H, W, normalize, fade):
+    """The reference's own float32 evaluation (the torch restatement in float32, autograd)."""
+    means, conics, colors, radii, op = field_tensors(f, "cpu")
+    leaves = [t.clone().requires_grad_() for t in (means, conics, colors)]
+    opl = op.clone().requires_grad_() if op is not None else None
+    img = R.accumulate(*leaves, radii, H, W, 1, opl, normalize, bool(fade), 3.0)
+    (img * torch.from_numpy(G.astype(np.float32))).sum().backward()
+    grads = [t.grad.double().numpy() for t in leaves]
+    grads.append(opl.grad.double().numpy() if opl is not None else None)
+    return img.detach().double().numpy(), grads, (means, conics, colors, radii, op)
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+@pytest.mark.parametrize("fade", [False, True])
+@pytest.mark.parametrize("uniform_colors", [False, True])
+def test_conditioning_bounds_the_float32_error(normalize, fade, uniform_colors):
+    """forward_conditioning and backward_conditioning bound the error of the reference's own
+    float32 evaluation against float64: |f32 - f64| <= 64 u sens for every pixel and gradient.
+    Needle-like Gaussians (aspect up to 40) make the quadratic's terms cancel; equal colors make
+    the normalized per-pixel chain dw = sum G (c - out) / D cancel to zero in exact arithmetic,
+    so a bound without those cancellation terms fails here."""
+    H, W = 21, 27
+    f = random_field(5, 40, H, W, opacity=True, aniso=(0.4, 1.2, 4.0, 16.0))
+    if uniform_colors:
+        f["colors"][:] = np.float32([0.3, 0.6, 0.9])
+    G = np.random.default_rng(9).standard_normal((H, W, 3))
+    img32, g32, t32 = _f32_render_and_grads(f, G, H, W, normalize, fade)
+    args = tuple(to_np(t) for t in t32)
+    img64, _ = exact64.forward(*args, H, W, normalize, 1e-8, fade, 3.0)
+    sens_f = exact64.forward_conditioning(*args, H, W, normalize, 1e-8, fade, 3.0)
+    assert np.all(np.abs(img32 - img64) <= 64 * U32 * sens_f + 1e-30)
+    g64 = exact64.backward(G, *args, H, W, normalize, 1e-8, fade, 3.0)
+    sens_b = exact64.backward_conditioning(G, *args, H, W, normalize, 1e-8, fade, 3.0)
+    for name, a, b, s in zip(("means", "conics", "colors", "opacities"), g32, g64, sens_b):
+        err = np.abs(a - b)
+        assert np.all(err <= 64 * U32 * s + 1e-30), (name, float(np.max(err / (U32 * s + 1e-300))))
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+def test_conditioning_bounds_the_float32_error_of_needles(normalize):
+    """Extreme needles (minor scale 0.08, major up to 24, any angle): the quadratic's terms are
+    thousands of times q itself, so raw's float32 rounding is relative to qabs, not to q; a bound
+    without the qabs term fails here."""
+    H, W = 25, 25
+    f = random_field(6, 12, H, W, opacity=True, aniso=(0.08, 0.08, 8.0, 24.0))
+    G = np.random.default_rng(10).standard_normal((H, W, 3))
+    img32, g32, t32 = _f32_render_and_grads(f, G, H, W, normalize, False)
+    args = tuple(to_np(t) for t in t32)
+    img64, _ = exact64.forward(*args, H, W, normalize, 1e-8, False, 3.0)
+    sens_f = exact64.forward_conditioning(*args, H, W, normalize, 1e-8, False, 3.0)
+    assert np.all(np.abs(img32 - img64) <= 64 * U32 * sens_f + 1e-30)
+    g64 = exact64.backward(G, *args, H, W, normalize, 1e-8, False, 3.0)
+    sens_b = exact64.backward_conditioning(G, *args, H, W, normalize, 1e-8, False, 3.0)
+    for name, a, b, s in zip(("means", "conics", "colors", "opacities"), g32, g64, sens_b):
+        err = np.abs(a - b)
+        assert np.all(err <= 64 * U32 * s + 1e-30), (name, float(np.max(err / (U32 * s + 1e-300))))
+
+
+def test_backward_conditioning_is_close_to_the_gradient_where_well_conditioned():
+    """Round Gaussians, additive, all-positive G: no cancellation anywhere, so the sensitivity is
+    a small multiple of the gradient's own magnitude (the bound is not vacuous)."""
+    H, W = 17, 19
+    f = random_field(3, 20, H, W, scale_lo=1.0, scale_hi=2.0, opacity=True)
+    f["angles"][:] = 0.0
+    f["scales"][:, 1] = f["scales"][:, 0]
+    G = np.random.default_rng(4).random((H, W, 3))
+    means, conics, colors, radii, op = (to_np(t) for t in field_tensors(f, "cpu"))
+    g64 = exact64.backward(G, means, conics, colors, radii, op, H, W, False, 1e-8, False, 3.0)
+    s = exact64.backward_conditioning(G, means, conics, colors, radii, op, H, W, False, 1e-8,
+                                      False, 3.0)
+    # colors: sum G w, sensitivity sum G w (1 + qabs/2) with qabs <= 9 inside the 3-sigma box.
+    assert np.all(s[2] >= np.abs(g64[2]) * (1 - 1e-12))
+    assert np.all(s[2] <= 5.5 * np.abs(g64[2]) + 1e-30)
+    assert np.all(s[3] >= np.abs(g64[3]) * (1 - 1e-12))

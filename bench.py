@@ -57,6 +57,13 @@ if ROOT not in sys.path:
 
 RECORD_BYTES = 44  # algorithmic bytes of one Gaussian's parameters
 HBM_PEAK_GBS = 8000.0
+# Vector FP32 issue rate of MI355X without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
+VALU_PEAK_LANE_OPS = 39.3e12
+# Useful FP32 lane-operations per in-rectangle (pixel, Gaussian) visit, counted from the kernels'
+# arithmetic: forward dx, dy, the quadratic (5), exp2, the denominator and 3 color FMAs = 12;
+# backward's gradient visit 23 (the quadratic and exp2 again, dw 3, 4 color/opacity sums, 7 for
+# the moment sums) plus the recomputed denominator's 8.
+VISIT_OPS = {"fwd": 12, "bwd_stored": 23, "bwd_recompute": 31}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 WORKLOADS = {
@@ -68,6 +75,24 @@ WORKLOADS = {
     "stress": ("fwd+bwd 4096x4096 RGB fp32, 1,000,000 Gaussians: scales log-normal(ln 2, 1) "
                "clipped at 64 px, aspect log-U[1,20], seed 0", 4096, 4096, 1_000_000),
 }
+
+
+def rect_visits(means, radii, H: int, W: int) -> int:
+    """E: the (pixel, Gaussian) visits of the whole image, the pixels of every Gaussian's clipped
+    support rectangle (centre rounded half to even, +- radius, clipped; render.py:62-78)."""
+    import torch
+
+    m = means.double()
+    r = radii.long()
+    ok = torch.isfinite(m).all(dim=1)
+    ix = torch.round(torch.nan_to_num(m[:, 0])).long()
+    iy = torch.round(torch.nan_to_num(m[:, 1])).long()
+    x0 = (ix - r[:, 0]).clamp(min=0)
+    x1 = (ix + r[:, 0]).clamp(max=W - 1)
+    y0 = (iy - r[:, 1]).clamp(min=0)
+    y1 = (iy + r[:, 1]).clamp(max=H - 1)
+    area = (x1 - x0 + 1).clamp(min=0) * (y1 - y0 + 1).clamp(min=0)
+    return int((area * ok).sum().item())
 
 
 def backward_kernel_bytes(P: int, HW: int, stored: bool) -> int:
@@ -373,6 +398,25 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def valu_block(E: int, ms_per_step: float, kernel_ms: dict, stored: bool) -> dict:
+    """The compute-side roof next to the HBM one: useful FP32 lane-operations (E visits times
+    VISIT_OPS) per second against the vector issue rate."""
+    bwd_ops = VISIT_OPS["bwd_stored" if stored else "bwd_recompute"]
+    step_ops = E * (VISIT_OPS["fwd"] + bwd_ops)
+    bwd = E * bwd_ops / (kernel_ms["k_backward"] * 1e-3)
+    fwd = E * VISIT_OPS["fwd"] / (kernel_ms["k_forward"] * 1e-3)
+    return {
+        "visits": E,
+        "ops_per_visit": {"fwd": VISIT_OPS["fwd"], "bwd": bwd_ops},
+        "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+        "step_frac": round(step_ops / (ms_per_step * 1e-3) / VALU_PEAK_LANE_OPS, 4),
+        "k_forward_frac": round(fwd / VALU_PEAK_LANE_OPS, 4),
+        "k_backward_frac": round(bwd / VALU_PEAK_LANE_OPS, 4),
+        "note": ("useful lane-operations only: lanes outside a Gaussian's rectangle, reductions, "
+                 "staging and addressing are not counted"),
+    }
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from profiles/hbm_traffic.json, if it covers `workload`."""
     try:
@@ -579,6 +623,7 @@ def main() -> int:
     other_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     other_kernel_ms = kernel_times(not stored)
 
+    E = rect_visits(means, radii, H, W)
     P_now = int(offsets[-1].item())
     if P_now != P:
         raise RuntimeError(f"pair count changed between binnings: {P} -> {P_now}")
@@ -635,6 +680,7 @@ def main() -> int:
             "frac": round(bwd_gbs / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(args.workload, bwd_kernel),
         },
+        "valu": valu_block(E, ms_per_step, kernel_ms, stored),
     }
     _progress(f"fwd+bwd {ms_per_step:.4f} ms per step")
     if args.bands > 0:

@@ -236,10 +236,11 @@ def cpu_baseline(workload: str, H: int, W: int, bytes_step: float) -> dict:
         "gpu_timing": "render_hip_exact + autograd backward per call (binning, forward, "
                       "backward and the autograd node's allocations), 3 warm-ups, median of 10",
     }
-def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
+def batched_fit(steps: int, num_fields: int = 64, seed: int = 0, plain_loop: bool = True) -> dict:
     """Batched fitting: `num_fields` fields, sides uniform over {256, 512, 768, 1024}, Gaussian
     counts log-uniform in [1k, 60k], fit-target recipe; the batched graph-captured fit against
-    the plain per-image loop, same steps, same initial fields (fields sorted by size)."""
+    the plain per-image loop, same steps, same initial fields (fields sorted by size).  Without
+    `plain_loop` only the batched fit runs (for profiling it alone)."""
     import numpy as np
     import torch
 
@@ -259,6 +260,17 @@ def batched_fit(steps: int, num_fields: int = 64, seed: int = 0) -> dict:
 
     cfg = F.FitConfig(iters=steps)
     batched = B.fit_batch(fields(), targets, cfg, graph=True)
+    if not plain_loop:
+        return {
+            "fields": num_fields,
+            "steps_per_field": steps,
+            "batched_steps_per_s": round(steps / batched.seconds, 3),
+            "batched_replay_ms_per_step": round(batched.step_seconds * 1e3
+                                                / max(batched.timed_steps, 1), 4),
+            "mean_psnr_db": round(float(np.mean(batched.psnr)), 3),
+            "tile_size": HR.tile_edge(batched.tile_size),
+            "schedule": "persistent" if batched.tile_size & HR.TILE_PERSISTENT else "grid",
+        }
     # The plain loop at the batched fit's (autotuned) tile size: the same arithmetic per field.
     plain = B.fit_each(fields(), targets, dataclasses.replace(cfg, tile_size=batched.tile_size))
     diff = float(np.max(np.abs(np.array(batched.psnr) - np.array(plain.psnr))))
@@ -463,6 +475,8 @@ def main() -> int:
                     help="Adam steps of the 2048x2048/200k fit loop (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched-vs-plain fit (--batched-steps steps)")
+    ap.add_argument("--no-plain-loop", action="store_true",
+                    help="with --batched-only: the batched fit alone, without the plain loop")
     ap.add_argument("--bands", type=int, default=8,
                     help="row bands timed as one GPU's share each (0 skips)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -475,7 +489,8 @@ def main() -> int:
     if args.copy_probe > 0:
         return copy_probe(args.copy_probe)
     if args.batched_only:
-        print(json.dumps({"batched": batched_fit(max(args.batched_steps, 1))}), flush=True)
+        print(json.dumps({"batched": batched_fit(max(args.batched_steps, 1),
+                                                 plain_loop=not args.no_plain_loop)}), flush=True)
         return 0
 
     import torch

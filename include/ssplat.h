@@ -418,8 +418,9 @@ int ssplat_l1_ssim_loss(const float* image, const float* target, int height, int
 
 /*
  * The target's window moments for a fit whose target does not change: moments receives two
- * (H, W, 3) float32 maps, G*t then G*t^2 (the 11-tap window per channel, zero-padded).  No
- * reference counterpart (the reference recomputes them in every SSIM call).
+ * (H, W, 3) float32 maps, G*t then G*t^2 (the 11-tap window per channel, zero-padded): the
+ * target-side statistics the reference caches across a fit in SSIMTargetStats
+ * (metrics.py:93-128).
  */
 int ssplat_loss_target_moments(const float* target, int height, int width, float* moments,
                                void* stream);

@@ -70,7 +70,8 @@ constexpr int kGVTasks = 3 * kGT * (kGT / kGVSeg);     // 192
 static_assert(kGHTasks <= kLossThreads && kGVTasks <= kLossThreads, "one task per thread");
 
 // What the stats pass blurs.  The target's window moments mu_t = G*t and G*t^2 do not change
-// while a fit runs, so a fit computes them once (kStatsTarget) and its steps read them
+// while a fit runs (the reference caches them in SSIMTargetStats, metrics.py:93-128), so a fit
+// computes them once (kStatsTarget) and its steps read them
 // (kStatsMoments), blurring three maps of the image instead of five.  The blurs are the same
 // arithmetic in every mode, so the moments read equal the ones computed inline bit for bit.
 constexpr int kStatsFull = 0;     // p, t, p^2, t^2, p t

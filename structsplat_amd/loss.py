@@ -8,7 +8,8 @@ them into the image gradient, scaled on the device by the incoming gradient.  A 
 the sum of its images' losses.  ``metrics.ssim`` is the torch statement of the same SSIM.
 
 A fit's target does not change, so its window moments (G*t and G*t^2 per channel) can be computed
-once (``target_moments``) and passed to every step's loss: the forward then blurs three maps of
+once (``target_moments``, the role of the reference's ``SSIMTargetStats``, metrics.py:93-128) and
+passed to every step's loss: the forward then blurs three maps of
 the image instead of five, with the same loss and gradient bit for bit.
 """
 from __future__ import annotations

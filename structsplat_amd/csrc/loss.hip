@@ -67,6 +67,8 @@ constexpr int kGHSeg = 8;
 constexpr int kGVSeg = 4;
 constexpr int kGHTasks = 3 * kGSpan * (kGT / kGHSeg);  // 156
 constexpr int kGVTasks = 3 * kGT * (kGT / kGVSeg);     // 192
+constexpr int kGStageStride = 28;            // floats per staged row: 26 used, 16-byte rows
+constexpr int kGBlurStride = 20;             // floats per blurred row: 16 used, 16-byte rows
 static_assert(kGHTasks <= kLossThreads && kGVTasks <= kLossThreads, "one task per thread");
 
 // What the stats pass blurs.  The target's window moments mu_t = G*t and G*t^2 do not change
@@ -407,10 +409,10 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_l1(LossArgs a, bool forwa
 // dL/dp = scale (w_l1 sign(p - t) / n - w_ssim (G*a + 2 p G*b + t G*c) / n); two workgroups per
 // unit, one per 16 x 16 half.
 __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
-  constexpr int kMapFloats = 3 * kGSpan * (kGSpan + 1);
+  constexpr int kMapFloats = 3 * kGSpan * kGStageStride;
   constexpr int kStaged = 3 * kMapFloats;
-  constexpr int kBlurred = 3 * 3 * kGSpan * (kGT + 1);
-  __shared__ float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
+  constexpr int kBlurred = 3 * 3 * kGSpan * kGBlurStride;
+  __shared__ __align__(16) float s_buf[kStaged > kBlurred ? kStaged : kBlurred];
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const UnitGeo g = unit_geo(a, wg >> 1);
   const int y0 = g.y0 + (wg & 1) * kGT;
@@ -422,17 +424,26 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
   const int h_row = h_rem >> 1;
   const int h_c0 = (h_rem & 1) * kGHSeg;
   {
-    stage_halo<3, kGT, kGSpan + 1, false>(src, g.x0, y0, g, s_buf);
+    stage_halo<3, kGT, kGStageStride, false>(src, g.x0, y0, g, s_buf);
     __syncthreads();
     float h[3][kGHSeg];
     if (threadIdx.x < kGHTasks) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const float (*s_m)[kGSpan][kGSpan + 1] =
-            reinterpret_cast<const float (*)[kGSpan][kGSpan + 1]>(s_buf + q * kMapFloats);
+        // The row's 18 inputs as four float4 and a float2 (16-byte staged rows).
+        const float* row = s_buf + q * kMapFloats + (h_ch * kGSpan + h_row) * kGStageStride + h_c0;
         float v[kGHSeg + kWin - 1];
 #pragma unroll
-        for (int i = 0; i < kGHSeg + kWin - 1; ++i) v[i] = s_m[h_ch][h_row][h_c0 + i];
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const float4 x = *reinterpret_cast<const float4*>(row + 4 * q4);
+          v[4 * q4] = x.x;
+          v[4 * q4 + 1] = x.y;
+          v[4 * q4 + 2] = x.z;
+          v[4 * q4 + 3] = x.w;
+        }
+        const float2 x2 = *reinterpret_cast<const float2*>(row + 16);
+        v[16] = x2.x;
+        v[17] = x2.y;
 #pragma unroll
         for (int o = 0; o < kGHSeg; ++o) {
           float acc = 0.0f;
@@ -443,12 +454,16 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
       }
     }
     __syncthreads();  // every read of the staged maps is done
-    float (*s_h)[3][kGSpan][kGT + 1] = reinterpret_cast<float (*)[3][kGSpan][kGT + 1]>(s_buf);
+    float (*s_h)[3][kGSpan][kGBlurStride] =
+        reinterpret_cast<float (*)[3][kGSpan][kGBlurStride]>(s_buf);
     if (threadIdx.x < kGHTasks) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
 #pragma unroll
-        for (int o = 0; o < kGHSeg; ++o) s_h[q][h_ch][h_row][h_c0 + o] = h[q][o];
+        for (int o = 0; o < kGHSeg; o += 4) {
+          *reinterpret_cast<float4*>(&s_h[q][h_ch][h_row][h_c0 + o]) =
+              make_float4(h[q][o], h[q][o + 1], h[q][o + 2], h[q][o + 3]);
+        }
       }
     }
     __syncthreads();

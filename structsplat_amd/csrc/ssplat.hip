@@ -1103,15 +1103,24 @@ struct StageLds {
   float4 c[kChunks][kS];
 };
 
+// The tile a gradient sweep stages records for, to find each pair's slot.
+struct SlotTile {
+  int tile_x, tile_y;  // the tile's position in its field's tile grid
+  int lg;              // log2 of the tile edge
+  int64_t capacity;    // slots the buffer holds
+};
+
 // Coalesced staging of up to kStage records of the tile list into LDS (kChunks 16-byte loads
 // per thread).  An id outside [0, n) (a list slot a capacity-limited binning left unwritten)
 // stages an empty rectangle, which no pixel and no quadrant test ever selects.
 // kAffine: the affine chunks follow in st.c[4 .. 4 + kAffineChunks).  kSlot: the gradient
-// sweep's records carry the Gaussian's absolute first slot in c3.w.
+// sweep's records carry in c3.w the pair's slot, the Gaussian's first slot plus this tile's
+// row-major index in its rectangle's tiles, or -1 where the pair has none (a list entry outside
+// the rectangle, a slot beyond the capacity, an empty record).
 template <int kLoad, int kChunks, int kS, bool kAffine = false, bool kSlot = false,
           bool kOpaque = false>
 __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunks, kS>& st,
-                                            int64_t batch, int m) {
+                                            int64_t batch, int m, SlotTile stile = {}) {
   static_assert(kLoad <= kChunks, "staging more chunks than the buffer holds");
   static_assert(!kAffine || (kLoad == 4 && kChunks == 4 + kAffineChunks), "affine layout");
   const int t = thread_index<kOpaque>();
@@ -1119,11 +1128,22 @@ __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunk
     const int g = a.gids[batch + t];
     if (g >= 0 && g < a.n) {
       const float4* src = reinterpret_cast<const float4*>(a.recs + g);
+      float4 v[kLoad];
 #pragma unroll
-      for (int c = 0; c < kLoad; ++c) st.c[c][t] = src[c];
+      for (int c = 0; c < kLoad; ++c) v[c] = src[c];
       if constexpr (kSlot) {
-        st.c[3][t].w = __int_as_float(__float_as_int(src[3].w) + a.rect_off[g >> 6]);
+        const RectTiles rt = rect_tiles(__float_as_uint(v[1].y), __float_as_uint(v[1].z), stile.lg);
+        const int kx = stile.tile_x - rt.tx0, ky = stile.tile_y - rt.ty0;
+        int slot = -1;
+        if (kx >= 0 && kx < rt.tw && ky >= 0 && ky < rt.th) {
+          const int64_t s64 = int64_t(__float_as_int(v[3].w)) + a.rect_off[g >> 6] +
+                              ky * rt.tw + kx;
+          if (s64 < stile.capacity) slot = static_cast<int>(s64);
+        }
+        v[3].w = __int_as_float(slot);
       }
+#pragma unroll
+      for (int c = 0; c < kLoad; ++c) st.c[c][t] = v[c];
       if constexpr (kAffine) {
 #pragma unroll
         for (int c = 0; c < kAffineChunks; ++c) {
@@ -1133,6 +1153,7 @@ __device__ __forceinline__ void stage_batch(const RenderArgs& a, StageLds<kChunk
     } else {
 #pragma unroll
       for (int c = 0; c < kChunks; ++c) st.c[c][t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if constexpr (kSlot) st.c[3][t].w = __int_as_float(-1);
     }
   }
 }
@@ -1547,7 +1568,9 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(a, st, batch, m);
+    stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(a, st, batch, m,
+                                                       SlotTile{tile_x, tile_y, kLg,
+                                                                g.slot_capacity});
     __syncthreads();
     for (int j = wave; j < m; j += kWaves) {
       const float4 v0 = st.c[0][j];
@@ -1559,8 +1582,9 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
       const int x0 = xr & 0xffff, x1 = xr >> 16, y0 = yr & 0xffff, y1 = yr >> 16;
       const float mx = v0.x, my = v0.y, qa = v0.z, qb = v0.w, qc = v1.x;
       const float cr = v3.x, cg = v3.y, cbl = v3.z;
-      float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
-      float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
+      // The accumulators; the first quadrant's visit (16-pixel tiles) or an explicit clear sets
+      // them.
+      float sx, sy, sxx, sxy, syy, gcr, gcg, gcb, gop;
       // Affine colors: the frame and color gradients, and their partials (unused otherwise).
       float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0, a2 = a0;
       if constexpr (kAffine) {
@@ -1568,15 +1592,32 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
         a1 = st.c[5][j];
         a2 = st.c[6][j];
       }
-      float g00 = 0.0f, g01 = 0.0f, g02 = 0.0f, g10 = 0.0f, g11 = 0.0f, g12 = 0.0f;
-      float sa = 0.0f, sb = 0.0f, cxx = 0.0f, cyy = 0.0f, cxy = 0.0f, cyx = 0.0f;
-      // One visit: pixel (px, py) inside the rectangle, with its adjoint.
-      auto visit = [&](int px, int py, const float4& ad) {
+      float g00, g01, g02, g10, g11, g12, sa, sb, cxx, cyy, cxy, cyx;
+      // The clear's moves stay on its own path (a volatile statement is not hoisted above the
+      // branch that selects it; a plain zero would be set ahead of the branch on every pair).
+      auto zero = [](float& v) { asm volatile("v_mov_b32 %0, 0" : "=v"(v)); };
+      auto clear = [&]() {
+        zero(sx), zero(sy), zero(sxx), zero(sxy), zero(syy);
+        zero(gcr), zero(gcg), zero(gcb), zero(gop);
+        if constexpr (kAffine) {
+          zero(g00), zero(g01), zero(g02), zero(g10), zero(g11), zero(g12);
+          zero(sa), zero(sb), zero(cxx), zero(cyy), zero(cxy), zero(cyx);
+        }
+      };
+      // One visit of pixel (px, py) with its adjoint.  kFirst: the visit sets the accumulators
+      // instead of adding to them and runs on every lane, a lane outside the rectangle evaluating
+      // its weight at exp2(-inf) = 0 so that its terms are zero (for finite adjoints).  Otherwise
+      // only lanes inside the rectangle run it.
+      auto visit = [&](auto first, int px, int py, const float4& ad, bool inside) {
+        constexpr bool kFirst = decltype(first)::value;
+        auto mad = [](float& acc, float x, float y) { acc = kFirst ? x * y : fmaf(x, y, acc); };
+        auto add = [](float& acc, float x) { acc = kFirst ? x : acc + x; };
         const float dx = static_cast<float>(px) - mx;
         const float dy = static_cast<float>(py) - my;
         const float t = fmaf(qa, dx, qb * dy);
         const float e = fmaf(t, dx, qc * dy * dy);
-        const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
+        const float arg = kFade ? e * kNegHalfLog2e : e;
+        const float raw = exp2_hw(kFirst && !inside ? -INFINITY : arg);
         float lxc = 0.0f, lyc = 0.0f;
         float pr = cr, pg = cg, pb = cbl;  // the pixel's color
         if constexpr (kAffine) {
@@ -1595,51 +1636,65 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
           base = raw;
           dqr = dw * raw;
         }
-        gcr = fmaf(ad.x, base, gcr);
-        gcg = fmaf(ad.y, base, gcg);
-        gcb = fmaf(ad.z, base, gcb);
-        gop = fmaf(dw, base, gop);
-        sx = fmaf(dqr, dx, sx);
-        sy = fmaf(dqr, dy, sy);
+        mad(gcr, ad.x, base);
+        mad(gcg, ad.y, base);
+        mad(gcb, ad.z, base);
+        mad(gop, dw, base);
+        mad(sx, dqr, dx);
+        mad(sy, dqr, dy);
         const float dqx = dqr * dx;
-        sxx = fmaf(dqx, dx, sxx);
-        sxy = fmaf(dqx, dy, sxy);
-        syy = fmaf(dqr * dy, dy, syy);
+        mad(sxx, dqx, dx);
+        mad(sxy, dqx, dy);
+        mad(syy, dqr * dy, dy);
         if constexpr (kAffine) {
           // Color-gradient partials sum base * local * dL/dN; the local-coordinate adjoints
           // base * (dL/dN . G0) and base * (dL/dN . G1) feed the mean, scale and rotation
           // gradients (times the opacity, applied per pair).
           const float bl = base * lxc, bly = base * lyc;
-          g00 = fmaf(ad.x, bl, g00);
-          g01 = fmaf(ad.y, bl, g01);
-          g02 = fmaf(ad.z, bl, g02);
-          g10 = fmaf(ad.x, bly, g10);
-          g11 = fmaf(ad.y, bly, g11);
-          g12 = fmaf(ad.z, bly, g12);
+          mad(g00, ad.x, bl);
+          mad(g01, ad.y, bl);
+          mad(g02, ad.z, bl);
+          mad(g10, ad.x, bly);
+          mad(g11, ad.y, bly);
+          mad(g12, ad.z, bly);
           const float ax = base * fmaf(ad.x, a1.x, fmaf(ad.y, a1.y, ad.z * a1.z));
           const float ay = base * fmaf(ad.x, a1.w, fmaf(ad.y, a2.x, ad.z * a2.y));
-          sa += ax;
-          sb += ay;
-          cxx = fmaf(ax, lxc, cxx);
-          cyy = fmaf(ay, lyc, cyy);
-          cxy = fmaf(ax, lyc, cxy);
-          cyx = fmaf(ay, lxc, cyx);
+          add(sa, ax);
+          add(sb, ay);
+          mad(cxx, ax, lxc);
+          mad(cyy, ay, lyc);
+          mad(cxy, ax, lyc);
+          mad(cyx, ay, lxc);
         }
       };
+      using First = std::true_type;
+      using Next = std::false_type;
       const unsigned rw = static_cast<unsigned>(x1 - x0), rh = static_cast<unsigned>(y1 - y0);
       if constexpr (kT == 16) {
         // One straight-line block per quadrant: the per-lane rectangle test alone decides which
         // lanes accumulate (a quadrant the rectangle misses has no lane inside and is skipped by
         // the exec-mask branch), so the accumulators stay in the same registers throughout.
+        // The first quadrant, when the rectangle meets it (a uniform test), sets the
+        // accumulators on every lane; otherwise they are cleared.
+        const bool meets0 = x0 < tx0 + 8 && x1 > tx0 && y0 < ty0 + 8 && y1 > ty0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int px = tx0 + (s & 1) * 8 + lx;
           const int py = ty0 + (s >> 1) * 8 + ly;
-          if ((static_cast<unsigned>(px - x0) < rw) & (static_cast<unsigned>(py - y0) < rh)) {
-            visit(px, py, adj[s]);
+          const bool inside =
+              (static_cast<unsigned>(px - x0) < rw) & (static_cast<unsigned>(py - y0) < rh);
+          if (s == 0) {
+            if (meets0) {
+              visit(First{}, px, py, adj[s], inside);
+            } else {
+              clear();
+            }
+          } else if (inside) {
+            visit(Next{}, px, py, adj[s], inside);
           }
         }
       } else {
+        clear();
         // Only the quadrants the rectangle meets (uniform bounds), adjoints read from LDS.
         const int cx0 = max(x0, tx0) - tx0, cx1 = min(x1, tx0 + kT) - tx0;
         const int cy0 = max(y0, ty0) - ty0, cy1 = min(y1, ty0 + kT) - ty0;
@@ -1649,7 +1704,7 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
               const int px = tx0 + qx * 8 + lx;
               const int py = ty0 + qy * 8 + ly;
               if ((static_cast<unsigned>(px - x0) < rw) & (static_cast<unsigned>(py - y0) < rh)) {
-                visit(px, py, s_adj[(qy * 8 + ly) * kT + qx * 8 + lx]);
+                visit(Next{}, px, py, s_adj[(qy * 8 + ly) * kT + qx * 8 + lx], true);
               }
             }
           }
@@ -1658,31 +1713,30 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
       // Reduce the eight partials to lanes 8k+7 (k = component), the opacity partial to lane 63.
       const float vals[8] = {sx, sy, sxx, sxy, syy, gcr, gcg, gcb};
       const float red = wave_sum8_transposed(vals, hi8);
-      const float partner = dpp_move<0x128>(red);  // lane 7 <-> lane 15: Sx and Sy meet
-      float gop_sum, red2 = 0.0f, red3 = 0.0f;
+      float gop_sum, red2 = 0.0f;
+      // The slot's components: without affine colors the pair's raw sums (Sx, Sy, Sxx, Sxy, Syy,
+      // Cr, Cg, Cb; the opacity and conic factors are per Gaussian and k_grad_reduce applies them
+      // once to the sums over its tiles); with affine colors the pair's gradients.
+      float gval = red;
+      float extra[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // affine: scale x, scale y, rotation
+      const float op = v2.w;
       if constexpr (kAffine) {
+        const float partner = dpp_move<0x128>(red);  // lane 7 <-> lane 15: Sx and Sy meet
         const float vals2[8] = {g00, g01, g02, g10, g11, g12, sa, sb};
         const float vals3[8] = {cxx, cyy, cxy, cyx, gop, 0.0f, 0.0f, 0.0f};
         red2 = wave_sum8_transposed(vals2, hi8);
-        red3 = wave_sum8_transposed(vals3, hi8);
+        const float red3 = wave_sum8_transposed(vals3, hi8);
         gop_sum = has_opacity ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red3), 39))
                               : 0.0f;
-      } else {
-        gop_sum = has_opacity ? wave_sum_lane63(gop) : 0.0f;
-      }
-      // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries:
-      //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
-      //   g_c = -op/2 Syy, g_color = op C.
-      // The record's quadratic is the conic times -log2(e)/2 when folded; undo that here.
-      const int gid = __float_as_int(v1.w);
-      const float op = v2.w;
-      const float unfold = kFade ? 1.0f : kInvNegHalfLog2e;
-      const float ca = qa * unfold, cc = qc * unfold, cb = (0.5f * unfold) * qb;
-      const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
-      const float cross = comp < 2 ? partner * cb : 0.0f;
-      float gval = op * fmaf(red, own_scale, cross);
-      float extra[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // affine: scale x, scale y, rotation
-      if constexpr (kAffine) {
+        // dL/dq = -op/2 * dw * raw per pixel; chain to the mean and conic entries:
+        //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
+        //   g_c = -op/2 Syy, g_color = op C.
+        // The record's quadratic is the conic times -log2(e)/2 when folded; undo that here.
+        const float unfold = kFade ? 1.0f : kInvNegHalfLog2e;
+        const float ca = qa * unfold, cc = qc * unfold, cb = (0.5f * unfold) * qb;
+        const float own_scale = comp == 0 ? ca : (comp == 1 ? cc : comp_scale);
+        const float cross = comp < 2 ? partner * cb : 0.0f;
+        gval = op * fmaf(red, own_scale, cross);
         // Local coordinates x = (c dx + s dy) isx, y = (c dy - s dx) isy (isx = 1/sx), d = p - mean,
         // with A, B the sums of the local-coordinate adjoints and Cuv = sum (adjoint u) * v:
         //   dL/dmean = -op (A (a0.x, a0.y) + B (-a0.w, a0.z));
@@ -1701,30 +1755,25 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
         extra[0] = a2.z < 0.0f ? 0.0f : -op * Cxx * isx;
         extra[1] = a2.w < 0.0f ? 0.0f : -op * Cyy * isy;
         extra[2] = op * (Cxy * (isx / isy) - Cyx * (isy / isx));
+      } else {
+        gop_sum = has_opacity ? wave_sum_lane63(gop) : 0.0f;
       }
-      // The pair's slot: the Gaussian's first slot plus this tile's row-major index in its
-      // rectangle (uniform values).  A list entry outside the rectangle has no pixel inside
-      // and no slot.
-      const RectTiles rt = rect_tiles(xr, yr, kLg);
-      const int kx = tile_x - rt.tx0, ky = tile_y - rt.ty0;
-      if (kx >= 0 && kx < rt.tw && ky >= 0 && ky < rt.th) {
-        const int64_t slot = int64_t(__builtin_amdgcn_readfirstlane(__float_as_int(v3.w))) +
-                             ky * rt.tw + kx;
-        if (slot < g.slot_capacity) {
-          float* dst = g.slots + slot * kWords;
-          if (result_lane) dst[comp] = gval;
-          if constexpr (kAffine) {
-            if (result_lane && comp < 6) dst[9 + comp] = op * red2;
-            if (lane == 63) {
-              dst[8] = gop_sum;
-              dst[15] = extra[0];
-              dst[16] = extra[1];
-              dst[17] = extra[2];
-              dst[kWords - 1] = tag;
-            }
-          } else {
-            if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
+      // The pair's slot, found at staging (-1: none).
+      const int slot = __builtin_amdgcn_readfirstlane(__float_as_int(v3.w));
+      if (slot >= 0) {
+        float* dst = g.slots + int64_t(slot) * kWords;
+        if (result_lane) dst[comp] = gval;
+        if constexpr (kAffine) {
+          if (result_lane && comp < 6) dst[9 + comp] = op * red2;
+          if (lane == 63) {
+            dst[8] = gop_sum;
+            dst[15] = extra[0];
+            dst[16] = extra[1];
+            dst[17] = extra[2];
+            dst[kWords - 1] = tag;
           }
+        } else {
+          if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
         }
       }
     }
@@ -1773,6 +1822,7 @@ struct ReduceArgs {
   float* grad_scales;       // affine only: (n, 2)
   float* grad_rotations;    // affine only: (n,)
   int32_t* status;          // may be NULL
+  float unfold;             // the records' conic factor undone: 1, or 1 / (-log2(e)/2) if folded
 };
 
 // Adds one slot's kVals partial gradients when its tag (word kWords - 1) is the call's: a slot
@@ -1915,6 +1965,25 @@ __global__ __launch_bounds__(256) void k_grad_reduce(ReduceArgs a) {
     }
   }
   if (i < a.n) {
+    if constexpr (kVals == 9) {
+      // The slots hold each pair's raw sums; the per-Gaussian factors (dL/dq = -op/2 dw raw per
+      // pixel, chained to the mean and conic entries) apply to their totals:
+      //   g_mx = op (a Sx + b Sy), g_my = op (b Sx + c Sy), g_a = -op/2 Sxx, g_b = -op Sxy,
+      //   g_c = -op/2 Syy, g_color = op C.
+      const GaussRec& r = a.recs[i];
+      const float4 c0 = r.c0;
+      const float op = r.c2.w;
+      const float ca = c0.z * a.unfold, cc = r.c1.x * a.unfold, cb = (0.5f * a.unfold) * c0.w;
+      const float sx = acc[0], sy = acc[1];
+      acc[0] = op * fmaf(sx, ca, sy * cb);
+      acc[1] = op * fmaf(sy, cc, sx * cb);
+      acc[2] = op * (-0.5f * acc[2]);
+      acc[3] = op * -acc[3];
+      acc[4] = op * (-0.5f * acc[4]);
+      acc[5] *= op;
+      acc[6] *= op;
+      acc[7] *= op;
+    }
     a.grad_means[2 * i + 0] = acc[0];
     a.grad_means[2 * i + 1] = acc[1];
     a.grad_conics[3 * i + 0] = acc[2];
@@ -2704,7 +2773,7 @@ int backward_tiled(const Plan& p, const float* grad_out, const float* means, con
                        epoch,       grad_means,            grad_conics,   grad_colors,
                        gop,         af ? af->grad_color_grads : nullptr,
                        af ? af->grad_scales : nullptr,      af ? af->grad_rotations : nullptr,
-                       status};
+                       status,      support_fade ? 1.0f : kInvNegHalfLog2e};
   const int blocks = static_cast<int>((p.n + 255) / 256);
   if (af != nullptr) {
     k_grad_reduce<kSlotWordsAffine, 18><<<blocks, 256, 0, s>>>(red);

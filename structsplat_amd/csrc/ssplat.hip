@@ -972,6 +972,676 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(int32_t* __restrict
 }
 
 // ------------------------------------------------------------------------------------------
+// Strip binning: the tile lists without a global atomic per (tile, Gaussian) pair
+// ------------------------------------------------------------------------------------------
+//
+// The global tile order (every field's tiles, row-major, fields in order) is cut into strips of
+// 2^lg consecutive tiles (lg = 4, 5 for large grids, down to 0 for small ones), and the Gaussians into chunks of consecutive ids, one
+// chunk per level-1 workgroup.  Two launches:
+//
+//   k_strip_level1, one workgroup per chunk: lists every (strip, Gaussian) pair of the chunk
+//     whose rectangle meets the strip, with the Gaussian's tile mask inside the strip (its pairs
+//     there, after the ellipse cull), first in LDS, then grouped by strip in the chunk's own
+//     region of the bucket.  The chunk's row of a chunks x (strips + 1) matrix gets, per strip,
+//     the chunk's first entry of the strip and the chunk's pairs in the strips before it.  LDS
+//     atomics only.
+//   k_strip_level2, one workgroup per strip: reads its matrix column, whose sum over the chunks
+//     is the number of pairs before the strip (its first list position), gathers its entries
+//     from every chunk's region, counts its pairs per tile, writes its tiles' offsets, places the
+//     pairs per tile in LDS and sorts every tile's pairs into its list, which gives the
+//     reference's ascending Gaussian ids.
+//
+// A chunk whose entries outgrow its region (Gaussians that meet very many strips) stores none;
+// level 2 then takes each of its Gaussians as a candidate and computes the masks itself.
+// Grids with more than kStripMax strips of 32 tiles use the per-pair kernels above.
+
+constexpr int kStripMax = 7680;         // strips of one binning (level-1 LDS bins)
+constexpr int kStripTarget = 512;       // narrower strips below this many
+constexpr int kStripL1Threads = 1024;   // level-1 workgroup
+constexpr int kStripL2Threads = 512;    // level-2 workgroup
+constexpr int kStripMaxChunks = 1024;   // chunks of one binning (level-2 LDS table)
+constexpr int kStripChunkUnit = 1024;   // chunks hold a multiple of this many Gaussians
+constexpr int kStripIdBits = 19;        // level 1 packs (strip, id in chunk) into 32 bits
+constexpr int kStripMaxChunk = 1 << kStripIdBits;
+constexpr int kStripL1Lds = 65024;      // level-1 dynamic LDS: counters plus the entry list
+constexpr int kStripEntriesPerGaussian = 8;  // a chunk's region: 8 entries per Gaussian
+constexpr int kStripCache = 1024;       // level-2 candidates kept in LDS between passes
+constexpr int kStripPairsLds = 4096;    // pairs a strip places in LDS (more: in the lists)
+static_assert(kStripMax <= kStripL1Threads * 8, "level 1 scans 8 strips per thread");
+static_assert(kStripMaxChunks <= kStripL2Threads * 2, "level 2 scans 2 chunks per thread");
+static_assert(kStripMax < (1 << (32 - kStripIdBits)), "strip numbers fit the packed entry");
+static_assert(kStripPairsLds >= kSortLds, "long lists sort in the placement buffer");
+
+struct StripArgs {
+  int lg;                // log2 of the tiles per strip
+  int strips;            // ceil(T / 2^lg)
+  int num_tiles;         // T
+  int chunks;            // ceil(n / chunk), at least 1
+  int chunk;             // Gaussians per chunk (a multiple of kStripChunkUnit)
+  int chunk_cap;         // entries of a chunk's region
+  int list_cap;          // entries of level 1's LDS list
+  int2* offs;            // chunks x (strips + 1): per strip, the chunk's first entry of the
+                         // strip and its pairs before the strip; [strips] = its totals
+  int2* bucket;          // chunks x chunk_cap entries (gid, tile mask)
+};
+
+// One Gaussian's tile rectangle in its field's tile grid (inclusive; empty when tx1 < tx0), its
+// clipped pixel rectangle and mean (the cull's inputs), and its field's tile range.
+struct GaussTiles {
+  int tx0, tx1, ty0, ty1;
+  int first_tile, tiles_x, num_tiles;
+  int x0, x1, y0, y1;  // half-open clipped support rectangle
+  float mx, my;
+};
+
+__device__ __forceinline__ GaussTiles empty_tiles() {
+  GaussTiles t;
+  t.tx0 = 0, t.tx1 = -1, t.ty0 = 0, t.ty1 = -1;
+  t.first_tile = 0, t.tiles_x = 1, t.num_tiles = 0;
+  t.x0 = 0, t.x1 = 0, t.y0 = 0, t.y1 = 0;
+  t.mx = 0.0f, t.my = 0.0f;
+  return t;
+}
+
+// Same rectangle and tile bounds as enumerate_pairs.
+__device__ __forceinline__ GaussTiles gauss_tiles(const BinArgs& a, int g) {
+  GaussTiles t = empty_tiles();
+  const ssplat_field f = field_of_gaussian(a.fields, g);
+  t.first_tile = static_cast<int>(f.first_tile);
+  t.tiles_x = f.tiles_x;
+  t.num_tiles = f.num_tiles;
+  t.mx = a.means[2 * g + 0];
+  t.my = a.means[2 * g + 1];
+  const Rect r = support_rect(t.mx, t.my, a.radii[2 * g + 0], a.radii[2 * g + 1], f.height,
+                              f.width);
+  if (r.x1 > r.x0 && r.y1 > r.y0) {
+    t.x0 = r.x0, t.x1 = r.x1, t.y0 = r.y0, t.y1 = r.y1;
+    t.tx0 = r.x0 / a.ts;
+    t.tx1 = (r.x1 - 1) / a.ts;
+    t.ty0 = r.y0 / a.ts;
+    t.ty1 = (r.y1 - 1) / a.ts;
+  }
+  return t;
+}
+
+// The tiles of strip s (global tiles [s << lg, (s + 1) << lg)) that hold a (tile, Gaussian) pair
+// of this Gaussian: the tiles of its rectangle inside the strip, less the culled ones (the
+// per-pair kernels' test, operation for operation).  Bit k stands for tile (s << lg) + k.
+__device__ __forceinline__ uint32_t strip_mask(const BinArgs& a, const GaussTiles& t, float ca,
+                                               float cb, float cc, int s, int lg) {
+  if (t.tx1 < t.tx0) return 0u;
+  const int s0 = s << lg;
+  // The strip's tiles inside this Gaussian's field, as field-local indices [l0, l1).
+  const int l0 = max(s0, t.first_tile) - t.first_tile;
+  const int l1 = min(s0 + (1 << lg), t.first_tile + t.num_tiles) - t.first_tile;
+  if (l1 <= l0) return 0u;
+  const int rlo = max(l0 / t.tiles_x, t.ty0);
+  const int rhi = min((l1 - 1) / t.tiles_x, t.ty1);
+  uint32_t m = 0u;
+  for (int row = rlo; row <= rhi; ++row) {
+    const int rb = row * t.tiles_x;  // field-local index of the row's first tile
+    const int xa = max(t.tx0, max(l0 - rb, 0));
+    const int xb = min(t.tx1, min(l1 - 1 - rb, t.tiles_x - 1));
+    if (xb < xa) continue;
+    const int k0 = t.first_tile + rb + xa - s0;  // in [0, 2^lg), and k0 + xb - xa < 2^lg
+    if (!a.cull) {
+      m |= static_cast<uint32_t>(((uint64_t(1) << (xb - xa + 1)) - 1) << k0);
+    } else {
+      const int ry0 = max(t.y0, row * a.ts);
+      const int ry1 = min(t.y1 - 1, (row + 1) * a.ts - 1);
+      for (int tx = xa; tx <= xb; ++tx) {
+        const int rx0 = max(t.x0, tx * a.ts);
+        const int rx1 = min(t.x1 - 1, (tx + 1) * a.ts - 1);
+        const float mq = tile_min_q(t.mx, t.my, ca, cb, cc, static_cast<float>(rx0),
+                                    static_cast<float>(rx1), static_cast<float>(ry0),
+                                    static_cast<float>(ry1));
+        if (!(mq > a.cull_thr)) m |= 1u << (k0 + tx - xa);
+      }
+    }
+  }
+  return m;
+}
+
+// Gaussian g's mask for strip s, from its inputs.
+__device__ __forceinline__ uint32_t entry_mask(const BinArgs& a, int g, int s, int lg) {
+  const GaussTiles t = gauss_tiles(a, g);
+  float ca = 0.0f, cb = 0.0f, cc = 0.0f;
+  if (a.cull) {
+    ca = a.conics[3 * g + 0];
+    cb = a.conics[3 * g + 1];
+    cc = a.conics[3 * g + 2];
+  }
+  return strip_mask(a, t, ca, cb, cc, s, lg);
+}
+
+// The owner lane's copy of every GaussTiles field (all lanes active).
+__device__ __forceinline__ GaussTiles shfl_tiles(const GaussTiles& t, int o) {
+  GaussTiles r;
+  r.tx0 = __shfl(t.tx0, o), r.tx1 = __shfl(t.tx1, o);
+  r.ty0 = __shfl(t.ty0, o), r.ty1 = __shfl(t.ty1, o);
+  r.first_tile = __shfl(t.first_tile, o), r.tiles_x = __shfl(t.tiles_x, o);
+  r.num_tiles = __shfl(t.num_tiles, o);
+  r.x0 = __shfl(t.x0, o), r.x1 = __shfl(t.x1, o), r.y0 = __shfl(t.y0, o), r.y1 = __shfl(t.y1, o);
+  r.mx = __shfl(t.mx, o), r.my = __shfl(t.my, o);
+  return r;
+}
+
+// Gaussian g for level 1 (empty past n) and its conic when kFull and culling.
+template <bool kFull>
+__device__ __forceinline__ void strip_gaussian(const BinArgs& a, int64_t g, GaussTiles* t,
+                                               float* ca, float* cb, float* cc) {
+  *t = empty_tiles();
+  *ca = 0.0f, *cb = 0.0f, *cc = 0.0f;
+  if (g < a.n) {
+    const int i = static_cast<int>(g);
+    *t = gauss_tiles(a, i);
+    if (kFull && a.cull) {
+      *ca = a.conics[3 * i + 0];
+      *cb = a.conics[3 * i + 1];
+      *cc = a.conics[3 * i + 2];
+    }
+  }
+}
+
+// One pass of level 1 over 1024 Gaussians of a chunk (ids g0 + sub + thread): the Gaussians' tile
+// rows are numbered by a wave prefix sum and dealt to the lanes round-robin, so a tall Gaussian
+// does not serialize its wave.  A lane handling row `row` of a Gaussian takes the strips of that
+// row's tiles, less the first when the previous row's last strip is the same (a strip met by two
+// rows is listed once: tile indices grow with the row, so a strip meeting rows r and r + 2 also
+// meets r + 1).  step(s_lo, ne, owner tiles, ca, cb, cc, id in chunk) runs on every lane of each
+// step (ne = 0 on a lane without a row): the lane's strips are s_lo .. s_lo + ne - 1.
+template <class Step>
+__device__ __forceinline__ void strip_steps(const BinArgs& a, int lg, int64_t g0, int sub,
+                                            Step&& step) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  GaussTiles t;
+  float ca, cb, cc;
+  strip_gaussian<true>(a, g0 + sub + tid, &t, &ca, &cb, &cc);
+  const int local0 = sub + (tid & ~63);  // the wave's first Gaussian, relative to the chunk
+  const int units = t.tx1 >= t.tx0 ? t.ty1 - t.ty0 + 1 : 0;
+  const int incl = wave_inclusive_scan(units);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  for (int base = 0; base < total; base += 64) {
+    const int k = min(base + lane, total - 1);
+    int o = 0;
+#pragma unroll
+    for (int sz = 32; sz >= 1; sz >>= 1) {
+      if (__shfl(incl, o + sz - 1) <= k) o += sz;
+    }
+    const int before = __shfl(incl, o > 0 ? o - 1 : 0);
+    const int excl = o > 0 ? before : 0;
+    const GaussTiles ot = shfl_tiles(t, o);
+    const float oa = __shfl(ca, o), ob = __shfl(cb, o), oc = __shfl(cc, o);
+    int s_lo = 0, s_hi = -1;
+    if (base + lane < total) {
+      const int row = ot.ty0 + (k - excl);
+      const int rowbase = ot.first_tile + row * ot.tiles_x;
+      s_lo = (rowbase + ot.tx0) >> lg;
+      s_hi = (rowbase + ot.tx1) >> lg;
+      if (row > ot.ty0 && s_lo == ((rowbase - ot.tiles_x + ot.tx1) >> lg)) ++s_lo;
+    }
+    step(s_lo, s_hi >= s_lo ? s_hi - s_lo + 1 : 0, ot, oa, ob, oc, local0 + o);
+  }
+}
+
+// Level 1: one workgroup per chunk.  Every (strip, Gaussian) entry gets its mask and is appended
+// to the chunk's LDS list (one LDS atomic per wave and step for the list, per entry for the
+// strip's entry and pair counters); one block scan of the packed counters (pairs in the high
+// word, entries in the low) gives the chunk's offsets, written as its matrix row; the list is
+// then copied into the chunk's region, grouped by strip.  A chunk with more entries than its
+// list holds places them by a second pass; one with more than its region holds stores none
+// (level 2 recomputes its masks).
+__global__ __launch_bounds__(kStripL1Threads) void k_strip_level1(BinArgs a, StripArgs sa,
+                                                                  int32_t* __restrict__ status) {
+  extern __shared__ int32_t s_lds[];
+  int32_t* s_ent = s_lds;                   // strips + 1: entry counts, offsets, then cursors
+  int32_t* s_pairs = s_lds + sa.strips + 1;  // strips + 1: pair counts, then their prefix
+  int2* s_list = reinterpret_cast<int2*>(s_lds + 2 * ((sa.strips + 2) & ~1));
+  __shared__ int s_count;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int c = blockIdx.x;
+  if (c == 0 && tid == 0) *status = SSPLAT_OK;
+  for (int s = tid; s <= sa.strips; s += kStripL1Threads) {
+    s_ent[s] = 0;
+    s_pairs[s] = 0;
+  }
+  if (tid == 0) s_count = 0;
+  __syncthreads();
+  const int64_t g0 = int64_t(c) * sa.chunk;
+  const int lg = sa.lg;
+  for (int sub = 0; sub < sa.chunk; sub += kStripL1Threads) {
+    strip_steps(a, lg, g0, sub,
+                [&](int s_lo, int ne, const GaussTiles& ot, float oa, float ob, float oc, int id) {
+                  const int ne_incl = wave_inclusive_scan(ne);
+                  const int step_total = __builtin_amdgcn_readlane(ne_incl, 63);
+                  int wbase = 0;
+                  if (lane == 0 && step_total > 0) wbase = atomicAdd(&s_count, step_total);
+                  wbase = __builtin_amdgcn_readfirstlane(wbase) + ne_incl - ne;
+                  for (int i = 0; i < ne; ++i) {
+                    const int s = s_lo + i;
+                    const uint32_t m = strip_mask(a, ot, oa, ob, oc, s, lg);
+                    atomicAdd(&s_ent[s], 1);
+                    if (m != 0u) atomicAdd(&s_pairs[s], __popc(m));
+                    if (wbase + i < sa.list_cap) {
+                      const uint32_t key = (static_cast<uint32_t>(s) << kStripIdBits) | uint32_t(id);
+                      s_list[wbase + i] = make_int2(static_cast<int>(key), static_cast<int>(m));
+                    }
+                  }
+                });
+  }
+  __syncthreads();
+  // Exclusive scans of the entry and pair counts in one: pairs << 32 | entries (the entries of a
+  // chunk stay below 2^32, so the low word never carries).  8 strips a thread.
+  constexpr int kPer = 8;
+  const int b0 = tid * kPer;
+  uint64_t x[kPer];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    x[k] = b0 + k < sa.strips
+               ? (uint64_t(uint32_t(s_pairs[b0 + k])) << 32) | uint32_t(s_ent[b0 + k])
+               : 0ull;
+    sum += x[k];
+  }
+  int64_t total64;
+  uint64_t run = static_cast<uint64_t>(
+      block_exclusive_scan<kStripL1Threads>(static_cast<int64_t>(sum), &total64));
+  const uint64_t total = static_cast<uint64_t>(total64);
+  const int64_t entries = static_cast<int64_t>(total & 0xffffffffull);
+  const int64_t pairs = static_cast<int64_t>(total >> 32);
+  int2* row = sa.offs + int64_t(c) * (sa.strips + 1);
+  auto clamp32 = [](uint64_t v) { return static_cast<int32_t>(v < INT32_MAX ? v : INT32_MAX); };
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (b0 + k < sa.strips) {
+      const int32_t e = clamp32(run & 0xffffffffull);
+      s_ent[b0 + k] = e;  // the strip's cursor in the region
+      row[b0 + k] = make_int2(e, clamp32(run >> 32));
+    }
+    run += x[k];
+  }
+  if (tid == 0) row[sa.strips] = make_int2(clamp32(entries), clamp32(pairs));
+  if (entries > sa.chunk_cap) return;  // level 2 recomputes this chunk's masks
+  __syncthreads();
+  int2* region = sa.bucket + int64_t(c) * sa.chunk_cap;
+  if (entries <= sa.list_cap) {
+    for (int i = tid; i < entries; i += kStripL1Threads) {
+      const int2 e = s_list[i];
+      const int s = static_cast<int>(static_cast<uint32_t>(e.x) >> kStripIdBits);
+      const int g = static_cast<int>(g0 + (e.x & (kStripMaxChunk - 1)));
+      region[atomicAdd(&s_ent[s], 1)] = make_int2(g, e.y);
+    }
+    return;
+  }
+  for (int sub = 0; sub < sa.chunk; sub += kStripL1Threads) {
+    strip_steps(a, lg, g0, sub,
+                [&](int s_lo, int ne, const GaussTiles& ot, float oa, float ob, float oc, int id) {
+                  for (int i = 0; i < ne; ++i) {
+                    const int s = s_lo + i;
+                    const uint32_t m = strip_mask(a, ot, oa, ob, oc, s, lg);
+                    region[atomicAdd(&s_ent[s], 1)] =
+                        make_int2(static_cast<int>(g0 + id), static_cast<int>(m));
+                  }
+                });
+  }
+}
+
+// OR of a value over the 64 lanes of a wave (all lanes active).
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  v |= __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+  v |= __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+  v |= __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v |= __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v |= __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+// Sorts the list g[0, L) ascending by one wave in registers (L <= 1024); longer lists are left
+// to the caller.
+__device__ __forceinline__ void wave_sort_list(int32_t* g, int L, int lane) {
+  if (L <= 1) return;
+  if (L <= 64) {
+    sort_tile_in_registers<1>(g, L, lane);
+  } else if (L <= 128) {
+    sort_tile_in_registers<2>(g, L, lane);
+  } else if (L <= 256) {
+    sort_tile_in_registers<4>(g, L, lane);
+  } else if (L <= 512) {
+    sort_tile_in_registers<8>(g, L, lane);
+  } else if (L <= 1024) {
+    sort_tile_in_registers<16>(g, L, lane);
+  }
+}
+
+constexpr int kStripCountOnly = 0;  // level 2 writes the strip's offsets
+constexpr int kStripBuild = 1;      // offsets and lists
+constexpr int kStripFill = 2;       // lists, placed by the caller's offsets
+
+// Sorts the list src[0, L) (L <= 64 kR) ascending by one wave in registers and writes the first
+// n <= L values to dst.
+template <int kR>
+__device__ __forceinline__ void sort_list_to(const int32_t* src, int32_t* dst, int L, int n,
+                                             int lane) {
+  int32_t v[kR];
+#pragma unroll
+  for (int r = 0; r < kR; ++r) v[r] = 64 * r + lane < L ? src[64 * r + lane] : INT32_MAX;
+  wave_sort_regs<kR>(v, lane);
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    if (64 * r + lane < n) dst[64 * r + lane] = v[r];
+  }
+}
+
+// One wave: src[0, L) sorted into dst[0, n) (L <= 1024; returns false for longer lists).
+__device__ __forceinline__ bool wave_sort_to(const int32_t* src, int32_t* dst, int L, int n,
+                                             int lane) {
+  if (L <= 64) {
+    sort_list_to<1>(src, dst, L, n, lane);
+  } else if (L <= 128) {
+    sort_list_to<2>(src, dst, L, n, lane);
+  } else if (L <= 256) {
+    sort_list_to<4>(src, dst, L, n, lane);
+  } else if (L <= 512) {
+    sort_list_to<8>(src, dst, L, n, lane);
+  } else if (L <= 1024) {
+    sort_list_to<16>(src, dst, L, n, lane);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// Level 2: one workgroup per strip.  Its matrix column gives, per chunk, the strip's entries in
+// the chunk's region (or, for an overflowed chunk, its Gaussians as candidates, whose masks are
+// computed here) and the chunk's pairs before the strip, whose sum is the strip's first list
+// position.  Candidates are in chunk order (ascending ids between chunks, any order inside one).
+//
+// Usual case (at most kStripCache candidates): the candidates are loaded into LDS once, and each
+// wave compacts, for each of its tiles, the candidates holding that tile into the tile's LDS
+// segment (a ballot and a bit count per 64 candidates; the counts come out exact even when a
+// segment is full); a wave scan of the counts gives the tiles' offsets; each tile's pairs are
+// then sorted by one wave in registers straight into its list.  Otherwise (more candidates, or
+// a tile beyond its segment): ballots count the pairs per tile, the pairs are placed per tile
+// in LDS in any order (in the lists themselves when the strip has more than kStripPairsLds) and
+// each list is sorted.  Fill mode places by the caller's offsets (clamped to gid_capacity) and
+// flags *status when a tile holds more pairs than its range.  Eight waves a SIMD: every
+// workgroup of a 2048x2048 grid at 16-pixel tiles is resident at once.
+template <int kMode>
+__global__ __launch_bounds__(kStripL2Threads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_strip_level2(BinArgs a, StripArgs sa, int32_t* offsets, int32_t* gids,
+                    int64_t gid_capacity, int32_t* __restrict__ ws_status,
+                    int32_t* __restrict__ status) {
+  constexpr int kWaves = kStripL2Threads / 64;
+  __shared__ int s_cnt[32];
+  __shared__ int s_cur[32];
+  __shared__ int s_loc[32];   // the tile's first pair in the strip
+  __shared__ int s_len[32];   // pairs written to its list
+  __shared__ int64_t s_out[32];
+  __shared__ int s_full;      // a tile's pairs outgrew its segment
+  // Per chunk: the exclusive prefix of its candidates and the first entry of this strip in its
+  // region (-1: overflowed chunk, candidates are its Gaussians).
+  __shared__ int32_t s_tab[2 * kStripMaxChunks];
+  __shared__ int2 s_cache[kStripCache];
+  __shared__ int32_t s_buf[kStripPairsLds];  // the strip's pairs, per tile
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int S = 1 << sa.lg;
+  const int seg = kStripPairsLds >> sa.lg;  // a tile's LDS segment in the usual case
+  const int s = xcd_remap(blockIdx.x, gridDim.x);
+  const int s0 = s << sa.lg;
+  if (tid < 32) {
+    s_cnt[tid] = 0;
+    s_cur[tid] = 0;
+  }
+  if (tid == 0) s_full = 0;
+  // The level-1 status is final here: merge it into the caller's sticky status word.
+  if (kMode != kStripCountOnly && status != nullptr && blockIdx.x == 0 && tid == 0 &&
+      *ws_status != SSPLAT_OK) {
+    *status = *ws_status;
+  }
+
+  // Candidates per chunk and their exclusive prefix; the pairs before the strip (2 chunks a
+  // thread); the last strip also sums every chunk's pairs.
+  constexpr int kPer = 2;
+  const bool last = s == sa.strips - 1;
+  int nc[kPer];
+  int64_t sum = 0, before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int c = tid * kPer + k;
+    nc[k] = 0;
+    if (c < sa.chunks) {
+      const int2* row = sa.offs + int64_t(c) * (sa.strips + 1);
+      const int2 lo = row[s], hi = row[s + 1], total = row[sa.strips];
+      if (total.x > sa.chunk_cap) {
+        const int64_t left = int64_t(a.n) - int64_t(c) * sa.chunk;
+        nc[k] = static_cast<int>(left < sa.chunk ? left : sa.chunk);
+        s_tab[2 * c + 1] = -1;
+      } else {
+        nc[k] = hi.x - lo.x;
+        s_tab[2 * c + 1] = lo.x;
+      }
+      before += lo.y;
+      all += total.y;
+    }
+    sum += nc[k];
+  }
+  int64_t count, first, total_pairs = 0;
+  int64_t run = block_exclusive_scan<kStripL2Threads>(sum, &count);  // ends with a barrier
+  block_exclusive_scan<kStripL2Threads>(before, &first);
+  if (last) block_exclusive_scan<kStripL2Threads>(all, &total_pairs);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int c = tid * kPer + k;
+    if (c < sa.chunks) s_tab[2 * c] = static_cast<int32_t>(run);
+    run += nc[k];
+  }
+  __syncthreads();
+
+  // Candidate j: its chunk by binary search over the prefixes, then its entry (or Gaussian).
+  auto candidate = [&](int64_t j, int* g, uint32_t* m) {
+    *g = -1;
+    *m = 0u;
+    if (j >= count) return;
+    int lo = 0, hi = sa.chunks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_tab[2 * mid] <= j) {
+        lo = mid;
+      } else {
+        hi = mid - 1;
+      }
+    }
+    const int i = static_cast<int>(j - s_tab[2 * lo]);
+    const int entry0 = s_tab[2 * lo + 1];
+    if (entry0 < 0) {
+      const int id = static_cast<int>(int64_t(lo) * sa.chunk + i);  // below n
+      *g = id;
+      *m = entry_mask(a, id, s, sa.lg);
+    } else {
+      const int2 e = sa.bucket[int64_t(lo) * sa.chunk_cap + entry0 + i];
+      if (e.x >= 0 && e.x < a.n) {
+        *g = e.x;
+        *m = static_cast<uint32_t>(e.y);
+      }
+    }
+  };
+
+  // Pairs per tile.
+  const bool cached = count <= kStripCache;
+  if (cached) {
+    for (int64_t base = 0; base < count; base += kStripL2Threads) {
+      const int64_t j = base + tid;
+      int g;
+      uint32_t m;
+      candidate(j, &g, &m);
+      if (j < count) s_cache[j] = make_int2(g, static_cast<int>(m));
+    }
+    __syncthreads();
+    // Compaction: a wave per tile, candidates in order (64 a step).
+    const int rounds = static_cast<int>((count + 63) >> 6);
+    for (int k = wave; k < S; k += kWaves) {
+      int32_t* dst = s_buf + k * seg;
+      int n = 0;
+      for (int r = 0; r < rounds; ++r) {
+        const int j = (r << 6) + lane;
+        const int2 e = j < count ? s_cache[j] : make_int2(-1, 0);
+        const bool hit = (static_cast<uint32_t>(e.y) >> k) & 1u;
+        const uint64_t b = __ballot(hit);
+        const int pos = n + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                static_cast<uint32_t>(b >> 32),
+                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u)));
+        if (hit && pos < seg) dst[pos] = e.x;
+        n += static_cast<int>(__popcll(b));
+      }
+      if (lane == 0) {
+        s_cnt[k] = n;
+        if (n > seg) s_full = 1;
+      }
+    }
+  } else {
+    for (int64_t base = 0; base < count; base += kStripL2Threads) {
+      const int64_t j = base + tid;
+      int g;
+      uint32_t m;
+      candidate(j, &g, &m);
+      uint32_t wm = wave_or(m);
+      while (wm != 0u) {
+        const int k = __ffs(wm) - 1;
+        wm &= wm - 1;
+        const uint64_t b = __ballot((m >> k) & 1u);
+        if (lane == 0) atomicAdd(&s_cnt[k], static_cast<int>(__popcll(b)));
+      }
+    }
+  }
+  __syncthreads();
+
+  // Offsets: a wave scan of the per-tile counts from the strip's first pair.
+  if (wave == 0) {
+    const int c = lane < S ? s_cnt[lane] : 0;
+    const int incl = wave_inclusive_scan(c);
+    const int excl = incl - c;
+    if (lane < 32) s_loc[lane] = excl;
+    if (kMode != kStripFill && last && lane == 0) {
+      offsets[sa.num_tiles] =
+          static_cast<int32_t>(total_pairs > INT32_MAX ? INT32_MAX : total_pairs);
+      if (total_pairs > INT32_MAX || (kMode == kStripBuild && total_pairs > gid_capacity)) {
+        *ws_status = SSPLAT_ERR_CAPACITY;
+        if (status != nullptr) *status = SSPLAT_ERR_CAPACITY;
+      }
+    }
+    if (lane < S) {
+      const int tile = s0 + lane;
+      if (kMode == kStripFill) {
+        int64_t lo = 0, hi = 0;
+        if (tile < sa.num_tiles) {
+          lo = offsets[tile];
+          hi = offsets[tile + 1];
+        }
+        const int64_t cap = gid_capacity;
+        const int64_t b = lo < 0 ? 0 : (lo > cap ? cap : lo);
+        const int64_t end = hi < b ? b : (hi > cap ? cap : hi);
+        s_out[lane] = b;
+        s_len[lane] = static_cast<int>(c < end - b ? c : end - b);
+        if (c > end - b && status != nullptr) *status = SSPLAT_ERR_CAPACITY;
+      } else {
+        const int64_t pos = first + excl;
+        if (tile < sa.num_tiles) {
+          offsets[tile] = static_cast<int32_t>(pos > INT32_MAX ? INT32_MAX : pos);
+        }
+        const int64_t cap = kMode == kStripBuild ? gid_capacity : 0;
+        s_out[lane] = pos;
+        s_len[lane] = static_cast<int>(pos >= cap ? 0 : (c < cap - pos ? c : cap - pos));
+      }
+    }
+  }
+  if (kMode == kStripCountOnly) return;
+  __syncthreads();
+
+  if (cached && s_full == 0) {
+    // Each tile's compacted pairs sorted by one wave into its list.
+    for (int k = wave; k < S; k += kWaves) {
+      wave_sort_to(s_buf + k * seg, gids + s_out[k], s_cnt[k], s_len[k], lane);
+    }
+    return;
+  }
+
+  // The strip's pairs fit the LDS buffer: place there, sort into the lists.  Otherwise place
+  // straight into the lists (up to each list's length) and sort them in place.
+  const bool in_lds = s_loc[S - 1] + s_cnt[S - 1] <= kStripPairsLds;
+  // Placement: ranks per tile by ballots, one LDS atomic per wave and tile.
+  for (int64_t base = 0; base < count; base += kStripL2Threads) {
+    const int64_t j = base + tid;
+    int g;
+    uint32_t m;
+    if (cached) {
+      const int2 e = j < count ? s_cache[j] : make_int2(-1, 0);
+      g = e.x;
+      m = static_cast<uint32_t>(e.y);
+    } else {
+      candidate(j, &g, &m);
+    }
+    uint32_t wm = wave_or(m);
+    while (wm != 0u) {
+      const int k = __ffs(wm) - 1;
+      wm &= wm - 1;
+      const uint64_t b = __ballot((m >> k) & 1u);
+      const int leader = __ffsll(static_cast<unsigned long long>(b)) - 1;
+      int r0 = 0;
+      if (lane == leader) r0 = atomicAdd(&s_cur[k], static_cast<int>(__popcll(b)));
+      r0 = __builtin_amdgcn_readlane(r0, leader);
+      if ((m >> k) & 1u) {
+        const int rank =
+            r0 + static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(
+                                                                static_cast<uint32_t>(b), 0u)));
+        if (in_lds) {
+          s_buf[s_loc[k] + rank] = g;
+        } else if (rank < s_len[k]) {
+          gids[s_out[k] + rank] = g;
+        }
+      }
+    }
+  }
+  // The barrier's workgroup-scope fences order the LDS and global stores before the reads of
+  // other waves of this workgroup (a device-scope fence would write back this XCD's whole L2,
+  // per workgroup).
+  __syncthreads();
+
+  // Sort every tile's pairs into its list: a wave per tile in registers, the block for the rare
+  // lists above 1024 entries (in LDS up to kSortLds, else in global memory).
+  for (int k = wave; k < S; k += kWaves) {
+    if (in_lds) {
+      wave_sort_to(s_buf + s_loc[k], gids + s_out[k], s_cnt[k], s_len[k], lane);
+    } else {
+      wave_sort_list(gids + s_out[k], s_len[k], lane);
+    }
+  }
+  for (int k = 0; k < S; ++k) {
+    const int L = in_lds ? s_cnt[k] : s_len[k];
+    if (L <= 1024) continue;
+    __syncthreads();
+    int32_t* g = gids + s_out[k];
+    if (in_lds) {
+      bitonic_sort<false>(s_buf + s_loc[k], L);
+      const int n = s_len[k];
+      for (int i = tid; i < n; i += kStripL2Threads) g[i] = s_buf[s_loc[k] + i];
+    } else if (L <= kSortLds) {
+      for (int i = tid; i < L; i += kStripL2Threads) s_buf[i] = g[i];
+      __syncthreads();
+      bitonic_sort<false>(s_buf, L);
+      for (int i = tid; i < L; i += kStripL2Threads) g[i] = s_buf[i];
+    } else {
+      bitonic_sort<true>(g, L);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Rendering
 // ------------------------------------------------------------------------------------------
 
@@ -2207,9 +2877,49 @@ struct Workspace {
   int32_t* rect_status;  // int32 overflow of that scan
   int32_t* queue_fwd;    // kQueues work-queue counters of a persistent forward
   int32_t* queue_bwd;    // and of a persistent backward
+  StripArgs strip;       // strip binning buffers (strip.strips == 0: the per-pair kernels bin)
   int num_waves;
+  size_t render_bytes;   // the part the render calls use (the strip buffers follow it)
   size_t bytes;
 };
+
+// Level-1 LDS of the two counter arrays (strips + 1 each, padded to 8 bytes) and its list.
+int strip_counter_bytes(int strips) { return 2 * ((strips + 2) & ~1) * int(sizeof(int32_t)); }
+
+// Strips of a tile grid: 16 tiles each, or 32 when 16 would make more than kStripMax strips
+// (none when 32 does too: the per-pair kernels bin such grids); fewer tiles a strip, down to
+// one, when 16 would leave fewer than kStripTarget strips, so a small grid still spreads its
+// pairs over many level-2 workgroups.  Chunks: kStripChunkUnit Gaussians, or the multiple of
+// it that keeps them within kStripMaxChunks.
+void strip_geometry(int64_t n, int num_tiles, StripArgs* sa) {
+  sa->lg = 0;
+  sa->strips = 0;
+  sa->num_tiles = num_tiles;
+  auto strips_at = [&](int l) { return (int64_t(num_tiles) + (1 << l) - 1) >> l; };
+  int lg = 4;
+  if (strips_at(4) > kStripMax) {
+    if (strips_at(5) > kStripMax) return;
+    lg = 5;
+  } else {
+    while (lg > 0 && strips_at(lg) < kStripTarget) --lg;
+  }
+  sa->lg = lg;
+  sa->strips = static_cast<int>(strips_at(lg));
+  const int64_t per = int64_t(kStripMaxChunks) * kStripChunkUnit;
+  const int64_t units = (n + per - 1) / per;
+  const int64_t chunk = (units > 1 ? units : 1) * kStripChunkUnit;
+  if (chunk > kStripMaxChunk) {  // beyond the packed entry's ids: the per-pair kernels bin
+    sa->lg = 0;
+    sa->strips = 0;
+    return;
+  }
+  sa->chunk = static_cast<int>(chunk);
+  const int64_t chunks = (n + sa->chunk - 1) / sa->chunk;
+  sa->chunks = static_cast<int>(chunks > 1 ? chunks : 1);
+  sa->chunk_cap = kStripEntriesPerGaussian * sa->chunk;
+  // The level-1 list takes what its LDS leaves after the two counter arrays.
+  sa->list_cap = (kStripL1Lds - strip_counter_bytes(sa->strips)) / int(sizeof(int2));
+}
 
 Workspace layout(void* base, int64_t n, int num_tiles) {
   Workspace w{};
@@ -2234,6 +2944,15 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   off = align_up(off + sizeof(int32_t), 256);
   const size_t queues = off;
   off = align_up(off + 2 * kQueues * sizeof(int32_t), 256);
+  const size_t render_bytes = off;
+  // Strip binning: the chunks' offset matrix and regions.
+  StripArgs sa{};
+  strip_geometry(n, num_tiles, &sa);
+  const bool strips = sa.strips > 0;
+  const size_t offs = off;
+  if (strips) off = align_up(off + size_t(sa.chunks) * (sa.strips + 1) * sizeof(int2), 256);
+  const size_t bucket = off;
+  if (strips) off = align_up(off + size_t(sa.chunks) * sa.chunk_cap * sizeof(int2), 256);
   char* b = static_cast<char*>(base);
   w.recs = reinterpret_cast<GaussRec*>(b + recs);
   w.counts = reinterpret_cast<int32_t*>(b + counts);
@@ -2245,20 +2964,27 @@ Workspace layout(void* base, int64_t n, int num_tiles) {
   w.rect_status = reinterpret_cast<int32_t*>(b + rect_status);
   w.queue_fwd = reinterpret_cast<int32_t*>(b + queues);
   w.queue_bwd = w.queue_fwd + kQueues;
+  w.strip = sa;
+  w.strip.offs = reinterpret_cast<int2*>(b + offs);
+  w.strip.bucket = reinterpret_cast<int2*>(b + bucket);
   w.num_waves = static_cast<int>(waves);
+  w.render_bytes = render_bytes;
   w.bytes = off;
   return w;
 }
 
-int check_workspace(void* ws, size_t ws_bytes, const Plan& p, Workspace* w) {
+// binning: the call bins (it needs the strip buffers too, not only the render part).
+int check_workspace(void* ws, size_t ws_bytes, const Plan& p, Workspace* w,
+                    bool binning = false) {
   *w = layout(ws, p.n, p.num_tiles);
   if (ws == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace is NULL");
   if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
   }
-  if (ws_bytes < w->bytes) {
+  const size_t need = binning ? w->bytes : w->render_bytes;
+  if (ws_bytes < need) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "workspace has %zu bytes, needs %zu", ws_bytes,
-                     w->bytes);
+                     need);
   }
   return SSPLAT_OK;
 }
@@ -2357,16 +3083,44 @@ int run_fill(const BinArgs& a, const Plan& p, const Workspace& w, const int32_t*
   return SSPLAT_OK;
 }
 
+// Strip binning (two launches): mode kStripCountOnly writes offsets, kStripBuild offsets and
+// gids, kStripFill gids placed by the given offsets.
+int run_strip_bin(const BinArgs& a, const Workspace& w, int mode, int32_t* offsets,
+                  int32_t* gids, int64_t capacity, int32_t* status, hipStream_t stream) {
+  const StripArgs& sa = w.strip;
+  const size_t lds = kStripL1Lds;
+  k_strip_level1<<<sa.chunks, kStripL1Threads, lds, stream>>>(a, sa, w.status);
+  SSPLAT_LAUNCH_CHECK("k_strip_level1");
+  if (mode == kStripCountOnly) {
+    k_strip_level2<kStripCountOnly><<<sa.strips, kStripL2Threads, 0, stream>>>(
+        a, sa, offsets, gids, capacity, w.status, status);
+  } else if (mode == kStripBuild) {
+    k_strip_level2<kStripBuild><<<sa.strips, kStripL2Threads, 0, stream>>>(
+        a, sa, offsets, gids, capacity, w.status, status);
+  } else {
+    k_strip_level2<kStripFill><<<sa.strips, kStripL2Threads, 0, stream>>>(
+        a, sa, offsets, gids, capacity, w.status, status);
+  }
+  SSPLAT_LAUNCH_CHECK("k_strip_level2");
+  return SSPLAT_OK;
+}
+
 int tile_index_count(const float* means, const float* conics, const int64_t* radii,
                      const Plan& p, int cull, double sigma_cutoff, int32_t* offsets, void* ws,
                      size_t ws_bytes, void* stream) {
   int rc;
   if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
   Workspace w;
-  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w, true))) return rc;
   BinArgs a;
   if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
-  if ((rc = run_count(a, p, w, offsets, static_cast<hipStream_t>(stream)))) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (w.strip.strips > 0) {
+    rc = run_strip_bin(a, w, kStripCountOnly, offsets, nullptr, 0, nullptr, s);
+  } else {
+    rc = run_count(a, p, w, offsets, s);
+  }
+  if (rc) return rc;
   return success();
 }
 
@@ -2380,11 +3134,42 @@ int tile_index_fill(const float* means, const float* conics, const int64_t* radi
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or capacity negative");
   }
   Workspace w;
-  if ((rc = check_workspace(ws, ws_bytes, p, &w))) return rc;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w, true))) return rc;
   BinArgs a;
   if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
-  if ((rc = run_fill(a, p, w, offsets, gids, capacity, status, rebuild_cursors,
-                     static_cast<hipStream_t>(stream)))) {
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (w.strip.strips > 0) {
+    rc = run_strip_bin(a, w, kStripFill, const_cast<int32_t*>(offsets), gids, capacity, status, s);
+  } else {
+    rc = run_fill(a, p, w, offsets, gids, capacity, status, rebuild_cursors, s);
+  }
+  if (rc) return rc;
+  return success();
+}
+
+// Both binning phases in one call: offsets and lists, no host synchronization.
+int build_tile_index(const float* means, const float* conics, const int64_t* radii,
+                     const Plan& p, int cull, double sigma_cutoff, int32_t* offsets,
+                     int32_t* gids, int64_t capacity, int32_t* status, void* ws, size_t ws_bytes,
+                     void* stream) {
+  int rc;
+  if (offsets == nullptr) return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_offsets is NULL");
+  if (capacity < 0 || (capacity > 0 && gids == nullptr)) {
+    return set_error(SSPLAT_ERR_INVALID_ARGUMENT, "tile_gids is NULL or capacity negative");
+  }
+  Workspace w;
+  if ((rc = check_workspace(ws, ws_bytes, p, &w, true))) return rc;
+  if (w.strip.strips == 0) {
+    rc = tile_index_count(means, conics, radii, p, cull, sigma_cutoff, offsets, ws, ws_bytes,
+                          stream);
+    if (rc) return rc;
+    return tile_index_fill(means, conics, radii, p, cull, sigma_cutoff, offsets, gids, capacity,
+                           status, ws, ws_bytes, false, stream);
+  }
+  BinArgs a;
+  if ((rc = make_bin_args(means, conics, radii, p, cull, sigma_cutoff, &a))) return rc;
+  if ((rc = run_strip_bin(a, w, kStripBuild, offsets, gids, capacity, status,
+                          static_cast<hipStream_t>(stream)))) {
     return rc;
   }
   return success();
@@ -2968,14 +3753,10 @@ int ssplat_build_tile_index(const float* means, const float* conics, const int64
                             int64_t gid_capacity, int32_t* status, void* workspace,
                             size_t workspace_bytes, void* stream) {
   Plan p;
-  int rc = make_single_plan(n, height, width, tile_size, &p);
+  const int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  rc = tile_index_count(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                        workspace, workspace_bytes, stream);
-  if (rc) return rc;
-  return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                         tile_gids, gid_capacity, status, workspace, workspace_bytes, false,
-                         stream);
+  return build_tile_index(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                          tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
 }
 
 int ssplat_forward_tiled(const float* means, const float* conics, const float* colors,
@@ -3001,7 +3782,7 @@ int ssplat_forward_tiled_noalloc(const float* means, const float* conics, const 
   Plan p;
   int rc = make_single_plan(n, height, width, tile_size, &p);
   if (rc) return rc;
-  const size_t need = layout(nullptr, p.n, p.num_tiles).bytes;
+  const size_t need = layout(nullptr, p.n, p.num_tiles).render_bytes;
   if (p.n > kNoAllocMaxGaussians || p.num_tiles > kNoAllocMaxTiles ||
       need > kNoAllocScratchBytes) {
     return set_error(SSPLAT_ERR_INVALID_ARGUMENT,
@@ -3185,14 +3966,10 @@ int ssplat_batch_build_tile_index(const ssplat_field* fields, int num_fields,
                                   int32_t* tile_gids, int64_t gid_capacity, int32_t* status,
                                   void* workspace, size_t workspace_bytes, void* stream) {
   Plan p;
-  int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
+  const int rc = make_batch_plan(fields, num_fields, total_gaussians, total_tiles, tile_size, &p);
   if (rc) return rc;
-  rc = tile_index_count(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                        workspace, workspace_bytes, stream);
-  if (rc) return rc;
-  return tile_index_fill(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
-                         tile_gids, gid_capacity, status, workspace, workspace_bytes, false,
-                         stream);
+  return build_tile_index(means, conics, radii, p, ellipse_cull, sigma_cutoff, tile_offsets,
+                          tile_gids, gid_capacity, status, workspace, workspace_bytes, stream);
 }
 
 int ssplat_batch_forward_tiled(const ssplat_field* fields, int num_fields,

@@ -36,10 +36,11 @@ def test_tile_index_bit_exact(cuda, ts, cull):
 
 
 @pytest.mark.parametrize("fused", [False, True])
-@pytest.mark.parametrize("H,W,ts", [(1100, 1100, 4), (300, 200, 4)])
+@pytest.mark.parametrize("H,W,ts", [(1100, 1100, 4), (300, 200, 4), (1100, 1100, 2)])
 def test_tile_index_scan_paths_bit_exact(cuda, H, W, ts, fused):
-    """Both scans of the counts: 75,626 counters take the three-launch scan, 3,751 the
-    one-workgroup scan; both through the two-call and the one-call binning."""
+    """Both binning schemes: 75,625 and 3,750 tiles bin by strips of 16 tiles (4,727 and 235
+    strips); 302,500 tiles, more than 7,680 strips of 32, bin by the per-pair kernels, whose
+    counters take the three-launch scan.  Through the two-call and the one-call binning."""
     _print_case(test="tile_index_scan_paths", H=H, W=W, ts=ts, fused=fused)
     f = random_field(33, 3000, H, W, scale_lo=0.6, scale_hi=12.0, margin=8.0)
     means, conics, colors, radii, _ = field_tensors(f, cuda)
@@ -56,6 +57,44 @@ def test_tile_index_scan_paths_bit_exact(cuda, H, W, ts, fused):
         torch.cuda.synchronize()
     assert to_np(offsets).tolist() == ref_o.tolist()
     assert to_np(gids).tolist() == ref_g.tolist()
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("cull", [False, True])
+def test_tile_index_bucket_overflow_bit_exact(cuda, cull, fused):
+    """Gaussians whose rectangles cover the whole image meet every strip: 100 needles 300-600 px
+    long at 1024x1024 with 4-pixel tiles (4,096 strips of 16 tiles) make ~400,000 strip entries,
+    beyond the bucket's 8 N + 65,536 = 68,736 (N = 400), so the strips past it take every
+    Gaussian as a candidate.  The ellipse cull drops most of the needles' pairs.  The lists equal
+    the oracle's."""
+    _print_case(test="bucket_overflow", cull=cull, fused=fused)
+    rng = np.random.default_rng(5)
+    H = W = 1024
+    nb, ns = 100, 300
+    means_np = np.concatenate([
+        np.stack([rng.uniform(0, W, nb), rng.uniform(0, H, nb)], 1),
+        np.stack([rng.uniform(0, W, ns), rng.uniform(0, H, ns)], 1)]).astype(np.float32)
+    scales = np.concatenate([
+        np.stack([rng.uniform(300, 600, nb), rng.uniform(20, 60, nb)], 1),
+        np.exp(rng.uniform(np.log(0.6), np.log(8.0), (ns, 2)))]).astype(np.float32)
+    order = rng.permutation(nb + ns)
+    f = {"means": means_np[order], "scales": scales[order],
+         "angles": rng.uniform(0, np.pi, nb + ns).astype(np.float32),
+         "colors": rng.random((nb + ns, 3)).astype(np.float32), "opacity": None}
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    ref_g, ref_o = tile_index.build_tile_index(to_np(means), to_np(conics), to_np(radii), H, W,
+                                               4, cull, 3.0)
+    if fused:
+        gids, offsets, status = HR.build_tile_index_capacity(means, conics, radii, H, W,
+                                                             len(ref_g) + 3, 4, cull, 3.0)
+        torch.cuda.synchronize()
+        assert int(status.item()) == 0
+        gids = gids[:len(ref_g)]
+    else:
+        gids, offsets = HR.build_tile_index(means, conics, radii, H, W, 4, cull, 3.0)
+        torch.cuda.synchronize()
+    assert to_np(offsets).tolist() == ref_o.tolist()
+    assert np.array_equal(to_np(gids), ref_g)
 
 
 def test_tile_index_reference_golden(cuda):

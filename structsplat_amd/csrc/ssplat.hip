@@ -126,6 +126,9 @@ constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
 // The 16-pixel backward stages fewer records per batch: with the recomputing sweep's masks its
 // LDS then stays within 20 KB, so eight workgroups (eight waves per SIMD) fit on a CU.
 constexpr int kBwdStage16 = 128;
+// The backward sums a (tile, Gaussian) pair whose rectangle covers at most this many pixels of
+// the tile in one lane (no cross-lane reduction); larger pairs take a whole wave.
+constexpr int kLaneArea = 64;
 // Deterministic gradient accumulation.  Gaussian g owns R(g) slots, one per tile of its clipped
 // support rectangle (row-major); slots of consecutive Gaussians are consecutive.  The backward
 // writes each (tile, Gaussian) pair's nine partial gradients and the call's tag to the pair's
@@ -1007,7 +1010,8 @@ constexpr int kStripL1Lds = 65024;      // level-1 dynamic LDS: counters plus th
 constexpr int kStripEntriesPerGaussian = 8;  // a chunk's region: 8 entries per Gaussian
 constexpr int kStripCache = 1024;       // level-2 candidates kept in LDS between passes
 constexpr int kStripPairsLds = 4096;    // pairs a strip places in LDS (more: in the lists)
-static_assert(kStripMax <= kStripL1Threads * 8, "level 1 scans 8 strips per thread");
+constexpr int kStripL1PerThread = 8;    // strips a level-1 thread scans
+static_assert(kStripMax <= kStripL1Threads * kStripL1PerThread, "level 1 scans every strip");
 static_assert(kStripMaxChunks <= kStripL2Threads * 2, "level 2 scans 2 chunks per thread");
 static_assert(kStripMax < (1 << (32 - kStripIdBits)), "strip numbers fit the packed entry");
 static_assert(kStripPairsLds >= kSortLds, "long lists sort in the placement buffer");
@@ -1233,8 +1237,8 @@ __global__ __launch_bounds__(kStripL1Threads) void k_strip_level1(BinArgs a, Str
   }
   __syncthreads();
   // Exclusive scans of the entry and pair counts in one: pairs << 32 | entries (the entries of a
-  // chunk stay below 2^32, so the low word never carries).  8 strips a thread.
-  constexpr int kPer = 8;
+  // chunk stay below 2^32, so the low word never carries).
+  constexpr int kPer = kStripL1PerThread;
   const int b0 = tid * kPer;
   uint64_t x[kPer];
   uint64_t sum = 0;
@@ -1286,6 +1290,12 @@ __global__ __launch_bounds__(kStripL1Threads) void k_strip_level1(BinArgs a, Str
                   }
                 });
   }
+}
+
+// Set bits of a wave-wide mask below this lane (v_mbcnt).
+__device__ __forceinline__ int lanes_below(uint64_t b) {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+      static_cast<uint32_t>(b >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u)));
 }
 
 // OR of a value over the 64 lanes of a wave (all lanes active).
@@ -2216,15 +2226,8 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   }
   __syncthreads();
 
-  // Gradient sweep: each lane owns the pixel (lx, ly) of every quadrant; with 16-pixel tiles
-  // the four quadrants' adjoints stay in registers.
-  float4 adj[4];
-  if (kT == 16) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      adj[s] = s_adj[((s >> 1) * 8 + ly) * kT + (s & 1) * 8 + lx];
-    }
-  }
+  // Gradient sweep: each lane owns the pixel (lx, ly) of every quadrant; with 16-pixel tiles a
+  // wave record reads the four quadrants' adjoints into registers once.
   const bool has_opacity = g.has_opacity;
   const float tag = __uint_as_float(*g.epoch);
   constexpr int kLg = kT == 16 ? 4 : 5;
@@ -2234,6 +2237,86 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   const int comp = lane >> 3;
   const bool result_lane = (lane & 7) == 7;
   const float comp_scale = (comp == 2 || comp == 4) ? -0.5f : (comp == 3 ? -1.0f : 1.0f);
+  // The batch's small and large records (see below) and their counts.
+  __shared__ uint16_t s_small[kAffine ? 1 : kS];
+  __shared__ uint16_t s_large[kAffine ? 1 : kS];
+  __shared__ int s_nsmall, s_nlarge;
+  // One small staged record (index j, -1: none) by one quad of lanes: lane `sub` of the quad
+  // takes the record's pixels of this tile with row-major index sub, sub + 4, ... (the visit's
+  // arithmetic, summed in registers); a fixed two-step DPP sum gives every lane of the quad the
+  // same totals and lane 0 stores the pair's slot.  All lanes of the wave call it together.
+  auto quad_record = [&](int j, int sub) {
+    float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
+    int slot = -1;
+    if (j >= 0) {
+      const float4 v0 = st.c[0][j];
+      const float4 v1 = st.c[1][j];
+      const float4 v3 = st.c[3][j];
+      slot = __float_as_int(v3.w);
+      const uint32_t xr = __float_as_uint(v1.y), yr = __float_as_uint(v1.z);
+      const int ix0 = max(static_cast<int>(xr & 0xffff), tx0);
+      const int ix1 = min(static_cast<int>(xr >> 16), tx0 + kT);
+      const int iy0 = max(static_cast<int>(yr & 0xffff), ty0);
+      const int w = ix1 - ix0;
+      const int n = w * (min(static_cast<int>(yr >> 16), ty0 + kT) - iy0);
+      const float mx = v0.x, my = v0.y, qa = v0.z, qb = v0.w, qc = v1.x;
+      const float cr = v3.x, cg = v3.y, cbl = v3.z;
+      // Pixel `sub` of the row-major order, then steps of 4 pixels: dr rows and dc columns,
+      // with a carry into the next row.
+      const int dr = 4 / w, dc = 4 - dr * w;
+      int py = iy0 + sub / w;
+      int px = ix0 + sub - (py - iy0) * w;
+      for (int k = sub; k < n; k += 4) {
+        const float4 ad = s_adj[(py - ty0) * kT + (px - tx0)];
+        const float dx = static_cast<float>(px) - mx;
+        const float dy = static_cast<float>(py) - my;
+        const float t = fmaf(qa, dx, qb * dy);
+        const float e = fmaf(t, dx, qc * dy * dy);
+        const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
+        const float dw = fmaf(ad.x, cr, fmaf(ad.y, cg, fmaf(ad.z, cbl, ad.w)));
+        float base, dqr;
+        if (kFade) {
+          base = fmaxf(raw - tail, 0.0f);
+          dqr = raw > tail ? dw * raw : 0.0f;
+        } else {
+          base = raw;
+          dqr = dw * raw;
+        }
+        gcr = fmaf(ad.x, base, gcr);
+        gcg = fmaf(ad.y, base, gcg);
+        gcb = fmaf(ad.z, base, gcb);
+        gop = fmaf(dw, base, gop);
+        sx = fmaf(dqr, dx, sx);
+        sy = fmaf(dqr, dy, sy);
+        const float dqx = dqr * dx;
+        sxx = fmaf(dqx, dx, sxx);
+        sxy = fmaf(dqx, dy, sxy);
+        syy = fmaf(dqr * dy, dy, syy);
+        px += dc;
+        py += dr;
+        if (px >= ix1) {
+          px -= w;
+          ++py;
+        }
+      }
+    }
+    // Quad sums: (v0 + v1) + (v2 + v3) on every lane of the quad, bit for bit.
+    auto quad_sum = [](float& v) {
+      v += dpp_move<0xb1>(v);  // quad_perm [1,0,3,2]
+      v += dpp_move<0x4e>(v);  // quad_perm [2,3,0,1]
+    };
+    quad_sum(sx), quad_sum(sy), quad_sum(sxx), quad_sum(sxy), quad_sum(syy);
+    quad_sum(gcr), quad_sum(gcg), quad_sum(gcb), quad_sum(gop);
+    if (sub == 0 && slot >= 0) {
+      float2* dst = reinterpret_cast<float2*>(g.slots + int64_t(slot) * kSlotWords);
+      dst[0] = make_float2(sx, sy);
+      dst[1] = make_float2(sxx, sxy);
+      dst[2] = make_float2(syy, gcr);
+      dst[3] = make_float2(gcg, gcb);
+      dst[4] = make_float2(has_opacity ? gop : 0.0f, tag);
+    }
+  };
   int64_t start, end;
   tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kS) {
@@ -2242,7 +2325,10 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
                                                        SlotTile{tile_x, tile_y, kLg,
                                                                 g.slot_capacity});
     __syncthreads();
-    for (int j = wave; j < m; j += kWaves) {
+    // One staged record by the whole wave: each lane visits its pixel of every quadrant the
+    // rectangle meets, then the wave's transposed reduction and one slot store.
+    // With 16-pixel tiles `adj` holds the lane's adjoints of the four quadrants.
+    auto wave_record = [&](int j, const float4 (&adj)[4]) {
       const float4 v0 = st.c[0][j];
       const float4 v1 = st.c[1][j];
       const float4 v2 = st.c[2][j];
@@ -2445,6 +2531,65 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
         } else {
           if (lane == 63) *reinterpret_cast<float2*>(dst + 8) = make_float2(gop_sum, tag);
         }
+      }
+    };
+    // The lane's adjoints of the four quadrants (16-pixel tiles), read from LDS into registers
+    // once per batch, where a wave starts its large records (not live during the lane path).
+    auto load_adj = [&](float4 (&adj)[4]) {
+      if constexpr (kT == 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          adj[s] = s_adj[((s >> 1) * 8 + ly) * kT + (s & 1) * 8 + lx];
+        }
+      }
+    };
+    if constexpr (kAffine) {
+      float4 adj[4];
+      load_adj(adj);
+      for (int j = wave; j < m; j += kWaves) wave_record(j, adj);
+    } else {
+      // Records whose rectangle covers at most kLaneArea pixels of the tile are summed by a quad
+      // of lanes each over those pixels (adjoints from LDS): a wave takes 16 of them at a time,
+      // with only a quad sum to reduce; wave 0 sorts the batch into those and the larger ones
+      // (empty intersections have no slot and are dropped).  Both kinds are dealt round-robin
+      // to the waves: 16 small records a wave at a time, then one large record at a time.
+      if (wave == 0) {
+        int ns = 0, nl = 0;
+        for (int base = 0; base < m; base += 64) {
+          const int j = base + lane;
+          int area = 0;
+          if (j < m) {
+            const float4 v1 = st.c[1][j];
+            const uint32_t xr = __float_as_uint(v1.y), yr = __float_as_uint(v1.z);
+            const int w = min(static_cast<int>(xr >> 16), tx0 + kT) -
+                          max(static_cast<int>(xr & 0xffff), tx0);
+            const int h = min(static_cast<int>(yr >> 16), ty0 + kT) -
+                          max(static_cast<int>(yr & 0xffff), ty0);
+            area = w > 0 && h > 0 ? w * h : 0;
+          }
+          const bool small = area > 0 && area <= kLaneArea;
+          const bool large = area > kLaneArea;
+          const uint64_t bs = __ballot(small), bl = __ballot(large);
+          if (small) s_small[ns + lanes_below(bs)] = static_cast<uint16_t>(j);
+          if (large) s_large[nl + lanes_below(bl)] = static_cast<uint16_t>(j);
+          ns += static_cast<int>(__popcll(bs));
+          nl += static_cast<int>(__popcll(bl));
+        }
+        if (lane == 0) {
+          s_nsmall = ns;
+          s_nlarge = nl;
+        }
+      }
+      __syncthreads();
+      const int ns = s_nsmall, nl = s_nlarge;
+      for (int base = wave * 16; base < ns; base += kWaves * 16) {
+        const int i = base + (lane >> 2);
+        quad_record(i < ns ? static_cast<int>(s_small[i]) : -1, lane & 3);
+      }
+      if (wave < nl) {
+        float4 adj[4];
+        load_adj(adj);
+        for (int i = wave; i < nl; i += kWaves) wave_record(s_large[i], adj);
       }
     }
     __syncthreads();

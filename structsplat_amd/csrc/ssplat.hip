@@ -2308,13 +2308,13 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
     };
     quad_sum(sx), quad_sum(sy), quad_sum(sxx), quad_sum(sxy), quad_sum(syy);
     quad_sum(gcr), quad_sum(gcg), quad_sum(gcb), quad_sum(gop);
-    if (sub == 0 && slot >= 0) {
+    if (slot >= 0) {
+      // The slot's first 32 bytes by the quad's four lanes in one store, then opacity and tag.
       float2* dst = reinterpret_cast<float2*>(g.slots + int64_t(slot) * kSlotWords);
-      dst[0] = make_float2(sx, sy);
-      dst[1] = make_float2(sxx, sxy);
-      dst[2] = make_float2(syy, gcr);
-      dst[3] = make_float2(gcg, gcb);
-      dst[4] = make_float2(has_opacity ? gop : 0.0f, tag);
+      const float lo = sub == 0 ? sx : (sub == 1 ? sxx : (sub == 2 ? syy : gcg));
+      const float hi = sub == 0 ? sy : (sub == 1 ? sxy : (sub == 2 ? gcr : gcb));
+      dst[sub] = make_float2(lo, hi);
+      if (sub == 0) dst[4] = make_float2(has_opacity ? gop : 0.0f, tag);
     }
   };
   int64_t start, end;

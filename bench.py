@@ -536,13 +536,14 @@ def main() -> int:
              torch.empty_like(op))
 
     stream = torch.cuda.current_stream(dev)
-    ev = {k: [torch.cuda.Event(enable_timing=True) for _ in range(2)] for k in ("b", "f", "g")}
     phase_ms = {"bin": [], "fwd": [], "bwd": []}
 
     stored = args.normalizer == "stored"
 
-    def step(record: bool, saved: bool = stored):
-        """One fwd+bwd; `saved`: the forward stores den and the backward reads it."""
+    def step(ev: dict | None, saved: bool = stored):
+        """One fwd+bwd; `ev`: event pairs to record around its three phases; `saved`: the
+        forward stores den and the backward reads it."""
+        record = ev is not None
         if record:
             ev["b"][0].record(stream)
         HR.build_tile_index_capacity(means, conics, radii, H, W, capacity, ts,
@@ -562,7 +563,7 @@ def main() -> int:
             ev["g"][1].record(stream)
 
     for _ in range(args.warmup):
-        step(False)
+        step(None)
     torch.cuda.synchronize()
     if int(status.item()) != 0:
         raise RuntimeError("pair capacity exceeded during warmup")
@@ -570,7 +571,7 @@ def main() -> int:
     # Whole-step timing: K back-to-back steps bracketed by device synchronization.
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(False)
+        step(None)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
@@ -580,12 +581,12 @@ def main() -> int:
     side = torch.cuda.Stream(device=dev)
     side.wait_stream(stream)
     with torch.cuda.stream(side):
-        step(False)  # warm the side stream once before capture
+        step(None)  # warm the side stream once before capture
     stream.wait_stream(side)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
-        step(False)
+        step(None)
     for _ in range(args.warmup):
         graph.replay()
     torch.cuda.synchronize()
@@ -597,10 +598,16 @@ def main() -> int:
     if int(status.item()) != 0:
         raise RuntimeError("pair capacity exceeded in the graph replays")
 
-    # Per-phase timing with events on the launch stream (a separate pass over K steps).
-    for _ in range(args.steps):
-        step(True)
-        torch.cuda.synchronize()
+    # Per-phase timing with events on the launch stream: a separate pass of K back-to-back
+    # steps, each phase between its own event pair, one synchronization at the end. The device
+    # stays ahead of the host, so a phase's time is the device's (synchronizing after every
+    # step would add the host's launch latency to each step's first phase).
+    evs = [{k: [torch.cuda.Event(enable_timing=True) for _ in range(2)] for k in ("b", "f", "g")}
+           for _ in range(args.steps)]
+    for ev in evs:
+        step(ev)
+    torch.cuda.synchronize()
+    for ev in evs:
         phase_ms["bin"].append(ev["b"][0].elapsed_time(ev["b"][1]))
         phase_ms["fwd"].append(ev["f"][0].elapsed_time(ev["f"][1]))
         phase_ms["bwd"].append(ev["g"][0].elapsed_time(ev["g"][1]))
@@ -617,7 +624,7 @@ def main() -> int:
         _native.kernel_timing(True)
         try:
             for _ in range(args.steps):
-                step(False, saved)
+                step(None, saved)
             torch.cuda.synchronize()
         finally:
             _native.kernel_timing(False)
@@ -629,11 +636,11 @@ def main() -> int:
     kernel_ms = kernel_times(stored)
     # The other normalizer mode: whole-step wall time and kernel times.
     for _ in range(args.warmup):
-        step(False, not stored)
+        step(None, not stored)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(False, not stored)
+        step(None, not stored)
     torch.cuda.synchronize()
     other_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     other_kernel_ms = kernel_times(not stored)

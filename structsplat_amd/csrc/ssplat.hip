@@ -126,12 +126,16 @@ constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
 // The 16-pixel backward stages fewer records per batch: with the recomputing sweep's masks its
 // LDS then stays within 20 KB, so eight workgroups (eight waves per SIMD) fit on a CU.
 constexpr int kBwdStage16 = 128;
-// The backward sums a (tile, Gaussian) pair whose rectangle covers at most this many pixels of
-// the tile in one lane (no cross-lane reduction); larger pairs take a whole wave.
-constexpr int kLaneArea = 64;
+// The backward sums a (tile, Gaussian) pair whose rectangle covers at most
+// kLaneClassArea * lane_classes(tile) pixels of the tile by a quad of lanes (a quad sum instead
+// of the wave's transposed reduction); larger pairs take a whole wave.  The small pairs are
+// listed by area classes of kLaneClassArea pixels.  The limits were measured: 96 pixels with
+// 16-pixel tiles, 128 with 32-pixel tiles.
+constexpr int kLaneClassArea = 16;
+constexpr int lane_classes(int tile) { return tile == 16 ? 6 : 8; }
 // Deterministic gradient accumulation.  Gaussian g owns R(g) slots, one per tile of its clipped
 // support rectangle (row-major); slots of consecutive Gaussians are consecutive.  The backward
-// writes each (tile, Gaussian) pair's nine partial gradients and the call's tag to the pair's
+// writes each (tile, Gaussian) pair's raw sums (nine values) and the call's tag to the pair's
 // slot, and the reduction sums every Gaussian's slots in a fixed order, so the gradients do not
 // depend on scheduling.  A slot whose tag is not the current call's (a tile the lists do not
 // hold, e.g. culled) counts as zero.
@@ -2244,7 +2248,8 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   // One small staged record (index j, -1: none) by one quad of lanes: lane `sub` of the quad
   // takes the record's pixels of this tile with row-major index sub, sub + 4, ... (the visit's
   // arithmetic, summed in registers); a fixed two-step DPP sum gives every lane of the quad the
-  // same totals and lane 0 stores the pair's slot.  All lanes of the wave call it together.
+  // same totals and the quad's lanes store the pair's slot.  All lanes of the wave call it
+  // together.
   auto quad_record = [&](int j, int sub) {
     float sx = 0.0f, sy = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
     float gcr = 0.0f, gcg = 0.0f, gcb = 0.0f, gop = 0.0f;
@@ -2262,15 +2267,35 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
       const int n = w * (min(static_cast<int>(yr >> 16), ty0 + kT) - iy0);
       const float mx = v0.x, my = v0.y, qa = v0.z, qb = v0.w, qc = v1.x;
       const float cr = v3.x, cg = v3.y, cbl = v3.z;
-      // Pixel `sub` of the row-major order, then steps of 4 pixels: dr rows and dc columns,
-      // with a carry into the next row.
-      const int dr = 4 / w, dc = 4 - dr * w;
-      int py = iy0 + sub / w;
-      int px = ix0 + sub - (py - iy0) * w;
+      // Pixel `sub` of the row-major order, then steps of 4 pixels: dr = 4 / w rows and dc
+      // columns, or, past the row's end, one row more and w columns fewer.  The pixel's
+      // coordinates are kept as floats (small integers, exact) and its adjoints' LDS address in
+      // bytes; the quotients by w (1 <= w) are spelled out for the numerators 0-4.
+      const int dr = w >= 5 ? 0 : (w == 1 ? 4 : (w == 2 ? 2 : 1));
+      const int dc = 4 - dr * w;
+      const int r0 = w >= 4 ? 0 : (w == 1 ? sub : (w == 2 ? sub >> 1 : (sub == 3 ? 1 : 0)));
+      float fx = static_cast<float>(ix0 + sub - r0 * w);
+      float fy = static_cast<float>(iy0 + r0);
+      const float fx_end = static_cast<float>(ix1 - dc);  // at or past it, the step wraps
+      // The steps stay registers (opaque values: otherwise the select of two conversions is
+      // folded into a conversion of a select, two more instructions per pixel).
+      float sx_n = static_cast<float>(dc), sx_w = static_cast<float>(dc - w);
+      float sy_n = static_cast<float>(dr), sy_w = static_cast<float>(dr + 1);
+      asm volatile("" : "+v"(sx_n), "+v"(sx_w), "+v"(sy_n), "+v"(sy_w));
+      constexpr int kRowBytes = kT * static_cast<int>(sizeof(float4));
+      const int sa_n = dr * kRowBytes + dc * static_cast<int>(sizeof(float4));
+      const int sa_w = sa_n + kRowBytes - w * static_cast<int>(sizeof(float4));
+      int addr = ((iy0 + r0 - ty0) * kT + (ix0 + sub - r0 * w - tx0)) *
+                 static_cast<int>(sizeof(float4));
+      const char* adj_base = reinterpret_cast<const char*>(s_adj);
       for (int k = sub; k < n; k += 4) {
-        const float4 ad = s_adj[(py - ty0) * kT + (px - tx0)];
-        const float dx = static_cast<float>(px) - mx;
-        const float dy = static_cast<float>(py) - my;
+        const float4 ad = *reinterpret_cast<const float4*>(adj_base + addr);
+        const float dx = fx - mx;
+        const float dy = fy - my;
+        const bool wrap = fx >= fx_end;
+        fx += wrap ? sx_w : sx_n;
+        fy += wrap ? sy_w : sy_n;
+        addr += wrap ? sa_w : sa_n;
         const float t = fmaf(qa, dx, qb * dy);
         const float e = fmaf(t, dx, qc * dy * dy);
         const float raw = exp2_hw(kFade ? e * kNegHalfLog2e : e);
@@ -2293,12 +2318,6 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
         sxx = fmaf(dqx, dx, sxx);
         sxy = fmaf(dqx, dy, sxy);
         syy = fmaf(dqr * dy, dy, syy);
-        px += dc;
-        py += dr;
-        if (px >= ix1) {
-          px -= w;
-          ++py;
-        }
       }
     }
     // Quad sums: (v0 + v1) + (v2 + v3) on every lane of the quad, bit for bit.
@@ -2548,15 +2567,21 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
       load_adj(adj);
       for (int j = wave; j < m; j += kWaves) wave_record(j, adj);
     } else {
-      // Records whose rectangle covers at most kLaneArea pixels of the tile are summed by a quad
-      // of lanes each over those pixels (adjoints from LDS): a wave takes 16 of them at a time,
-      // with only a quad sum to reduce; wave 0 sorts the batch into those and the larger ones
-      // (empty intersections have no slot and are dropped).  Both kinds are dealt round-robin
-      // to the waves: 16 small records a wave at a time, then one large record at a time.
+      // Records whose rectangle covers at most kClasses * kLaneClassArea pixels of the tile are
+      // summed by a quad of lanes each over those pixels (adjoints from LDS): a wave takes 16 of
+      // them at a time, with only a quad sum to reduce; wave 0 sorts the batch into those and
+      // the larger ones (empty intersections have no slot and are dropped).  The small ones are
+      // listed by area class (1-16, 17-32, ... pixels), so the 16 records a wave takes together
+      // need about the same number of steps (a wave runs as many as its largest record needs).
+      // Both kinds are dealt round-robin to the waves: 16 small records a wave at a time, then
+      // one large record at a time.
+      constexpr int kClasses = lane_classes(kT);
       if (wave == 0) {
-        int ns = 0, nl = 0;
-        for (int base = 0; base < m; base += 64) {
-          const int j = base + lane;
+        constexpr int kPerLane = kS / 64;
+        int cls[kPerLane];  // 0: no pixels, 1..kClasses: small, kClasses + 1: large
+#pragma unroll
+        for (int r = 0; r < kPerLane; ++r) {
+          const int j = r * 64 + lane;
           int area = 0;
           if (j < m) {
             const float4 v1 = st.c[1][j];
@@ -2567,13 +2592,25 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
                           max(static_cast<int>(yr & 0xffff), ty0);
             area = w > 0 && h > 0 ? w * h : 0;
           }
-          const bool small = area > 0 && area <= kLaneArea;
-          const bool large = area > kLaneArea;
-          const uint64_t bs = __ballot(small), bl = __ballot(large);
-          if (small) s_small[ns + lanes_below(bs)] = static_cast<uint16_t>(j);
-          if (large) s_large[nl + lanes_below(bl)] = static_cast<uint16_t>(j);
-          ns += static_cast<int>(__popcll(bs));
-          nl += static_cast<int>(__popcll(bl));
+          cls[r] = min((area + kLaneClassArea - 1) / kLaneClassArea, kClasses + 1);
+        }
+        int ns = 0, nl = 0;
+#pragma unroll
+        for (int c = 1; c <= kClasses; ++c) {
+#pragma unroll
+          for (int r = 0; r < kPerLane; ++r) {
+            const bool in = cls[r] == c;
+            const uint64_t b = __ballot(in);
+            if (in) s_small[ns + lanes_below(b)] = static_cast<uint16_t>(r * 64 + lane);
+            ns += static_cast<int>(__popcll(b));
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kPerLane; ++r) {
+          const bool in = cls[r] == kClasses + 1;
+          const uint64_t b = __ballot(in);
+          if (in) s_large[nl + lanes_below(b)] = static_cast<uint16_t>(r * 64 + lane);
+          nl += static_cast<int>(__popcll(b));
         }
         if (lane == 0) {
           s_nsmall = ns;

@@ -122,6 +122,15 @@ __device__ __forceinline__ UnitGeo unit_geo(const LossArgs& a, int unit) {
   return g;
 }
 
+// A float at a byte offset from a base: with a uniform base and a 32-bit offset the access
+// takes the scalar-base form of the memory instructions (no 64-bit address arithmetic per lane).
+__device__ __forceinline__ float load_at(const float* base, uint32_t byte_off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void store_at(float* base, uint32_t byte_off, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 // Sum over the workgroup (returned to thread 0).  Every thread calls it.
 __device__ float block_sum(float v) {
   __shared__ float s_red[kLossThreads / 64];
@@ -159,8 +168,13 @@ __device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], in
   const int gx = x0 - kHalo + col;
   const bool x_ok = active && gx >= 0 && gx < g.W;
   const bool x_center = col >= kHalo && col < kHalo + kUnitW;
-  const int64_t row_stride = int64_t(g.W) * 3;
-  const int64_t at0 = (g.first_pixel + int64_t(y0 - kHalo + r0) * g.W + gx) * 3 + ch;
+  // Addresses: the staged window's first element (uniform; it may lie outside the image) plus a
+  // 32-bit byte offset per lane, which is non-negative for every staged position.  A position outside
+  // the image loads the unit's own first element (offset `safe`, inside the image) and stores 0.
+  const int64_t origin = (g.first_pixel + int64_t(y0 - kHalo) * g.W + (x0 - kHalo)) * 3;
+  const uint32_t row_bytes = static_cast<uint32_t>(g.W) * 12u;
+  const uint32_t safe = kHalo * row_bytes + kHalo * 12u;
+  uint32_t off = static_cast<uint32_t>(r0) * row_bytes + static_cast<uint32_t>(c3) * 4u;
   const int lds0 = (ch * kSpan + r0) * kStride + col;
   float v[kMaps][kPasses];
   bool ok[kPasses];
@@ -169,12 +183,13 @@ __device__ __forceinline__ float stage_halo(const float* const (&src)[kMaps], in
     const int row = r0 + kPerPass * k;
     const int gy = y0 - kHalo + row;
     ok[k] = x_ok && row < kSpan && gy >= 0 && gy < g.H;
-    const int64_t at = ok[k] ? at0 + int64_t(kPerPass * k) * row_stride : 0;
+    const uint32_t at = ok[k] ? off : safe;
 #pragma unroll
     for (int m = 0; m < kMaps; ++m) {
-      const float x = src[m][at];
+      const float x = load_at(src[m] + origin, at);
       v[m][k] = ok[k] ? x : 0.0f;
     }
+    off += kPerPass * row_bytes;
   }
   float l1 = 0.0f;
 #pragma unroll
@@ -303,15 +318,24 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     const int col = threadIdx.x & (kUnitW - 1);
     const int r0 = ((threadIdx.x >> 4) & 3) * kVSeg;
     const int x = g.x0 + col;
+    // Output addresses: the unit's first element (uniform) plus a 32-bit byte offset per lane;
+    // an output outside the image reads the unit's first element instead (offset 0).
+    const int64_t origin = (g.first_pixel + int64_t(g.y0) * g.W + g.x0) * 3;
+    const int64_t plane = a.pixels * 3;
+    const uint32_t row_bytes = static_cast<uint32_t>(g.W) * 12u;
+    const uint32_t off0 =
+        static_cast<uint32_t>(r0) * row_bytes + static_cast<uint32_t>(col * 3 + ch) * 4u;
     // The target's moments at the outputs, loaded before the blur so it hides their latency.
     float om[2][kVSeg];
     if constexpr (kMode == kStatsMoments) {
+      const float* m0_base = a.tmom + origin;
+      const float* m1_base = a.tmom + plane + origin;
 #pragma unroll
       for (int o = 0; o < kVSeg; ++o) {
         const int y = g.y0 + r0 + o;
         const bool ok = x < g.W && y < g.H;
-        const int64_t at = ok ? (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch : 0;
-        const float m0 = a.tmom[at], m1 = a.tmom[a.pixels * 3 + at];
+        const uint32_t at = ok ? off0 + o * row_bytes : 0u;
+        const float m0 = load_at(m0_base, at), m1 = load_at(m1_base, at);
         om[0][o] = m0;
         om[1][o] = m1;
       }
@@ -334,10 +358,10 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
     for (int o = 0; o < kVSeg; ++o) {
       const int y = g.y0 + r0 + o;
       if (x >= g.W || y >= g.H) continue;
-      const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+      const uint32_t at = off0 + o * row_bytes;
       if constexpr (kTarget) {
-        a.tmom_out[at] = st[0][o];
-        a.tmom_out[a.pixels * 3 + at] = st[1][o];
+        store_at(a.tmom_out + origin, at, st[0][o]);
+        store_at(a.tmom_out + plane + origin, at, st[1][o]);
       } else {
         // No contraction into fused multiply-adds here: both ways of obtaining the target's
         // moments then give the same S and derivatives bit for bit.
@@ -363,9 +387,10 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_stats(LossArgs a) {
         const float inv_b1 = B2 * inv_d, inv_b2 = B1 * inv_d;
         ssim_sum += S;
         if (a.keep) {
-          a.abc[at] = 2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2);
-          a.abc[a.pixels * 3 + at] = -S * inv_b2;
-          a.abc[2 * a.pixels * 3 + at] = 2.0f * A1 * inv_d;
+          store_at(a.abc + origin, at,
+                   2.0f * mu_t * (A2 - A1) * inv_d - 2.0f * mu_p * S * (inv_b1 - inv_b2));
+          store_at(a.abc + plane + origin, at, -S * inv_b2);
+          store_at(a.abc + 2 * plane + origin, at, 2.0f * A1 * inv_d);
         }
       }
     }
@@ -472,14 +497,20 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
       const int col = threadIdx.x & (kGT - 1);
       const int r0 = ((threadIdx.x >> 4) & 3) * kGVSeg;
       const int x = g.x0 + col;
+      // Output addresses: the half's first element (uniform) plus a 32-bit byte offset per
+      // lane; an output outside the image reads the half's first element instead (offset 0).
+      const int64_t origin = (g.first_pixel + int64_t(y0) * g.W + g.x0) * 3;
+      const uint32_t row_bytes = static_cast<uint32_t>(g.W) * 12u;
+      const uint32_t off0 =
+          static_cast<uint32_t>(r0) * row_bytes + static_cast<uint32_t>(col * 3 + ch) * 4u;
       float pv[kGVSeg], tv[kGVSeg];  // p and t at the outputs, loaded before the blur
 #pragma unroll
       for (int o = 0; o < kGVSeg; ++o) {
         const int y = y0 + r0 + o;
         const bool ok = x < g.W && y < g.H;
-        const int64_t at = ok ? (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch : 0;
-        pv[o] = a.img[at];
-        tv[o] = a.target[at];
+        const uint32_t at = ok ? off0 + o * row_bytes : 0u;
+        pv[o] = load_at(a.img + origin, at);
+        tv[o] = load_at(a.target + origin, at);
       }
       float bl[3][kGVSeg];
 #pragma unroll
@@ -499,12 +530,12 @@ __global__ __launch_bounds__(kLossThreads) void k_loss_grad(LossArgs a) {
       for (int o = 0; o < kGVSeg; ++o) {
         const int y = y0 + r0 + o;
         if (x >= g.W || y >= g.H) continue;
-        const int64_t at = (g.first_pixel + int64_t(y) * g.W + x) * 3 + ch;
+        const uint32_t at = off0 + o * row_bytes;
         const float p = pv[o], t = tv[o];
         const float d = p - t;
         const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
         const float dssim = bl[0][o] + 2.0f * p * bl[1][o] + t * bl[2][o];
-        a.grad[at] = scale * (g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim));
+        store_at(a.grad + origin, at, scale * (g.inv_n * (a.w_l1 * sg - a.w_ssim * dssim)));
       }
     }
   }

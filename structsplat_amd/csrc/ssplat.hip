@@ -1330,6 +1330,8 @@ __device__ __forceinline__ void wave_sort_list(int32_t* g, int L, int lane) {
   }
 }
 
+constexpr int kWaveSortMax = 1024;  // the longest list one wave sorts in registers
+
 constexpr int kStripCountOnly = 0;  // level 2 writes the strip's offsets
 constexpr int kStripBuild = 1;      // offsets and lists
 constexpr int kStripFill = 2;       // lists, placed by the caller's offsets
@@ -1349,7 +1351,7 @@ __device__ __forceinline__ void sort_list_to(const int32_t* src, int32_t* dst, i
   }
 }
 
-// One wave: src[0, L) sorted into dst[0, n) (L <= 1024; returns false for longer lists).
+// One wave: src[0, L) sorted into dst[0, n) (L <= kWaveSortMax; returns false for longer lists).
 __device__ __forceinline__ bool wave_sort_to(const int32_t* src, int32_t* dst, int L, int n,
                                              int lane) {
   if (L <= 64) {
@@ -1360,7 +1362,7 @@ __device__ __forceinline__ bool wave_sort_to(const int32_t* src, int32_t* dst, i
     sort_list_to<4>(src, dst, L, n, lane);
   } else if (L <= 512) {
     sort_list_to<8>(src, dst, L, n, lane);
-  } else if (L <= 1024) {
+  } else if (L <= kWaveSortMax) {
     sort_list_to<16>(src, dst, L, n, lane);
   } else {
     return false;
@@ -1373,12 +1375,11 @@ __device__ __forceinline__ bool wave_sort_to(const int32_t* src, int32_t* dst, i
 // computed here) and the chunk's pairs before the strip, whose sum is the strip's first list
 // position.  Candidates are in chunk order (ascending ids between chunks, any order inside one).
 //
-// Usual case (at most kStripCache candidates): the candidates are loaded into LDS once, and each
-// wave compacts, for each of its tiles, the candidates holding that tile into the tile's LDS
-// segment (a ballot and a bit count per 64 candidates; the counts come out exact even when a
-// segment is full); a wave scan of the counts gives the tiles' offsets; each tile's pairs are
-// then sorted by one wave in registers straight into its list.  Otherwise (more candidates, or
-// a tile beyond its segment): ballots count the pairs per tile, the pairs are placed per tile
+// The candidates pass through LDS kStripCache at a time, and each wave compacts, for each of its
+// tiles, the candidates holding that tile into the tile's LDS segment (a ballot and a bit count
+// per 64 candidates; the counts come out exact even when a segment is full); a wave scan of the
+// counts gives the tiles' offsets; each tile's pairs are then sorted by one wave in registers
+// straight into its list.  When a tile's pairs outgrow its segment, the pairs are placed per tile
 // in LDS in any order (in the lists themselves when the strip has more than kStripPairsLds) and
 // each list is sorted.  Fill mode places by the caller's offsets (clamped to gid_capacity) and
 // flags *status when a tile holds more pairs than its range.  Eight waves a SIMD: every
@@ -1485,25 +1486,31 @@ void k_strip_level2(BinArgs a, StripArgs sa, int32_t* offsets, int32_t* gids,
     }
   };
 
-  // Pairs per tile.
+  // Pairs per tile: the candidates pass through the LDS cache kStripCache at a time, and each
+  // wave compacts, for each of its tiles, the cached candidates holding that tile into the
+  // tile's LDS segment after the earlier rounds' (a ballot and a bit count per 64 candidates;
+  // the counts come out exact even when a segment is full).  `cached`: the cache ends up holding
+  // every candidate.
   const bool cached = count <= kStripCache;
-  if (cached) {
-    for (int64_t base = 0; base < count; base += kStripL2Threads) {
-      const int64_t j = base + tid;
+  for (int64_t r0 = 0; r0 < count; r0 += kStripCache) {
+    const int m = static_cast<int>(count - r0 < kStripCache ? count - r0 : kStripCache);
+    if (r0 > 0) __syncthreads();  // the previous round's compaction is done with the cache
+    for (int i = tid; i < m; i += kStripL2Threads) {
       int g;
-      uint32_t m;
-      candidate(j, &g, &m);
-      if (j < count) s_cache[j] = make_int2(g, static_cast<int>(m));
+      uint32_t mk;
+      candidate(r0 + i, &g, &mk);
+      s_cache[i] = make_int2(g, static_cast<int>(mk));
     }
     __syncthreads();
-    // Compaction: a wave per tile, candidates in order (64 a step).
-    const int rounds = static_cast<int>((count + 63) >> 6);
+    // Compaction: a wave per tile, candidates in order (64 a step).  Tile k's count is read and
+    // written by its wave only (lane 0 writes it, the wave's later LDS reads are ordered after).
+    const int rounds = (m + 63) >> 6;
     for (int k = wave; k < S; k += kWaves) {
       int32_t* dst = s_buf + k * seg;
-      int n = 0;
+      int n = s_cnt[k];
       for (int r = 0; r < rounds; ++r) {
         const int j = (r << 6) + lane;
-        const int2 e = j < count ? s_cache[j] : make_int2(-1, 0);
+        const int2 e = j < m ? s_cache[j] : make_int2(-1, 0);
         const bool hit = (static_cast<uint32_t>(e.y) >> k) & 1u;
         const uint64_t b = __ballot(hit);
         const int pos = n + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
@@ -1514,21 +1521,8 @@ void k_strip_level2(BinArgs a, StripArgs sa, int32_t* offsets, int32_t* gids,
       }
       if (lane == 0) {
         s_cnt[k] = n;
-        if (n > seg) s_full = 1;
-      }
-    }
-  } else {
-    for (int64_t base = 0; base < count; base += kStripL2Threads) {
-      const int64_t j = base + tid;
-      int g;
-      uint32_t m;
-      candidate(j, &g, &m);
-      uint32_t wm = wave_or(m);
-      while (wm != 0u) {
-        const int k = __ffs(wm) - 1;
-        wm &= wm - 1;
-        const uint64_t b = __ballot((m >> k) & 1u);
-        if (lane == 0) atomicAdd(&s_cnt[k], static_cast<int>(__popcll(b)));
+        // A tile beyond its segment, or beyond what one wave sorts: the general path below.
+        if (n > seg || n > kWaveSortMax) s_full = 1;
       }
     }
   }
@@ -1576,7 +1570,7 @@ void k_strip_level2(BinArgs a, StripArgs sa, int32_t* offsets, int32_t* gids,
   if (kMode == kStripCountOnly) return;
   __syncthreads();
 
-  if (cached && s_full == 0) {
+  if (s_full == 0) {
     // Each tile's compacted pairs sorted by one wave into its list.
     for (int k = wave; k < S; k += kWaves) {
       wave_sort_to(s_buf + k * seg, gids + s_out[k], s_cnt[k], s_len[k], lane);

@@ -16,13 +16,13 @@
 // and, the window being symmetric, dSSIM/dp = (G*a + 2 p G*b + t G*c) / n for the three maps
 // a, b, c above (zero outside the image).
 //
-// Kernels, one 192-thread workgroup per unit, a strip 16 columns wide and up to 128 rows tall
-// (ssplat_loss_plan numbers the units of every field):
-//   k_loss_stats : walks down the strip 16 rows at a time: stages p and t with a 5-pixel halo
-//                  in LDS, blurs p, p^2, p t (and t, t^2 without precomputed target moments)
-//                  horizontally into a ring of rows, then vertically, writes a, b, c and the
-//                  unit's partial loss.
-//   k_loss_grad  : the same walk over a, b, c; adds the SSIM gradient to the L1 gradient.
+// Kernels, over units of 16 columns by 32 rows (ssplat_loss_plan numbers the units of every
+// field):
+//   k_loss_stats : one 256-thread workgroup per unit: stages p and t with a 5-pixel halo in LDS,
+//                  blurs p, p^2, p t (and t, t^2 without precomputed target moments)
+//                  horizontally, then vertically, writes a, b, c and the unit's partial loss.
+//   k_loss_grad  : two 256-thread workgroups per unit, one per 16 x 16 half: the same blurs of
+//                  a, b, c; adds the SSIM gradient to the L1 gradient.
 //   k_loss_l1    : the loss and gradient without SSIM.
 //   k_loss_total : one workgroup sums the unit partials in a fixed order (deterministic loss).
 
@@ -35,11 +35,11 @@ namespace {
 
 using namespace ssplat_detail;
 
-// Geometry.  A unit of work is 16 columns by 32 rows of one image (clipped by the image), one
-// 256-thread workgroup.  The stats pass blurs the unit's 42 staged rows horizontally (42 / 32 rows
-// per output row; a 16 x 16 tile would blur 26 / 16) and its 32 output rows vertically.  The
-// gradient pass works on the unit's two 16 x 16 halves in turn (three staged maps of a 16 x 32
-// unit would not leave room in LDS for more than three workgroups per CU).
+// Geometry.  A unit of work is 16 columns by 32 rows of one image (clipped by the image).  The
+// stats pass blurs the unit's 42 staged rows horizontally (42 / 32 rows per output row; a
+// 16 x 16 tile would blur 26 / 16) and its 32 output rows vertically.  The gradient pass works on
+// the unit's two 16 x 16 halves in two workgroups (three staged maps of a 16 x 32 unit would not
+// leave room in LDS for more than three workgroups per CU).
 constexpr int kUnitW = 16;
 constexpr int kUnitH = 32;
 constexpr int kHalo = 5;                     // window radius

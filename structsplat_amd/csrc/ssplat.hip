@@ -1870,11 +1870,13 @@ __device__ __forceinline__ uint64_t quadrant_mask(uint32_t xr, uint32_t yr, int 
 // into its LDS slice; a ballot keeps the records that meet the quadrant, and each lane reads its
 // bit of their masks.
 // kDenOnly: only the denominator D (acc.w) is accumulated (the backward's normalizer
-// recomputation); it is the same sum, in the same order, as the forward's D.
+// recomputation); it is the same sum, in the same order, as the forward's D.  staged: the
+// caller has staged the tile's only batch already (with a barrier after it).
 template <bool kFade, bool kDenOnly, int kChunks, int kS, bool kAffine = false, bool kOpaque = false>
 __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLds<kChunks, kS>& st,
                                                     uint32_t* s_mask, int tile, int qx0, int qy0,
-                                                    int px, int py, float tail, float4& acc) {
+                                                    int px, int py, float tail, float4& acc,
+                                                    bool staged = false) {
   const int lane = thread_index<kOpaque>() & 63;
   const float fx = static_cast<float>(px);
   const float fy = static_cast<float>(py);
@@ -1883,12 +1885,14 @@ __device__ __forceinline__ void accumulate_quadrant(const RenderArgs& a, StageLd
   float nr = 0.0f, ng = 0.0f, nb = 0.0f, d = 0.0f;
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    if constexpr (kAffine && !kDenOnly) {
-      stage_batch<4, kChunks, kS, true, false, kOpaque>(a, st, batch, m);
-    } else {
-      stage_batch<3, kChunks, kS, false, false, kOpaque>(a, st, batch, m);
+    if (!staged) {
+      if constexpr (kAffine && !kDenOnly) {
+        stage_batch<4, kChunks, kS, true, false, kOpaque>(a, st, batch, m);
+      } else {
+        stage_batch<3, kChunks, kS, false, false, kOpaque>(a, st, batch, m);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     uint64_t hit[kS / 64];
 #pragma unroll
     for (int k = 0; k < kS / 64; ++k) {
@@ -2166,6 +2170,20 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   const int ty0 = tile_y * kT;
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
 
+  // A tile whose list fits one staged batch (most tiles of dense fields) stages it once, with
+  // the gradient sweep's chunks and slots, before the normalizer sweep, which reads the same
+  // records; the gradient sweep then skips its staging.
+  constexpr int kLg = kT == 16 ? 4 : 5;
+  int64_t start, end;
+  tile_range(a, tile, &start, &end);
+  const bool single = kRecompute && end - start <= kS;
+  if (single) {
+    stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(
+        a, st, start, static_cast<int>(end - start),
+        SlotTile{tile_x, tile_y, kLg, g.slot_capacity});
+    __syncthreads();
+  }
+
   // Per-pixel adjoints (render_ext.cu:496-510):
   //   dL/dN_k = G_k/(D+eps), dL/dD = -sum_k G_k out_k/(D+eps)  (additive: G_k and 0).
   if (kRecompute) {
@@ -2178,8 +2196,7 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
     uint32_t* s_mask = reinterpret_cast<uint32_t*>(s_scratch) + wave * (2 * kS);
     float4 acc;
     accumulate_quadrant<kFade, kNorm == kNormRecompD, kChunks, kS, kAffine, kOpaque>(
-        a, st, s_mask, tile, qx0, qy0, px, py,
-                                                      tail, acc);
+        a, st, s_mask, tile, qx0, qy0, px, py, tail, acc, single);
     // Every wave is past the sweep's last barrier: the mask space can take the adjoints.
     float4 adj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (px < f.width && py < f.height) {
@@ -2228,7 +2245,6 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
   // wave record reads the four quadrants' adjoints into registers once.
   const bool has_opacity = g.has_opacity;
   const float tag = __uint_as_float(*g.epoch);
-  constexpr int kLg = kT == 16 ? 4 : 5;
   // Lane roles of the transposed reduction: lane 8k+7 owns gradient component k
   // (0-1 mean x/y, 2-4 conic a/b/c, 5-7 color r/g/b); lane 63 also the opacity and the tag.
   const bool hi8 = (lane & 8) != 0;
@@ -2330,14 +2346,14 @@ __device__ __forceinline__ void backward_tile(const RenderArgs& a, const GradArg
       if (sub == 0) dst[4] = make_float2(has_opacity ? gop : 0.0f, tag);
     }
   };
-  int64_t start, end;
-  tile_range(a, tile, &start, &end);
   for (int64_t batch = start; batch < end; batch += kS) {
     const int m = static_cast<int>(end - batch < kS ? end - batch : kS);
-    stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(a, st, batch, m,
-                                                       SlotTile{tile_x, tile_y, kLg,
-                                                                g.slot_capacity});
-    __syncthreads();
+    if (!single) {
+      stage_batch<4, kChunks, kS, kAffine, true, kOpaque>(a, st, batch, m,
+                                                         SlotTile{tile_x, tile_y, kLg,
+                                                                  g.slot_capacity});
+      __syncthreads();
+    }
     // One staged record by the whole wave: each lane visits its pixel of every quadrant the
     // rectangle meets, then the wave's transposed reduction and one slot store.
     // With 16-pixel tiles `adj` holds the lane's adjoints of the four quadrants.

@@ -126,6 +126,9 @@ constexpr int kStage = 256;  // Gaussian records staged in LDS per batch
 // The 16-pixel backward stages fewer records per batch: with the recomputing sweep's masks its
 // LDS then stays within 20 KB, so eight workgroups (eight waves per SIMD) fit on a CU.
 constexpr int kBwdStage16 = 128;
+// The 16-pixel forward too: its filter pass walks the batch 64 records at a time, and tile lists
+// are mostly shorter than 128.
+constexpr int kFwdStage16 = 128;
 // The backward sums a (tile, Gaussian) pair whose rectangle covers at most
 // kLaneClassArea * lane_classes(tile) pixels of the tile by a quad of lanes (a quad sum instead
 // of the wave's transposed reduction); larger pairs take a whole wave.  The small pairs are
@@ -2074,8 +2077,9 @@ template <int kT, bool kFade, bool kNormalize, bool kAffine, bool kOpaque = fals
 __device__ __forceinline__ void forward_tile(const RenderArgs& a, int tile, float* __restrict__ out,
                                              float* __restrict__ den) {
   constexpr int kQ = kT / 8;  // quadrants per tile row
-  __shared__ StageLds<kAffine ? 4 + kAffineChunks : 3> st;
-  __shared__ uint32_t s_mask[kQ * kQ][2 * kStage];  // per-wave quadrant masks
+  constexpr int kS = kT == 16 ? kFwdStage16 : kStage;  // records per staged batch
+  __shared__ StageLds<kAffine ? 4 + kAffineChunks : 3, kS> st;
+  __shared__ uint32_t s_mask[kQ * kQ][2 * kS];  // per-wave quadrant masks
   const ssplat_field f = field_of_tile(a.fields, tile);
   const int local = tile - static_cast<int>(f.first_tile);
   const int tile_x = local % f.tiles_x;
@@ -2088,7 +2092,7 @@ __device__ __forceinline__ void forward_tile(const RenderArgs& a, int tile, floa
   const int py = qy0 + (lane >> 3);
   const float tail = kFade ? a.fade_alpha * exp2_hw(a.tail_exp) : 0.0f;
   float4 acc;
-  accumulate_quadrant<kFade, false, kAffine ? 4 + kAffineChunks : 3, kStage, kAffine, kOpaque>(
+  accumulate_quadrant<kFade, false, kAffine ? 4 + kAffineChunks : 3, kS, kAffine, kOpaque>(
       a, st, s_mask[wave], tile, qx0, qy0, px, py, tail, acc);
   if (px < f.width && py < f.height) {
     const int64_t flat = f.first_pixel + int64_t(py) * f.width + px;

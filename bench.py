@@ -459,8 +459,11 @@ def _progress(msg: str) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 100 untimed steps bring the host's launch path and the GPU's clocks to a steady state (with
+    # 5 the eager step read about 10% slower at identical kernel times); 100 timed steps are
+    # about 35 ms of work at 2048x2048/200k.
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="metric", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--normalizer", default="recompute", choices=["recompute", "stored"],

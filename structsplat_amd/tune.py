@@ -58,7 +58,7 @@ def _time_ms(fn, reps: int) -> float:
 @torch.no_grad()
 def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, normalize=True,
                    eps=1e-8, support_fade=False, sigma_cutoff=3.0, recompute_normalizer=True,
-                   candidates=CANDIDATE_TILE_SIZES, reps=3):
+                   candidates=CANDIDATE_TILE_SIZES, reps=5):
     """Fastest tile configuration (edge, with render.TILE_PERSISTENT for the persistent
     schedule) for rendering (fwd+bwd) the fields `sizes` [(H, W)] with `counts` Gaussians each
     (rows of `means`, ... concatenated in field order).
@@ -85,7 +85,7 @@ def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, no
     hit = _CACHE.get(key)
     if hit is not None:
         return hit
-    timings = {}
+    steps = []
     for ts in candidates:
         b = Batch(sizes, counts, dev, tile_size=ts)
         pairs = b.pair_count(means, conics, radii, cull, sigma_cutoff)
@@ -99,15 +99,23 @@ def tune_tile_size(sizes, counts, means, conics, colors, radii, opacities, *, no
         bufs = (gids, offsets, status)
         slots = HR.grad_slots(pairs + 1, dev)
 
-        def step():
-            b.build_tile_index(means, conics, radii, pairs + 1, cull, sigma_cutoff, out=bufs)
+        def step(b=b, gids=gids, offsets=offsets, bufs=bufs, den=den, grad=grad, out=out,
+                 slots=slots, status=status, cap=pairs + 1):
+            b.build_tile_index(means, conics, radii, cap, cull, sigma_cutoff, out=bufs)
             b.forward(means, conics, colors, radii, op, gids, offsets, normalize, eps, alpha,
                       sigma_cutoff, out=out, den=den)
             b.backward(grad, means, conics, colors, radii, op, den, out, gids, offsets,
-                       normalize, eps, alpha, sigma_cutoff, slots=slots, slot_capacity=pairs + 1,
+                       normalize, eps, alpha, sigma_cutoff, slots=slots, slot_capacity=cap,
                        status=status, after_forward=True)
 
-        timings[ts] = _time_ms(step, reps)
+        steps.append((ts, step))
+    # Every candidate runs twice before any is timed: the first launches of a process meet a GPU
+    # that is still raising its clocks, which would penalize whichever candidate came first.
+    for _, step in steps:
+        step()
+        step()
+    torch.cuda.synchronize()
+    timings = {ts: _time_ms(step, reps) for ts, step in steps}
     best = min(timings, key=timings.get)
     _CACHE[key] = (best, timings)
     return best, timings

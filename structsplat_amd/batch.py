@@ -27,7 +27,7 @@ from torch.autograd.function import once_differentiable
 from . import _native
 from . import fit as F
 from .gaussians import GaussianField
-from .loss import batch_l1_ssim_loss, batch_target_moments, loss_plan
+from .loss import batch_l1_ssim_loss, loss_plan
 from .metrics import psnr_tensor
 from .tune import tune_field, tune_tile_size
 from .render import _fade_alpha, _ptr, _stream, grad_slots, tile_edge
@@ -279,7 +279,7 @@ def fit_batch(fields: list[GaussianField], targets: list[torch.Tensor], cfg: F.F
 
     batch = Batch(sizes, counts, dev, tile_size=F.resolve_tile_size(cfg, tune))
     targets_flat = torch.cat([t.reshape(-1, 3) for t in targets], 0).contiguous()
-    moments = batch_target_moments(batch, targets_flat) if cfg.ssim_weight > 0.0 else None
+    moments = None  # blurred inline by the loss, as in fit.fit (the same bits, less traffic)
     opt = F.make_optimizer(big, cfg)
     # Per-field log-scale upper bounds (one per element when a field caps its scales).
     uppers = [F.log_scale_upper(f, h, w) for f, (h, w) in zip(fields, sizes)]

@@ -31,7 +31,7 @@ from torch.autograd.function import once_differentiable
 
 from . import render as HR
 from .gaussians import GaussianField
-from .loss import l1_ssim_loss, target_moments
+from .loss import l1_ssim_loss
 from .optim import FusedAdam
 from .tune import tune_field
 from .metrics import psnr_tensor
@@ -229,7 +229,7 @@ def loss_fn(img: torch.Tensor, target: torch.Tensor, w: float,
             moments: torch.Tensor | None = None) -> torch.Tensor:
     """(1 - w) * L1 + w * (1 - SSIM) (fit.py:3219-3244 with pixel_loss="l1"), computed with its
     gradient by the fused HIP loss kernels; ``moments``: the target's window moments
-    (``loss.target_moments``), computed once per fit."""
+    (``loss.target_moments``), or None to blur them inline (the same bits)."""
     return l1_ssim_loss(img, target, w, moments)
 
 
@@ -317,7 +317,10 @@ def fit(field: GaussianField, target: torch.Tensor, cfg: FitConfig, graph: bool 
     plan = new_plan(_capacity(pair_count(field, H, W, cfg, ts), cfg))
     state = {"loss": None}
     target = target.contiguous()
-    moments = target_moments(target) if cfg.ssim_weight > 0.0 else None
+    # The loss blurs the target's window moments inline (five maps a step) rather than reading
+    # them precomputed (loss.target_moments): its stats pass is bound by memory traffic, and the
+    # precomputed moments are 24 more bytes a pixel to read each step.  Both give the same bits.
+    moments = None
 
     def step():
         # Gradients are assigned, not accumulated: no fill and add kernels per parameter.

@@ -332,6 +332,38 @@ def test_tile_index_long_lists_bit_exact(cuda, n_dense):
     assert to_np(gids).tolist() == ref_g.tolist()
 
 
+@pytest.mark.parametrize("cull", [False, True])
+def test_tile_index_strip_cache_rounds_bit_exact(cuda, cull):
+    """One strip met by about 2000 Gaussians: at 2048x2048 with 16-pixel tiles a strip is 16
+    tiles of one tile row, so the strip of tile row 10, tiles 0-15, gets more candidates than
+    level 2's LDS cache holds (1024) and takes them in two rounds, while every tile's list stays
+    within its LDS segment (256); the lists must equal the oracle's."""
+    _print_case(test="strip cache rounds", cull=cull)
+    rng = np.random.default_rng(5)
+    H = W = 2048
+    n_dense = 2000
+    dense = np.stack([rng.uniform(4, 252, n_dense), rng.uniform(163, 173, n_dense)], 1)
+    sparse = np.stack([rng.uniform(0, W, 3000), rng.uniform(0, H, 3000)], 1)
+    means_np = np.concatenate([dense, sparse]).astype(np.float32)
+    order = rng.permutation(len(means_np))
+    n = len(order)
+    f = {"means": means_np[order],
+         "scales": np.exp(rng.uniform(np.log(0.5), np.log(1.0), (n, 2))).astype(np.float32),
+         "angles": rng.uniform(0, np.pi, n).astype(np.float32),
+         "colors": rng.random((n, 3)).astype(np.float32), "opacity": None}
+    means, conics, colors, radii, _ = field_tensors(f, cuda)
+    gids, offsets = HR.build_tile_index(means, conics, radii, H, W, 16, ellipse_cull=cull)
+    torch.cuda.synchronize()
+    ref_g, ref_o = tile_index.build_tile_index(to_np(means), to_np(conics), to_np(radii), H, W,
+                                               16, ellipse_cull=cull)
+    strip = range(10 * 128, 10 * 128 + 16)
+    members = {int(g) for t in strip for g in ref_g[ref_o[t]:ref_o[t + 1]]}
+    assert len(members) > 1024
+    assert max(int(ref_o[t + 1] - ref_o[t]) for t in strip) <= 256
+    assert to_np(offsets).tolist() == ref_o.tolist()
+    assert to_np(gids).tolist() == ref_g.tolist()
+
+
 @pytest.mark.parametrize("fade", [False, True, 0.5])
 def test_normalized_render_is_invariant_to_opacity_scale(cuda, fade):
     """Scaling every opacity by k = 4 (a power of two, so every weight and partial sum scales

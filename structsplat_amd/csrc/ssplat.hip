@@ -1015,7 +1015,7 @@ constexpr int kStripIdBits = 19;        // level 1 packs (strip, id in chunk) in
 constexpr int kStripMaxChunk = 1 << kStripIdBits;
 constexpr int kStripL1Lds = 65024;      // level-1 dynamic LDS: counters plus the entry list
 constexpr int kStripEntriesPerGaussian = 8;  // a chunk's region: 8 entries per Gaussian
-constexpr int kStripCache = 1024;       // level-2 candidates kept in LDS between passes
+constexpr int kStripCache = 1024;       // level-2 candidates in LDS per round
 constexpr int kStripPairsLds = 4096;    // pairs a strip places in LDS (more: in the lists)
 constexpr int kStripL1PerThread = 8;    // strips a level-1 thread scans
 static_assert(kStripMax <= kStripL1Threads * kStripL1PerThread, "level 1 scans every strip");
@@ -1408,7 +1408,7 @@ void k_strip_level2(BinArgs a, StripArgs sa, int32_t* offsets, int32_t* gids,
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int S = 1 << sa.lg;
-  const int seg = kStripPairsLds >> sa.lg;  // a tile's LDS segment in the usual case
+  const int seg = kStripPairsLds >> sa.lg;  // a tile's LDS segment in the compaction path
   const int s = xcd_remap(blockIdx.x, gridDim.x);
   const int s0 = s << sa.lg;
   if (tid < 32) {
